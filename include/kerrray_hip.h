@@ -1,0 +1,168 @@
+/*
+ * kerrray_hip.h - C ABI of the MI355X (gfx950) null-geodesic backend.
+ *
+ * Plain C: pointers, sizes and POD structs only.  Every entry point returns
+ * a kr_status; kr_last_error() gives the message of the most recent failure
+ * on the calling thread.  No entry point aborts or throws across the ABI.
+ *
+ * The reference this replaces is the Python/Numba "numba" backend of KerrRay
+ * (src/kerrray/raytracing/backends/numba_backend.py); each declaration below
+ * cites the reference interface it stands in for.
+ *
+ * Conventions
+ *  - Photon state y = [t, r, theta, phi, p_t, p_r, p_theta, p_phi]
+ *    (src/kerrray/geodesics/state.py:35), float64 inputs.
+ *  - "_dev" entry points take device pointers and a hipStream_t (as void*)
+ *    and only enqueue work; the others take caller-owned host arrays,
+ *    copy through library-owned device memory and return after the work is
+ *    done.
+ *  - A handle owns one device, one stream and a grow-only device buffer; it
+ *    is not re-entrant (one call at a time per handle).
+ */
+#ifndef KERRRAY_HIP_H
+#define KERRRAY_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KR_ABI_VERSION 1
+
+typedef enum kr_status {
+    KR_OK = 0,
+    KR_ERR_INVALID = 1,     /* bad argument: maps to ValueError */
+    KR_ERR_DEVICE = 2,      /* HIP runtime failure / no device: BackendUnavailable */
+    KR_ERR_RADICAND = 3,    /* negative Carter radicand in an initial condition: ValueError */
+    KR_ERR_UNSUPPORTED = 4  /* option the kernels do not implement: ValueError */
+} kr_status;
+
+typedef struct kr_context* kr_handle;
+
+/* Working dtype of the integration (IntegratorOptions.dtype,
+ * src/kerrray/geodesics/integrators.py:95). */
+enum { KR_FLOAT64 = 0, KR_FLOAT32 = 1 };
+
+/* Kernel constants, packed exactly as the reference's host wrapper packs
+ * them: geom/consts/params/ipar = _kernel_constants
+ * (numba_backend.py:87-115, layouts numba_kernel.py:89-107; consts hold the
+ * float64 values that the reference then casts to the working dtype) and
+ * the dense tableaus of _tableaus (numba_backend.py:118-128).  Only the
+ * Dormand-Prince 5(4) and classical RK4 tableaus of
+ * src/kerrray/geodesics/tableaus.py:59-86 are accepted. */
+typedef struct kr_params {
+    double geom[7];
+    double consts[21];
+    double params[3];
+    int64_t ipar[4];
+    double tab_a[49];
+    double tab_e[7];
+    double rk4_b[4];
+    double rk4_c[4];
+    int32_t dtype; /* KR_FLOAT64 or KR_FLOAT32 */
+    int32_t reserved;
+} kr_params;
+
+/* Output arrays of a batch: the BatchResult fields
+ * (src/kerrray/photons/trajectories.py:100-128).  Y and event_Y hold the
+ * working dtype; event_Y/event_hit may be NULL unless ipar[3] (disk) is set. */
+typedef struct kr_outputs {
+    void* Y;                  /* (n, 8) */
+    double* lam;              /* (n) */
+    int64_t* n_steps;         /* (n) */
+    int64_t* n_rejected;      /* (n) */
+    int64_t* state;           /* (n) TerminationState codes */
+    double* max_null_error;   /* (n) */
+    double* max_energy_drift; /* (n) */
+    double* max_lz_drift;     /* (n) */
+    double* max_carter_drift; /* (n) */
+    void* event_Y;            /* (n, 8) or NULL */
+    uint8_t* event_hit;       /* (n) or NULL */
+} kr_outputs;
+
+/* A square image of Bardeen image-plane initial conditions at the observer:
+ * pixel centres fov (2j+1-n)/n (src/kerrray/raytracing/camera.py:206-222),
+ * alpha along columns, beta = -centre_i along rows (row 0 on top), and per
+ * pixel the photon of src/kerrray/benchmarks/rayset.py:158-165 with E = 1,
+ * inward, sign_theta = -1 for beta >= 0 and +1 otherwise. */
+typedef struct kr_image {
+    double mass;
+    double spin;        /* a / M */
+    double r_o;         /* observer radius */
+    double theta_o;     /* observer polar angle, radians */
+    double fov;         /* half-width of the image plane */
+    int64_t resolution; /* n: the image has n x n pixels, row-major */
+} kr_image;
+
+/* ABI version (KR_ABI_VERSION). */
+int kr_abi_version(void);
+
+/* Number of visible HIP devices, or a negative value on runtime failure. */
+int kr_device_count(void);
+
+/* Open a handle on `device`.  Stands in for the availability check of
+ * NumbaBackend.__init__ (numba_backend.py:223-226): failure here is the
+ * backend being unavailable. */
+kr_status kr_create(int device, kr_handle* out);
+kr_status kr_destroy(kr_handle h);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* kr_last_error(void);
+
+/* Backend.integrate_batch over caller-supplied initial states:
+ * integrate_batch_numba (numba_backend.py:131-207) ->
+ * integrate_rays_kernel (numba_kernel.py:466-474).  Y0 is (n, 8) float64
+ * host memory; outputs are host arrays.  kernel_ms (may be NULL) receives
+ * the device time of the integration kernel. */
+kr_status kr_integrate_batch(kr_handle h, const kr_params* p, const double* Y0, int64_t n, const kr_outputs* out,
+                             double* kernel_ms);
+
+/* Same on device pointers, enqueued on `stream` (hipStream_t). */
+kr_status kr_integrate_batch_dev(kr_handle h, const kr_params* p, const double* d_Y0, int64_t n,
+                                 const kr_outputs* d_out, void* stream);
+
+/* Device-side error flags of the image and point entry points: the kernel
+ * ORs KR_FLAG_RADICAND into *d_flags (when d_flags is not NULL) if some
+ * initial condition had a Carter radicand below -1e-12 x its scale, the
+ * case photon_from_constants rejects with ValueError
+ * (src/kerrray/geodesics/initial_conditions.py:71-74).  The caller zeroes
+ * the flag word before the launch. */
+#define KR_FLAG_RADICAND 1
+
+/* compute_shadow's trace (src/kerrray/raytracing/shadow.py:179-211 ->
+ * rays.py:72-117) for a Bardeen image: initial conditions are generated
+ * on the device, one pixel per lane, outputs row-major over the image. */
+kr_status kr_trace_image(kr_handle h, const kr_params* p, const kr_image* img, const kr_outputs* out,
+                         double* kernel_ms);
+kr_status kr_trace_image_dev(kr_handle h, const kr_params* p, const kr_image* img, const kr_outputs* d_out,
+                             int32_t* d_flags, void* stream);
+
+/* Bardeen photons at arbitrary image points (alpha[k], beta[k]) of the
+ * observer in `img` (resolution and fov are ignored); device pointers. */
+kr_status kr_trace_points_dev(kr_handle h, const kr_params* p, const kr_image* img, const double* d_alpha,
+                              const double* d_beta, int64_t n, const kr_outputs* d_out, int32_t* d_flags,
+                              void* stream);
+
+/* Shadow-edge search along polar directions about (centre_alpha,
+ * centre_beta): the loop of scripts/study_edge_bisection.py:53-68 on the
+ * device.  Direction k has unit vector (cos_angles[k], sin_angles[k]); its
+ * bracket [lo[k], hi[k]] (inside at lo, outside at hi) is updated in place
+ * for `iterations` rounds.  n_probe = 1 is plain bisection (mid =
+ * 0.5 (lo + hi); escaped -> hi = mid, else lo = mid); n_probe > 1 traces
+ * n_probe evenly spaced interior points per round and keeps the sub-bracket
+ * around the first escaping one (multi-probe form of
+ * src/kerrray/validation/kerr_bisection.py:12-18,230-246).  failures
+ * counts traced photons that neither escaped nor were captured;
+ * nonmonotone counts rounds whose probes were not inside...outside ordered.
+ * Host arrays. */
+kr_status kr_bisect_edge(kr_handle h, const kr_params* p, const kr_image* img, double centre_alpha,
+                         double centre_beta, const double* cos_angles, const double* sin_angles, int64_t n_angles,
+                         double* lo, double* hi, int32_t iterations, int32_t n_probe, int64_t* failures,
+                         int64_t* nonmonotone, double* kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KERRRAY_HIP_H */

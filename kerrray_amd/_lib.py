@@ -1,0 +1,189 @@
+"""ctypes binding of ``libkrhip.so`` (declarations in ``include/kerrray_hip.h``).
+
+The library is built in-tree by :mod:`kerrray_amd.build`. Loading it never
+falls back to anything else: a missing or unloadable library raises
+:class:`LibraryMissing`, which the backend registry reports as the ``hip``
+backend being unavailable.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+from kerrray_amd.build import LIB_PATH
+
+KR_OK, KR_ERR_INVALID, KR_ERR_DEVICE, KR_ERR_RADICAND, KR_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
+KR_FLOAT64, KR_FLOAT32 = 0, 1
+KR_FLAG_RADICAND = 1
+ABI_VERSION = 1
+
+# Every symbol the header declares (checked by the tests).
+EXPORTED = (
+    "kr_abi_version",
+    "kr_device_count",
+    "kr_create",
+    "kr_destroy",
+    "kr_last_error",
+    "kr_integrate_batch",
+    "kr_integrate_batch_dev",
+    "kr_trace_image",
+    "kr_trace_image_dev",
+    "kr_trace_points_dev",
+    "kr_bisect_edge",
+)
+
+
+class LibraryMissing(RuntimeError):
+    """``libkrhip.so`` is absent or cannot be loaded."""
+
+
+class KrError(RuntimeError):
+    """A non-OK status from the C ABI."""
+
+    def __init__(self, status: int, message: str) -> None:
+        super().__init__(message)
+        self.status = status
+
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_int64_p = ctypes.POINTER(ctypes.c_int64)
+
+
+class KrParams(ctypes.Structure):
+    _fields_ = [
+        ("geom", ctypes.c_double * 7),
+        ("consts", ctypes.c_double * 21),
+        ("params", ctypes.c_double * 3),
+        ("ipar", ctypes.c_int64 * 4),
+        ("tab_a", ctypes.c_double * 49),
+        ("tab_e", ctypes.c_double * 7),
+        ("rk4_b", ctypes.c_double * 4),
+        ("rk4_c", ctypes.c_double * 4),
+        ("dtype", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class KrOutputs(ctypes.Structure):
+    _fields_ = [
+        ("Y", ctypes.c_void_p),
+        ("lam", ctypes.c_void_p),
+        ("n_steps", ctypes.c_void_p),
+        ("n_rejected", ctypes.c_void_p),
+        ("state", ctypes.c_void_p),
+        ("max_null_error", ctypes.c_void_p),
+        ("max_energy_drift", ctypes.c_void_p),
+        ("max_lz_drift", ctypes.c_void_p),
+        ("max_carter_drift", ctypes.c_void_p),
+        ("event_Y", ctypes.c_void_p),
+        ("event_hit", ctypes.c_void_p),
+    ]
+
+
+class KrImage(ctypes.Structure):
+    _fields_ = [
+        ("mass", ctypes.c_double),
+        ("spin", ctypes.c_double),
+        ("r_o", ctypes.c_double),
+        ("theta_o", ctypes.c_double),
+        ("fov", ctypes.c_double),
+        ("resolution", ctypes.c_int64),
+    ]
+
+
+_lock = threading.Lock()
+_lib: ctypes.CDLL | None = None
+
+
+def _declare(L: ctypes.CDLL) -> None:
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    L.kr_abi_version.restype = ctypes.c_int
+    L.kr_abi_version.argtypes = []
+    L.kr_device_count.restype = ctypes.c_int
+    L.kr_device_count.argtypes = []
+    L.kr_create.restype = ctypes.c_int
+    L.kr_create.argtypes = [ctypes.c_int, P(vp)]
+    L.kr_destroy.restype = ctypes.c_int
+    L.kr_destroy.argtypes = [vp]
+    L.kr_last_error.restype = ctypes.c_char_p
+    L.kr_last_error.argtypes = []
+    L.kr_integrate_batch.restype = ctypes.c_int
+    L.kr_integrate_batch.argtypes = [vp, P(KrParams), vp, ctypes.c_int64, P(KrOutputs), c_double_p]
+    L.kr_integrate_batch_dev.restype = ctypes.c_int
+    L.kr_integrate_batch_dev.argtypes = [vp, P(KrParams), vp, ctypes.c_int64, P(KrOutputs), vp]
+    L.kr_trace_image.restype = ctypes.c_int
+    L.kr_trace_image.argtypes = [vp, P(KrParams), P(KrImage), P(KrOutputs), c_double_p]
+    L.kr_trace_image_dev.restype = ctypes.c_int
+    L.kr_trace_image_dev.argtypes = [vp, P(KrParams), P(KrImage), P(KrOutputs), vp, vp]
+    L.kr_trace_points_dev.restype = ctypes.c_int
+    L.kr_trace_points_dev.argtypes = [vp, P(KrParams), P(KrImage), vp, vp, ctypes.c_int64, P(KrOutputs), vp, vp]
+    L.kr_bisect_edge.restype = ctypes.c_int
+    L.kr_bisect_edge.argtypes = [
+        vp, P(KrParams), P(KrImage), ctypes.c_double, ctypes.c_double, vp, vp, ctypes.c_int64, vp, vp,
+        ctypes.c_int32, ctypes.c_int32, c_int64_p, c_int64_p, c_double_p,
+    ]
+
+
+def lib() -> ctypes.CDLL:
+    """Load ``libkrhip.so`` once (raises :class:`LibraryMissing`)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise LibraryMissing(
+                    f"{LIB_PATH} is not built; run `python -m kerrray_amd.build` (needs ROCm hipcc)"
+                )
+            try:
+                L = ctypes.CDLL(LIB_PATH)
+            except OSError as exc:
+                raise LibraryMissing(f"cannot load {LIB_PATH}: {exc}") from exc
+            _declare(L)
+            if L.kr_abi_version() != ABI_VERSION:
+                raise LibraryMissing(f"{LIB_PATH} has ABI {L.kr_abi_version()}, expected {ABI_VERSION}")
+            _lib = L
+        return _lib
+
+
+def last_error() -> str:
+    msg = lib().kr_last_error()
+    return msg.decode("utf-8", "replace") if msg else ""
+
+
+def check(status: int, what: str) -> None:
+    """Raise the Python exception the reference raises for this kind of failure."""
+    if status == KR_OK:
+        return
+    msg = f"{what}: {last_error()}"
+    if status in (KR_ERR_INVALID, KR_ERR_RADICAND, KR_ERR_UNSUPPORTED):
+        raise ValueError(msg)
+    raise KrError(status, msg)
+
+
+class Handle:
+    """One ``kr_handle`` (device, stream, scratch buffer); closed on garbage collection."""
+
+    def __init__(self, device: int = 0) -> None:
+        L = lib()
+        h = ctypes.c_void_p()
+        check(L.kr_create(device, ctypes.byref(h)), "kr_create")
+        self._h = h
+        self.device = device
+        self._lock = threading.Lock()
+
+    @property
+    def ptr(self) -> ctypes.c_void_p:
+        return self._h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().kr_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,559 @@
+// Device-side building blocks of the per-photon null-geodesic integrator.
+//
+// One lane integrates one photon; its whole state lives in VGPRs.  Every
+// function reproduces the reference's scalar Numba kernel
+// (src/kerrray/raytracing/backends/numba_kernel.py) in the same
+// floating-point operation order; the library is compiled with
+// -ffp-contract=off so no multiply-add is fused.  The working type T is
+// double (the reference's float64) or float (its float32 mode, where the
+// metric is still evaluated in double and cast, numba_kernel.py:165-167).
+//
+// Two exact simplifications relative to the reference's eight-component
+// loops (both leave every output bit unchanged):
+//  * dp_t/dlambda and dp_phi/dlambda are identically zero
+//    (numba_kernel.py:185,197), so every stage value of p_t and p_phi is
+//    y + h * (+0) = y + 0.0 (the first tableau coefficient of every stage row
+//    is positive, so the zero sum is +0); only the six components
+//    t, r, theta, phi, p_r, p_theta are integrated and p_t, p_phi are
+//    carried as y + 0.0.
+//  * the error-norm terms of p_t and p_phi are +0, so NumPy's pairwise
+//    sum ((q0+q1)+(q2+q3))+((q4+q5)+(q6+q7)) equals ((q0+q1)+(q2+q3))+(q5+q6)
+//    bit for bit.
+// sin and cos of theta computed inside the metric of the last stage (at the
+// accepted state) are reused by the classifier and the Carter diagnostic,
+// which evaluate exactly the same functions of exactly the same argument.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kr {
+
+// Termination codes (src/kerrray/photons/classification.py:60-69).
+enum : int { RUNNING = 0, ESCAPED = 1, CAPTURED = 2, MAX_AFFINE = 3, FAILURE = 4, OUT_OF_DOMAIN = 5, DISK_HIT = 6 };
+
+// Indices of the reference packing (numba_kernel.py:89-107).
+enum { G_MASS, G_A, G_A2, G_NEG_2MA, G_POS_2MA, G_TWO_M, G_TWO_A2 };
+enum {
+    C_ZERO, C_ONE, C_TWO, C_NEG_HALF, C_EIGHT, C_SAFETY, C_FMIN, C_FMAX, C_EXP, C_RTOL,
+    C_ATOL, C_HMIN, C_RCAP, C_RESC, C_THLO, C_THHI, C_AXIS, C_H0, C_RIN, C_ROUT, C_HALFPI
+};
+enum { P_LAMMAX, P_HALFPI, P_HMAX };
+enum { I_MAX_STEPS, I_GUARD, I_ADAPTIVE, I_DISK };
+
+// Dormand-Prince 5(4) tableau (Dormand & Prince 1980; HNW 1993 Table 5.2),
+// the values of src/kerrray/geodesics/tableaus.py:63-86 evaluated in double.
+// The host entry points reject any other tableau, so these literals are what
+// the reference passes in (numba_backend.py:118-128).
+namespace dp {
+constexpr double a10 = 1.0 / 5.0;
+constexpr double a20 = 3.0 / 40.0, a21 = 9.0 / 40.0;
+constexpr double a30 = 44.0 / 45.0, a31 = -56.0 / 15.0, a32 = 32.0 / 9.0;
+constexpr double a40 = 19372.0 / 6561.0, a41 = -25360.0 / 2187.0, a42 = 64448.0 / 6561.0, a43 = -212.0 / 729.0;
+constexpr double a50 = 9017.0 / 3168.0, a51 = -355.0 / 33.0, a52 = 46732.0 / 5247.0, a53 = 49.0 / 176.0,
+                 a54 = -5103.0 / 18656.0;
+constexpr double a60 = 35.0 / 384.0, a62 = 500.0 / 1113.0, a63 = 125.0 / 192.0, a64 = -2187.0 / 6784.0,
+                 a65 = 11.0 / 84.0;
+constexpr double e0 = 5179.0 / 57600.0 - 35.0 / 384.0, e2 = 7571.0 / 16695.0 - 500.0 / 1113.0,
+                 e3 = 393.0 / 640.0 - 125.0 / 192.0, e4 = -92097.0 / 339200.0 - -2187.0 / 6784.0,
+                 e5 = 187.0 / 2100.0 - 11.0 / 84.0, e6 = 1.0 / 40.0 - 0.0;
+}  // namespace dp
+namespace rk4 {
+constexpr double c1 = 0.5, c2 = 0.5, c3 = 1.0;
+constexpr double b0 = 1.0 / 6.0, b1 = 1.0 / 3.0, b2 = 1.0 / 3.0, b3 = 1.0 / 6.0;
+}  // namespace rk4
+
+// Everything one launch needs, passed by value in the kernel arguments
+// (lands in the scalar cache; every lane reads the same words).
+struct Params {
+    double geom[7];
+    double consts[21];  // float64 values; cast to the working type where used
+    double params[3];
+    int64_t ipar[4];
+};
+
+// The six integrated components of y = [t, r, th, ph, p_t, p_r, p_th, p_ph]
+// (state.py:35) are stored as v[0..5] = t, r, th, ph, p_r, p_th.
+constexpr int NA = 6;
+__host__ __device__ constexpr int full_index(int i) { return i < 4 ? i : i + 1; }
+
+template <typename T>
+struct Photon {
+    T v[NA];   // t, r, th, ph, p_r, p_th
+    T p_t;     // constant of motion (-E)
+    T p_ph;    // constant of motion (L_z)
+};
+
+template <typename T> __device__ __forceinline__ T pymax(T a, T b) { return b > a ? b : a; }
+template <typename T> __device__ __forceinline__ T pymin(T a, T b) { return b < a ? b : a; }
+
+__device__ __forceinline__ double wsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float wsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double wpow(double x, double y) { return pow(x, y); }
+__device__ __forceinline__ float wpow(float x, float y) { return powf(x, y); }
+__device__ __forceinline__ double wabs(double x) { return fabs(x); }
+__device__ __forceinline__ float wabs(float x) { return fabsf(x); }
+__device__ __forceinline__ bool wfinite(double x) { return isfinite(x); }
+__device__ __forceinline__ bool wfinite(float x) { return isfinite(x); }
+
+// numba_kernel.py:111-154 (= geometry/metric.py:166-195,265-313). Also
+// returns sin(th), cos(th) for reuse by the classifier and diagnostics.
+__device__ __forceinline__ void inverse_metric_terms(const Params& P, double r, double th, double (&out)[15],
+                                                     double& s_out, double& c_out) {
+    const double m = P.geom[G_MASS];
+    const double a2 = P.geom[G_A2];
+    const double s = sin(th);
+    const double c = cos(th);
+    const double s2 = s * s;
+    const double sin2th = sin(2.0 * th);
+    const double r2 = r * r;
+    const double sig = r2 + a2 * (c * c);
+    const double dlt = r2 - P.geom[G_TWO_M] * r + a2;
+    const double r2a2 = r2 + a2;
+    const double big_a = r2a2 * r2a2 - a2 * dlt * s2;
+    const double sd = sig * dlt;
+    out[0] = -big_a / sd;
+    out[1] = P.geom[G_NEG_2MA] * r / sd;
+    out[2] = dlt / sig;
+    out[3] = 1.0 / sig;
+    out[4] = (dlt - a2 * s2) / (sd * s2);
+    const double sig_r = 2.0 * r;
+    const double sig_th = (-a2) * sin2th;
+    const double dlt_r = 2.0 * (r - m);
+    const double a_r = 4.0 * r * r2a2 - P.geom[G_TWO_A2] * (r - m) * s2;
+    const double a_th = (-a2) * dlt * sin2th;
+    const double sd_r = sig_r * dlt + sig * dlt_r;
+    const double sd_th = sig_th * dlt;
+    const double sd2 = sd * sd;
+    const double sig2 = sig * sig;
+    out[5] = -(a_r * sd - big_a * sd_r) / sd2;
+    out[6] = P.geom[G_NEG_2MA] * (sd - r * sd_r) / sd2;
+    out[7] = (dlt_r * sig - dlt * sig_r) / sig2;
+    out[8] = -sig_r / sig2;
+    out[9] = -sig_r / (sig2 * s2) + a2 * sd_r / sd2;
+    out[10] = -(a_th * sd - big_a * sd_th) / sd2;
+    out[11] = P.geom[G_POS_2MA] * r * sd_th / sd2;
+    out[12] = -dlt * sig_th / sig2;
+    out[13] = -sig_th / sig2;
+    out[14] = -(sig_th * s2 + sig * sin2th) / (sig2 * (s2 * s2)) + a2 * sd_th / sd2;
+    s_out = s;
+    c_out = c;
+}
+
+// numba_kernel.py:158-197: f = dy/dlambda for the six integrated components
+// (f[p_t] = f[p_phi] = 0 identically).
+template <typename T>
+__device__ __forceinline__ void rhs(const Params& P, const T (&v)[NA], T p_t, T p_ph, T (&f)[NA], double& s,
+                                    double& c) {
+    double g64[15];
+    inverse_metric_terms(P, static_cast<double>(v[1]), static_cast<double>(v[2]), g64, s, c);
+    T g[15];
+#pragma unroll
+    for (int i = 0; i < 15; ++i) g[i] = static_cast<T>(g64[i]);
+    const T zero = T(0);
+    const T two = T(2);
+    const T p_r = v[4], p_th = v[5];
+    T gphph = g[4], r_gphph = g[9], t_gphph = g[14];
+    if (p_ph == zero) {  // polar-axis guard (numba_kernel.py:177-180)
+        gphph = zero;
+        r_gphph = zero;
+        t_gphph = zero;
+    }
+    f[0] = g[0] * p_t + g[1] * p_ph;
+    f[1] = g[2] * p_r;
+    f[2] = g[3] * p_th;
+    f[3] = g[1] * p_t + gphph * p_ph;
+    const T pt2 = p_t * p_t, ptpph = p_t * p_ph, pr2 = p_r * p_r, pth2 = p_th * p_th, pph2 = p_ph * p_ph;
+    f[4] = T(-0.5) * (g[5] * pt2 + two * g[6] * ptpph + g[7] * pr2 + g[8] * pth2 + r_gphph * pph2);
+    f[5] = T(-0.5) * (g[10] * pt2 + two * g[11] * ptpph + g[12] * pr2 + g[13] * pth2 + t_gphph * pph2);
+}
+
+template <typename T>
+__device__ __forceinline__ bool all_finite(const T (&a)[NA], const T (&b)[NA]) {
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < NA; ++i) ok = ok && wfinite(a[i]) && wfinite(b[i]);
+    return ok;
+}
+
+// numba_kernel.py:201-238.  k0 = f(y) on entry (FSAL); on exit y_new, err and
+// k6 = f(y_new), plus sin/cos of theta at y_new.
+template <typename T>
+__device__ __forceinline__ void dp5_step(const Params& P, const T (&y)[NA], T p_t, T p_ph, T h, const T (&k0)[NA],
+                                         T (&y_new)[NA], T (&err)[NA], T (&k6)[NA], double& s_new, double& c_new) {
+    using namespace dp;
+    T k1[NA], k2[NA], k3[NA], k4[NA], k5[NA], ys[NA];
+    double s, c;
+    const T pt = p_t + T(0), pph = p_ph + T(0);
+#pragma unroll
+    for (int i = 0; i < NA; ++i) ys[i] = y[i] + h * (T(a10) * k0[i]);
+    rhs(P, ys, pt, pph, k1, s, c);
+#pragma unroll
+    for (int i = 0; i < NA; ++i) ys[i] = y[i] + h * (T(a20) * k0[i] + T(a21) * k1[i]);
+    rhs(P, ys, pt, pph, k2, s, c);
+#pragma unroll
+    for (int i = 0; i < NA; ++i) ys[i] = y[i] + h * (T(a30) * k0[i] + T(a31) * k1[i] + T(a32) * k2[i]);
+    rhs(P, ys, pt, pph, k3, s, c);
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+        ys[i] = y[i] + h * (T(a40) * k0[i] + T(a41) * k1[i] + T(a42) * k2[i] + T(a43) * k3[i]);
+    rhs(P, ys, pt, pph, k4, s, c);
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+        ys[i] = y[i] + h * (T(a50) * k0[i] + T(a51) * k1[i] + T(a52) * k2[i] + T(a53) * k3[i] + T(a54) * k4[i]);
+    rhs(P, ys, pt, pph, k5, s, c);
+    T esum[NA];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+        y_new[i] = y[i] + h * (T(a60) * k0[i] + T(a62) * k2[i] + T(a63) * k3[i] + T(a64) * k4[i] + T(a65) * k5[i]);
+        esum[i] = T(e0) * k0[i] + T(e2) * k2[i] + T(e3) * k3[i] + T(e4) * k4[i] + T(e5) * k5[i];
+    }
+    rhs(P, y_new, pt, pph, k6, s_new, c_new);
+#pragma unroll
+    for (int i = 0; i < NA; ++i) err[i] = h * (esum[i] + T(e6) * k6[i]);
+}
+
+// numba_kernel.py:242-257.
+template <typename T>
+__device__ __forceinline__ void rk4_step(const Params& P, const T (&y)[NA], T p_t, T p_ph, T h, const T (&k0)[NA],
+                                         T (&y_new)[NA], T (&k6)[NA], double& s_new, double& c_new) {
+    using namespace rk4;
+    T k1[NA], k2[NA], k3[NA], ys[NA];
+    double s, c;
+    const T pt = p_t + T(0), pph = p_ph + T(0);
+#pragma unroll
+    for (int i = 0; i < NA; ++i) ys[i] = y[i] + h * (T(c1) * k0[i]);
+    rhs(P, ys, pt, pph, k1, s, c);
+#pragma unroll
+    for (int i = 0; i < NA; ++i) ys[i] = y[i] + h * (T(c2) * k1[i]);
+    rhs(P, ys, pt, pph, k2, s, c);
+#pragma unroll
+    for (int i = 0; i < NA; ++i) ys[i] = y[i] + h * (T(c3) * k2[i]);
+    rhs(P, ys, pt, pph, k3, s, c);
+#pragma unroll
+    for (int i = 0; i < NA; ++i) y_new[i] = y[i] + h * (T(b0) * k0[i] + T(b1) * k1[i] + T(b2) * k2[i] + T(b3) * k3[i]);
+    rhs(P, y_new, pt, pph, k6, s_new, c_new);
+}
+
+// numba_kernel.py:261-268 (scale uses max(|y|, |y_new|); p_t/p_phi terms are +0).
+template <typename T>
+__device__ __forceinline__ T error_norm(T rtol, T atol, const T (&err)[NA], const T (&y)[NA], const T (&y_new)[NA]) {
+    T q[NA];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+        const T scale = atol + rtol * pymax(wabs(y[i]), wabs(y_new[i]));
+        const T v = err[i] / scale;
+        q[i] = v * v;
+    }
+    // q index: 0 t, 1 r, 2 th, 3 ph, 4 p_r (full 5), 5 p_th (full 6)
+    const T total = ((q[0] + q[1]) + (q[2] + q[3])) + (q[4] + q[5]);
+    return wsqrt(total / T(8));
+}
+
+// numba_kernel.py:272-281.
+template <typename T>
+__device__ __forceinline__ T step_factor(T safety, T fmin, T fmax, T expo, T en, bool rejected_before) {
+    T factor;
+    if (en == T(0)) {
+        factor = fmax;
+    } else {
+        const T raw = safety * wpow(en, expo);
+        factor = pymin(fmax, pymax(fmin, raw));
+    }
+    if (rejected_before) factor = pymin(factor, T(1));
+    return factor;
+}
+
+// numba_kernel.py:293-311.  sin_th is sin(y[th]) in the working type.
+template <typename T>
+__device__ __forceinline__ int classify(const Params& P, const T (&v)[NA], T p_ph, const T (&f)[NA], double lam,
+                                        int64_t n_steps, T h, bool check_h, T sin_th) {
+    if (!all_finite(v, f)) return FAILURE;
+    if (check_h && h < static_cast<T>(P.consts[C_HMIN])) return FAILURE;
+    const T r = v[1];
+    const T th = v[2];
+    if (r < T(0) || th < static_cast<T>(P.consts[C_THLO]) || th > static_cast<T>(P.consts[C_THHI]))
+        return OUT_OF_DOMAIN;
+    if (p_ph != T(0) && wabs(sin_th) < static_cast<T>(P.consts[C_AXIS])) return OUT_OF_DOMAIN;
+    if (r <= static_cast<T>(P.consts[C_RCAP])) return CAPTURED;
+    if (r >= static_cast<T>(P.consts[C_RESC]) && f[1] > T(0)) return ESCAPED;
+    if (lam >= P.params[P_LAMMAX] || n_steps >= P.ipar[I_MAX_STEPS]) return MAX_AFFINE;
+    return RUNNING;
+}
+
+template <typename T> __device__ __forceinline__ T working_sin(double s_double, T th);
+template <> __device__ __forceinline__ double working_sin<double>(double s_double, double) { return s_double; }
+template <> __device__ __forceinline__ float working_sin<float>(double, float th) { return sinf(th); }
+
+// numba_kernel.py:315-321.
+template <typename T>
+__device__ __forceinline__ double null_error(const T (&v)[NA], T p_t, T p_ph, const T (&f)[NA]) {
+    double total = static_cast<double>(p_t) * static_cast<double>(f[0]);
+    total = total + static_cast<double>(v[4]) * static_cast<double>(f[1]);
+    total = total + static_cast<double>(v[5]) * static_cast<double>(f[2]);
+    total = total + static_cast<double>(p_ph) * static_cast<double>(f[3]);
+    return fabs(0.5 * total);
+}
+
+// numba_kernel.py:325-336 with s = sin(th), c = cos(th) of double(th).
+template <typename T>
+__device__ __forceinline__ double carter(double a2, const T (&v)[NA], T p_t, T p_ph, double s, double c) {
+    const double p_th = static_cast<double>(v[5]);
+    const double lz = static_cast<double>(p_ph);
+    const double e = -static_cast<double>(p_t);
+    double lz2_over_s2 = 0.0;
+    if (lz != 0.0) lz2_over_s2 = (lz * lz) / (s * s);
+    return p_th * p_th + (c * c) * (lz2_over_s2 - a2 * e * e);
+}
+
+template <typename T>
+struct RayResult {
+    T y[8];
+    double lam;
+    int64_t n_steps;
+    int64_t n_rej;
+    int state;
+    double m_null, m_e, m_lz, m_q;
+    bool ev_hit;
+    T ev[8];
+};
+
+// numba_kernel.py:340-463 for one photon whose float64 initial state is y0.
+// (Y0 arrives in float64 and is cast to T, numba_backend.py:167; the
+// reference drift baselines e0, lz0, q0 and floors are taken from the
+// float64 Y0, numba_backend.py:174-178.)
+template <typename T, bool ADAPTIVE, bool DISK>
+__device__ void integrate_ray(const Params& P, const double (&y64)[8], RayResult<T>& out) {
+    const double a2 = P.geom[G_A2];
+    const double lambda_max = P.params[P_LAMMAX];
+    const double half_pi = P.params[P_HALFPI];
+    const double h_cap = P.params[P_HMAX];
+    const T rtol = static_cast<T>(P.consts[C_RTOL]), atol = static_cast<T>(P.consts[C_ATOL]);
+    const T safety = static_cast<T>(P.consts[C_SAFETY]), fmin = static_cast<T>(P.consts[C_FMIN]);
+    const T fmax = static_cast<T>(P.consts[C_FMAX]), expo = static_cast<T>(P.consts[C_EXP]);
+    const T h_min = static_cast<T>(P.consts[C_HMIN]);
+
+    T y[NA];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) y[i] = static_cast<T>(y64[full_index(i)]);
+    T p_t = static_cast<T>(y64[4]);
+    T p_ph = static_cast<T>(y64[7]);
+
+    // Drift baselines from the float64 initial state (photons/constants.py:65-90,163-171).
+    const double e0 = -y64[4];
+    const double lz0 = y64[7];
+    double fe = fabs(e0);
+    fe = fe > 0.0 ? fe : 1.0;
+    const double mass = P.geom[G_MASS];
+    const double fl = fe * mass;
+    const double fq = fe * fe * mass * mass;
+    const double fe2 = fe * fe;
+
+    T f[NA];
+    double s_cur, c_cur;
+    rhs(P, y, p_t, p_ph, f, s_cur, c_cur);
+    double q0;
+    {
+        // carter_constant(st, Y0) on the float64 Y0 (same expression as numba_kernel.py:325-336).
+        const double th0 = y64[2];
+        double s0 = s_cur, c0 = c_cur;
+        if (static_cast<double>(static_cast<T>(th0)) != th0) {
+            s0 = sin(th0);
+            c0 = cos(th0);
+        }
+        const double lz = y64[7], e = -y64[4];
+        double lz2_over_s2 = 0.0;
+        if (lz != 0.0) lz2_over_s2 = (lz * lz) / (s0 * s0);
+        q0 = y64[6] * y64[6] + (c0 * c0) * (lz2_over_s2 - a2 * e * e);
+    }
+    double m_null = null_error(y, p_t, p_ph, f) / fe2;
+    double m_e = 0.0, m_lz = 0.0, m_q = 0.0;
+    double lam = 0.0;
+    T h = static_cast<T>(P.consts[C_H0]);
+    if (ADAPTIVE && static_cast<double>(h) > h_cap) h = static_cast<T>(h_cap);
+    int64_t n_steps = 0, n_rej = 0, attempts = 0;
+    const int64_t guard = P.ipar[I_GUARD];
+    bool rejected_before = false;
+    bool ev_hit = false;
+    T ev[8];
+    int state = classify(P, y, p_ph, f, lam, n_steps, h, false, working_sin<T>(s_cur, y[2]));
+    while (state == RUNNING) {
+        attempts += 1;
+        if (attempts > guard) {
+            state = FAILURE;
+            break;
+        }
+        const double remaining = lambda_max - lam;
+        const bool last = static_cast<double>(h) >= remaining;
+        T h_eff = h;
+        if (last) h_eff = static_cast<T>(remaining);
+        T y_new[NA], k6[NA], err[NA];
+        double s_new, c_new;
+        if (ADAPTIVE)
+            dp5_step(P, y, p_t, p_ph, h_eff, f, y_new, err, k6, s_new, c_new);
+        else
+            rk4_step(P, y, p_t, p_ph, h_eff, f, y_new, k6, s_new, c_new);
+        bool finite = all_finite(y_new, k6);
+        bool accept = finite;
+        T h_new = h;
+        if (ADAPTIVE && finite) {
+            const T en = error_norm(rtol, atol, err, y, y_new);
+            if (!wfinite(en)) {
+                finite = false;
+                accept = false;
+            } else {
+                accept = en <= T(1);
+                const T factor = step_factor(safety, fmin, fmax, expo, en, rejected_before);
+                if (!(accept && last)) {
+                    h_new = h_eff * factor;
+                    if (static_cast<double>(h_new) > h_cap) h_new = static_cast<T>(h_cap);
+                }
+            }
+        }
+        if (!finite) {
+            state = FAILURE;
+            break;
+        }
+        const bool rejected = !accept;
+        if (rejected) n_rej += 1;
+        rejected_before = rejected;
+        h = h_new;
+        if (rejected) {
+            if (h_new < h_min) {
+                state = FAILURE;
+                break;
+            }
+            continue;
+        }
+        T y_prev[NA];
+        T p_t_prev = p_t, p_ph_prev = p_ph;
+        if (DISK) {
+#pragma unroll
+            for (int i = 0; i < NA; ++i) y_prev[i] = y[i];
+        }
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            y[i] = y_new[i];
+            f[i] = k6[i];
+        }
+        p_t = p_t + T(0);
+        p_ph = p_ph + T(0);
+        s_cur = s_new;
+        c_cur = c_new;
+        const double lam_old = lam;
+        if (last)
+            lam = lambda_max;
+        else
+            lam = lam + static_cast<double>(h_eff);
+        n_steps += 1;
+        if (sizeof(T) != sizeof(double)) {
+            // In float64 E and L_z are bit-constant and both drifts stay 0.
+            const double e = -static_cast<double>(p_t);
+            const double lz = static_cast<double>(p_ph);
+            m_e = pymax(m_e, fabs(e - e0) / pymax(fabs(e0), fe));
+            m_lz = pymax(m_lz, fabs(lz - lz0) / pymax(fabs(lz0), fl));
+        }
+        m_q = pymax(m_q, fabs(carter(a2, y, p_t, p_ph, s_cur, c_cur) - q0) / pymax(fabs(q0), fq));
+        m_null = pymax(m_null, null_error(y, p_t, p_ph, f) / fe2);
+        int code = classify(P, y, p_ph, f, lam, n_steps, h, true, working_sin<T>(s_cur, y[2]));
+        if (DISK) {
+            // numba_kernel.py:439-454
+            const double th_old = static_cast<double>(y_prev[2]);
+            const double th_new = static_cast<double>(y[2]);
+            if ((th_old - half_pi) * (th_new - half_pi) < 0.0) {
+                const double sfrac = (half_pi - th_old) / (th_new - th_old);
+                T yc[8];
+                T full_new[8] = {y[0], y[1], y[2], y[3], p_t, y[4], y[5], p_ph};
+                T full_old[8] = {y_prev[0], y_prev[1], y_prev[2], y_prev[3], p_t_prev, y_prev[4], y_prev[5], p_ph_prev};
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const T cast = static_cast<T>(sfrac * static_cast<double>(full_new[i] - full_old[i]));
+                    yc[i] = full_old[i] + cast;
+                }
+                yc[2] = static_cast<T>(P.consts[C_HALFPI]);
+                if (yc[1] >= static_cast<T>(P.consts[C_RIN]) && yc[1] <= static_cast<T>(P.consts[C_ROUT])) {
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) ev[i] = yc[i];
+#pragma unroll
+                    for (int i = 0; i < NA; ++i) y[i] = yc[full_index(i)];
+                    p_t = yc[4];
+                    p_ph = yc[7];
+                    ev_hit = true;
+                    lam = lam_old + sfrac * static_cast<double>(h_eff);
+                    code = DISK_HIT;
+                }
+            }
+        }
+        state = code;
+    }
+#pragma unroll
+    for (int i = 0; i < NA; ++i) out.y[full_index(i)] = y[i];
+    out.y[4] = p_t;
+    out.y[7] = p_ph;
+    out.lam = lam;
+    out.n_steps = n_steps;
+    out.n_rej = n_rej;
+    out.state = state;
+    out.m_null = m_null;
+    out.m_e = m_e;
+    out.m_lz = m_lz;
+    out.m_q = m_q;
+    out.ev_hit = ev_hit;
+    if (DISK) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) out.ev[i] = ev_hit ? ev[i] : T(0);
+    }
+}
+
+// Per-image scalars of the Bardeen initial conditions, computed on the host
+// with the reference's own Python float expressions (rayset.py:158-165,
+// initial_conditions.py:50-112, horizons via delta()):
+struct BardeenImage {
+    double r_o, theta_o;     // observer radius and polar angle
+    double sin_o;            // math.sin(theta_o)
+    double a;                // spin * mass
+    double a2c2;             // a**2 * cos_o**2
+    double cot2;             // (cos_o / sin_o)**2
+    double r2a2;             // E * (r*r + a*a) with E = 1
+    double dlt;              // r**2 - 2.0*m*r + a**2 at r_o
+    double c2, s2;           // math.cos(theta)**2, math.sin(theta)**2 at theta_o
+    double aaEE;             // a * a * E * E
+    double fov;              // half-width of the image plane (M)
+    int64_t n;               // resolution (n x n pixels)
+};
+
+// _off_equatorial_state + photon_from_constants for one image-plane point.
+// Python's x**2 on a float is glibc pow(x, 2.0), which returns x*x exactly.
+// Returns false when a radicand is negative beyond round-off
+// (initial_conditions.py:71-74).
+__device__ __forceinline__ bool bardeen_state(const BardeenImage& B, double alpha, double beta, double (&y)[8]) {
+    const double xi = -alpha * B.sin_o;
+    const double eta = beta * beta - B.a2c2 + xi * xi * B.cot2;
+    const double term1 = B.r2a2 - B.a * xi;
+    const double d = xi - B.a * 1.0;
+    const double term2 = B.dlt * (eta + d * d);
+    const double big_r = term1 * term1 - term2;
+    const double scale_r = term1 * term1 + fabs(term2);
+    const double lz_term = (xi == 0.0) ? 0.0 : xi * xi / B.s2;
+    const double big_theta = eta - B.c2 * (lz_term - B.aaEE);
+    const double scale_theta = fabs(eta) + B.c2 * (lz_term + B.aaEE);
+    const bool ok = !(big_r < -1e-12 * pymax(scale_r, 1.0)) && !(big_theta < -1e-12 * pymax(scale_theta, 1.0));
+    const double sth = beta >= 0.0 ? -1.0 : 1.0;
+    y[0] = 0.0;
+    y[1] = B.r_o;
+    y[2] = B.theta_o;
+    y[3] = 0.0;
+    y[4] = -1.0;
+    y[5] = -1.0 * sqrt(pymax(big_r, 0.0)) / B.dlt;
+    y[6] = sth * sqrt(pymax(big_theta, 0.0));
+    y[7] = xi;
+    return ok;
+}
+
+// Camera.pixel_coordinates (camera.py:206-222): fov (2 j + 1 - n) / n.
+__device__ __forceinline__ double pixel_centre(double fov, int64_t j, int64_t n) {
+    const double nd = static_cast<double>(n);
+    return (fov * ((2.0 * static_cast<double>(j) + 1.0) - nd)) / nd;
+}
+
+}  // namespace kr

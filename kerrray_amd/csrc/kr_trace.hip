@@ -1,0 +1,542 @@
+// Plain per-photon kernels (one lane = one photon) and the C ABI around
+// them: batches of given initial states, Bardeen images generated on the
+// device, arbitrary image points, and the device-side edge bisection.
+//
+// Reference: src/kerrray/raytracing/backends/numba_kernel.py:466-474 (one ray
+// per prange iteration) behind numba_backend.py:131-256.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "kerrray_hip.h"
+#include "kr_device.h"
+#include "kr_host.h"
+
+namespace kr {
+
+template <typename T>
+struct Outputs {
+    T* Y;
+    double* lam;
+    int64_t* n_steps;
+    int64_t* n_rej;
+    int64_t* state;
+    double* m_null;
+    double* m_e;
+    double* m_lz;
+    double* m_q;
+    T* ev_Y;
+    uint8_t* ev_hit;
+};
+
+enum { IC_Y0 = 0, IC_IMAGE = 1, IC_POINTS = 2 };
+
+struct ICArgs {
+    const double* Y0;     // IC_Y0: (n, 8)
+    BardeenImage B;       // IC_IMAGE / IC_POINTS
+    const double* alpha;  // IC_POINTS
+    const double* beta;
+    int32_t* flags;       // may be null
+};
+
+template <typename T, bool DISK>
+__device__ __forceinline__ void write_result(const Outputs<T>& o, int64_t idx, const RayResult<T>& r) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) o.Y[idx * 8 + c] = r.y[c];
+    o.lam[idx] = r.lam;
+    o.n_steps[idx] = r.n_steps;
+    o.n_rej[idx] = r.n_rej;
+    o.state[idx] = r.state;
+    o.m_null[idx] = r.m_null;
+    o.m_e[idx] = r.m_e;
+    o.m_lz[idx] = r.m_lz;
+    o.m_q[idx] = r.m_q;
+    if (DISK) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) o.ev_Y[idx * 8 + c] = r.ev[c];
+        o.ev_hit[idx] = r.ev_hit ? 1 : 0;
+    }
+}
+
+// Image lanes: each 64-lane wavefront takes an 8 x 8 pixel tile so that a
+// wave's photons are neighbours on the sky and have similar step counts.
+__device__ __forceinline__ bool image_pixel(int64_t n, int64_t& i, int64_t& j) {
+    const int64_t wave = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const int64_t tiles_x = (n + 7) >> 3;
+    const int64_t ty = wave / tiles_x, tx = wave - ty * tiles_x;
+    i = ty * 8 + (lane >> 3);
+    j = tx * 8 + (lane & 7);
+    return i < n && j < n;
+}
+
+template <typename T, bool ADAPTIVE, bool DISK, int IC>
+__global__ __launch_bounds__(256) void trace_kernel(Params P, ICArgs ic, Outputs<T> out, int64_t n) {
+    int64_t idx;
+    double y0[8];
+    if (IC == IC_IMAGE) {
+        int64_t i, j;
+        if (!image_pixel(ic.B.n, i, j)) return;
+        idx = i * ic.B.n + j;
+        const double alpha = pixel_centre(ic.B.fov, j, ic.B.n);
+        const double beta = -pixel_centre(ic.B.fov, i, ic.B.n);
+        if (!bardeen_state(ic.B, alpha, beta, y0) && ic.flags) atomicOr(ic.flags, KR_FLAG_RADICAND);
+    } else {
+        idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+        if (idx >= n) return;
+        if (IC == IC_Y0) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) y0[c] = ic.Y0[idx * 8 + c];
+        } else {
+            if (!bardeen_state(ic.B, ic.alpha[idx], ic.beta[idx], y0) && ic.flags)
+                atomicOr(ic.flags, KR_FLAG_RADICAND);
+        }
+    }
+    RayResult<T> r;
+    integrate_ray<T, ADAPTIVE, DISK>(P, y0, r);
+    write_result<T, DISK>(out, idx, r);
+}
+
+// One bisection round, probe kernel: lane (k, j) traces probe j of direction k.
+__device__ __forceinline__ double probe_radius(double lo, double hi, int j, int n_probe) {
+    if (n_probe == 1) return 0.5 * (lo + hi);
+    return lo + (hi - lo) * (static_cast<double>(j + 1) / static_cast<double>(n_probe + 1));
+}
+
+template <typename T, bool ADAPTIVE>
+__global__ __launch_bounds__(256) void bisect_probe_kernel(Params P, BardeenImage B, double ca, double cb,
+                                                          const double* cosk, const double* sink, const double* lo,
+                                                          const double* hi, int64_t n_angles, int n_probe,
+                                                          uint8_t* escaped, unsigned long long* failures,
+                                                          int32_t* flags) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n_angles * n_probe) return;
+    const int64_t k = t / n_probe;
+    const int j = static_cast<int>(t - k * n_probe);
+    const double mid = probe_radius(lo[k], hi[k], j, n_probe);
+    const double alpha = ca + mid * cosk[k];
+    const double beta = cb + mid * sink[k];
+    double y0[8];
+    if (!bardeen_state(B, alpha, beta, y0)) atomicOr(flags, KR_FLAG_RADICAND);
+    RayResult<T> r;
+    integrate_ray<T, ADAPTIVE, false>(P, y0, r);
+    escaped[t] = r.state == ESCAPED ? 1 : 0;
+    if (r.state != ESCAPED && r.state != CAPTURED) atomicAdd(failures, 1ull);
+}
+
+// Bracket update (study_edge_bisection.py:66-67 for n_probe = 1).
+__global__ void bisect_update_kernel(double* lo, double* hi, const uint8_t* escaped, int64_t n_angles, int n_probe,
+                                     unsigned long long* nonmonotone) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n_angles) return;
+    const double l = lo[k], h = hi[k];
+    int first = -1;
+    bool bad = false;
+    for (int j = 0; j < n_probe; ++j) {
+        const bool e = escaped[k * n_probe + j] != 0;
+        if (e && first < 0) first = j;
+        if (!e && first >= 0) bad = true;
+    }
+    if (bad) atomicAdd(nonmonotone, 1ull);
+    if (first < 0) {
+        lo[k] = probe_radius(l, h, n_probe - 1, n_probe);
+    } else {
+        hi[k] = probe_radius(l, h, first, n_probe);
+        if (first > 0) lo[k] = probe_radius(l, h, first - 1, n_probe);
+    }
+}
+
+__global__ void fill_u64_kernel(unsigned long long* p, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ void fill_i32_kernel(int32_t* p, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ---------------------------------------------------------------- launches
+
+template <typename T>
+Outputs<T> typed(const kr_outputs& o) {
+    return Outputs<T>{static_cast<T*>(o.Y), o.lam, o.n_steps, o.n_rejected, o.state, o.max_null_error,
+                      o.max_energy_drift, o.max_lz_drift, o.max_carter_drift, static_cast<T*>(o.event_Y),
+                      o.event_hit};
+}
+
+template <typename T, bool ADAPTIVE, bool DISK, int IC>
+hipError_t launch_trace_t(const Params& P, const ICArgs& ic, const kr_outputs& o, int64_t n, hipStream_t s) {
+    int64_t threads;
+    if (IC == IC_IMAGE) {
+        const int64_t tiles = ((ic.B.n + 7) >> 3) * ((ic.B.n + 7) >> 3);
+        threads = tiles * 64;
+    } else {
+        threads = n;
+    }
+    if (threads == 0) return hipSuccess;
+    const int64_t blocks = (threads + 255) / 256;
+    hipLaunchKernelGGL((trace_kernel<T, ADAPTIVE, DISK, IC>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,
+                       P, ic, typed<T>(o), n);
+    return hipGetLastError();
+}
+
+template <int IC>
+hipError_t launch_trace(const Params& P, int dtype, const ICArgs& ic, const kr_outputs& o, int64_t n, hipStream_t s) {
+    const bool adaptive = P.ipar[I_ADAPTIVE] != 0;
+    const bool disk = P.ipar[I_DISK] != 0;
+    if (dtype == KR_FLOAT64) {
+        if (adaptive)
+            return disk ? launch_trace_t<double, true, true, IC>(P, ic, o, n, s)
+                        : launch_trace_t<double, true, false, IC>(P, ic, o, n, s);
+        return disk ? launch_trace_t<double, false, true, IC>(P, ic, o, n, s)
+                    : launch_trace_t<double, false, false, IC>(P, ic, o, n, s);
+    }
+    if (adaptive)
+        return disk ? launch_trace_t<float, true, true, IC>(P, ic, o, n, s)
+                    : launch_trace_t<float, true, false, IC>(P, ic, o, n, s);
+    return disk ? launch_trace_t<float, false, true, IC>(P, ic, o, n, s)
+                : launch_trace_t<float, false, false, IC>(P, ic, o, n, s);
+}
+
+hipError_t fill_i32(int32_t* p, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t fill_u64(unsigned long long* p, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+}  // namespace kr
+
+// ------------------------------------------------------------------ C ABI
+
+using namespace kr;
+
+extern "C" int kr_abi_version(void) { return KR_ABI_VERSION; }
+
+extern "C" int kr_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return -1;
+    return n;
+}
+
+extern "C" const char* kr_last_error(void) { return host::last_error().c_str(); }
+
+extern "C" kr_status kr_create(int device, kr_handle* out) {
+    host::clear_error();
+    if (!out) return host::fail(KR_ERR_INVALID, "kr_create: out is NULL");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0)
+        return host::fail(KR_ERR_DEVICE, std::string("no HIP device available: ") + hipGetErrorString(e));
+    if (device < 0 || device >= count) return host::fail(KR_ERR_INVALID, "kr_create: device index out of range");
+    kr_context* h = new (std::nothrow) kr_context();
+    if (!h) return host::fail(KR_ERR_DEVICE, "kr_create: out of host memory");
+    h->device = device;
+    KR_HIP(hipSetDevice(device));
+    KR_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    KR_HIP(hipEventCreate(&h->ev0));
+    KR_HIP(hipEventCreate(&h->ev1));
+    *out = h;
+    return KR_OK;
+}
+
+extern "C" kr_status kr_destroy(kr_handle h) {
+    host::clear_error();
+    if (!h) return KR_OK;
+    hipSetDevice(h->device);
+    if (h->stream) hipStreamSynchronize(h->stream);
+    if (h->buf) hipFree(h->buf);
+    if (h->ev0) hipEventDestroy(h->ev0);
+    if (h->ev1) hipEventDestroy(h->ev1);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return KR_OK;
+}
+
+namespace {
+
+// Device layout of one batch in the handle's buffer.
+struct BatchBuffers {
+    double* Y0;
+    kr_outputs d;
+    int32_t* flags;
+};
+
+kr_status carve(kr_handle h, int64_t n, int dtype, bool need_y0, bool disk, BatchBuffers& b) {
+    const size_t ts = dtype == KR_FLOAT64 ? 8 : 4;
+    host::Carver c;
+    const size_t oY0 = need_y0 ? c.take(n * 64) : 0;
+    const size_t oY = c.take(n * 8 * ts);
+    const size_t olam = c.take(n * 8), osteps = c.take(n * 8), orej = c.take(n * 8), ostate = c.take(n * 8);
+    const size_t om0 = c.take(n * 8), om1 = c.take(n * 8), om2 = c.take(n * 8), om3 = c.take(n * 8);
+    const size_t oev = disk ? c.take(n * 8 * ts) : 0;
+    const size_t ohit = disk ? c.take(n) : 0;
+    const size_t oflags = c.take(256);
+    kr_status st = host::reserve(h, c.total);
+    if (st != KR_OK) return st;
+    char* base = static_cast<char*>(h->buf);
+    b.Y0 = need_y0 ? reinterpret_cast<double*>(base + oY0) : nullptr;
+    b.d.Y = base + oY;
+    b.d.lam = reinterpret_cast<double*>(base + olam);
+    b.d.n_steps = reinterpret_cast<int64_t*>(base + osteps);
+    b.d.n_rejected = reinterpret_cast<int64_t*>(base + orej);
+    b.d.state = reinterpret_cast<int64_t*>(base + ostate);
+    b.d.max_null_error = reinterpret_cast<double*>(base + om0);
+    b.d.max_energy_drift = reinterpret_cast<double*>(base + om1);
+    b.d.max_lz_drift = reinterpret_cast<double*>(base + om2);
+    b.d.max_carter_drift = reinterpret_cast<double*>(base + om3);
+    b.d.event_Y = disk ? base + oev : nullptr;
+    b.d.event_hit = disk ? reinterpret_cast<uint8_t*>(base + ohit) : nullptr;
+    b.flags = reinterpret_cast<int32_t*>(base + oflags);
+    return KR_OK;
+}
+
+kr_status copy_back(kr_handle h, const BatchBuffers& b, const kr_outputs& out, int64_t n, int dtype, bool disk) {
+    const size_t ts = dtype == KR_FLOAT64 ? 8 : 4;
+    hipStream_t s = h->stream;
+    KR_HIP(hipMemcpyAsync(out.Y, b.d.Y, n * 8 * ts, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(out.lam, b.d.lam, n * 8, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(out.n_steps, b.d.n_steps, n * 8, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(out.n_rejected, b.d.n_rejected, n * 8, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(out.state, b.d.state, n * 8, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(out.max_null_error, b.d.max_null_error, n * 8, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(out.max_energy_drift, b.d.max_energy_drift, n * 8, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(out.max_lz_drift, b.d.max_lz_drift, n * 8, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(out.max_carter_drift, b.d.max_carter_drift, n * 8, hipMemcpyDeviceToHost, s));
+    if (disk) {
+        if (out.event_Y) KR_HIP(hipMemcpyAsync(out.event_Y, b.d.event_Y, n * 8 * ts, hipMemcpyDeviceToHost, s));
+        if (out.event_hit) KR_HIP(hipMemcpyAsync(out.event_hit, b.d.event_hit, n, hipMemcpyDeviceToHost, s));
+    }
+    KR_HIP(hipStreamSynchronize(s));
+    return KR_OK;
+}
+
+kr_status check_outputs(const kr_outputs* o, bool disk) {
+    if (!o) return host::fail(KR_ERR_INVALID, "outputs struct is NULL");
+    if (!o->Y || !o->lam || !o->n_steps || !o->n_rejected || !o->state || !o->max_null_error ||
+        !o->max_energy_drift || !o->max_lz_drift || !o->max_carter_drift)
+        return host::fail(KR_ERR_INVALID, "an output array is NULL");
+    if (disk && (!o->event_Y || !o->event_hit))
+        return host::fail(KR_ERR_INVALID, "disk event enabled but event_Y/event_hit is NULL");
+    return KR_OK;
+}
+
+}  // namespace
+
+extern "C" kr_status kr_integrate_batch_dev(kr_handle h, const kr_params* p, const double* d_Y0, int64_t n,
+                                            const kr_outputs* d_out, void* stream) {
+    host::clear_error();
+    Params P;
+    kr_status st = host::validate(h, p, n, &P);
+    if (st != KR_OK) return st;
+    const bool disk = P.ipar[I_DISK] != 0;
+    if ((st = check_outputs(d_out, disk)) != KR_OK) return st;
+    if (n > 0 && !d_Y0) return host::fail(KR_ERR_INVALID, "Y0 is NULL");
+    KR_HIP(hipSetDevice(h->device));
+    ICArgs ic{};
+    ic.Y0 = d_Y0;
+    KR_HIP(launch_trace<IC_Y0>(P, p->dtype, ic, *d_out, n, static_cast<hipStream_t>(stream)));
+    return KR_OK;
+}
+
+extern "C" kr_status kr_integrate_batch(kr_handle h, const kr_params* p, const double* Y0, int64_t n,
+                                        const kr_outputs* out, double* kernel_ms) {
+    host::clear_error();
+    Params P;
+    kr_status st = host::validate(h, p, n, &P);
+    if (st != KR_OK) return st;
+    const bool disk = P.ipar[I_DISK] != 0;
+    if ((st = check_outputs(out, disk)) != KR_OK) return st;
+    if (n > 0 && !Y0) return host::fail(KR_ERR_INVALID, "Y0 is NULL");
+    KR_HIP(hipSetDevice(h->device));
+    BatchBuffers b;
+    if ((st = carve(h, n, p->dtype, true, disk, b)) != KR_OK) return st;
+    hipStream_t s = h->stream;
+    if (n > 0) KR_HIP(hipMemcpyAsync(b.Y0, Y0, n * 64, hipMemcpyHostToDevice, s));
+    ICArgs ic{};
+    ic.Y0 = b.Y0;
+    KR_HIP(hipEventRecord(h->ev0, s));
+    KR_HIP(launch_trace<IC_Y0>(P, p->dtype, ic, b.d, n, s));
+    KR_HIP(hipEventRecord(h->ev1, s));
+    if ((st = copy_back(h, b, *out, n, p->dtype, disk)) != KR_OK) return st;
+    if (kernel_ms) {
+        float ms = 0.f;
+        KR_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        *kernel_ms = ms;
+    }
+    return KR_OK;
+}
+
+extern "C" kr_status kr_trace_image_dev(kr_handle h, const kr_params* p, const kr_image* img,
+                                        const kr_outputs* d_out, int32_t* d_flags, void* stream) {
+    host::clear_error();
+    BardeenImage B;
+    kr_status st = host::bardeen_image(img, true, &B);
+    if (st != KR_OK) return st;
+    Params P;
+    if ((st = host::validate(h, p, B.n * B.n, &P)) != KR_OK) return st;
+    if (P.ipar[I_DISK] != 0) return host::fail(KR_ERR_UNSUPPORTED, "the image entry points have no disk event");
+    if ((st = check_outputs(d_out, false)) != KR_OK) return st;
+    KR_HIP(hipSetDevice(h->device));
+    ICArgs ic{};
+    ic.B = B;
+    ic.flags = d_flags;
+    KR_HIP(launch_trace<IC_IMAGE>(P, p->dtype, ic, *d_out, B.n * B.n, static_cast<hipStream_t>(stream)));
+    return KR_OK;
+}
+
+extern "C" kr_status kr_trace_image(kr_handle h, const kr_params* p, const kr_image* img, const kr_outputs* out,
+                                    double* kernel_ms) {
+    host::clear_error();
+    BardeenImage B;
+    kr_status st = host::bardeen_image(img, true, &B);
+    if (st != KR_OK) return st;
+    const int64_t n = B.n * B.n;
+    Params P;
+    if ((st = host::validate(h, p, n, &P)) != KR_OK) return st;
+    if (P.ipar[I_DISK] != 0) return host::fail(KR_ERR_UNSUPPORTED, "the image entry points have no disk event");
+    if ((st = check_outputs(out, false)) != KR_OK) return st;
+    KR_HIP(hipSetDevice(h->device));
+    BatchBuffers b;
+    if ((st = carve(h, n, p->dtype, false, false, b)) != KR_OK) return st;
+    hipStream_t s = h->stream;
+    KR_HIP(fill_i32(b.flags, 1, s));
+    ICArgs ic{};
+    ic.B = B;
+    ic.flags = b.flags;
+    KR_HIP(hipEventRecord(h->ev0, s));
+    KR_HIP(launch_trace<IC_IMAGE>(P, p->dtype, ic, b.d, n, s));
+    KR_HIP(hipEventRecord(h->ev1, s));
+    int32_t flags = 0;
+    KR_HIP(hipMemcpyAsync(&flags, b.flags, 4, hipMemcpyDeviceToHost, s));
+    if ((st = copy_back(h, b, *out, n, p->dtype, false)) != KR_OK) return st;
+    if (flags & KR_FLAG_RADICAND)
+        return host::fail(KR_ERR_RADICAND, "a Carter radicand is negative beyond round-off at the observer");
+    if (kernel_ms) {
+        float ms = 0.f;
+        KR_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        *kernel_ms = ms;
+    }
+    return KR_OK;
+}
+
+extern "C" kr_status kr_trace_points_dev(kr_handle h, const kr_params* p, const kr_image* img, const double* d_alpha,
+                                         const double* d_beta, int64_t n, const kr_outputs* d_out, int32_t* d_flags,
+                                         void* stream) {
+    host::clear_error();
+    BardeenImage B;
+    kr_status st = host::bardeen_image(img, false, &B);
+    if (st != KR_OK) return st;
+    Params P;
+    if ((st = host::validate(h, p, n, &P)) != KR_OK) return st;
+    if (P.ipar[I_DISK] != 0) return host::fail(KR_ERR_UNSUPPORTED, "the point entry points have no disk event");
+    if ((st = check_outputs(d_out, false)) != KR_OK) return st;
+    if (n > 0 && (!d_alpha || !d_beta)) return host::fail(KR_ERR_INVALID, "alpha/beta is NULL");
+    KR_HIP(hipSetDevice(h->device));
+    ICArgs ic{};
+    ic.B = B;
+    ic.alpha = d_alpha;
+    ic.beta = d_beta;
+    ic.flags = d_flags;
+    KR_HIP(launch_trace<IC_POINTS>(P, p->dtype, ic, *d_out, n, static_cast<hipStream_t>(stream)));
+    return KR_OK;
+}
+
+extern "C" kr_status kr_bisect_edge(kr_handle h, const kr_params* p, const kr_image* img, double centre_alpha,
+                                    double centre_beta, const double* cos_angles, const double* sin_angles,
+                                    int64_t n_angles, double* lo, double* hi, int32_t iterations, int32_t n_probe,
+                                    int64_t* failures, int64_t* nonmonotone, double* kernel_ms) {
+    host::clear_error();
+    BardeenImage B;
+    kr_status st = host::bardeen_image(img, false, &B);
+    if (st != KR_OK) return st;
+    if (n_angles < 0 || iterations < 0 || n_probe < 1 || n_probe > 4096)
+        return host::fail(KR_ERR_INVALID, "need n_angles >= 0, iterations >= 0 and 1 <= n_probe <= 4096");
+    Params P;
+    if ((st = host::validate(h, p, n_angles * n_probe, &P)) != KR_OK) return st;
+    if (P.ipar[I_DISK] != 0) return host::fail(KR_ERR_UNSUPPORTED, "edge bisection has no disk event");
+    if (n_angles > 0 && (!cos_angles || !sin_angles || !lo || !hi))
+        return host::fail(KR_ERR_INVALID, "an angle or bracket array is NULL");
+    KR_HIP(hipSetDevice(h->device));
+    host::Carver c;
+    const int64_t nt = n_angles * n_probe;
+    const size_t ocos = c.take(n_angles * 8), osin = c.take(n_angles * 8), olo = c.take(n_angles * 8),
+                 ohi = c.take(n_angles * 8), oesc = c.take(nt), ocnt = c.take(16), oflags = c.take(4);
+    if ((st = host::reserve(h, c.total)) != KR_OK) return st;
+    char* base = static_cast<char*>(h->buf);
+    double* d_cos = reinterpret_cast<double*>(base + ocos);
+    double* d_sin = reinterpret_cast<double*>(base + osin);
+    double* d_lo = reinterpret_cast<double*>(base + olo);
+    double* d_hi = reinterpret_cast<double*>(base + ohi);
+    uint8_t* d_esc = reinterpret_cast<uint8_t*>(base + oesc);
+    unsigned long long* d_cnt = reinterpret_cast<unsigned long long*>(base + ocnt);
+    int32_t* d_flags = reinterpret_cast<int32_t*>(base + oflags);
+    hipStream_t s = h->stream;
+    if (n_angles > 0) {
+        KR_HIP(hipMemcpyAsync(d_cos, cos_angles, n_angles * 8, hipMemcpyHostToDevice, s));
+        KR_HIP(hipMemcpyAsync(d_sin, sin_angles, n_angles * 8, hipMemcpyHostToDevice, s));
+        KR_HIP(hipMemcpyAsync(d_lo, lo, n_angles * 8, hipMemcpyHostToDevice, s));
+        KR_HIP(hipMemcpyAsync(d_hi, hi, n_angles * 8, hipMemcpyHostToDevice, s));
+    }
+    KR_HIP(fill_u64(d_cnt, 2, s));
+    KR_HIP(fill_i32(d_flags, 1, s));
+    KR_HIP(hipEventRecord(h->ev0, s));
+    const bool adaptive = P.ipar[I_ADAPTIVE] != 0;
+    for (int it = 0; it < iterations && nt > 0; ++it) {
+        const unsigned blocks = static_cast<unsigned>((nt + 255) / 256);
+        if (p->dtype == KR_FLOAT64) {
+            if (adaptive)
+                hipLaunchKernelGGL((bisect_probe_kernel<double, true>), dim3(blocks), dim3(256), 0, s, P, B,
+                                   centre_alpha, centre_beta, d_cos, d_sin, d_lo, d_hi, n_angles, n_probe, d_esc,
+                                   d_cnt, d_flags);
+            else
+                hipLaunchKernelGGL((bisect_probe_kernel<double, false>), dim3(blocks), dim3(256), 0, s, P, B,
+                                   centre_alpha, centre_beta, d_cos, d_sin, d_lo, d_hi, n_angles, n_probe, d_esc,
+                                   d_cnt, d_flags);
+        } else {
+            if (adaptive)
+                hipLaunchKernelGGL((bisect_probe_kernel<float, true>), dim3(blocks), dim3(256), 0, s, P, B,
+                                   centre_alpha, centre_beta, d_cos, d_sin, d_lo, d_hi, n_angles, n_probe, d_esc,
+                                   d_cnt, d_flags);
+            else
+                hipLaunchKernelGGL((bisect_probe_kernel<float, false>), dim3(blocks), dim3(256), 0, s, P, B,
+                                   centre_alpha, centre_beta, d_cos, d_sin, d_lo, d_hi, n_angles, n_probe, d_esc,
+                                   d_cnt, d_flags);
+        }
+        KR_HIP(hipGetLastError());
+        hipLaunchKernelGGL(bisect_update_kernel, dim3(static_cast<unsigned>((n_angles + 255) / 256)), dim3(256), 0,
+                           s, d_lo, d_hi, d_esc, n_angles, n_probe, d_cnt + 1);
+        KR_HIP(hipGetLastError());
+    }
+    KR_HIP(hipEventRecord(h->ev1, s));
+    unsigned long long cnt[2] = {0, 0};
+    int32_t flags = 0;
+    if (n_angles > 0) {
+        KR_HIP(hipMemcpyAsync(lo, d_lo, n_angles * 8, hipMemcpyDeviceToHost, s));
+        KR_HIP(hipMemcpyAsync(hi, d_hi, n_angles * 8, hipMemcpyDeviceToHost, s));
+    }
+    KR_HIP(hipMemcpyAsync(cnt, d_cnt, 16, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(&flags, d_flags, 4, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipStreamSynchronize(s));
+    if (failures) *failures = static_cast<int64_t>(cnt[0]);
+    if (nonmonotone) *nonmonotone = static_cast<int64_t>(cnt[1]);
+    if (flags & KR_FLAG_RADICAND)
+        return host::fail(KR_ERR_RADICAND, "a Carter radicand is negative beyond round-off at the observer");
+    if (kernel_ms) {
+        float ms = 0.f;
+        KR_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        *kernel_ms = ms;
+    }
+    return KR_OK;
+}

@@ -1,0 +1,198 @@
+"""Shadow images: trace every pixel, classify, keep the captured mask.
+
+Mirrors ``ShadowImage``, ``state_counts`` and ``compute_shadow`` of the
+reference (src/kerrray/raytracing/shadow.py:59-211), including its ``.npz``
+file layout (format version 1). For a :class:`~kerrray_amd.camera.BardeenCamera`
+on the ``hip`` backend the initial states are generated on the device
+(``kr_trace_image``: one pixel per lane, 8 x 8 pixel tiles per wavefront)
+instead of being built on the host and copied over.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Any
+
+import numpy as np
+from numpy.typing import NDArray
+
+from kerrray_amd import _lib
+from kerrray_amd.backends import Backend, HipBackend, HostBatch, get_backend, kernel_params
+from kerrray_amd.camera import BardeenCamera, bardeen_initial_states
+from kerrray_amd.geodesics import IntegratorOptions, TerminationOptions
+from kerrray_amd.geometry import Spacetime
+from kerrray_amd.photons import FAILURE_STATES, BatchResult, TerminationState
+from kerrray_amd.rays import DEFAULT_CHUNK_SIZE, ProgressCallback, trace_rays
+
+__all__ = ["NPZ_FORMAT_VERSION", "ShadowImage", "compute_shadow", "state_counts", "trace_image"]
+
+NPZ_FORMAT_VERSION = 1
+_CAMERA_FIELDS = ("radius", "inclination_deg", "phi_deg", "fov", "resolution")
+_RESULT_ARRAYS = (
+    "Y", "state", "n_steps", "lam", "max_null_error", "max_energy_drift", "max_lz_drift", "max_carter_drift",
+)
+_OPTIONAL_RESULT_ARRAYS = ("event_Y", "event_hit", "n_rejected")
+
+
+def state_counts(state: NDArray[np.integer]) -> dict[str, int]:
+    """Pixels per termination-state name (every member, zeros included)."""
+    codes = np.asarray(state)
+    return {member.name: int(np.count_nonzero(codes == int(member))) for member in TerminationState}
+
+
+@dataclass
+class ShadowImage:
+    alpha: NDArray[np.float64]
+    beta: NDArray[np.float64]
+    state: NDArray[np.int64]
+    captured: NDArray[np.bool_]
+    counts: dict[str, int]
+    result: BatchResult
+    camera: BardeenCamera
+    spacetime: Spacetime
+
+    @property
+    def shape(self) -> tuple[int, int]:
+        return self.camera.shape
+
+    @property
+    def n_rays(self) -> int:
+        return self.camera.n_rays
+
+    @property
+    def pixel_size(self) -> float:
+        return self.camera.pixel_size
+
+    def count(self, state: TerminationState) -> int:
+        return self.counts[state.name]
+
+    @property
+    def n_failed(self) -> int:
+        return sum(self.count(s) for s in FAILURE_STATES)
+
+    @property
+    def n_other(self) -> int:
+        settled = self.count(TerminationState.CAPTURED) + self.count(TerminationState.ESCAPED)
+        return self.n_rays - settled - self.n_failed
+
+    def save(self, path: Path | str) -> Path:
+        """Compressed ``.npz`` in the reference's version-1 layout (shadow.py:119-142)."""
+        target = Path(path)
+        target.parent.mkdir(parents=True, exist_ok=True)
+        payload: dict[str, Any] = {
+            "format_version": np.int64(NPZ_FORMAT_VERSION),
+            "alpha": self.alpha,
+            "beta": self.beta,
+            "image_state": self.state,
+            "captured": self.captured,
+            "mass": np.float64(self.spacetime.mass),
+            "spin": np.float64(self.spacetime.spin),
+            "runtime_s": np.float64(self.result.runtime_s),
+        }
+        for name in _CAMERA_FIELDS:
+            payload[f"camera_{name}"] = np.asarray(getattr(self.camera, name))
+        for name in _RESULT_ARRAYS:
+            payload[f"result_{name}"] = np.asarray(getattr(self.result, name))
+        for name in _OPTIONAL_RESULT_ARRAYS:
+            value = getattr(self.result, name)
+            if value is not None:
+                payload[f"result_{name}"] = np.asarray(value)
+        np.savez_compressed(target, **payload)
+        return target
+
+    @classmethod
+    def load(cls, path: Path | str) -> ShadowImage:
+        """Read a file written by :meth:`save` (pickle-free)."""
+        with np.load(Path(path), allow_pickle=False) as data:
+            version = int(data["format_version"])
+            if version != NPZ_FORMAT_VERSION:
+                raise ValueError(f"unsupported shadow file version {version} (expected {NPZ_FORMAT_VERSION})")
+            camera = BardeenCamera(
+                radius=float(data["camera_radius"]),
+                inclination_deg=float(data["camera_inclination_deg"]),
+                phi_deg=float(data["camera_phi_deg"]),
+                fov=float(data["camera_fov"]),
+                resolution=int(data["camera_resolution"]),
+            )
+            spacetime = Spacetime(mass=float(data["mass"]), spin=float(data["spin"]))
+            fields: dict[str, Any] = {name: data[f"result_{name}"] for name in _RESULT_ARRAYS}
+            for name in _OPTIONAL_RESULT_ARRAYS:
+                key = f"result_{name}"
+                fields[name] = data[key] if key in data.files else None
+            result = BatchResult(runtime_s=float(data["runtime_s"]), **fields)
+            state = np.asarray(data["image_state"], dtype=np.int64)
+            return cls(
+                alpha=np.asarray(data["alpha"], dtype=np.float64),
+                beta=np.asarray(data["beta"], dtype=np.float64),
+                state=state,
+                captured=np.asarray(data["captured"], dtype=bool),
+                counts=state_counts(state),
+                result=result,
+                camera=camera,
+                spacetime=spacetime,
+            )
+
+
+def kr_image(st: Spacetime, r_o: float, inclination_deg: float, fov: float = 1.0, resolution: int = 2) -> _lib.KrImage:
+    import math
+
+    img = _lib.KrImage()
+    img.mass = st.mass
+    img.spin = st.spin
+    img.r_o = float(r_o)
+    img.theta_o = math.radians(inclination_deg)
+    img.fov = float(fov)
+    img.resolution = int(resolution)
+    return img
+
+
+def trace_image(engine: HipBackend, st: Spacetime, cam: BardeenCamera, integ: IntegratorOptions,
+                term: TerminationOptions) -> BatchResult:
+    """All pixels of ``cam`` on the device in one launch (``kr_trace_image``), row-major."""
+    if integ.method == "dop853":
+        raise ValueError("method 'dop853' is available in integrate() only; the hip backend supports 'rk4' and 'rk45'")
+    t0 = time.perf_counter()
+    params = kernel_params(st, integ, term, None)
+    img = kr_image(st, cam.radius, cam.inclination_deg, cam.fov, cam.resolution)
+    out = HostBatch(cam.n_rays, integ.np_dtype, False)
+    ms = ctypes.c_double(0.0)
+    with engine.handle._lock:
+        _lib.check(
+            _lib.lib().kr_trace_image(engine.handle.ptr, ctypes.byref(params), ctypes.byref(img),
+                                      ctypes.byref(out.outputs), ctypes.byref(ms)),
+            "kr_trace_image",
+        )
+    engine.last_kernel_ms = ms.value
+    return out.result(time.perf_counter() - t0)
+
+
+def compute_shadow(
+    st: Spacetime,
+    cam: BardeenCamera,
+    integ: IntegratorOptions,
+    term: TerminationOptions,
+    *,
+    backend: str | Backend = "hip",
+    progress: ProgressCallback | None = None,
+    chunk_size: int = DEFAULT_CHUNK_SIZE,
+) -> ShadowImage:
+    """Trace every pixel of ``cam`` and classify it (shadow.py:179-211)."""
+    if not isinstance(cam, BardeenCamera):
+        raise TypeError("compute_shadow needs a BardeenCamera (image-plane initial conditions)")
+    engine = get_backend(backend) if isinstance(backend, str) else backend
+    if isinstance(engine, HipBackend):
+        result = trace_image(engine, st, cam, integ, term)
+        if progress is not None:
+            progress(1.0)
+    else:
+        result = trace_rays(st, bardeen_initial_states(cam, st), integ, term, backend=engine, progress=progress,
+                            chunk_size=chunk_size)
+    alpha, beta = cam.pixel_coordinates()
+    state = np.asarray(result.state, dtype=np.int64).reshape(cam.shape)
+    return ShadowImage(
+        alpha=alpha, beta=beta, state=state, captured=state == int(TerminationState.CAPTURED),
+        counts=state_counts(state), result=result, camera=cam, spacetime=st,
+    )

@@ -1,0 +1,531 @@
+// CPU oracle for the per-photon null-geodesic path.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker: it is loaded by the
+// tests, by the smoke check and by the benchmark's CPU-baseline leg, and by
+// nothing else. The product path (kerrray_amd/) never links or calls it.
+//
+// It restates, operation by operation and in the same floating-point order,
+// the reference's scalar Numba kernel
+//   src/kerrray/raytracing/backends/numba_kernel.py:111-474
+// (inverse_metric_terms :111-154, geodesic_rhs_scalar :158-197,
+//  dormand_prince_step :201-238, rk4_step :242-257, _error_norm :261-268,
+//  _step_factor :272-281, _all_finite :285-289, _classify :293-311,
+//  _null_error :315-321, _carter :325-336, _integrate_ray :340-463,
+//  integrate_rays_kernel :466-474)
+// and the Bardeen image-plane initial conditions
+//   src/kerrray/raytracing/camera.py:206-222          (pixel centres)
+//   src/kerrray/benchmarks/rayset.py:158-165          (_off_equatorial_state)
+//   src/kerrray/geodesics/initial_conditions.py:50-112 (carter_potentials,
+//                                                      photon_from_constants)
+// Build: oracle/Makefile (g++ -O2 -fno-fast-math -ffp-contract=off -fopenmp),
+// so that no contraction or reassociation changes a rounding.  Transcendentals
+// are glibc's sin, cos, pow (the same libm the compiled Numba kernel calls).
+//
+// Working dtype: the reference specialises the kernel on float32/float64;
+// here the template parameter T plays that role.  The metric is always
+// evaluated in double and cast, as in numba_kernel.py:165-167.
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <omp.h>
+
+namespace {
+
+enum { RUNNING = 0, ESCAPED = 1, CAPTURED = 2, MAX_AFFINE = 3, FAILURE = 4, OUT_OF_DOMAIN = 5, DISK_HIT = 6 };
+enum { G_MASS, G_A, G_A2, G_NEG_2MA, G_POS_2MA, G_TWO_M, G_TWO_A2 };
+enum {
+    C_ZERO, C_ONE, C_TWO, C_NEG_HALF, C_EIGHT, C_SAFETY, C_FMIN, C_FMAX, C_EXP, C_RTOL,
+    C_ATOL, C_HMIN, C_RCAP, C_RESC, C_THLO, C_THHI, C_AXIS, C_H0, C_RIN, C_ROUT, C_HALFPI,
+    N_CONSTS
+};
+enum { P_LAMMAX, P_HALFPI, P_HMAX };
+enum { I_MAX_STEPS, I_GUARD, I_ADAPTIVE, I_DISK };
+enum { IDX_T, IDX_R, IDX_TH, IDX_PH, IDX_PT, IDX_PR, IDX_PTH, IDX_PPH };
+constexpr int S = 8;
+
+// Python's builtin max/min on two floats: the first argument unless the
+// second compares strictly greater (smaller).
+template <typename T> inline T pymax(T a, T b) { return b > a ? b : a; }
+template <typename T> inline T pymin(T a, T b) { return b < a ? b : a; }
+
+inline double wpow(double x, double y) { return std::pow(x, y); }
+inline float wpow(float x, float y) { return powf(x, y); }
+inline double wsqrt(double x) { return std::sqrt(x); }
+inline float wsqrt(float x) { return sqrtf(x); }
+inline double wsin(double x) { return std::sin(x); }
+inline float wsin(float x) { return sinf(x); }
+
+// numba_kernel.py:111-154
+void inverse_metric_terms(const double* geom, double r, double th, double* out) {
+    const double m = geom[G_MASS];
+    const double a2 = geom[G_A2];
+    const double s = std::sin(th);
+    const double c = std::cos(th);
+    const double s2 = s * s;
+    const double sin2th = std::sin(2.0 * th);
+    const double r2 = r * r;
+    const double sig = r2 + a2 * (c * c);
+    const double dlt = r2 - geom[G_TWO_M] * r + a2;
+    const double r2a2 = r2 + a2;
+    const double big_a = r2a2 * r2a2 - a2 * dlt * s2;
+    const double sd = sig * dlt;
+    out[0] = -big_a / sd;
+    out[1] = geom[G_NEG_2MA] * r / sd;
+    out[2] = dlt / sig;
+    out[3] = 1.0 / sig;
+    out[4] = (dlt - a2 * s2) / (sd * s2);
+    const double sig_r = 2.0 * r;
+    const double sig_th = (-a2) * sin2th;
+    const double dlt_r = 2.0 * (r - m);
+    const double a_r = 4.0 * r * r2a2 - geom[G_TWO_A2] * (r - m) * s2;
+    const double a_th = (-a2) * dlt * sin2th;
+    const double sd_r = sig_r * dlt + sig * dlt_r;
+    const double sd_th = sig_th * dlt;
+    const double sd2 = sd * sd;
+    const double sig2 = sig * sig;
+    out[5] = -(a_r * sd - big_a * sd_r) / sd2;
+    out[6] = geom[G_NEG_2MA] * (sd - r * sd_r) / sd2;
+    out[7] = (dlt_r * sig - dlt * sig_r) / sig2;
+    out[8] = -sig_r / sig2;
+    out[9] = -sig_r / (sig2 * s2) + a2 * sd_r / sd2;
+    out[10] = -(a_th * sd - big_a * sd_th) / sd2;
+    out[11] = geom[G_POS_2MA] * r * sd_th / sd2;
+    out[12] = -dlt * sig_th / sig2;
+    out[13] = -sig_th / sig2;
+    out[14] = -(sig_th * s2 + sig * sin2th) / (sig2 * (s2 * s2)) + a2 * sd_th / sd2;
+}
+
+// numba_kernel.py:158-197
+template <typename T>
+void geodesic_rhs_scalar(const double* geom, const T* consts, const T* y, T* f) {
+    double g64[15];
+    T g[15];
+    inverse_metric_terms(geom, static_cast<double>(y[IDX_R]), static_cast<double>(y[IDX_TH]), g64);
+    for (int i = 0; i < 15; ++i) g[i] = static_cast<T>(g64[i]);
+    const T zero = consts[C_ZERO];
+    const T two = consts[C_TWO];
+    const T p_t = y[IDX_PT], p_r = y[IDX_PR], p_th = y[IDX_PTH], p_ph = y[IDX_PPH];
+    T gphph = g[4], r_gphph = g[9], t_gphph = g[14];
+    if (p_ph == zero) {
+        gphph = zero;
+        r_gphph = zero;
+        t_gphph = zero;
+    }
+    f[IDX_T] = g[0] * p_t + g[1] * p_ph;
+    f[IDX_R] = g[2] * p_r;
+    f[IDX_TH] = g[3] * p_th;
+    f[IDX_PH] = g[1] * p_t + gphph * p_ph;
+    f[IDX_PT] = zero;
+    const T pt2 = p_t * p_t, ptpph = p_t * p_ph, pr2 = p_r * p_r, pth2 = p_th * p_th, pph2 = p_ph * p_ph;
+    f[IDX_PR] = consts[C_NEG_HALF] * (g[5] * pt2 + two * g[6] * ptpph + g[7] * pr2 + g[8] * pth2 + r_gphph * pph2);
+    f[IDX_PTH] = consts[C_NEG_HALF] * (g[10] * pt2 + two * g[11] * ptpph + g[12] * pr2 + g[13] * pth2 + t_gphph * pph2);
+    f[IDX_PPH] = zero;
+}
+
+// numba_kernel.py:201-238 (k[0] = f(y) on entry; fills k[1..6], y_new, err)
+template <typename T>
+void dormand_prince_step(const double* geom, const T* consts, const T* A, const T* E, const T* y, T h,
+                         T (*k)[S], T* ys, T* y_new, T* err) {
+#define KA(i, j) A[(i) * 7 + (j)]
+    for (int i = 0; i < S; ++i) ys[i] = y[i] + h * (KA(1, 0) * k[0][i]);
+    geodesic_rhs_scalar(geom, consts, ys, k[1]);
+    for (int i = 0; i < S; ++i) ys[i] = y[i] + h * (KA(2, 0) * k[0][i] + KA(2, 1) * k[1][i]);
+    geodesic_rhs_scalar(geom, consts, ys, k[2]);
+    for (int i = 0; i < S; ++i) ys[i] = y[i] + h * (KA(3, 0) * k[0][i] + KA(3, 1) * k[1][i] + KA(3, 2) * k[2][i]);
+    geodesic_rhs_scalar(geom, consts, ys, k[3]);
+    for (int i = 0; i < S; ++i)
+        ys[i] = y[i] + h * (KA(4, 0) * k[0][i] + KA(4, 1) * k[1][i] + KA(4, 2) * k[2][i] + KA(4, 3) * k[3][i]);
+    geodesic_rhs_scalar(geom, consts, ys, k[4]);
+    for (int i = 0; i < S; ++i)
+        ys[i] = y[i] + h * (KA(5, 0) * k[0][i] + KA(5, 1) * k[1][i] + KA(5, 2) * k[2][i] + KA(5, 3) * k[3][i] +
+                            KA(5, 4) * k[4][i]);
+    geodesic_rhs_scalar(geom, consts, ys, k[5]);
+    for (int i = 0; i < S; ++i)
+        y_new[i] = y[i] + h * (KA(6, 0) * k[0][i] + KA(6, 2) * k[2][i] + KA(6, 3) * k[3][i] + KA(6, 4) * k[4][i] +
+                               KA(6, 5) * k[5][i]);
+    geodesic_rhs_scalar(geom, consts, y_new, k[6]);
+    for (int i = 0; i < S; ++i)
+        err[i] = h * (E[0] * k[0][i] + E[2] * k[2][i] + E[3] * k[3][i] + E[4] * k[4][i] + E[5] * k[5][i] +
+                      E[6] * k[6][i]);
+#undef KA
+}
+
+// numba_kernel.py:242-257
+template <typename T>
+void rk4_step(const double* geom, const T* consts, const T* b, const T* c, const T* y, T h, T (*k)[S], T* ys,
+              T* y_new) {
+    for (int i = 0; i < S; ++i) ys[i] = y[i] + h * (c[1] * k[0][i]);
+    geodesic_rhs_scalar(geom, consts, ys, k[1]);
+    for (int i = 0; i < S; ++i) ys[i] = y[i] + h * (c[2] * k[1][i]);
+    geodesic_rhs_scalar(geom, consts, ys, k[2]);
+    for (int i = 0; i < S; ++i) ys[i] = y[i] + h * (c[3] * k[2][i]);
+    geodesic_rhs_scalar(geom, consts, ys, k[3]);
+    for (int i = 0; i < S; ++i) y_new[i] = y[i] + h * (b[0] * k[0][i] + b[1] * k[1][i] + b[2] * k[2][i] + b[3] * k[3][i]);
+    geodesic_rhs_scalar(geom, consts, y_new, k[6]);
+}
+
+// numba_kernel.py:261-268: HNW scaled RMS, NumPy's pairwise order over 8 terms.
+template <typename T>
+T error_norm(const T* consts, const T* err, const T* y, const T* y_new) {
+    T q[S];
+    for (int i = 0; i < S; ++i) {
+        const T scale = consts[C_ATOL] + consts[C_RTOL] * pymax(std::fabs(y[i]), std::fabs(y_new[i]));
+        const T v = err[i] / scale;
+        q[i] = v * v;
+    }
+    const T total = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
+    return wsqrt(total / consts[C_EIGHT]);
+}
+
+// numba_kernel.py:272-281
+template <typename T>
+T step_factor(const T* consts, T en, bool rejected_before) {
+    T factor;
+    if (en == consts[C_ZERO]) {
+        factor = consts[C_FMAX];
+    } else {
+        const T raw = consts[C_SAFETY] * wpow(en, consts[C_EXP]);
+        factor = pymin(consts[C_FMAX], pymax(consts[C_FMIN], raw));
+    }
+    if (rejected_before) factor = pymin(factor, consts[C_ONE]);
+    return factor;
+}
+
+template <typename T>
+bool all_finite(const T* y, const T* f) {
+    for (int i = 0; i < S; ++i)
+        if (!(std::isfinite(y[i]) && std::isfinite(f[i]))) return false;
+    return true;
+}
+
+// numba_kernel.py:293-311
+template <typename T>
+int classify(const T* consts, const double* params, const int64_t* ipar, const T* y, const T* f, double lam,
+             int64_t n_steps, T h, bool check_h) {
+    if (!all_finite(y, f)) return FAILURE;
+    if (check_h && h < consts[C_HMIN]) return FAILURE;
+    const T r = y[IDX_R];
+    const T th = y[IDX_TH];
+    if (r < consts[C_ZERO] || th < consts[C_THLO] || th > consts[C_THHI]) return OUT_OF_DOMAIN;
+    if (y[IDX_PPH] != consts[C_ZERO] && std::fabs(wsin(th)) < consts[C_AXIS]) return OUT_OF_DOMAIN;
+    if (r <= consts[C_RCAP]) return CAPTURED;
+    if (r >= consts[C_RESC] && f[IDX_R] > consts[C_ZERO]) return ESCAPED;
+    if (lam >= params[P_LAMMAX] || n_steps >= ipar[I_MAX_STEPS]) return MAX_AFFINE;
+    return RUNNING;
+}
+
+// numba_kernel.py:315-321
+template <typename T>
+double null_error(const T* y, const T* f) {
+    double total = static_cast<double>(y[IDX_PT]) * static_cast<double>(f[IDX_T]);
+    total = total + static_cast<double>(y[IDX_PR]) * static_cast<double>(f[IDX_R]);
+    total = total + static_cast<double>(y[IDX_PTH]) * static_cast<double>(f[IDX_TH]);
+    total = total + static_cast<double>(y[IDX_PPH]) * static_cast<double>(f[IDX_PH]);
+    return std::fabs(0.5 * total);
+}
+
+// numba_kernel.py:325-336
+template <typename T>
+double carter(double a2, const T* y) {
+    const double th = static_cast<double>(y[IDX_TH]);
+    const double p_th = static_cast<double>(y[IDX_PTH]);
+    const double lz = static_cast<double>(y[IDX_PPH]);
+    const double e = -static_cast<double>(y[IDX_PT]);
+    const double s = std::sin(th);
+    const double c = std::cos(th);
+    double lz2_over_s2 = 0.0;
+    if (lz != 0.0) lz2_over_s2 = (lz * lz) / (s * s);
+    return p_th * p_th + (c * c) * (lz2_over_s2 - a2 * e * e);
+}
+
+struct RayArgs {
+    const double* geom;
+    const double* params;
+    const int64_t* ipar;
+    // working-dtype copies are built per call (below)
+    double* lam_out;
+    int64_t* n_steps_out;
+    int64_t* n_rej_out;
+    int64_t* state_out;
+    double* max_null;
+    double* max_e;
+    double* max_lz;
+    double* max_q;
+    const double* e0;
+    const double* lz0;
+    const double* q0;
+    const double* fe;
+    const double* fl;
+    const double* fq;
+    const double* fe2;
+    uint8_t* event_hit;
+};
+
+// numba_kernel.py:340-463
+template <typename T>
+void integrate_ray(int64_t n, const RayArgs& a, const T* consts, const T* tab_a, const T* tab_e, const T* rk4_b,
+                   const T* rk4_c, T* Y, T* F, T* event_Y) {
+    T* y = Y + n * S;
+    T* f = F + n * S;
+    T k[7][S], ys[S], y_new[S], y_prev[S], err[S];
+    const double* geom = a.geom;
+    const double a2 = geom[G_A2];
+    const bool adaptive = a.ipar[I_ADAPTIVE] != 0;
+    const bool disk = a.ipar[I_DISK] != 0;
+    const double lambda_max = a.params[P_LAMMAX];
+    const double half_pi = a.params[P_HALFPI];
+    geodesic_rhs_scalar(geom, consts, y, f);
+    double m_null = null_error(y, f) / a.fe2[n];
+    double m_e = 0.0, m_lz = 0.0, m_q = 0.0;
+    double lam = 0.0;
+    T h = consts[C_H0];
+    const double h_cap = a.params[P_HMAX];
+    if (adaptive && static_cast<double>(h) > h_cap) h = static_cast<T>(h_cap);
+    int64_t n_steps = 0, n_rej = 0, attempts = 0;
+    bool rejected_before = false;
+    int state = classify(consts, a.params, a.ipar, y, f, lam, n_steps, h, false);
+    while (state == RUNNING) {
+        attempts += 1;
+        if (attempts > a.ipar[I_GUARD]) {
+            state = FAILURE;
+            break;
+        }
+        const double remaining = lambda_max - lam;
+        const bool last = static_cast<double>(h) >= remaining;
+        T h_eff = h;
+        if (last) h_eff = static_cast<T>(remaining);
+        for (int i = 0; i < S; ++i) {
+            k[0][i] = f[i];
+            y_prev[i] = y[i];
+        }
+        if (adaptive)
+            dormand_prince_step(geom, consts, tab_a, tab_e, y, h_eff, k, ys, y_new, err);
+        else
+            rk4_step(geom, consts, rk4_b, rk4_c, y, h_eff, k, ys, y_new);
+        bool finite = all_finite(y_new, k[6]);
+        bool accept = finite;
+        T h_new = h;
+        if (adaptive && finite) {
+            const T en = error_norm(consts, err, y, y_new);
+            if (!std::isfinite(en)) {
+                finite = false;
+                accept = false;
+            } else {
+                accept = en <= consts[C_ONE];
+                const T factor = step_factor(consts, en, rejected_before);
+                if (!(accept && last)) {
+                    h_new = h_eff * factor;
+                    if (static_cast<double>(h_new) > h_cap) h_new = static_cast<T>(h_cap);
+                }
+            }
+        }
+        if (!finite) {
+            state = FAILURE;
+            break;
+        }
+        const bool rejected = !accept;
+        if (rejected) n_rej += 1;
+        rejected_before = rejected;
+        h = h_new;
+        if (rejected) {
+            if (h_new < consts[C_HMIN]) {
+                state = FAILURE;
+                break;
+            }
+            continue;
+        }
+        for (int i = 0; i < S; ++i) {
+            y[i] = y_new[i];
+            f[i] = k[6][i];
+        }
+        const double lam_old = lam;
+        if (last)
+            lam = lambda_max;
+        else
+            lam = lam + static_cast<double>(h_eff);
+        n_steps += 1;
+        const double e = -static_cast<double>(y[IDX_PT]);
+        const double lz = static_cast<double>(y[IDX_PPH]);
+        m_e = pymax(m_e, std::fabs(e - a.e0[n]) / pymax(std::fabs(a.e0[n]), a.fe[n]));
+        m_lz = pymax(m_lz, std::fabs(lz - a.lz0[n]) / pymax(std::fabs(a.lz0[n]), a.fl[n]));
+        m_q = pymax(m_q, std::fabs(carter(a2, y) - a.q0[n]) / pymax(std::fabs(a.q0[n]), a.fq[n]));
+        m_null = pymax(m_null, null_error(y, f) / a.fe2[n]);
+        int code = classify(consts, a.params, a.ipar, y, f, lam, n_steps, h, true);
+        if (disk) {
+            const double th_old = static_cast<double>(y_prev[IDX_TH]);
+            const double th_new = static_cast<double>(y[IDX_TH]);
+            if ((th_old - half_pi) * (th_new - half_pi) < 0.0) {
+                const double s = (half_pi - th_old) / (th_new - th_old);
+                for (int i = 0; i < S; ++i) {
+                    const T cast = static_cast<T>(s * static_cast<double>(y[i] - y_prev[i]));
+                    ys[i] = y_prev[i] + cast;
+                }
+                ys[IDX_TH] = consts[C_HALFPI];
+                if (ys[IDX_R] >= consts[C_RIN] && ys[IDX_R] <= consts[C_ROUT]) {
+                    for (int i = 0; i < S; ++i) {
+                        event_Y[n * S + i] = ys[i];
+                        y[i] = ys[i];
+                    }
+                    a.event_hit[n] = 1;
+                    lam = lam_old + s * static_cast<double>(h_eff);
+                    code = DISK_HIT;
+                }
+            }
+        }
+        state = code;
+    }
+    a.state_out[n] = state;
+    a.n_steps_out[n] = n_steps;
+    a.n_rej_out[n] = n_rej;
+    a.lam_out[n] = lam;
+    a.max_null[n] = m_null;
+    a.max_e[n] = m_e;
+    a.max_lz[n] = m_lz;
+    a.max_q[n] = m_q;
+}
+
+template <typename T>
+void run_all(int64_t n_rays, const RayArgs& a, const double* consts64, const double* tab_a64, const double* tab_e64,
+             const double* rk4_b64, const double* rk4_c64, void* Y, void* F, void* event_Y, int n_threads,
+             int dynamic_schedule) {
+    // Working-dtype copies, as np.array(..., dtype=dtype) does in numba_backend.py:100-128.
+    T consts[N_CONSTS], tab_a[49], tab_e[7], rk4_b[4], rk4_c[4];
+    for (int i = 0; i < N_CONSTS; ++i) consts[i] = static_cast<T>(consts64[i]);
+    for (int i = 0; i < 49; ++i) tab_a[i] = static_cast<T>(tab_a64[i]);
+    for (int i = 0; i < 7; ++i) tab_e[i] = static_cast<T>(tab_e64[i]);
+    for (int i = 0; i < 4; ++i) {
+        rk4_b[i] = static_cast<T>(rk4_b64[i]);
+        rk4_c[i] = static_cast<T>(rk4_c64[i]);
+    }
+    T* Yt = static_cast<T*>(Y);
+    T* Ft = static_cast<T*>(F);
+    T* Et = static_cast<T*>(event_Y);
+    if (n_threads <= 0) n_threads = omp_get_max_threads();
+    if (dynamic_schedule) {
+#pragma omp parallel for num_threads(n_threads) schedule(dynamic, 16)
+        for (int64_t n = 0; n < n_rays; ++n) integrate_ray<T>(n, a, consts, tab_a, tab_e, rk4_b, rk4_c, Yt, Ft, Et);
+    } else {
+#pragma omp parallel for num_threads(n_threads) schedule(static)
+        for (int64_t n = 0; n < n_rays; ++n) integrate_ray<T>(n, a, consts, tab_a, tab_e, rk4_b, rk4_c, Yt, Ft, Et);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// integrate_rays_kernel (numba_kernel.py:466-474) with the same argument
+// meaning.  dtype: 0 = float64, 1 = float32 (Y, F, event_Y in that dtype).
+// Y (n, 8) holds the initial states on entry and the final ones on return.
+int kro_integrate_rays(int dtype, const double* geom, const double* consts, const double* params, const int64_t* ipar,
+                       const double* tab_a, const double* tab_e, const double* rk4_b, const double* rk4_c, void* Y,
+                       void* F, double* lam, int64_t* n_steps, int64_t* n_rej, int64_t* state, double* max_null,
+                       double* max_e, double* max_lz, double* max_q, const double* e0, const double* lz0,
+                       const double* q0, const double* fe, const double* fl, const double* fq, const double* fe2,
+                       void* event_Y, uint8_t* event_hit, int64_t n_rays, int n_threads, int dynamic_schedule) {
+    RayArgs a{geom, params, ipar, lam, n_steps, n_rej, state, max_null, max_e, max_lz, max_q,
+              e0, lz0, q0, fe, fl, fq, fe2, event_hit};
+    if (dtype == 0)
+        run_all<double>(n_rays, a, consts, tab_a, tab_e, rk4_b, rk4_c, Y, F, event_Y, n_threads, dynamic_schedule);
+    else if (dtype == 1)
+        run_all<float>(n_rays, a, consts, tab_a, tab_e, rk4_b, rk4_c, Y, F, event_Y, n_threads, dynamic_schedule);
+    else
+        return 1;
+    return 0;
+}
+
+// Camera.pixel_coordinates (camera.py:206-222): centre_j = fov (2 j + 1 - n) / n,
+// evaluated as NumPy does: ((fov * ((2.0 * j + 1.0) - n)) / n).
+void kro_pixel_centres(double fov, int64_t n, double* centres) {
+    const double nd = static_cast<double>(n);
+    for (int64_t j = 0; j < n; ++j) centres[j] = (fov * ((2.0 * static_cast<double>(j) + 1.0) - nd)) / nd;
+}
+
+// Bardeen-constant photon at the observer (rayset.py:158-165 ->
+// initial_conditions.py:83-112 with E = 1, sign_r = -1).  Python's float
+// `x ** 2` is glibc pow(x, 2.0) and is written as such.  Returns 0, or 1 when
+// a radicand is negative beyond RADICAND_TOLERANCE (initial_conditions.py:71-74).
+int kro_bardeen_state(double mass, double a, double r0, double theta_o, double alpha, double beta, int sign_theta,
+                      double* y) {
+    const double sin_o = std::sin(theta_o), cos_o = std::cos(theta_o);
+    const double xi = -alpha * sin_o;
+    const double eta = beta * beta - std::pow(a, 2.0) * std::pow(cos_o, 2.0) + xi * xi * std::pow(cos_o / sin_o, 2.0);
+    const double E = 1.0, Lz = xi, Q = eta;
+    // carter_potentials (initial_conditions.py:50-68); delta() is NumPy: r**2 is r*r.
+    const double dlt = r0 * r0 - 2.0 * mass * r0 + std::pow(a, 2.0);
+    const double term1 = E * (r0 * r0 + a * a) - a * Lz;
+    const double term2 = dlt * (Q + std::pow(Lz - a * E, 2.0));
+    const double big_r = term1 * term1 - term2;
+    const double scale_r = term1 * term1 + std::fabs(term2);
+    const double c2 = std::pow(std::cos(theta_o), 2.0), s2 = std::pow(std::sin(theta_o), 2.0);
+    const double lz_term = (Lz == 0.0) ? 0.0 : Lz * Lz / s2;
+    const double big_theta = Q - c2 * (lz_term - a * a * E * E);
+    const double scale_theta = std::fabs(Q) + c2 * (lz_term + a * a * E * E);
+    const double tol = 1e-12;
+    if (big_r < -tol * pymax(scale_r, 1.0)) return 1;
+    if (big_theta < -tol * pymax(scale_theta, 1.0)) return 1;
+    const double p_r = -1.0 * std::sqrt(pymax(big_r, 0.0)) / dlt;
+    const double p_th = static_cast<double>(sign_theta) * std::sqrt(pymax(big_theta, 0.0));
+    y[0] = 0.0;
+    y[1] = r0;
+    y[2] = theta_o;
+    y[3] = 0.0;
+    y[4] = -E;
+    y[5] = p_r;
+    y[6] = p_th;
+    y[7] = Lz;
+    return 0;
+}
+
+// Row-major image of Bardeen states (pixel (i, j) -> row i*n + j), with
+// alpha = centre_j, beta = -centre_i and sign_theta = -1 for beta >= 0, +1
+// otherwise (the backward ray leaves towards the image's own side).
+int kro_bardeen_image(double mass, double a, double r0, double theta_o, double fov, int64_t n, double* Y) {
+    double* centres = new double[n];
+    kro_pixel_centres(fov, n, centres);
+    int bad = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double beta = -centres[i];
+        for (int64_t j = 0; j < n; ++j) {
+            const double alpha = centres[j];
+            bad |= kro_bardeen_state(mass, a, r0, theta_o, alpha, beta, beta >= 0.0 ? -1 : 1, Y + (i * n + j) * S);
+        }
+    }
+    delete[] centres;
+    return bad;
+}
+
+// Single-call building blocks, exposed for known-answer tests.
+void kro_inverse_metric_terms(const double* geom, double r, double th, double* out) {
+    inverse_metric_terms(geom, r, th, out);
+}
+
+void kro_rhs(const double* geom, const double* consts64, const double* y, double* f) {
+    double consts[N_CONSTS];
+    std::memcpy(consts, consts64, sizeof(consts));
+    geodesic_rhs_scalar<double>(geom, consts, y, f);
+}
+
+double kro_error_norm(const double* consts64, const double* err, const double* y, const double* y_new) {
+    return error_norm<double>(consts64, err, y, y_new);
+}
+
+double kro_step_factor(const double* consts64, double en, int rejected_before) {
+    return step_factor<double>(consts64, en, rejected_before != 0);
+}
+
+void kro_dp5_step(const double* geom, const double* consts64, const double* tab_a, const double* tab_e,
+                  const double* y, double h, const double* f0, double* y_new, double* err, double* f_new) {
+    double k[7][S], ys[S];
+    std::memcpy(k[0], f0, sizeof(double) * S);
+    dormand_prince_step<double>(geom, consts64, tab_a, tab_e, y, h, k, ys, y_new, err);
+    std::memcpy(f_new, k[6], sizeof(double) * S);
+}
+
+int kro_classify(const double* consts64, const double* params, const int64_t* ipar, const double* y,
+                 const double* f, double lam, int64_t n_steps, double h, int check_h) {
+    return classify<double>(consts64, params, ipar, y, f, lam, n_steps, h, check_h != 0);
+}
+
+}  // extern "C"

@@ -1,0 +1,240 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` import this module; the product path never does. It packs the
+kernel constants exactly as the reference's Numba host wrapper does
+(``src/kerrray/raytracing/backends/numba_backend.py:87-128,167-207``) and
+runs the C++ restatement in ``kr_oracle.cpp``.
+
+Inputs are duck-typed: any object with the attribute names of the
+reference's ``Spacetime`` (``mass``, ``spin``), ``IntegratorOptions``
+(``method``, ``rtol``, ``atol``, ``step_size``, ``max_steps``, ``lambda_max``,
+``h_min``, ``dtype``, ``h_max``), ``TerminationOptions``
+(``horizon_epsilon``, ``escape_radius``) and ``EventOptions``
+(``disk_plane``, ``r_in``, ``r_out``) works.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkr_oracle.so")
+
+# Dormand-Prince 5(4) and RK4 tableaus (tableaus.py:59-91; Dormand & Prince 1980;
+# Hairer, Norsett & Wanner 1993 Table 5.2), evaluated in float64 as Python does.
+RK4_C = (0.0, 0.5, 0.5, 1.0)
+RK4_B = (1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0)
+DP_A = (
+    (),
+    (1.0 / 5.0,),
+    (3.0 / 40.0, 9.0 / 40.0),
+    (44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0),
+    (19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0),
+    (9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0),
+    (35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0),
+)
+DP_B = (35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0, 0.0)
+DP_B_HAT = (5179.0 / 57600.0, 0.0, 7571.0 / 16695.0, 393.0 / 640.0, -92097.0 / 339200.0, 187.0 / 2100.0, 1.0 / 40.0)
+DP_E = tuple(bh - b for bh, b in zip(DP_B_HAT, DP_B))
+SAFETY, FACTOR_MIN, FACTOR_MAX, ERROR_EXPONENT = 0.9, 0.2, 5.0, -1.0 / 5.0
+THETA_DOMAIN_TOLERANCE = 1e-9
+AXIS_SIN_TOLERANCE = 1e-12
+
+_lib = None
+
+
+def build() -> str:
+    """Compile ``libkr_oracle.so`` with ``make`` (idempotent) and return its path."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int64)
+        vp = ctypes.c_void_p
+        L.kro_integrate_rays.restype = ctypes.c_int
+        L.kro_integrate_rays.argtypes = [ctypes.c_int] + [vp] * 27 + [ctypes.c_int64, ctypes.c_int, ctypes.c_int]
+        L.kro_bardeen_image.restype = ctypes.c_int
+        L.kro_bardeen_image.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int64, vp]
+        L.kro_bardeen_state.restype = ctypes.c_int
+        L.kro_bardeen_state.argtypes = [ctypes.c_double] * 6 + [ctypes.c_int, vp]
+        L.kro_pixel_centres.restype = None
+        L.kro_pixel_centres.argtypes = [ctypes.c_double, ctypes.c_int64, vp]
+        L.kro_inverse_metric_terms.restype = None
+        L.kro_inverse_metric_terms.argtypes = [vp, ctypes.c_double, ctypes.c_double, vp]
+        L.kro_rhs.restype = None
+        L.kro_rhs.argtypes = [vp, vp, vp, vp]
+        L.kro_error_norm.restype = ctypes.c_double
+        L.kro_error_norm.argtypes = [vp, vp, vp, vp]
+        L.kro_step_factor.restype = ctypes.c_double
+        L.kro_step_factor.argtypes = [vp, ctypes.c_double, ctypes.c_int]
+        L.kro_dp5_step.restype = None
+        L.kro_dp5_step.argtypes = [vp, vp, vp, vp, vp, ctypes.c_double, vp, vp, vp, vp]
+        L.kro_classify.restype = ctypes.c_int
+        L.kro_classify.argtypes = [vp, vp, vp, vp, vp, ctypes.c_double, ctypes.c_int64, ctypes.c_double, ctypes.c_int]
+        del dp, ip
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray | None) -> ctypes.c_void_p:
+    return ctypes.c_void_p(0 if a is None else a.ctypes.data)
+
+
+def outer_horizon(mass: float, spin: float) -> float:
+    """``r_+ = M + sqrt(M^2 - a^2)`` (geometry/horizons.py:33-42)."""
+    a = spin * mass
+    return mass + math.sqrt(mass * mass - a * a)
+
+
+def max_attempts(integ) -> int:
+    """Defensive loop guard (geodesics/integrators.py:151-162)."""
+    h_max = max(integ.lambda_max, integ.step_size)
+    rej = (math.log(h_max / integ.h_min) + integ.max_steps * math.log(FACTOR_MAX)) / math.log(1.0 / SAFETY)
+    return integ.max_steps + int(math.ceil(rej)) + 1
+
+
+def kernel_constants(st, integ, term, events):
+    """``(geom, consts, params, ipar)`` as numba_backend.py:87-115 builds them (consts kept in float64)."""
+    m = float(st.mass)
+    a = float(st.spin) * m
+    geom = np.array([m, a, a**2, -2.0 * m * a, 2.0 * m * a, 2.0 * m, 2.0 * a**2], dtype=np.float64)
+    disk = events is not None and bool(events.disk_plane)
+    r_in = float(events.r_in) if events is not None else 0.0
+    r_out = float(events.r_out) if events is not None else 0.0
+    consts = np.array(
+        [
+            0.0, 1.0, 2.0, -0.5, 8.0, SAFETY, FACTOR_MIN, FACTOR_MAX, ERROR_EXPONENT,
+            integ.rtol, integ.atol, integ.h_min, outer_horizon(m, float(st.spin)) + term.horizon_epsilon,
+            term.escape_radius, -THETA_DOMAIN_TOLERANCE, np.pi + THETA_DOMAIN_TOLERANCE,
+            AXIS_SIN_TOLERANCE, integ.step_size, r_in, r_out, 0.5 * np.pi,
+        ],
+        dtype=np.float64,
+    )
+    h_max = np.inf if integ.h_max is None else integ.h_max
+    params = np.array([integ.lambda_max, 0.5 * np.pi, h_max], dtype=np.float64)
+    ipar = np.array([integ.max_steps, max_attempts(integ), int(integ.method == "rk45"), int(disk)], dtype=np.int64)
+    return geom, consts, params, ipar
+
+
+def tableaus():
+    """Dense DP ``A`` (7x7), ``E`` (7,), RK4 ``B``, ``C`` in float64 (numba_backend.py:118-128)."""
+    tab_a = np.zeros((7, 7), dtype=np.float64)
+    for i, row in enumerate(DP_A):
+        tab_a[i, : len(row)] = row
+    return tab_a, np.asarray(DP_E, np.float64), np.asarray(RK4_B, np.float64), np.asarray(RK4_C, np.float64)
+
+
+@dataclass
+class OracleResult:
+    """The reference ``BatchResult`` fields (photons/trajectories.py:100-128)."""
+
+    Y: np.ndarray
+    state: np.ndarray
+    n_steps: np.ndarray
+    lam: np.ndarray
+    max_null_error: np.ndarray
+    max_energy_drift: np.ndarray
+    max_lz_drift: np.ndarray
+    max_carter_drift: np.ndarray
+    runtime_s: float
+    event_Y: np.ndarray | None
+    event_hit: np.ndarray | None
+    n_rejected: np.ndarray
+
+
+def integrate_batch(st, Y0, integ, term, events=None, *, n_threads: int = 0, dynamic: bool = False) -> OracleResult:
+    """Run the restated kernel over the rows of ``Y0`` (N, 8)."""
+    if integ.method == "dop853":
+        raise ValueError("method 'dop853' is not supported by the batched kernel")
+    y0 = np.array(Y0, dtype=np.float64)
+    if y0.ndim != 2 or y0.shape[1] != 8:
+        raise ValueError(f"Y0 must have shape (N, 8), got {y0.shape}")
+    if not np.all(np.isfinite(y0)):
+        raise ValueError("Y0 must be finite")
+    n = y0.shape[0]
+    dtype = np.dtype(integ.dtype)
+    geom, consts, params, ipar = kernel_constants(st, integ, term, events)
+    tab_a, tab_e, rk4_b, rk4_c = tableaus()
+    Y = np.ascontiguousarray(y0.astype(dtype))
+    F = np.empty_like(Y)
+    lam = np.zeros(n)
+    n_steps = np.zeros(n, np.int64)
+    n_rej = np.zeros(n, np.int64)
+    state = np.zeros(n, np.int64)
+    maxima = [np.zeros(n) for _ in range(4)]
+    # numba_backend.py:174-178 (photons/constants.py:65-90,163-171)
+    a = float(st.spin) * float(st.mass)
+    e0 = np.ascontiguousarray(-y0[:, 4])
+    lz0 = np.ascontiguousarray(y0[:, 7])
+    th, p_th = y0[:, 2], y0[:, 6]
+    s2 = np.sin(th) ** 2
+    lz2_over_s2 = np.divide(lz0 * lz0, s2, out=np.zeros(n), where=lz0 != 0)
+    q0 = np.ascontiguousarray(p_th * p_th + np.cos(th) ** 2 * (lz2_over_s2 - a**2 * e0 * e0))
+    fe = np.abs(e0)
+    fe = np.where(fe > 0.0, fe, 1.0)
+    fl = fe * float(st.mass)
+    fq = fe * fe * float(st.mass) * float(st.mass)
+    fe2 = fe**2
+    disk = bool(ipar[3])
+    event_Y = np.zeros((n if disk else 0, 8), dtype=dtype)
+    event_hit = np.zeros(n if disk else 0, dtype=np.uint8)
+    t0 = time.perf_counter()
+    rc = lib().kro_integrate_rays(
+        0 if dtype == np.float64 else 1,
+        *(_ptr(x) for x in (geom, consts, params, ipar, tab_a, tab_e, rk4_b, rk4_c, Y, F, lam, n_steps, n_rej, state,
+                            *maxima, e0, lz0, q0, fe, fl, fq, fe2, event_Y, event_hit)),
+        n, n_threads, 1 if dynamic else 0,
+    )
+    if rc != 0:
+        raise ValueError("oracle rejected its inputs")
+    return OracleResult(
+        Y=Y, state=state, n_steps=n_steps, lam=lam, max_null_error=maxima[0], max_energy_drift=maxima[1],
+        max_lz_drift=maxima[2], max_carter_drift=maxima[3], runtime_s=time.perf_counter() - t0,
+        event_Y=event_Y if disk else None, event_hit=event_hit.astype(bool) if disk else None, n_rejected=n_rej,
+    )
+
+
+def pixel_centres(fov: float, n: int) -> np.ndarray:
+    out = np.empty(n)
+    lib().kro_pixel_centres(fov, n, _ptr(out))
+    return out
+
+
+def bardeen_image(mass: float, spin: float, r_o: float, inclination_deg: float, fov: float, n: int) -> np.ndarray:
+    """Row-major (n*n, 8) Bardeen initial states of an n x n image (restated in C)."""
+    Y = np.empty((n * n, 8))
+    rc = lib().kro_bardeen_image(mass, spin * mass, r_o, math.radians(inclination_deg), fov, n, _ptr(Y))
+    if rc != 0:
+        raise ValueError("negative radicand in the Bardeen initial conditions")
+    return Y
+
+
+def bardeen_points(mass: float, spin: float, r_o: float, inclination_deg: float, alpha, beta) -> np.ndarray:
+    """Bardeen initial states for arbitrary image points (sign_theta = -1 for beta >= 0)."""
+    alpha = np.asarray(alpha, np.float64).ravel()
+    beta = np.asarray(beta, np.float64).ravel()
+    Y = np.empty((alpha.size, 8))
+    th = math.radians(inclination_deg)
+    L = lib()
+    for k in range(alpha.size):
+        yk = np.empty(8)
+        if L.kro_bardeen_state(mass, spin * mass, r_o, th, float(alpha[k]), float(beta[k]),
+                               -1 if beta[k] >= 0.0 else 1, _ptr(yk)):
+            raise ValueError("negative radicand in the Bardeen initial conditions")
+        Y[k] = yk
+    return Y

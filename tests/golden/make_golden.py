@@ -1,0 +1,234 @@
+"""Write the golden vectors of the per-photon path from the reference itself.
+
+Run in the development container, where the read-only reference checkout is
+mounted at /root/reference:
+
+    python tests/golden/make_golden.py --workers 8
+
+What it imports from the reference, and why: the exact discrete trajectory
+of the adaptive integrator (accepted/rejected step counts, bitwise final
+states, hence the captured mask on pixels next to the shadow edge) is fixed
+only by running the reference's own code; closed forms pin the mask only
+away from the edge. Two reference entry points are run:
+
+* ``kerrray.geodesics.integrate_batch`` (NumPy batched integrator,
+  src/kerrray/geodesics/integrators_batch.py:109-241), the reference's
+  numerical reference implementation;
+* ``integrate_batch_numba(..., interpreted=True)``
+  (src/kerrray/raytracing/backends/numba_backend.py:131-207), the Numba
+  kernel source (numba_kernel.py:340-474) executed as plain Python because
+  Numba is not installed here. Interpreted, its scalar arithmetic calls the
+  same glibc ``sin``/``cos``/``pow`` the compiled kernel calls, so these rows
+  are the "Numba" answers.
+
+Initial states come from the reference's Bardeen-constant constructor
+``kerrray.benchmarks.rayset._off_equatorial_state`` (rayset.py:158-165) on
+the pixel centres of ``Camera.pixel_coordinates`` (camera.py:206-222), with
+``sign_theta = -1`` for ``beta >= 0`` and ``+1`` otherwise.
+
+Outputs (compressed .npz, data only) go next to this script.
+"""
+
+from __future__ import annotations
+
+import argparse
+import hashlib
+import math
+import os
+import sys
+import time
+from concurrent.futures import ProcessPoolExecutor
+
+import numpy as np
+
+REF_SRC = "/root/reference/src"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _ref():
+    if REF_SRC not in sys.path:
+        sys.path.insert(0, REF_SRC)
+    import kerrray.benchmarks.rayset as rayset
+    from kerrray.geodesics import IntegratorOptions, TerminationOptions, integrate_batch
+    from kerrray.geometry import Spacetime
+    from kerrray.raytracing.backends.numba_backend import integrate_batch_numba
+    from kerrray.raytracing.camera import Camera
+
+    return rayset, IntegratorOptions, TerminationOptions, integrate_batch, Spacetime, integrate_batch_numba, Camera
+
+
+def pixel_grid(fov: float, n: int):
+    """alpha, beta (H, W) of the reference camera grid (camera.py:206-222)."""
+    *_, Camera = _ref()
+    cam = Camera(radius=1000.0, inclination_deg=60.0, fov=fov, resolution=n)
+    return cam.pixel_coordinates()
+
+
+def bardeen_states(spin: float, r_o: float, incl_deg: float, alpha: np.ndarray, beta: np.ndarray) -> np.ndarray:
+    rayset, _, _, _, Spacetime, _, _ = _ref()
+    st = Spacetime(1.0, spin)
+    th = math.radians(incl_deg)
+    a, b = alpha.ravel(), beta.ravel()
+    out = np.empty((a.size, 8))
+    for k in range(a.size):
+        out[k] = rayset._off_equatorial_state(st, r_o, th, float(a[k]), float(b[k]), -1 if b[k] >= 0.0 else 1)
+    return out
+
+
+def _numba_interp_chunk(args):
+    spin, Y0, rtol, atol = args
+    _, IntegratorOptions, TerminationOptions, _, Spacetime, integrate_batch_numba, _ = _ref()
+    res = integrate_batch_numba(
+        Spacetime(1.0, spin), Y0, IntegratorOptions(method="rk45", rtol=rtol, atol=atol), TerminationOptions(),
+        interpreted=True,
+    )
+    return _fields(res)
+
+
+def _numpy_chunk(args):
+    spin, Y0, rtol, atol = args
+    _, IntegratorOptions, TerminationOptions, integrate_batch, Spacetime, _, _ = _ref()
+    res = integrate_batch(Spacetime(1.0, spin), Y0, IntegratorOptions(method="rk45", rtol=rtol, atol=atol),
+                          TerminationOptions())
+    return _fields(res)
+
+
+def _fields(res) -> dict:
+    return {
+        "Y": res.Y, "state": res.state, "n_steps": res.n_steps, "n_rejected": res.n_rejected, "lam": res.lam,
+        "max_null_error": res.max_null_error, "max_carter_drift": res.max_carter_drift,
+    }
+
+
+def run_parallel(fn, spin, Y0, rtol, atol, workers: int, chunk: int) -> dict:
+    jobs = [(spin, Y0[i:i + chunk], rtol, atol) for i in range(0, len(Y0), chunk)]
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        parts = list(ex.map(fn, jobs))
+    return {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def config1(workers: int) -> None:
+    """Schwarzschild a=0, i=60 deg, r_o=1000, 64x64 on [-10,10]^2, rtol 1e-8 / atol 1e-10: every pixel."""
+    spin, incl, r_o, fov, n, rtol, atol = 0.0, 60.0, 1000.0, 10.0, 64, 1e-8, 1e-10
+    alpha, beta = pixel_grid(fov, n)
+    Y0 = bardeen_states(spin, r_o, incl, alpha, beta)
+    t0 = time.time()
+    nb = run_parallel(_numba_interp_chunk, spin, Y0, rtol, atol, workers, 64)
+    t1 = time.time()
+    npy = run_parallel(_numpy_chunk, spin, Y0, rtol, atol, workers, 512)
+    t2 = time.time()
+    print(f"config1: numba-interpreted {t1 - t0:.1f}s, numpy {t2 - t1:.1f}s", flush=True)
+    np.savez_compressed(
+        os.path.join(HERE, "config1_a0_i60_64.npz"),
+        spin=spin, inclination_deg=incl, r_o=r_o, fov=fov, resolution=n, rtol=rtol, atol=atol,
+        Y0=Y0,
+        **{f"numba_{k}": v for k, v in nb.items()},
+        numpy_state=npy["state"], numpy_n_steps=npy["n_steps"], numpy_n_rejected=npy["n_rejected"], numpy_Y=npy["Y"],
+    )
+
+
+def band_indices(spin: float, incl: float, fov: float, n: int, width_px: float) -> np.ndarray:
+    """Row-major pixel indices within ``width_px`` pixels of the reference's Bardeen curve."""
+    if REF_SRC not in sys.path:
+        sys.path.insert(0, REF_SRC)
+    from kerrray.geometry import Spacetime
+    from kerrray.photons.orbits import shadow_curve
+
+    ca, cb = shadow_curve(Spacetime(1.0, spin), incl, 4000)
+    alpha, beta = pixel_grid(fov, n)
+    d = 2.0 * fov / n
+    pts = np.stack([alpha.ravel(), beta.ravel()], axis=1)
+    # Distance from each pixel centre to the dense polyline (vertex distance is enough at 4000 points).
+    best = np.full(pts.shape[0], np.inf)
+    for k in range(0, ca.size, 256):
+        dx = pts[:, 0:1] - ca[None, k:k + 256]
+        dy = pts[:, 1:2] - cb[None, k:k + 256]
+        best = np.minimum(best, np.sqrt(dx * dx + dy * dy).min(axis=1))
+    return np.flatnonzero(best <= width_px * d)
+
+
+def config2(workers: int) -> None:
+    """Kerr a=0.9, i=60 deg, r_o=1000, 512x512 on [-10,10]^2, rtol 1e-6 / atol 1e-8."""
+    spin, incl, r_o, fov, n, rtol, atol = 0.9, 60.0, 1000.0, 10.0, 512, 1e-6, 1e-8
+    alpha, beta = pixel_grid(fov, n)
+    t0 = time.time()
+    Y0 = bardeen_states(spin, r_o, incl, alpha, beta)
+    t1 = time.time()
+    npy = run_parallel(_numpy_chunk, spin, Y0, rtol, atol, workers, 8192)
+    t2 = time.time()
+    band = band_indices(spin, incl, fov, n, 2.5)
+    sample = np.arange(0, n * n, 251)
+    pick = np.union1d(band, sample)
+    nb = run_parallel(_numba_interp_chunk, spin, Y0[pick], rtol, atol, workers, 32)
+    t3 = time.time()
+    print(f"config2: IC {t1 - t0:.1f}s, numpy {t2 - t1:.1f}s, numba-interpreted {len(pick)} rays {t3 - t2:.1f}s",
+          flush=True)
+    state = npy["state"].astype(np.int8)
+    np.savez_compressed(
+        os.path.join(HERE, "config2_a09_i60_512.npz"),
+        spin=spin, inclination_deg=incl, r_o=r_o, fov=fov, resolution=n, rtol=rtol, atol=atol,
+        Y0_sha256=sha(Y0), Y0_pick=Y0[pick], pick=pick, band=band,
+        numpy_state=state, numpy_n_steps=npy["n_steps"].astype(np.int32),
+        numpy_n_rejected=npy["n_rejected"].astype(np.int32), numpy_Y_pick=npy["Y"][pick],
+        **{f"numba_{k}": v for k, v in nb.items()},
+    )
+
+
+def bisection(workers: int) -> None:
+    """Edge bisection semantics of scripts/study_edge_bisection.py:53-68 on Bardeen states.
+
+    a=0.9, i=60 deg, r_o=1000, rtol 1e-6 (atol 1e-8), 16 directions (linspace + 0.1 rad) about
+    the centroid of the reference's Bardeen curve, bracket [0.98, 1.02] of the curve's radii,
+    34 iterations, each iteration one reference NumPy batch; escaped -> hi, else -> lo.
+    """
+    if REF_SRC not in sys.path:
+        sys.path.insert(0, REF_SRC)
+    from kerrray.geometry import Spacetime
+    from kerrray.photons.orbits import shadow_curve
+    from kerrray.raytracing.boundary import polygon_centroid, polygon_radii
+
+    _, IntegratorOptions, TerminationOptions, integrate_batch, _, _, _ = _ref()
+    spin, incl, r_o, rtol, n_dirs, iters = 0.9, 60.0, 1000.0, 1e-6, 16, 34
+    st = Spacetime(1.0, spin)
+    ca, cb = shadow_curve(st, incl, 40000)
+    centre = polygon_centroid(ca, cb)
+    angles = np.linspace(0.0, 2 * math.pi, n_dirs, endpoint=False) + 0.1
+    r_ana = polygon_radii(ca, cb, centre, angles)
+    integ = IntegratorOptions(method="rk45", rtol=rtol, atol=rtol * 1e-2, max_steps=2_000_000, lambda_max=4.0 * r_o)
+    term = TerminationOptions(horizon_epsilon=1e-6, escape_radius=r_o)
+    lo, hi = 0.98 * r_ana, 1.02 * r_ana
+    t0 = time.time()
+    for _ in range(iters):
+        mid = 0.5 * (lo + hi)
+        a = centre[0] + mid * np.cos(angles)
+        b = centre[1] + mid * np.sin(angles)
+        Y0 = bardeen_states(spin, r_o, incl, a, b)
+        res = integrate_batch(st, Y0, integ, term)
+        escaped = res.state == 1
+        hi = np.where(escaped, mid, hi)
+        lo = np.where(escaped, lo, mid)
+    print(f"bisection: {time.time() - t0:.1f}s", flush=True)
+    np.savez_compressed(
+        os.path.join(HERE, "bisect_a09_i60_16dirs.npz"),
+        spin=spin, inclination_deg=incl, r_o=r_o, rtol=rtol, atol=rtol * 1e-2, max_steps=2_000_000,
+        lambda_max=4.0 * r_o, iterations=iters, angles=angles, centre=np.asarray(centre), r_analytic=r_ana,
+        lo=lo, hi=hi, curve_alpha=ca[::100], curve_beta=cb[::100],
+    )
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--only", choices=["config1", "config2", "bisection"], default=None)
+    args = ap.parse_args()
+    for name, fn in (("config1", config1), ("config2", config2), ("bisection", bisection)):
+        if args.only in (None, name):
+            fn(args.workers)
+
+
+if __name__ == "__main__":
+    main()

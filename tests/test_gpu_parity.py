@@ -1,0 +1,214 @@
+"""Parity of the hip backend with the reference, on an MI355X.
+
+Bar (the reference's own backend-agreement pattern, tests/test_numba_backend.py:92-177):
+termination states and the captured mask identical; step counts identical for
+(almost) every photon; where the step sequence is identical, final states
+within 10 x rtol (relative, floor 1). The reference numbers come from
+tests/golden (the Numba kernel source and the NumPy integrator run on the same
+Bardeen initial states) and from the CPU oracle.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+from conftest import options
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    a64, b64 = a.astype(np.float64), b.astype(np.float64)
+    return np.max(np.abs(a64 - b64) / np.maximum(1.0, np.abs(b64)), axis=-1)
+
+
+def report(tag: str, mine, ref_Y, ref_steps, ref_rej, rtol: float) -> tuple[np.ndarray, float]:
+    same_seq = (mine.n_steps == ref_steps) & (mine.n_rejected == ref_rej)
+    bitwise = np.all(mine.Y == ref_Y, axis=1)
+    r = rel(mine.Y, ref_Y)
+    print(f"[{tag}] photons {len(r)}  same step sequence {same_seq.mean():.5f}  bitwise {bitwise.mean():.5f}  "
+          f"max rel (same seq) {r[same_seq].max() if same_seq.any() else 0:.3e}", flush=True)
+    assert np.all(r[same_seq] <= 10.0 * rtol), (tag, r[same_seq].max())
+    return same_seq, float(bitwise.mean())
+
+
+def test_config1_schwarzschild_64(hip, golden1) -> None:
+    from kerrray_amd.camera import BardeenCamera
+    from kerrray_amd.geometry import Spacetime
+    from kerrray_amd.shadow import compute_shadow
+
+    integ, term = options(rtol=1e-8, atol=1e-10)
+    cam = BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=10.0, resolution=64)
+    img = compute_shadow(Spacetime(1.0, 0.0), cam, integ, term, backend=hip)
+    b2 = img.alpha**2 + img.beta**2
+    assert np.array_equal(img.captured, b2 < 27.0), "closed-form Schwarzschild mask"
+    assert np.array_equal(img.result.state, golden1["numba_state"])
+    assert img.n_failed == 0 and img.n_other == 0
+    same, _ = report("config1", img.result, golden1["numba_Y"], golden1["numba_n_steps"],
+                     golden1["numba_n_rejected"], 1e-8)
+    assert same.mean() >= 0.99
+    assert np.all(img.result.max_energy_drift == 0.0) and np.all(img.result.max_lz_drift == 0.0)
+
+
+def test_config2_kerr_512_mask_is_bit_identical(hip, golden2) -> None:
+    from kerrray_amd.camera import BardeenCamera
+    from kerrray_amd.geometry import Spacetime
+    from kerrray_amd.shadow import compute_shadow
+
+    integ, term = options(rtol=1e-6, atol=1e-8)
+    cam = BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=10.0, resolution=512)
+    img = compute_shadow(Spacetime(1.0, 0.9), cam, integ, term, backend=hip)
+    ref_state = golden2["numpy_state"].astype(np.int64)
+    diff = np.flatnonzero(img.result.state != ref_state)
+    print(f"[config2] kernel {hip.last_kernel_ms:.3f} ms, {len(diff)} pixels differ from the reference mask", flush=True)
+    assert np.array_equal(img.result.state, ref_state)
+    assert np.array_equal(img.captured[::-1, :], img.captured)
+    steps_same = np.mean(img.result.n_steps == golden2["numpy_n_steps"])
+    print(f"[config2] n_steps identical to the reference on {steps_same:.5f} of pixels", flush=True)
+    assert steps_same >= 0.99
+    pick = golden2["pick"]
+    sub = type("R", (), {})()
+    sub.Y, sub.n_steps, sub.n_rejected = img.result.Y[pick], img.result.n_steps[pick], img.result.n_rejected[pick]
+    same, _ = report("config2 band+sample", sub, golden2["numba_Y"], golden2["numba_n_steps"],
+                     golden2["numba_n_rejected"], 1e-6)
+    assert same.mean() >= 0.99
+    lam_rel = np.abs(img.result.lam[pick] - golden2["numba_lam"]) / np.maximum(1.0, golden2["numba_lam"])
+    assert np.all(lam_rel[same] <= 1e-5)
+
+
+def test_y0_path_equals_image_path(hip) -> None:
+    from kerrray_amd.camera import BardeenCamera, bardeen_initial_states
+    from kerrray_amd.geometry import Spacetime
+    from kerrray_amd.shadow import compute_shadow
+
+    st = Spacetime(1.0, 0.9)
+    integ, term = options(rtol=1e-7, atol=1e-9)
+    cam = BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=8.0, resolution=24)
+    img = compute_shadow(st, cam, integ, term, backend=hip)
+    batch = hip.integrate_batch(st, bardeen_initial_states(cam, st), integ, term)
+    for name in ("Y", "state", "n_steps", "n_rejected", "lam", "max_null_error", "max_carter_drift"):
+        assert np.array_equal(getattr(img.result, name), getattr(batch, name)), name
+
+
+def test_batch_matches_oracle_kerr(hip, oracle) -> None:
+    from kerrray_amd.camera import BardeenCamera, bardeen_initial_states
+    from kerrray_amd.geometry import Spacetime
+
+    st = Spacetime(1.0, 0.9)
+    integ, term = options(rtol=1e-8, atol=1e-10)
+    Y0 = bardeen_initial_states(BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=8.0, resolution=32), st)
+    mine = hip.integrate_batch(st, Y0, integ, term)
+    ref = oracle.integrate_batch(st, Y0, integ, term)
+    assert np.array_equal(mine.state, ref.state)
+    same, bitwise = report("oracle a=0.9 rtol 1e-8", mine, ref.Y, ref.n_steps, ref.n_rejected, 1e-8)
+    assert same.mean() >= 0.99
+    for name in ("max_null_error", "max_carter_drift"):
+        a, b = getattr(mine, name)[same], getattr(ref, name)[same]
+        assert np.all(np.abs(a - b) <= 1e-6 * np.maximum(np.abs(b), 1e-12)), name
+
+
+def test_float32_variant_matches_oracle(hip, oracle) -> None:
+    from kerrray_amd.camera import BardeenCamera, bardeen_initial_states
+    from kerrray_amd.geometry import Spacetime
+
+    st = Spacetime(1.0, 0.9)
+    integ, term = options(rtol=1e-6, atol=1e-8, dtype="float32", horizon_epsilon=1e-3)
+    Y0 = bardeen_initial_states(BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=12.0, resolution=16), st)
+    mine = hip.integrate_batch(st, Y0, integ, term)
+    ref = oracle.integrate_batch(st, Y0, integ, term)
+    assert mine.Y.dtype == np.float32 and mine.lam.dtype == np.float64
+    assert np.array_equal(mine.state, ref.state)
+    assert np.max(rel(mine.Y, ref.Y)) < 1e-3
+    assert np.mean(mine.n_steps == ref.n_steps) >= 0.9
+    assert np.all(mine.max_energy_drift == ref.max_energy_drift)
+
+
+def test_rk4_and_budgets_match_oracle(hip, oracle) -> None:
+    from kerrray_amd.camera import BardeenCamera, bardeen_initial_states
+    from kerrray_amd.geometry import Spacetime
+
+    st = Spacetime(1.0, 0.9)
+    Y0 = bardeen_initial_states(BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=12.0, resolution=4), st)
+    integ4, far = options(method="rk4", step_size=0.05, lambda_max=20.0, escape_radius=1e6)
+    a, b = hip.integrate_batch(st, Y0, integ4, far), oracle.integrate_batch(st, Y0, integ4, far)
+    assert np.all(a.state == 3) and np.all(a.n_steps == 400)
+    assert np.max(rel(a.Y, b.Y)) < 1e-12 and np.array_equal(a.lam, b.lam)
+    integ, term = options(rtol=1e-8, atol=1e-10, max_steps=30)
+    a, b = hip.integrate_batch(st, Y0, integ, term), oracle.integrate_batch(st, Y0, integ, term)
+    assert np.all(a.state == 3) and np.all(a.n_steps == 30) and np.max(rel(a.Y, b.Y)) < 1e-12
+    integ, term = options(rtol=1e-8, atol=1e-10, lambda_max=500.0)
+    a, b = hip.integrate_batch(st, Y0, integ, term), oracle.integrate_batch(st, Y0, integ, term)
+    assert np.all(a.lam == 500.0) and np.array_equal(a.state, b.state) and np.max(rel(a.Y, b.Y)) < 1e-9
+    integ, term = options(rtol=1e-8, atol=1e-10, h_max=0.5)
+    a, b = hip.integrate_batch(st, Y0, integ, term), oracle.integrate_batch(st, Y0, integ, term)
+    assert np.array_equal(a.state, b.state) and np.array_equal(a.n_steps, b.n_steps) and np.max(rel(a.Y, b.Y)) < 1e-9
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_disk_event_matches_oracle(hip, oracle, dtype: str) -> None:
+    """States crossing theta = pi/2 inside [r_in, r_out] stop as DISK_HIT (numba_kernel.py:439-454)."""
+    from kerrray_amd.geodesics import EventOptions
+    from kerrray_amd.geometry import Spacetime
+
+    import math
+
+    st = Spacetime(1.0, 0.9)
+    # The rays of the reference's test (r = 200, theta = 1.2, E = 1, Q = 15, L_z in {-6, 3, 6, 9},
+    # inward, p_theta > 0), written as image points of an observer at theta_o = 1.2.
+    th = 1.2
+    lz = np.array([-6.0, 3.0, 6.0, 9.0])
+    alpha = -lz / math.sin(th)
+    beta = -np.sqrt(15.0 + 0.81 * math.cos(th) ** 2 - lz**2 / math.tan(th) ** 2)
+    Y0 = oracle.bardeen_points(1.0, 0.9, 200.0, math.degrees(th), alpha, beta)
+    events = EventOptions(disk_plane=True, r_in=6.0, r_out=30.0)
+    integ, far = options(rtol=1e-7, atol=1e-9, lambda_max=400.0, dtype=dtype, escape_radius=1e6)
+    a = hip.integrate_batch(st, Y0, integ, far, events=events)
+    b = oracle.integrate_batch(st, Y0, integ, far, events=events)
+    assert np.array_equal(a.state, b.state) and np.array_equal(a.event_hit, b.event_hit)
+    assert a.event_hit.any()
+    hit = a.event_hit
+    assert np.max(rel(a.event_Y[hit], b.event_Y[hit])) < 1e-5
+    assert np.max(np.abs(a.lam - b.lam)) < 1e-5
+
+
+def test_schwarzschild_limit_matches_oracle(hip, oracle) -> None:
+    from kerrray_amd.camera import BardeenCamera, bardeen_initial_states
+    from kerrray_amd.geometry import Spacetime
+
+    st = Spacetime(1.0, 0.0)
+    integ, term = options(rtol=1e-9, atol=1e-11)
+    Y0 = bardeen_initial_states(BardeenCamera(radius=1000.0, inclination_deg=89.0, fov=8.0, resolution=6), st)
+    a, b = hip.integrate_batch(st, Y0, integ, term), oracle.integrate_batch(st, Y0, integ, term)
+    assert np.array_equal(a.state, b.state) and np.max(rel(a.Y, b.Y)) < 1e-8
+
+
+def test_input_validation(hip) -> None:
+    from kerrray_amd.geometry import Spacetime
+
+    st = Spacetime(1.0, 0.9)
+    integ, term = options()
+    with pytest.raises(ValueError):
+        hip.integrate_batch(st, np.ones((2, 7)), integ, term)
+    with pytest.raises(ValueError):
+        hip.integrate_batch(st, np.full((1, 8), np.nan), integ, term)
+    with pytest.raises(ValueError):
+        hip.integrate_batch(st, np.ones((1, 8)), options(method="dop853")[0], term)
+    assert hip.warm_up(st, integ, term) > 0.0
+    empty = hip.integrate_batch(st, np.zeros((0, 8)), integ, term)
+    assert empty.Y.shape == (0, 8) and empty.state.shape == (0,)
+
+
+def test_chunked_equals_unchunked(hip) -> None:
+    from kerrray_amd.camera import BardeenCamera, bardeen_initial_states
+    from kerrray_amd.geometry import Spacetime
+    from kerrray_amd.rays import trace_rays
+
+    st = Spacetime(1.0, 0.9)
+    integ, term = options(rtol=1e-7, atol=1e-9)
+    Y0 = bardeen_initial_states(BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=8.0, resolution=8), st)
+    fractions: list[float] = []
+    one = trace_rays(st, Y0, integ, term, backend=hip)
+    chunked = trace_rays(st, Y0, integ, term, backend=hip, chunk_size=20, progress=fractions.append)
+    assert fractions == [20 / 64, 40 / 64, 60 / 64, 1.0]
+    for name in ("Y", "state", "n_steps", "n_rejected", "lam", "max_null_error", "max_carter_drift"):
+        assert np.array_equal(getattr(one, name), getattr(chunked, name)), name

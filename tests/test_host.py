@@ -1,0 +1,161 @@
+"""Host-side logic of the hip backend (no GPU): option validation, constant packing,
+initial conditions, registry, chunking and the C ABI library's exported symbols."""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+from conftest import ROOT, ns
+
+from kerrray_amd import _lib
+from kerrray_amd.backends import (
+    BACKEND_NAMES, Backend, BackendUnavailable, get_backend, kernel_params,
+)
+from kerrray_amd.camera import BardeenCamera, bardeen_initial_states, bardeen_points
+from kerrray_amd.geodesics import EventOptions, IntegratorOptions, TerminationOptions, max_attempts
+from kerrray_amd.geometry import Spacetime, outer_horizon
+from kerrray_amd.photons import BatchResult
+from kerrray_amd.rays import concatenate_results, trace_rays
+
+
+def test_option_validation() -> None:
+    with pytest.raises(ValueError):
+        IntegratorOptions(method="euler")
+    with pytest.raises(ValueError):
+        IntegratorOptions(rtol=0.0)
+    with pytest.raises(ValueError):
+        IntegratorOptions(h_max=-1.0)
+    with pytest.raises(ValueError):
+        IntegratorOptions(max_steps=0)
+    with pytest.raises(ValueError):
+        IntegratorOptions(dtype="float16")
+    with pytest.raises(ValueError):
+        TerminationOptions(horizon_epsilon=-1.0)
+    with pytest.raises(ValueError):
+        EventOptions(r_in=5.0, r_out=4.0)
+    with pytest.raises(ValueError):
+        Spacetime(1.0, 1.0)
+    assert max_attempts(IntegratorOptions()) > 1_600_000
+
+
+def test_kernel_params_match_the_reference_packing(oracle) -> None:
+    """kernel_params == numba_backend._kernel_constants + _tableaus (restated in the oracle front end)."""
+    for spin, integ, term, ev in [
+        (0.9, IntegratorOptions(rtol=1e-6, atol=1e-8), TerminationOptions(), None),
+        (-0.7, IntegratorOptions(method="rk4", h_max=0.3, dtype="float32"), TerminationOptions(1e-3, 500.0),
+         EventOptions(True, 6.0, 30.0)),
+    ]:
+        st = Spacetime(1.0, spin)
+        p = kernel_params(st, integ, term, ev)
+        geom, consts, params, ipar = oracle.kernel_constants(st, integ, term, ev)
+        tab_a, tab_e, rk4_b, rk4_c = oracle.tableaus()
+        assert np.array_equal(np.frombuffer(p.geom, np.float64), geom)
+        assert np.array_equal(np.frombuffer(p.consts, np.float64), consts)
+        assert np.array_equal(np.frombuffer(p.params, np.float64), params)
+        assert np.array_equal(np.frombuffer(p.ipar, np.int64), ipar)
+        assert np.array_equal(np.frombuffer(p.tab_a, np.float64), tab_a.ravel())
+        assert np.array_equal(np.frombuffer(p.tab_e, np.float64), tab_e)
+        assert np.array_equal(np.frombuffer(p.rk4_b, np.float64), rk4_b)
+        assert np.array_equal(np.frombuffer(p.rk4_c, np.float64), rk4_c)
+        assert p.dtype == (0 if integ.dtype == "float64" else 1)
+
+
+def test_bardeen_states_match_the_reference_golden(oracle, golden1, golden2) -> None:
+    cam = BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=10.0, resolution=64)
+    assert np.array_equal(bardeen_initial_states(cam, Spacetime(1.0, 0.0)), golden1["Y0"])
+    cam2 = BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=10.0, resolution=512)
+    Y2 = bardeen_initial_states(cam2, Spacetime(1.0, 0.9))
+    import hashlib
+
+    assert hashlib.sha256(Y2.tobytes()).hexdigest() == str(golden2["Y0_sha256"])
+    assert np.array_equal(Y2, oracle.bardeen_image(1.0, 0.9, 1000.0, 60.0, 10.0, 512))
+
+
+def test_pixel_grid_is_the_reference_grid() -> None:
+    cam = BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=10.0, resolution=8)
+    alpha, beta = cam.pixel_coordinates()
+    assert alpha.shape == (8, 8) and np.all(alpha[:, ::-1] == -alpha) and np.all(beta[::-1, :] == -beta)
+    assert np.all(alpha != 0) and beta[0, 0] > 0 and cam.pixel_size == 2.5
+    with pytest.raises(ValueError):
+        BardeenCamera(radius=1000.0, inclination_deg=60.0, resolution=7)
+
+
+def test_bardeen_points_reject_inconsistent_constants() -> None:
+    st = Spacetime(1.0, 0.9)
+    with pytest.raises(ValueError):
+        bardeen_points(st, 1.2, 60.0, [0.5], [0.5])  # inside the horizon (r_+ = 1.436)
+    # An image point farther than r_o on the sky has R(r_o) < 0.
+    with pytest.raises(ValueError):
+        bardeen_points(st, 10.0, 60.0, [40.0], [40.0])
+
+
+def test_registry() -> None:
+    assert BACKEND_NAMES == ("numpy", "numba", "cuda", "hip")
+    with pytest.raises(ValueError, match="unknown backend"):
+        get_backend("opencl")
+    for name in ("numpy", "numba", "cuda"):
+        with pytest.raises(BackendUnavailable):
+            get_backend(name)
+    assert issubclass(BackendUnavailable, RuntimeError)
+
+
+class _FakeBackend:
+    """Host-logic stand-in: returns the input states (exercises chunking only)."""
+
+    name = "fake"
+
+    def integrate_batch(self, st, Y0, integ, term, events=None):
+        y = np.asarray(Y0, float)
+        n = len(y)
+        z = np.zeros(n)
+        return BatchResult(Y=y.copy(), state=np.arange(n, dtype=np.int64), n_steps=z.astype(np.int64), lam=z,
+                           max_null_error=z, max_energy_drift=z, max_lz_drift=z, max_carter_drift=z,
+                           runtime_s=1e-3, n_rejected=z.astype(np.int64))
+
+
+def test_trace_rays_chunking_and_progress() -> None:
+    assert isinstance(_FakeBackend(), Backend)
+    Y0 = np.random.default_rng(0).normal(size=(64, 8))
+    fractions: list[float] = []
+    res = trace_rays(Spacetime(), Y0, IntegratorOptions(), TerminationOptions(), backend=_FakeBackend(),
+                     chunk_size=20, progress=fractions.append)
+    assert fractions == [20 / 64, 40 / 64, 60 / 64, 1.0]
+    assert np.array_equal(res.Y, Y0) and res.n_rays == 64
+    with pytest.raises(ValueError):
+        trace_rays(Spacetime(), np.zeros((0, 8)), IntegratorOptions(), TerminationOptions(), backend=_FakeBackend())
+    with pytest.raises(ValueError):
+        concatenate_results([])
+
+
+def _header_symbols() -> list[str]:
+    text = open(os.path.join(ROOT, "include", "kerrray_hip.h")).read()
+    return sorted(set(re.findall(r"^(?:kr_status|int|const char\*)\s+(kr_\w+)\(", text, flags=re.M)))
+
+
+def test_library_loads_and_exports_every_declared_symbol() -> None:
+    L = _lib.lib()
+    declared = _header_symbols()
+    assert set(declared) == set(_lib.EXPORTED)
+    for sym in declared:
+        assert hasattr(L, sym), sym
+    assert L.kr_abi_version() == _lib.ABI_VERSION
+
+
+def test_null_handle_and_bad_arguments_are_errors_not_crashes() -> None:
+    L = _lib.lib()
+    p = kernel_params(Spacetime(1.0, 0.9), IntegratorOptions(), TerminationOptions())
+    out = _lib.KrOutputs()
+    st = L.kr_integrate_batch(None, ctypes.byref(p), None, 0, ctypes.byref(out), None)
+    assert st == _lib.KR_ERR_INVALID and "handle" in _lib.last_error()
+    assert L.kr_create(0, None) == _lib.KR_ERR_INVALID
+    h = ctypes.c_void_p()
+    st = L.kr_create(0, ctypes.byref(h))
+    if st != _lib.KR_OK:  # no GPU here: the backend is unavailable, not broken
+        assert st == _lib.KR_ERR_DEVICE and h.value is None
+    else:
+        L.kr_destroy(h)
