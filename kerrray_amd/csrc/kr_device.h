@@ -24,8 +24,26 @@
 // which evaluate exactly the same functions of exactly the same argument.
 #pragma once
 
-#include <hip/hip_runtime.h>
 #include <stdint.h>
+
+// KR_HOST_BUILD compiles these functions as plain host C++ (the test suite
+// builds them with g++ against glibc to check the kernel logic bit for bit
+// against the oracle on the CPU); otherwise they are HIP device functions.
+#ifdef KR_HOST_BUILD
+#include <math.h>
+
+#include <cmath>
+#define KR_HD inline
+using std::cos;
+using std::fabs;
+using std::isfinite;
+using std::pow;
+using std::sin;
+using std::sqrt;
+#else
+#include <hip/hip_runtime.h>
+#define KR_HD __host__ __device__ __forceinline__
+#endif
 
 namespace kr {
 
@@ -75,7 +93,7 @@ struct Params {
 // The six integrated components of y = [t, r, th, ph, p_t, p_r, p_th, p_ph]
 // (state.py:35) are stored as v[0..5] = t, r, th, ph, p_r, p_th.
 constexpr int NA = 6;
-__host__ __device__ constexpr int full_index(int i) { return i < 4 ? i : i + 1; }
+constexpr int full_index(int i) { return i < 4 ? i : i + 1; }
 
 template <typename T>
 struct Photon {
@@ -84,21 +102,21 @@ struct Photon {
     T p_ph;    // constant of motion (L_z)
 };
 
-template <typename T> __device__ __forceinline__ T pymax(T a, T b) { return b > a ? b : a; }
-template <typename T> __device__ __forceinline__ T pymin(T a, T b) { return b < a ? b : a; }
+template <typename T> KR_HD T pymax(T a, T b) { return b > a ? b : a; }
+template <typename T> KR_HD T pymin(T a, T b) { return b < a ? b : a; }
 
-__device__ __forceinline__ double wsqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float wsqrt(float x) { return sqrtf(x); }
-__device__ __forceinline__ double wpow(double x, double y) { return pow(x, y); }
-__device__ __forceinline__ float wpow(float x, float y) { return powf(x, y); }
-__device__ __forceinline__ double wabs(double x) { return fabs(x); }
-__device__ __forceinline__ float wabs(float x) { return fabsf(x); }
-__device__ __forceinline__ bool wfinite(double x) { return isfinite(x); }
-__device__ __forceinline__ bool wfinite(float x) { return isfinite(x); }
+KR_HD double wsqrt(double x) { return sqrt(x); }
+KR_HD float wsqrt(float x) { return sqrtf(x); }
+KR_HD double wpow(double x, double y) { return pow(x, y); }
+KR_HD float wpow(float x, float y) { return powf(x, y); }
+KR_HD double wabs(double x) { return fabs(x); }
+KR_HD float wabs(float x) { return fabsf(x); }
+KR_HD bool wfinite(double x) { return isfinite(x); }
+KR_HD bool wfinite(float x) { return isfinite(x); }
 
 // numba_kernel.py:111-154 (= geometry/metric.py:166-195,265-313). Also
 // returns sin(th), cos(th) for reuse by the classifier and diagnostics.
-__device__ __forceinline__ void inverse_metric_terms(const Params& P, double r, double th, double (&out)[15],
+KR_HD void inverse_metric_terms(const Params& P, double r, double th, double (&out)[15],
                                                      double& s_out, double& c_out) {
     const double m = P.geom[G_MASS];
     const double a2 = P.geom[G_A2];
@@ -143,7 +161,7 @@ __device__ __forceinline__ void inverse_metric_terms(const Params& P, double r, 
 // numba_kernel.py:158-197: f = dy/dlambda for the six integrated components
 // (f[p_t] = f[p_phi] = 0 identically).
 template <typename T>
-__device__ __forceinline__ void rhs(const Params& P, const T (&v)[NA], T p_t, T p_ph, T (&f)[NA], double& s,
+KR_HD void rhs(const Params& P, const T (&v)[NA], T p_t, T p_ph, T (&f)[NA], double& s,
                                     double& c) {
     double g64[15];
     inverse_metric_terms(P, static_cast<double>(v[1]), static_cast<double>(v[2]), g64, s, c);
@@ -169,7 +187,7 @@ __device__ __forceinline__ void rhs(const Params& P, const T (&v)[NA], T p_t, T 
 }
 
 template <typename T>
-__device__ __forceinline__ bool all_finite(const T (&a)[NA], const T (&b)[NA]) {
+KR_HD bool all_finite(const T (&a)[NA], const T (&b)[NA]) {
     bool ok = true;
 #pragma unroll
     for (int i = 0; i < NA; ++i) ok = ok && wfinite(a[i]) && wfinite(b[i]);
@@ -179,7 +197,7 @@ __device__ __forceinline__ bool all_finite(const T (&a)[NA], const T (&b)[NA]) {
 // numba_kernel.py:201-238.  k0 = f(y) on entry (FSAL); on exit y_new, err and
 // k6 = f(y_new), plus sin/cos of theta at y_new.
 template <typename T>
-__device__ __forceinline__ void dp5_step(const Params& P, const T (&y)[NA], T p_t, T p_ph, T h, const T (&k0)[NA],
+KR_HD void dp5_step(const Params& P, const T (&y)[NA], T p_t, T p_ph, T h, const T (&k0)[NA],
                                          T (&y_new)[NA], T (&err)[NA], T (&k6)[NA], double& s_new, double& c_new) {
     using namespace dp;
     T k1[NA], k2[NA], k3[NA], k4[NA], k5[NA], ys[NA];
@@ -215,7 +233,7 @@ __device__ __forceinline__ void dp5_step(const Params& P, const T (&y)[NA], T p_
 
 // numba_kernel.py:242-257.
 template <typename T>
-__device__ __forceinline__ void rk4_step(const Params& P, const T (&y)[NA], T p_t, T p_ph, T h, const T (&k0)[NA],
+KR_HD void rk4_step(const Params& P, const T (&y)[NA], T p_t, T p_ph, T h, const T (&k0)[NA],
                                          T (&y_new)[NA], T (&k6)[NA], double& s_new, double& c_new) {
     using namespace rk4;
     T k1[NA], k2[NA], k3[NA], ys[NA];
@@ -237,7 +255,7 @@ __device__ __forceinline__ void rk4_step(const Params& P, const T (&y)[NA], T p_
 
 // numba_kernel.py:261-268 (scale uses max(|y|, |y_new|); p_t/p_phi terms are +0).
 template <typename T>
-__device__ __forceinline__ T error_norm(T rtol, T atol, const T (&err)[NA], const T (&y)[NA], const T (&y_new)[NA]) {
+KR_HD T error_norm(T rtol, T atol, const T (&err)[NA], const T (&y)[NA], const T (&y_new)[NA]) {
     T q[NA];
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
@@ -252,7 +270,7 @@ __device__ __forceinline__ T error_norm(T rtol, T atol, const T (&err)[NA], cons
 
 // numba_kernel.py:272-281.
 template <typename T>
-__device__ __forceinline__ T step_factor(T safety, T fmin, T fmax, T expo, T en, bool rejected_before) {
+KR_HD T step_factor(T safety, T fmin, T fmax, T expo, T en, bool rejected_before) {
     T factor;
     if (en == T(0)) {
         factor = fmax;
@@ -266,7 +284,7 @@ __device__ __forceinline__ T step_factor(T safety, T fmin, T fmax, T expo, T en,
 
 // numba_kernel.py:293-311.  sin_th is sin(y[th]) in the working type.
 template <typename T>
-__device__ __forceinline__ int classify(const Params& P, const T (&v)[NA], T p_ph, const T (&f)[NA], double lam,
+KR_HD int classify(const Params& P, const T (&v)[NA], T p_ph, const T (&f)[NA], double lam,
                                         int64_t n_steps, T h, bool check_h, T sin_th) {
     if (!all_finite(v, f)) return FAILURE;
     if (check_h && h < static_cast<T>(P.consts[C_HMIN])) return FAILURE;
@@ -281,13 +299,13 @@ __device__ __forceinline__ int classify(const Params& P, const T (&v)[NA], T p_p
     return RUNNING;
 }
 
-template <typename T> __device__ __forceinline__ T working_sin(double s_double, T th);
-template <> __device__ __forceinline__ double working_sin<double>(double s_double, double) { return s_double; }
-template <> __device__ __forceinline__ float working_sin<float>(double, float th) { return sinf(th); }
+template <typename T> KR_HD T working_sin(double s_double, T th);
+template <> KR_HD double working_sin<double>(double s_double, double) { return s_double; }
+template <> KR_HD float working_sin<float>(double, float th) { return sinf(th); }
 
 // numba_kernel.py:315-321.
 template <typename T>
-__device__ __forceinline__ double null_error(const T (&v)[NA], T p_t, T p_ph, const T (&f)[NA]) {
+KR_HD double null_error(const T (&v)[NA], T p_t, T p_ph, const T (&f)[NA]) {
     double total = static_cast<double>(p_t) * static_cast<double>(f[0]);
     total = total + static_cast<double>(v[4]) * static_cast<double>(f[1]);
     total = total + static_cast<double>(v[5]) * static_cast<double>(f[2]);
@@ -297,7 +315,7 @@ __device__ __forceinline__ double null_error(const T (&v)[NA], T p_t, T p_ph, co
 
 // numba_kernel.py:325-336 with s = sin(th), c = cos(th) of double(th).
 template <typename T>
-__device__ __forceinline__ double carter(double a2, const T (&v)[NA], T p_t, T p_ph, double s, double c) {
+KR_HD double carter(double a2, const T (&v)[NA], T p_t, T p_ph, double s, double c) {
     const double p_th = static_cast<double>(v[5]);
     const double lz = static_cast<double>(p_ph);
     const double e = -static_cast<double>(p_t);
@@ -323,7 +341,7 @@ struct RayResult {
 // reference drift baselines e0, lz0, q0 and floors are taken from the
 // float64 Y0, numba_backend.py:174-178.)
 template <typename T, bool ADAPTIVE, bool DISK>
-__device__ void integrate_ray(const Params& P, const double (&y64)[8], RayResult<T>& out) {
+KR_HD void integrate_ray(const Params& P, const double (&y64)[8], RayResult<T>& out) {
     const double a2 = P.geom[G_A2];
     const double lambda_max = P.params[P_LAMMAX];
     const double half_pi = P.params[P_HALFPI];
@@ -526,7 +544,7 @@ struct BardeenImage {
 // Python's x**2 on a float is glibc pow(x, 2.0), which returns x*x exactly.
 // Returns false when a radicand is negative beyond round-off
 // (initial_conditions.py:71-74).
-__device__ __forceinline__ bool bardeen_state(const BardeenImage& B, double alpha, double beta, double (&y)[8]) {
+KR_HD bool bardeen_state(const BardeenImage& B, double alpha, double beta, double (&y)[8]) {
     const double xi = -alpha * B.sin_o;
     const double eta = beta * beta - B.a2c2 + xi * xi * B.cot2;
     const double term1 = B.r2a2 - B.a * xi;
@@ -551,7 +569,7 @@ __device__ __forceinline__ bool bardeen_state(const BardeenImage& B, double alph
 }
 
 // Camera.pixel_coordinates (camera.py:206-222): fov (2 j + 1 - n) / n.
-__device__ __forceinline__ double pixel_centre(double fov, int64_t j, int64_t n) {
+KR_HD double pixel_centre(double fov, int64_t j, int64_t n) {
     const double nd = static_cast<double>(n);
     return (fov * ((2.0 * static_cast<double>(j) + 1.0) - nd)) / nd;
 }

@@ -1,0 +1,123 @@
+"""The device kernel source, compiled for the CPU, is bit-identical to the oracle.
+
+tests/hostref builds kerrray_amd/csrc/kr_device.h with g++ against glibc: the
+exact per-photon code the GPU runs (six-component state, reused sin/cos, the
+same operation order), with the host's libm in place of the device's. Every
+BatchResult field must then equal the oracle's (and the reference's golden
+vectors) bit for bit, for every option the kernel supports. On the GPU the
+only remaining difference is the device math library (sin, cos, pow round
+differently from glibc in a few percent of calls), which the GPU tests bound.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+from conftest import ROOT, options
+
+from kerrray_amd.backends import HostBatch, kernel_params
+from kerrray_amd.camera import BardeenCamera, bardeen_initial_states, bardeen_points
+from kerrray_amd.geodesics import EventOptions
+from kerrray_amd.geometry import Spacetime
+from kerrray_amd.shadow import kr_image
+
+FIELDS = ("Y", "state", "n_steps", "n_rejected", "lam", "max_null_error", "max_energy_drift", "max_lz_drift",
+          "max_carter_drift")
+HOSTREF = os.path.join(ROOT, "tests", "hostref")
+
+
+@pytest.fixture(scope="module")
+def hostref():
+    subprocess.run(["make", "-s", "-C", HOSTREF], check=True)
+    L = ctypes.CDLL(os.path.join(HOSTREF, "libkr_hostref.so"))
+    L.krh_integrate.restype = ctypes.c_int
+    L.krh_integrate.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int]
+    L.krh_trace_image.restype = ctypes.c_int
+    L.krh_trace_image.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    return L
+
+
+def run_y0(L, st, Y0, integ, term, events=None):
+    Y0 = np.ascontiguousarray(Y0, dtype=np.float64)
+    p = kernel_params(st, integ, term, events)
+    out = HostBatch(len(Y0), integ.np_dtype, events is not None and events.disk_plane)
+    assert L.krh_integrate(ctypes.addressof(p), Y0.ctypes.data, len(Y0), ctypes.addressof(out.outputs), 0) == 0
+    return out.result(0.0)
+
+
+def run_image(L, st, cam, integ, term):
+    p = kernel_params(st, integ, term)
+    img = kr_image(st, cam.radius, cam.inclination_deg, cam.fov, cam.resolution)
+    out = HostBatch(cam.n_rays, integ.np_dtype, False)
+    assert L.krh_trace_image(ctypes.addressof(p), ctypes.addressof(img), ctypes.addressof(out.outputs), 0) == 0
+    return out.result(0.0)
+
+
+def same(a, b, fields=FIELDS) -> None:
+    for name in fields:
+        x, y = getattr(a, name), getattr(b, name)
+        if x is None and y is None:
+            continue
+        assert np.array_equal(np.asarray(x), np.asarray(y)), name
+
+
+def test_config1_image_is_bitwise_the_reference(hostref, golden1) -> None:
+    integ, term = options(rtol=1e-8, atol=1e-10)
+    cam = BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=10.0, resolution=64)
+    r = run_image(hostref, Spacetime(1.0, 0.0), cam, integ, term)
+    for name in ("Y", "state", "n_steps", "n_rejected", "lam", "max_null_error", "max_carter_drift"):
+        assert np.array_equal(getattr(r, name), golden1[f"numba_{name}"]), name
+
+
+def test_config2_image_is_bitwise_the_reference(hostref, golden2) -> None:
+    integ, term = options(rtol=1e-6, atol=1e-8)
+    cam = BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=10.0, resolution=512)
+    r = run_image(hostref, Spacetime(1.0, 0.9), cam, integ, term)
+    assert np.array_equal(r.state, golden2["numpy_state"].astype(np.int64))
+    pick = golden2["pick"]
+    for name in ("Y", "state", "n_steps", "n_rejected", "lam", "max_null_error", "max_carter_drift"):
+        assert np.array_equal(getattr(r, name)[pick], golden2[f"numba_{name}"]), name
+
+
+@pytest.mark.parametrize("kw", [
+    dict(rtol=1e-8, atol=1e-10),
+    dict(rtol=1e-6, atol=1e-8, dtype="float32", horizon_epsilon=1e-3),
+    dict(method="rk4", step_size=0.05, lambda_max=20.0, escape_radius=1e6),
+    dict(rtol=1e-8, atol=1e-10, max_steps=30),
+    dict(rtol=1e-8, atol=1e-10, lambda_max=500.0),
+    dict(rtol=1e-8, atol=1e-10, h_max=0.5),
+    dict(method="rk4", step_size=0.05, lambda_max=3000.0, dtype="float32", horizon_epsilon=1e-3),
+])
+def test_options_are_bitwise_the_oracle(hostref, oracle, kw) -> None:
+    st = Spacetime(1.0, 0.9)
+    integ, term = options(**kw)
+    Y0 = bardeen_initial_states(BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=12.0, resolution=16), st)
+    same(run_y0(hostref, st, Y0, integ, term), oracle.integrate_batch(st, Y0, integ, term))
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_disk_event_is_bitwise_the_oracle(hostref, oracle, dtype) -> None:
+    st = Spacetime(1.0, 0.9)
+    th = 1.2
+    lz = np.array([-6.0, 3.0, 6.0, 9.0])
+    Y0 = bardeen_points(st, 200.0, math.degrees(th), -lz / math.sin(th),
+                        -np.sqrt(15.0 + 0.81 * math.cos(th) ** 2 - lz**2 / math.tan(th) ** 2))
+    events = EventOptions(disk_plane=True, r_in=6.0, r_out=30.0)
+    integ, far = options(rtol=1e-7, atol=1e-9, lambda_max=400.0, dtype=dtype, escape_radius=1e6)
+    a = run_y0(hostref, st, Y0, integ, far, events)
+    b = oracle.integrate_batch(st, Y0, integ, far, events=events)
+    same(a, b, FIELDS + ("event_Y", "event_hit"))
+    assert a.event_hit.all()
+
+
+def test_schwarzschild_and_negative_spin_are_bitwise_the_oracle(hostref, oracle) -> None:
+    for spin, incl in ((0.0, 89.0), (-0.7, 30.0), (0.99, 85.0)):
+        st = Spacetime(1.0, spin)
+        integ, term = options(rtol=1e-9, atol=1e-11)
+        Y0 = bardeen_initial_states(BardeenCamera(radius=1000.0, inclination_deg=incl, fov=8.0, resolution=8), st)
+        same(run_y0(hostref, st, Y0, integ, term), oracle.integrate_batch(st, Y0, integ, term))

@@ -1,0 +1,79 @@
+"""Analytic references used to bracket and score the edge: closed forms and invariants."""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from kerrray_amd.bisection import rounds_for_width
+from kerrray_amd.boundary import extract_boundary_from_mask, polygon_centroid, polygon_radii
+from kerrray_amd.camera import BardeenCamera
+from kerrray_amd.geometry import Spacetime
+from kerrray_amd.orbits import (
+    critical_impact_parameters, equatorial_photon_orbit_radius, equatorial_photon_orbit_radius_closed_form,
+    shadow_curve, spherical_orbit_constants,
+)
+
+
+def test_schwarzschild_critical_impact_parameter_is_3_sqrt3() -> None:
+    b_pro, b_ret = critical_impact_parameters(Spacetime(1.0, 0.0))
+    assert b_pro == pytest.approx(3.0 * math.sqrt(3.0), rel=1e-12) and b_ret == pytest.approx(b_pro, rel=1e-12)
+
+
+@pytest.mark.parametrize("spin", [0.3, 0.9, 0.99])
+def test_equatorial_orbits_match_bpt_closed_form(spin: float) -> None:
+    st = Spacetime(1.0, spin)
+    for prograde in (True, False):
+        assert equatorial_photon_orbit_radius(st, prograde) == pytest.approx(
+            equatorial_photon_orbit_radius_closed_form(st, prograde), rel=1e-12)
+
+
+@pytest.mark.parametrize("spin", [0.5, 0.9])
+def test_spherical_orbits_are_double_roots_of_R(spin: float) -> None:
+    """R(r) = (r^2 + a^2 - a xi)^2 - Delta (eta + (xi - a)^2) and dR/dr vanish on the orbit."""
+    st = Spacetime(1.0, spin)
+    a = st.a
+    for r in np.linspace(equatorial_photon_orbit_radius(st, True), equatorial_photon_orbit_radius(st, False), 7):
+        xi, eta = (float(v) for v in spherical_orbit_constants(st, r))
+
+        def R(x):
+            return (x * x + a * a - a * xi) ** 2 - (x * x - 2 * x + a * a) * (eta + (xi - a) ** 2)
+
+        h = 1e-5
+        assert abs(R(r)) < 1e-8 * r**4
+        assert abs((R(r + h) - R(r - h)) / (2 * h)) < 1e-6 * r**3
+
+
+def test_shadow_curve_properties() -> None:
+    st = Spacetime(1.0, 0.9)
+    a, b = shadow_curve(st, 60.0, 720)
+    assert a.size == 720 and np.all(np.diff(np.arctan2(b, a)) >= 0)
+    # beta -> -beta symmetric and displaced towards alpha > 0 for a > 0.
+    assert np.allclose(np.sort(b), np.sort(-b), atol=1e-12)
+    assert polygon_centroid(a, b)[0] > 0.0
+    a0, b0 = shadow_curve(Spacetime(1.0, 0.0), 60.0, 360)
+    assert np.allclose(np.hypot(a0, b0), 3.0 * math.sqrt(3.0), rtol=1e-12)
+
+
+def test_polygon_radii_of_a_circle() -> None:
+    ang = np.linspace(0, 2 * math.pi, 4000, endpoint=False)
+    x, y = 2.0 + 3.0 * np.cos(ang), -1.0 + 3.0 * np.sin(ang)
+    r = polygon_radii(x, y, (2.0, -1.0), np.array([0.1, 1.0, 2.0, 4.0]))
+    assert np.allclose(r, 3.0, rtol=1e-6)
+    cx, cy = polygon_centroid(x, y)
+    assert abs(cx - 2.0) < 1e-12 and abs(cy + 1.0) < 1e-12
+
+
+def test_mask_boundary_of_a_synthetic_disk() -> None:
+    cam = BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=10.0, resolution=128)
+    alpha, beta = cam.pixel_coordinates()
+    captured = (alpha - 0.5) ** 2 + beta**2 < 4.0**2
+    ang, rad = extract_boundary_from_mask(alpha, beta, captured, n_angles=90, centre=(0.5, 0.0))
+    assert np.max(np.abs(rad - 4.0)) < 0.5 * cam.pixel_size
+
+
+def test_rounds_for_width() -> None:
+    assert rounds_for_width(0.2, 0.2 / 2**34, 1) == 34
+    assert rounds_for_width(0.2, 0.2 / 64**5, 63) == 5
