@@ -107,12 +107,73 @@ def cpu_baseline(n_threads: int) -> dict:
     }
 
 
+def persistent_single(hip, ref_state) -> dict:
+    """The 512^2 image through the persistent work-stealing kernel (one job, hipGraph replay)."""
+    from kerrray_amd import flops
+    from kerrray_amd.batch import DeviceBatch, ImageJob
+
+    batch = DeviceBatch(hip, [ImageJob(SPIN, INCL, RES, RTOL, ATOL, fov=FOV, r_o=R_O)])
+    batch.run(graph=True)
+    (res,) = batch.results()
+    ms = batch.timed(repeats=10, graph=True)
+    batch.close()
+    fl = flops.batch_flops(res.n_steps, res.n_rejected)
+    ach = fl / (ms * 1e-3) / 1e12
+    return {"photons_per_s": RES * RES / (ms * 1e-3), "kernel_ms": ms, "roofline_frac": ach / FP64_VECTOR_PEAK_TFLOPS,
+            "mask_identical_to_reference": None if ref_state is None else bool(np.array_equal(res.state, ref_state)),
+            "tune": {"block_size": 256, "waves_per_simd": 2, "refill_interval": 4}}
+
+
+def ragged_batch(hip, log) -> dict:
+    """64 ragged images in one persistent launch: autotuned shape, hipGraph replay, checked against the plain kernel."""
+    import ctypes as _ct
+
+    from kerrray_amd import _lib, flops
+    from kerrray_amd.backends import HostBatch, kernel_params
+    from kerrray_amd.batch import DeviceBatch, autotune, ragged_workload
+    from kerrray_amd.shadow import kr_image
+
+    jobs = ragged_workload(0, 64)
+    t0 = time.perf_counter()
+    tune, table = autotune(hip, jobs, log=None)
+    tune_s = time.perf_counter() - t0
+    log(f"autotune {len(table)} shapes in {tune_s:.1f}s: block={tune.block_size} waves={tune.waves_per_simd} "
+        f"refill={tune.refill_interval}")
+    batch = DeviceBatch(hip, jobs, tune)
+    batch.run(graph=True)
+    res = batch.results()
+    ms = batch.timed(repeats=5, graph=True)
+    batch.close()
+    total = sum(j.n_rays for j in jobs)
+    fl = sum(flops.batch_flops(r.n_steps, r.n_rejected) for r in res)
+    ach = fl / (ms * 1e-3) / 1e12
+    # Same images, one plain launch each, for the mask check and the plain-kernel time of the batch.
+    plain_ms = 0.0
+    identical = True
+    for job, r in zip(jobs, res):
+        st, integ, term = job.options()
+        out = HostBatch(job.n_rays, np.dtype(np.float64), False)
+        msv = _ct.c_double(0.0)
+        _lib.check(_lib.lib().kr_trace_image(hip.handle.ptr, _ct.byref(kernel_params(st, integ, term)),
+                                             _ct.byref(kr_image(st, job.r_o, job.inclination_deg, job.fov,
+                                                                job.resolution)),
+                                             _ct.byref(out.outputs), _ct.byref(msv)), "kr_trace_image")
+        plain_ms += msv.value
+        identical &= bool(np.array_equal(out.state, r.state) and np.array_equal(out.Y, r.Y))
+    return {"photons": total, "photons_per_s": total / (ms * 1e-3), "kernel_ms": ms,
+            "plain_kernel_ms_sum": plain_ms, "speedup_vs_plain_per_image": plain_ms / ms,
+            "roofline_frac": ach / FP64_VECTOR_PEAK_TFLOPS, "masks_identical_to_plain_kernel": identical,
+            "tune": {"block_size": tune.block_size, "waves_per_simd": tune.waves_per_simd,
+                     "refill_interval": tune.refill_interval, "candidates": len(table), "seconds": tune_s}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the per-photon path has no cross-GPU step "
@@ -205,6 +266,14 @@ def main() -> int:
             "algorithmic_flops_per_launch": fl,
         },
     }
+    log("persistent kernel, single image (uniform constants)")
+    line["forms"] = {"fp64_plain": {"photons_per_s": n / (kernel_ms * 1e-3), "kernel_ms": kernel_ms,
+                                    "roofline_frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
+                                    "mask_identical_to_reference": mask_identical}}
+    line["forms"]["fp64_persistent"] = persistent_single(hip, ref)
+    if not args.no_batch:
+        log("ragged batch (64 images, autotuned, hipGraph)")
+        line["forms"]["batched_ragged64"] = ragged_batch(hip, log)
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
         threads = min(threads, len(os.sched_getaffinity(0)))

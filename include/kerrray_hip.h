@@ -161,6 +161,47 @@ kr_status kr_bisect_edge(kr_handle h, const kr_params* p, const kr_image* img, d
                          double* lo, double* hi, int32_t iterations, int32_t n_probe, int64_t* failures,
                          int64_t* nonmonotone, double* kernel_ms);
 
+/* ---------------------------------------------------------------------
+ * Ragged batches of images in one persistent launch.  No counterpart in the
+ * reference, which traces one image at a time (compute_shadow ->
+ * trace_rays, src/kerrray/raytracing/rays.py:72-117); each image here is
+ * integrated exactly as kr_trace_image integrates it.
+ * ------------------------------------------------------------------- */
+
+/* One image of a batch: its integration constants (float64, rk45, no disk
+ * event) and its Bardeen image. */
+typedef struct kr_job {
+    kr_params params;
+    kr_image image;
+} kr_job;
+
+/* Launch shape of the persistent kernel: threads per block (64, 128, 256),
+ * resident wavefronts per SIMD the grid is sized for (the register budget
+ * is set for 2 or >= 3), and the number of integration attempts between two
+ * refills of a wavefront's idle lanes. */
+typedef struct kr_tune {
+    int32_t block_size;
+    int32_t waves_per_simd;
+    int32_t refill_interval;
+    int32_t reserved;
+} kr_tune;
+
+typedef struct kr_batch_s* kr_batch;
+
+/* Uploads the job table (at most 128 images) and binds the device output
+ * arrays: rows are the images' pixels in job order, each image row-major. */
+kr_status kr_batch_create(kr_handle h, const kr_job* jobs, int32_t n_jobs, const kr_tune* tune,
+                          const kr_outputs* d_out, kr_batch* out);
+/* Total photons (output rows) of the batch, or -1. */
+int64_t kr_batch_photons(kr_batch b);
+/* Enqueue one run (counter reset + persistent kernel) on `stream`. */
+kr_status kr_batch_run(kr_batch b, void* stream);
+/* Same, as one hipGraph launch (captured on first use, then replayed). */
+kr_status kr_batch_graph_launch(kr_batch b, void* stream);
+/* KR_FLAG_* bits of the last completed run (synchronous read). */
+kr_status kr_batch_flags(kr_batch b, int32_t* flags);
+kr_status kr_batch_destroy(kr_batch b);
+
 #ifdef __cplusplus
 }
 #endif

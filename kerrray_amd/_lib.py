@@ -32,6 +32,12 @@ EXPORTED = (
     "kr_trace_image_dev",
     "kr_trace_points_dev",
     "kr_bisect_edge",
+    "kr_batch_create",
+    "kr_batch_photons",
+    "kr_batch_run",
+    "kr_batch_graph_launch",
+    "kr_batch_flags",
+    "kr_batch_destroy",
 )
 
 
@@ -93,6 +99,19 @@ class KrImage(ctypes.Structure):
     ]
 
 
+class KrJob(ctypes.Structure):
+    _fields_ = [("params", KrParams), ("image", KrImage)]
+
+
+class KrTune(ctypes.Structure):
+    _fields_ = [
+        ("block_size", ctypes.c_int32),
+        ("waves_per_simd", ctypes.c_int32),
+        ("refill_interval", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 _lock = threading.Lock()
 _lib: ctypes.CDLL | None = None
 
@@ -125,6 +144,18 @@ def _declare(L: ctypes.CDLL) -> None:
         vp, P(KrParams), P(KrImage), ctypes.c_double, ctypes.c_double, vp, vp, ctypes.c_int64, vp, vp,
         ctypes.c_int32, ctypes.c_int32, c_int64_p, c_int64_p, c_double_p,
     ]
+    L.kr_batch_create.restype = ctypes.c_int
+    L.kr_batch_create.argtypes = [vp, P(KrJob), ctypes.c_int32, P(KrTune), P(KrOutputs), P(vp)]
+    L.kr_batch_photons.restype = ctypes.c_int64
+    L.kr_batch_photons.argtypes = [vp]
+    L.kr_batch_run.restype = ctypes.c_int
+    L.kr_batch_run.argtypes = [vp, vp]
+    L.kr_batch_graph_launch.restype = ctypes.c_int
+    L.kr_batch_graph_launch.argtypes = [vp, vp]
+    L.kr_batch_flags.restype = ctypes.c_int
+    L.kr_batch_flags.argtypes = [vp, P(ctypes.c_int32)]
+    L.kr_batch_destroy.restype = ctypes.c_int
+    L.kr_batch_destroy.argtypes = [vp]
 
 
 def lib() -> ctypes.CDLL:

@@ -28,7 +28,7 @@ LIB_NAME = "libkrhip.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 ARCH = "gfx950"
 
-SOURCES = ["kr_trace.hip"]
+SOURCES = ["kr_trace.hip", "kr_batch.hip"]
 
 
 def hipcc() -> str:
@@ -44,7 +44,6 @@ def flags() -> list[str]:
         "-O3",
         "-std=c++17",
         "-fPIC",
-        "-shared",
         "-ffp-contract=off",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-Wno-unused-command-line-argument",
@@ -54,18 +53,37 @@ def flags() -> list[str]:
 
 
 def build(verbose: bool = False, extra: list[str] | None = None) -> str:
-    """Compile every HIP source into ``libkrhip.so`` (always recompiles) and return its path."""
-    tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc(), *flags(), *(extra or []), "-o", tmp, *[os.path.join(CSRC, s) for s in SOURCES]]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    proc = subprocess.run(cmd, capture_output=True, text=True)
-    if proc.returncode != 0:
-        sys.stderr.write(proc.stdout + proc.stderr)
-        raise RuntimeError(f"hipcc failed with exit code {proc.returncode}")
-    if verbose and proc.stderr:
-        sys.stderr.write(proc.stderr)
-    os.replace(tmp, LIB_PATH)
+    """Compile every HIP source (in parallel) and link ``libkrhip.so``; always recompiles."""
+    import tempfile
+
+    hc = hipcc()
+    with tempfile.TemporaryDirectory(prefix="krhip_") as tmpdir:
+        procs = []
+        objs = []
+        for src in SOURCES:
+            obj = os.path.join(tmpdir, src + ".o")
+            cmd = [hc, *flags(), *(extra or []), "-c", "-o", obj, os.path.join(CSRC, src)]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
+            objs.append(obj)
+        failed = False
+        for src, proc in procs:
+            out, err = proc.communicate()
+            if proc.returncode != 0:
+                failed = True
+                sys.stderr.write(f"--- {src}\n{out}{err}")
+            elif verbose and err:
+                sys.stderr.write(err)
+        if failed:
+            raise RuntimeError("hipcc failed")
+        tmp = LIB_PATH + ".tmp"
+        link = [hc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+        proc = subprocess.run(link, capture_output=True, text=True)
+        if proc.returncode != 0:
+            sys.stderr.write(proc.stdout + proc.stderr)
+            raise RuntimeError("hipcc link failed")
+        os.replace(tmp, LIB_PATH)
     return LIB_PATH
 
 

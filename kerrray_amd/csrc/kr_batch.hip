@@ -1,0 +1,339 @@
+// Persistent, work-stealing integration of a ragged batch of Bardeen images.
+//
+// One launch integrates every pixel of every image of the batch (mixed
+// spins, inclinations, resolutions and tolerances).  The grid is sized to
+// stay resident; each lane holds one photon at a time.  Every
+// `refill_interval` attempts a wavefront counts its idle lanes, takes that
+// many photon indices from a global counter with one atomic, and starts new
+// photons in those lanes, so finished photons are replaced instead of
+// leaving lanes idle until the slowest photon of the wavefront ends.  A
+// wavefront exits when the counter is exhausted and none of its lanes is
+// busy.  Per-photon arithmetic is that of the plain kernel (kr_device.h), so
+// every image's results equal the plain per-image kernel's bit for bit.
+//
+// The job table (integration constants and image scalars of every image)
+// is copied into LDS at kernel start; a lane reads its own job's constants
+// from there.  A batch of one image uses the uniform-constant variant
+// (constants in kernel arguments, scalar registers).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "kerrray_hip.h"
+#include "kr_device.h"
+#include "kr_host.h"
+
+namespace kr {
+
+struct JobDev {
+    Params P;
+    BardeenImage B;
+    int64_t first;  // first photon of this image in the batch numbering (= its first output row)
+    int64_t count;  // n * n
+};
+
+constexpr int MAX_LDS_JOBS = 128;
+
+template <typename T>
+struct BatchOut {
+    T* Y;
+    double* lam;
+    int64_t* n_steps;
+    int64_t* n_rej;
+    int64_t* state;
+    double* m_null;
+    double* m_e;
+    double* m_lz;
+    double* m_q;
+};
+
+template <typename T>
+__device__ __forceinline__ void batch_store(const BatchOut<T>& o, int64_t idx, const RayState<T>& R) {
+    RayResult<T> r;
+    ray_result<T, false>(R, r);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) o.Y[idx * 8 + c] = r.y[c];
+    o.lam[idx] = r.lam;
+    o.n_steps[idx] = r.n_steps;
+    o.n_rej[idx] = r.n_rej;
+    o.state[idx] = r.state;
+    o.m_null[idx] = r.m_null;
+    o.m_e[idx] = r.m_e;
+    o.m_lz[idx] = r.m_lz;
+    o.m_q[idx] = r.m_q;
+}
+
+__device__ __forceinline__ int find_job(const JobDev* jobs, int n_jobs, int64_t idx) {
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].first <= idx)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+struct BatchArgs {
+    const JobDev* jobs;  // device copy of the job table
+    int n_jobs;
+    int refill_interval;
+    int64_t total;  // photons in the batch
+    unsigned long long* counter;
+    int32_t* flags;
+};
+
+template <typename T, bool UNIFORM, int BLOCK, int WAVES>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void batch_kernel(
+    BatchArgs a, JobDev uniform_job, BatchOut<T> out) {
+    extern __shared__ JobDev lds_jobs[];
+    if (!UNIFORM) {
+        const int words = static_cast<int>(a.n_jobs * sizeof(JobDev) / sizeof(uint64_t));
+        const uint64_t* src = reinterpret_cast<const uint64_t*>(a.jobs);
+        uint64_t* dst = reinterpret_cast<uint64_t*>(lds_jobs);
+        for (int w = threadIdx.x; w < words; w += BLOCK) dst[w] = src[w];
+        __syncthreads();
+    }
+    const int lane = threadIdx.x & 63;
+    RayState<T> R;
+    R.state = RUNNING;
+    bool busy = false;
+    bool exhausted = false;  // wave-uniform: the counter has run past the end
+    int64_t idx = -1;
+    int job = 0;
+    int since = a.refill_interval;
+    while (true) {
+        const bool any_busy = __any(busy);
+        if (!exhausted && (since >= a.refill_interval || !any_busy)) {
+            since = 0;
+            const uint64_t idle = __ballot(!busy);
+            const int n_idle = __popcll(idle);
+            if (n_idle > 0) {
+                const int leader = __ffsll(static_cast<unsigned long long>(idle)) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(a.counter, static_cast<unsigned long long>(n_idle));
+                base = __shfl(base, leader);
+                if (base + static_cast<unsigned long long>(n_idle) >= static_cast<unsigned long long>(a.total))
+                    exhausted = true;
+                if (!busy) {
+                    const uint64_t below = (lane == 0) ? 0ull : (idle & (~0ull >> (64 - lane)));
+                    const int64_t mine = static_cast<int64_t>(base) + __popcll(below);
+                    if (mine < a.total) {
+                        idx = mine;
+                        job = UNIFORM ? 0 : find_job(lds_jobs, a.n_jobs, idx);
+                        const JobDev& J = UNIFORM ? uniform_job : lds_jobs[job];
+                        const int64_t local = idx - J.first;
+                        const int64_t i = local / J.B.n, j = local - i * J.B.n;
+                        double y0[8];
+                        if (!bardeen_state(J.B, pixel_centre(J.B.fov, j, J.B.n), -pixel_centre(J.B.fov, i, J.B.n),
+                                           y0) &&
+                            a.flags)
+                            atomicOr(a.flags, KR_FLAG_RADICAND);
+                        ray_init<T, true>(J.P, y0, R);
+                        if (R.state == RUNNING)
+                            busy = true;
+                        else
+                            batch_store(out, idx, R);
+                    }
+                }
+            }
+            continue;  // re-evaluate: new photons may have terminated at once
+        }
+        if (!any_busy) break;  // exhausted and nothing in flight
+        if (busy) {
+            const Params& P = UNIFORM ? uniform_job.P : lds_jobs[job].P;
+            ray_attempt<T, true, false>(P, R);
+            if (R.state != RUNNING) {
+                batch_store(out, idx, R);
+                busy = false;
+            }
+        }
+        ++since;
+    }
+}
+
+__global__ void batch_reset_kernel(unsigned long long* counter, int32_t* flags) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *counter = 0ull;
+        if (flags) *flags = 0;
+    }
+}
+
+using BatchFn = void (*)(BatchArgs, JobDev, BatchOut<double>);
+
+template <bool U>
+BatchFn pick(int block, int waves) {
+    if (block == 64) return waves >= 3 ? batch_kernel<double, U, 64, 3> : batch_kernel<double, U, 64, 2>;
+    if (block == 128) return waves >= 3 ? batch_kernel<double, U, 128, 3> : batch_kernel<double, U, 128, 2>;
+    return waves >= 3 ? batch_kernel<double, U, 256, 3> : batch_kernel<double, U, 256, 2>;
+}
+
+}  // namespace kr
+
+using namespace kr;
+
+struct kr_batch_s {
+    kr_context* h = nullptr;
+    void* dmem = nullptr;  // job table, counter, flags
+    JobDev* d_jobs = nullptr;
+    unsigned long long* d_counter = nullptr;
+    int32_t* d_flags = nullptr;
+    std::vector<JobDev> jobs;
+    int64_t total = 0;
+    kr_tune tune{};
+    kr_outputs out{};
+    int cus = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+kr_status check_tune(const kr_tune* t) {
+    if (!t) return host::fail(KR_ERR_INVALID, "tune is NULL");
+    if (t->block_size != 64 && t->block_size != 128 && t->block_size != 256)
+        return host::fail(KR_ERR_INVALID, "block_size must be 64, 128 or 256");
+    if (t->waves_per_simd < 1 || t->waves_per_simd > 8) return host::fail(KR_ERR_INVALID, "waves_per_simd in [1, 8]");
+    if (t->refill_interval < 1 || t->refill_interval > 4096)
+        return host::fail(KR_ERR_INVALID, "refill_interval in [1, 4096]");
+    return KR_OK;
+}
+
+kr_status build_jobs(kr_context* h, const kr_job* jobs, int32_t n_jobs, std::vector<JobDev>& out, int64_t& total) {
+    if (!jobs || n_jobs < 1) return host::fail(KR_ERR_INVALID, "need at least one job");
+    if (n_jobs > MAX_LDS_JOBS) return host::fail(KR_ERR_UNSUPPORTED, "at most 128 images per batch");
+    out.resize(n_jobs);
+    total = 0;
+    for (int k = 0; k < n_jobs; ++k) {
+        JobDev& J = out[k];
+        std::memset(&J, 0, sizeof(J));
+        const int64_t n = jobs[k].image.resolution;
+        kr_status st = host::validate(h, &jobs[k].params, n * n, &J.P);
+        if (st != KR_OK) return st;
+        if (jobs[k].params.dtype != KR_FLOAT64 || J.P.ipar[I_ADAPTIVE] != 1 || J.P.ipar[I_DISK] != 0)
+            return host::fail(KR_ERR_UNSUPPORTED, "batched images must be float64 rk45 without a disk event");
+        if ((st = host::bardeen_image(&jobs[k].image, true, &J.B)) != KR_OK) return st;
+        J.first = total;
+        J.count = n * n;
+        total += n * n;
+    }
+    return KR_OK;
+}
+
+hipError_t enqueue(kr_batch_s* b, hipStream_t s) {
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, b->d_counter, b->d_flags);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int cus = b->cus;
+    const int block = b->tune.block_size;
+    const int waves_per_cu = 4 * b->tune.waves_per_simd;
+    const int blocks_per_cu = (waves_per_cu * 64 + block - 1) / block;
+    const unsigned grid = static_cast<unsigned>(cus * blocks_per_cu);
+    const bool uniform = b->jobs.size() == 1;
+    BatchArgs args{b->d_jobs, static_cast<int>(b->jobs.size()), b->tune.refill_interval, b->total, b->d_counter,
+                   b->d_flags};
+    BatchOut<double> o{static_cast<double*>(b->out.Y), b->out.lam, b->out.n_steps, b->out.n_rejected, b->out.state,
+                       b->out.max_null_error, b->out.max_energy_drift, b->out.max_lz_drift, b->out.max_carter_drift};
+    const size_t lds = uniform ? 0 : b->jobs.size() * sizeof(JobDev);
+    BatchFn fn = uniform ? pick<true>(block, b->tune.waves_per_simd) : pick<false>(block, b->tune.waves_per_simd);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(block), lds, s, args, b->jobs[0], o);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" kr_status kr_batch_create(kr_handle h, const kr_job* jobs, int32_t n_jobs, const kr_tune* tune,
+                                     const kr_outputs* d_out, kr_batch* out) {
+    host::clear_error();
+    if (!h) return host::fail(KR_ERR_INVALID, "handle is NULL");
+    if (!out) return host::fail(KR_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    kr_status st = check_tune(tune);
+    if (st != KR_OK) return st;
+    if (!d_out || !d_out->Y || !d_out->lam || !d_out->n_steps || !d_out->n_rejected || !d_out->state ||
+        !d_out->max_null_error || !d_out->max_energy_drift || !d_out->max_lz_drift || !d_out->max_carter_drift)
+        return host::fail(KR_ERR_INVALID, "an output array is NULL");
+    kr_batch_s* b = new (std::nothrow) kr_batch_s();
+    if (!b) return host::fail(KR_ERR_DEVICE, "out of host memory");
+    b->h = h;
+    if ((st = build_jobs(h, jobs, n_jobs, b->jobs, b->total)) != KR_OK) {
+        delete b;
+        return st;
+    }
+    b->tune = *tune;
+    b->out = *d_out;
+    hipError_t e = hipSetDevice(h->device);
+    const size_t table = b->jobs.size() * sizeof(JobDev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, h->device);
+    if (e == hipSuccess) e = hipMalloc(&b->dmem, table + 512);
+    if (e == hipSuccess) {
+        b->d_jobs = static_cast<JobDev*>(b->dmem);
+        b->d_counter = reinterpret_cast<unsigned long long*>(static_cast<char*>(b->dmem) + ((table + 255) & ~size_t(255)));
+        b->d_flags = reinterpret_cast<int32_t*>(b->d_counter + 32);
+        e = hipMemcpy(b->d_jobs, b->jobs.data(), table, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        if (b->dmem) hipFree(b->dmem);
+        delete b;
+        return host::fail(KR_ERR_DEVICE, std::string("kr_batch_create: ") + hipGetErrorString(e));
+    }
+    *out = b;
+    return KR_OK;
+}
+
+extern "C" int64_t kr_batch_photons(kr_batch b) { return b ? b->total : -1; }
+
+extern "C" kr_status kr_batch_run(kr_batch b, void* stream) {
+    host::clear_error();
+    if (!b) return host::fail(KR_ERR_INVALID, "batch is NULL");
+    KR_HIP(hipSetDevice(b->h->device));
+    KR_HIP(enqueue(b, static_cast<hipStream_t>(stream)));
+    return KR_OK;
+}
+
+extern "C" kr_status kr_batch_graph_launch(kr_batch b, void* stream) {
+    host::clear_error();
+    if (!b) return host::fail(KR_ERR_INVALID, "batch is NULL");
+    KR_HIP(hipSetDevice(b->h->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!b->exec) {
+        // Capture on a private stream so the caller's stream is never in capture mode.
+        hipStream_t cap;
+        KR_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            hipError_t e2 = enqueue(b, cap);
+            e = hipStreamEndCapture(cap, &b->graph);
+            if (e2 != hipSuccess) e = e2;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        hipStreamDestroy(cap);
+        if (e != hipSuccess) return host::fail(KR_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    }
+    KR_HIP(hipGraphLaunch(b->exec, s));
+    return KR_OK;
+}
+
+extern "C" kr_status kr_batch_flags(kr_batch b, int32_t* flags) {
+    host::clear_error();
+    if (!b || !flags) return host::fail(KR_ERR_INVALID, "batch or flags is NULL");
+    KR_HIP(hipSetDevice(b->h->device));
+    KR_HIP(hipMemcpy(flags, b->d_flags, 4, hipMemcpyDeviceToHost));
+    return KR_OK;
+}
+
+extern "C" kr_status kr_batch_destroy(kr_batch b) {
+    host::clear_error();
+    if (!b) return KR_OK;
+    hipSetDevice(b->h->device);
+    hipDeviceSynchronize();
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->dmem) hipFree(b->dmem);
+    delete b;
+    return KR_OK;
+}

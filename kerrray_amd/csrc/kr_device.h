@@ -324,6 +324,193 @@ KR_HD double carter(double a2, const T (&v)[NA], T p_t, T p_ph, double s, double
     return p_th * p_th + (c * c) * (lz2_over_s2 - a2 * e * e);
 }
 
+
+// The per-photon integrator of numba_kernel.py:340-463 as a small state
+// machine: ray_init (:344-375), one loop iteration per ray_attempt
+// (:376-455), ray_result (:456-463).  integrate_ray runs them back to back;
+// the persistent kernel interleaves photons between attempts.  Either way
+// a photon sees exactly the same arithmetic.
+template <typename T>
+struct RayState {
+    T y[NA];
+    T p_t, p_ph;
+    T f[NA];
+    double s_cur, c_cur;  // sin, cos of the current theta (from the last right-hand side)
+    double e0, lz0, fe, fl, fq, fe2, q0;
+    double m_null, m_e, m_lz, m_q, lam;
+    T h;
+    int64_t n_steps, n_rej, attempts;
+    bool rejected_before;
+    bool ev_hit;
+    int state;
+    T ev[8];
+};
+
+// numba_kernel.py:344-375.  (Y0 arrives in float64 and is cast to T,
+// numba_backend.py:167; the drift baselines e0, lz0, q0 and the floors come
+// from the float64 Y0, numba_backend.py:174-178.)
+template <typename T, bool ADAPTIVE>
+KR_HD void ray_init(const Params& P, const double (&y64)[8], RayState<T>& R) {
+    const double a2 = P.geom[G_A2];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) R.y[i] = static_cast<T>(y64[full_index(i)]);
+    R.p_t = static_cast<T>(y64[4]);
+    R.p_ph = static_cast<T>(y64[7]);
+    R.e0 = -y64[4];
+    R.lz0 = y64[7];
+    double fe = fabs(R.e0);
+    fe = fe > 0.0 ? fe : 1.0;
+    const double mass = P.geom[G_MASS];
+    R.fe = fe;
+    R.fl = fe * mass;
+    R.fq = fe * fe * mass * mass;
+    R.fe2 = fe * fe;
+    rhs(P, R.y, R.p_t, R.p_ph, R.f, R.s_cur, R.c_cur);
+    {
+        // carter_constant(st, Y0) on the float64 Y0 (same expression as numba_kernel.py:325-336).
+        const double th0 = y64[2];
+        double s0 = R.s_cur, c0 = R.c_cur;
+        if (static_cast<double>(static_cast<T>(th0)) != th0) {
+            s0 = sin(th0);
+            c0 = cos(th0);
+        }
+        const double lz = y64[7], e = -y64[4];
+        double lz2_over_s2 = 0.0;
+        if (lz != 0.0) lz2_over_s2 = (lz * lz) / (s0 * s0);
+        R.q0 = y64[6] * y64[6] + (c0 * c0) * (lz2_over_s2 - a2 * e * e);
+    }
+    R.m_null = null_error(R.y, R.p_t, R.p_ph, R.f) / R.fe2;
+    R.m_e = 0.0;
+    R.m_lz = 0.0;
+    R.m_q = 0.0;
+    R.lam = 0.0;
+    R.h = static_cast<T>(P.consts[C_H0]);
+    if (ADAPTIVE && static_cast<double>(R.h) > P.params[P_HMAX]) R.h = static_cast<T>(P.params[P_HMAX]);
+    R.n_steps = 0;
+    R.n_rej = 0;
+    R.attempts = 0;
+    R.rejected_before = false;
+    R.ev_hit = false;
+    R.state = classify(P, R.y, R.p_ph, R.f, R.lam, R.n_steps, R.h, false, working_sin<T>(R.s_cur, R.y[2]));
+}
+
+// One iteration of the while loop of numba_kernel.py:376-455 (a `break`
+// there is a return here with R.state set; `continue` is a return with
+// R.state still RUNNING).
+template <typename T, bool ADAPTIVE, bool DISK>
+KR_HD void ray_attempt(const Params& P, RayState<T>& R) {
+    const double lambda_max = P.params[P_LAMMAX];
+    const double h_cap = P.params[P_HMAX];
+    R.attempts += 1;
+    if (R.attempts > P.ipar[I_GUARD]) {
+        R.state = FAILURE;
+        return;
+    }
+    const double remaining = lambda_max - R.lam;
+    const bool last = static_cast<double>(R.h) >= remaining;
+    T h_eff = R.h;
+    if (last) h_eff = static_cast<T>(remaining);
+    T y_new[NA], k6[NA], err[NA];
+    double s_new, c_new;
+    if (ADAPTIVE)
+        dp5_step(P, R.y, R.p_t, R.p_ph, h_eff, R.f, y_new, err, k6, s_new, c_new);
+    else
+        rk4_step(P, R.y, R.p_t, R.p_ph, h_eff, R.f, y_new, k6, s_new, c_new);
+    bool finite = all_finite(y_new, k6);
+    bool accept = finite;
+    T h_new = R.h;
+    if (ADAPTIVE && finite) {
+        const T en = error_norm(static_cast<T>(P.consts[C_RTOL]), static_cast<T>(P.consts[C_ATOL]), err, R.y, y_new);
+        if (!wfinite(en)) {
+            finite = false;
+            accept = false;
+        } else {
+            accept = en <= T(1);
+            const T factor = step_factor(static_cast<T>(P.consts[C_SAFETY]), static_cast<T>(P.consts[C_FMIN]),
+                                         static_cast<T>(P.consts[C_FMAX]), static_cast<T>(P.consts[C_EXP]), en,
+                                         R.rejected_before);
+            if (!(accept && last)) {
+                h_new = h_eff * factor;
+                if (static_cast<double>(h_new) > h_cap) h_new = static_cast<T>(h_cap);
+            }
+        }
+    }
+    if (!finite) {
+        R.state = FAILURE;
+        return;
+    }
+    const bool rejected = !accept;
+    if (rejected) R.n_rej += 1;
+    R.rejected_before = rejected;
+    R.h = h_new;
+    if (rejected) {
+        if (h_new < static_cast<T>(P.consts[C_HMIN])) R.state = FAILURE;
+        return;
+    }
+    T y_prev[NA];
+    const T p_t_prev = R.p_t, p_ph_prev = R.p_ph;
+    if (DISK) {
+#pragma unroll
+        for (int i = 0; i < NA; ++i) y_prev[i] = R.y[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+        R.y[i] = y_new[i];
+        R.f[i] = k6[i];
+    }
+    R.p_t = R.p_t + T(0);
+    R.p_ph = R.p_ph + T(0);
+    R.s_cur = s_new;
+    R.c_cur = c_new;
+    const double lam_old = R.lam;
+    if (last)
+        R.lam = lambda_max;
+    else
+        R.lam = R.lam + static_cast<double>(h_eff);
+    R.n_steps += 1;
+    if (sizeof(T) != sizeof(double)) {
+        // In float64 E and L_z are bit-constant and both drifts stay 0.
+        const double e = -static_cast<double>(R.p_t);
+        const double lz = static_cast<double>(R.p_ph);
+        R.m_e = pymax(R.m_e, fabs(e - R.e0) / pymax(fabs(R.e0), R.fe));
+        R.m_lz = pymax(R.m_lz, fabs(lz - R.lz0) / pymax(fabs(R.lz0), R.fl));
+    }
+    R.m_q = pymax(R.m_q, fabs(carter(P.geom[G_A2], R.y, R.p_t, R.p_ph, R.s_cur, R.c_cur) - R.q0) /
+                             pymax(fabs(R.q0), R.fq));
+    R.m_null = pymax(R.m_null, null_error(R.y, R.p_t, R.p_ph, R.f) / R.fe2);
+    int code = classify(P, R.y, R.p_ph, R.f, R.lam, R.n_steps, R.h, true, working_sin<T>(R.s_cur, R.y[2]));
+    if (DISK) {
+        // numba_kernel.py:439-454
+        const double half_pi = P.params[P_HALFPI];
+        const double th_old = static_cast<double>(y_prev[2]);
+        const double th_new = static_cast<double>(R.y[2]);
+        if ((th_old - half_pi) * (th_new - half_pi) < 0.0) {
+            const double sfrac = (half_pi - th_old) / (th_new - th_old);
+            T yc[8];
+            const T full_new[8] = {R.y[0], R.y[1], R.y[2], R.y[3], R.p_t, R.y[4], R.y[5], R.p_ph};
+            const T full_old[8] = {y_prev[0], y_prev[1], y_prev[2], y_prev[3], p_t_prev, y_prev[4], y_prev[5], p_ph_prev};
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const T cast = static_cast<T>(sfrac * static_cast<double>(full_new[i] - full_old[i]));
+                yc[i] = full_old[i] + cast;
+            }
+            yc[2] = static_cast<T>(P.consts[C_HALFPI]);
+            if (yc[1] >= static_cast<T>(P.consts[C_RIN]) && yc[1] <= static_cast<T>(P.consts[C_ROUT])) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) R.ev[i] = yc[i];
+#pragma unroll
+                for (int i = 0; i < NA; ++i) R.y[i] = yc[full_index(i)];
+                R.p_t = yc[4];
+                R.p_ph = yc[7];
+                R.ev_hit = true;
+                R.lam = lam_old + sfrac * static_cast<double>(h_eff);
+                code = DISK_HIT;
+            }
+        }
+    }
+    R.state = code;
+}
+
 template <typename T>
 struct RayResult {
     T y[8];
@@ -336,191 +523,34 @@ struct RayResult {
     T ev[8];
 };
 
-// numba_kernel.py:340-463 for one photon whose float64 initial state is y0.
-// (Y0 arrives in float64 and is cast to T, numba_backend.py:167; the
-// reference drift baselines e0, lz0, q0 and floors are taken from the
-// float64 Y0, numba_backend.py:174-178.)
-template <typename T, bool ADAPTIVE, bool DISK>
-KR_HD void integrate_ray(const Params& P, const double (&y64)[8], RayResult<T>& out) {
-    const double a2 = P.geom[G_A2];
-    const double lambda_max = P.params[P_LAMMAX];
-    const double half_pi = P.params[P_HALFPI];
-    const double h_cap = P.params[P_HMAX];
-    const T rtol = static_cast<T>(P.consts[C_RTOL]), atol = static_cast<T>(P.consts[C_ATOL]);
-    const T safety = static_cast<T>(P.consts[C_SAFETY]), fmin = static_cast<T>(P.consts[C_FMIN]);
-    const T fmax = static_cast<T>(P.consts[C_FMAX]), expo = static_cast<T>(P.consts[C_EXP]);
-    const T h_min = static_cast<T>(P.consts[C_HMIN]);
-
-    T y[NA];
+template <typename T, bool DISK>
+KR_HD void ray_result(const RayState<T>& R, RayResult<T>& out) {
 #pragma unroll
-    for (int i = 0; i < NA; ++i) y[i] = static_cast<T>(y64[full_index(i)]);
-    T p_t = static_cast<T>(y64[4]);
-    T p_ph = static_cast<T>(y64[7]);
-
-    // Drift baselines from the float64 initial state (photons/constants.py:65-90,163-171).
-    const double e0 = -y64[4];
-    const double lz0 = y64[7];
-    double fe = fabs(e0);
-    fe = fe > 0.0 ? fe : 1.0;
-    const double mass = P.geom[G_MASS];
-    const double fl = fe * mass;
-    const double fq = fe * fe * mass * mass;
-    const double fe2 = fe * fe;
-
-    T f[NA];
-    double s_cur, c_cur;
-    rhs(P, y, p_t, p_ph, f, s_cur, c_cur);
-    double q0;
-    {
-        // carter_constant(st, Y0) on the float64 Y0 (same expression as numba_kernel.py:325-336).
-        const double th0 = y64[2];
-        double s0 = s_cur, c0 = c_cur;
-        if (static_cast<double>(static_cast<T>(th0)) != th0) {
-            s0 = sin(th0);
-            c0 = cos(th0);
-        }
-        const double lz = y64[7], e = -y64[4];
-        double lz2_over_s2 = 0.0;
-        if (lz != 0.0) lz2_over_s2 = (lz * lz) / (s0 * s0);
-        q0 = y64[6] * y64[6] + (c0 * c0) * (lz2_over_s2 - a2 * e * e);
-    }
-    double m_null = null_error(y, p_t, p_ph, f) / fe2;
-    double m_e = 0.0, m_lz = 0.0, m_q = 0.0;
-    double lam = 0.0;
-    T h = static_cast<T>(P.consts[C_H0]);
-    if (ADAPTIVE && static_cast<double>(h) > h_cap) h = static_cast<T>(h_cap);
-    int64_t n_steps = 0, n_rej = 0, attempts = 0;
-    const int64_t guard = P.ipar[I_GUARD];
-    bool rejected_before = false;
-    bool ev_hit = false;
-    T ev[8];
-    int state = classify(P, y, p_ph, f, lam, n_steps, h, false, working_sin<T>(s_cur, y[2]));
-    while (state == RUNNING) {
-        attempts += 1;
-        if (attempts > guard) {
-            state = FAILURE;
-            break;
-        }
-        const double remaining = lambda_max - lam;
-        const bool last = static_cast<double>(h) >= remaining;
-        T h_eff = h;
-        if (last) h_eff = static_cast<T>(remaining);
-        T y_new[NA], k6[NA], err[NA];
-        double s_new, c_new;
-        if (ADAPTIVE)
-            dp5_step(P, y, p_t, p_ph, h_eff, f, y_new, err, k6, s_new, c_new);
-        else
-            rk4_step(P, y, p_t, p_ph, h_eff, f, y_new, k6, s_new, c_new);
-        bool finite = all_finite(y_new, k6);
-        bool accept = finite;
-        T h_new = h;
-        if (ADAPTIVE && finite) {
-            const T en = error_norm(rtol, atol, err, y, y_new);
-            if (!wfinite(en)) {
-                finite = false;
-                accept = false;
-            } else {
-                accept = en <= T(1);
-                const T factor = step_factor(safety, fmin, fmax, expo, en, rejected_before);
-                if (!(accept && last)) {
-                    h_new = h_eff * factor;
-                    if (static_cast<double>(h_new) > h_cap) h_new = static_cast<T>(h_cap);
-                }
-            }
-        }
-        if (!finite) {
-            state = FAILURE;
-            break;
-        }
-        const bool rejected = !accept;
-        if (rejected) n_rej += 1;
-        rejected_before = rejected;
-        h = h_new;
-        if (rejected) {
-            if (h_new < h_min) {
-                state = FAILURE;
-                break;
-            }
-            continue;
-        }
-        T y_prev[NA];
-        T p_t_prev = p_t, p_ph_prev = p_ph;
-        if (DISK) {
-#pragma unroll
-            for (int i = 0; i < NA; ++i) y_prev[i] = y[i];
-        }
-#pragma unroll
-        for (int i = 0; i < NA; ++i) {
-            y[i] = y_new[i];
-            f[i] = k6[i];
-        }
-        p_t = p_t + T(0);
-        p_ph = p_ph + T(0);
-        s_cur = s_new;
-        c_cur = c_new;
-        const double lam_old = lam;
-        if (last)
-            lam = lambda_max;
-        else
-            lam = lam + static_cast<double>(h_eff);
-        n_steps += 1;
-        if (sizeof(T) != sizeof(double)) {
-            // In float64 E and L_z are bit-constant and both drifts stay 0.
-            const double e = -static_cast<double>(p_t);
-            const double lz = static_cast<double>(p_ph);
-            m_e = pymax(m_e, fabs(e - e0) / pymax(fabs(e0), fe));
-            m_lz = pymax(m_lz, fabs(lz - lz0) / pymax(fabs(lz0), fl));
-        }
-        m_q = pymax(m_q, fabs(carter(a2, y, p_t, p_ph, s_cur, c_cur) - q0) / pymax(fabs(q0), fq));
-        m_null = pymax(m_null, null_error(y, p_t, p_ph, f) / fe2);
-        int code = classify(P, y, p_ph, f, lam, n_steps, h, true, working_sin<T>(s_cur, y[2]));
-        if (DISK) {
-            // numba_kernel.py:439-454
-            const double th_old = static_cast<double>(y_prev[2]);
-            const double th_new = static_cast<double>(y[2]);
-            if ((th_old - half_pi) * (th_new - half_pi) < 0.0) {
-                const double sfrac = (half_pi - th_old) / (th_new - th_old);
-                T yc[8];
-                T full_new[8] = {y[0], y[1], y[2], y[3], p_t, y[4], y[5], p_ph};
-                T full_old[8] = {y_prev[0], y_prev[1], y_prev[2], y_prev[3], p_t_prev, y_prev[4], y_prev[5], p_ph_prev};
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const T cast = static_cast<T>(sfrac * static_cast<double>(full_new[i] - full_old[i]));
-                    yc[i] = full_old[i] + cast;
-                }
-                yc[2] = static_cast<T>(P.consts[C_HALFPI]);
-                if (yc[1] >= static_cast<T>(P.consts[C_RIN]) && yc[1] <= static_cast<T>(P.consts[C_ROUT])) {
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) ev[i] = yc[i];
-#pragma unroll
-                    for (int i = 0; i < NA; ++i) y[i] = yc[full_index(i)];
-                    p_t = yc[4];
-                    p_ph = yc[7];
-                    ev_hit = true;
-                    lam = lam_old + sfrac * static_cast<double>(h_eff);
-                    code = DISK_HIT;
-                }
-            }
-        }
-        state = code;
-    }
-#pragma unroll
-    for (int i = 0; i < NA; ++i) out.y[full_index(i)] = y[i];
-    out.y[4] = p_t;
-    out.y[7] = p_ph;
-    out.lam = lam;
-    out.n_steps = n_steps;
-    out.n_rej = n_rej;
-    out.state = state;
-    out.m_null = m_null;
-    out.m_e = m_e;
-    out.m_lz = m_lz;
-    out.m_q = m_q;
-    out.ev_hit = ev_hit;
+    for (int i = 0; i < NA; ++i) out.y[full_index(i)] = R.y[i];
+    out.y[4] = R.p_t;
+    out.y[7] = R.p_ph;
+    out.lam = R.lam;
+    out.n_steps = R.n_steps;
+    out.n_rej = R.n_rej;
+    out.state = R.state;
+    out.m_null = R.m_null;
+    out.m_e = R.m_e;
+    out.m_lz = R.m_lz;
+    out.m_q = R.m_q;
+    out.ev_hit = R.ev_hit;
     if (DISK) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) out.ev[i] = ev_hit ? ev[i] : T(0);
+        for (int i = 0; i < 8; ++i) out.ev[i] = R.ev_hit ? R.ev[i] : T(0);
     }
+}
+
+// numba_kernel.py:340-463 for one photon whose float64 initial state is y64.
+template <typename T, bool ADAPTIVE, bool DISK>
+KR_HD void integrate_ray(const Params& P, const double (&y64)[8], RayResult<T>& out) {
+    RayState<T> R;
+    ray_init<T, ADAPTIVE>(P, y64, R);
+    while (R.state == RUNNING) ray_attempt<T, ADAPTIVE, DISK>(P, R);
+    ray_result<T, DISK>(R, out);
 }
 
 // Per-image scalars of the Bardeen initial conditions, computed on the host

@@ -134,7 +134,7 @@ def test_trace_rays_chunking_and_progress() -> None:
 
 def _header_symbols() -> list[str]:
     text = open(os.path.join(ROOT, "include", "kerrray_hip.h")).read()
-    return sorted(set(re.findall(r"^(?:kr_status|int|const char\*)\s+(kr_\w+)\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^(?:kr_status|int|int64_t|const char\*)\s+(kr_\w+)\(", text, flags=re.M)))
 
 
 def test_library_loads_and_exports_every_declared_symbol() -> None:
@@ -159,3 +159,18 @@ def test_null_handle_and_bad_arguments_are_errors_not_crashes() -> None:
         assert st == _lib.KR_ERR_DEVICE and h.value is None
     else:
         L.kr_destroy(h)
+
+
+def test_ragged_workload_is_the_committed_table() -> None:
+    import json
+    from dataclasses import asdict
+
+    from kerrray_amd.batch import ragged_workload
+
+    spec = json.load(open(os.path.join(ROOT, "tests", "golden", "ragged64_spec.json")))
+    jobs = ragged_workload(0, 64)
+    assert [asdict(j) for j in jobs] == spec["images"]
+    assert sum(j.n_rays for j in jobs) == 5_828_608
+    assert {j.inclination_deg for j in jobs} == {17.0, 60.0, 85.0}
+    assert {j.resolution for j in jobs} == {64, 128, 256, 512}
+    assert all(0.0 <= j.spin < 0.99 and j.atol == 1e-2 * j.rtol for j in jobs)
