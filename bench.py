@@ -124,6 +124,62 @@ def persistent_single(hip, ref_state) -> dict:
             "tune": {"block_size": 256, "waves_per_simd": 2, "refill_interval": 4}}
 
 
+def planned_image(hip, ref_state, fp32_bulk: bool, repeats: int = 10) -> dict:
+    """512^2 image through the planned kernels (device buffers, HIP events on the launch stream)."""
+    import ctypes as _ct
+
+    import torch
+
+    from kerrray_amd import _lib, flops
+    from kerrray_amd.backends import kernel_params
+    from kerrray_amd.camera import BardeenCamera
+    from kerrray_amd.geodesics import IntegratorOptions, TerminationOptions
+    from kerrray_amd.geometry import Spacetime
+    from kerrray_amd.planned import PlannedOptions, compute_shadow_planned, edge_table
+    from kerrray_amd.shadow import kr_image
+
+    st = Spacetime(1.0, SPIN)
+    integ, term = IntegratorOptions(method="rk45", rtol=RTOL, atol=ATOL), TerminationOptions()
+    opts = PlannedOptions(fp32_bulk=fp32_bulk)
+    cam = BardeenCamera(radius=R_O, inclination_deg=INCL, fov=FOV, resolution=RES)
+    image, stats = compute_shadow_planned(st, cam, integ, term, backend=hip, options=opts)
+    table = edge_table(st, INCL)
+    n = RES * RES
+    bufs, outs = device_outputs(torch, n, torch.float64)
+    radii = torch.from_numpy(table.radii).cuda()
+    nbytes = _lib.lib().kr_planned_workspace_bytes(n, table.n_angles)
+    work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    params, img, ko = kernel_params(st, integ, term), kr_image(st, R_O, INCL, FOV, RES), opts.kr()
+    stream = torch.cuda.current_stream()
+
+    def step():
+        _lib.check(_lib.lib().kr_trace_image_planned_dev(
+            hip.handle.ptr, _ct.byref(params), _ct.byref(img), _ct.byref(ko), table.centre[0], table.centre[1],
+            radii.data_ptr(), table.n_angles, _ct.byref(outs), work.data_ptr(), nbytes,
+            _ct.c_void_p(stream.cuda_stream)), "kr_trace_image_planned_dev")
+
+    step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(repeats):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / repeats
+    state = bufs["state"].cpu().numpy()
+    n_steps, n_rej = bufs["n_steps"].cpu().numpy(), bufs["n_rejected"].cpu().numpy()
+    fl = flops.batch_flops(n_steps, n_rej)
+    return {"photons_per_s": n / (ms * 1e-3), "kernel_ms": ms,
+            "roofline_frac": fl / (ms * 1e-3) / 1e12 / FP64_VECTOR_PEAK_TFLOPS,
+            "mask_identical_to_reference": None if ref_state is None else bool(np.array_equal(state, ref_state)),
+            "edge_band_pixels": stats.planned_band, "repaired_pixels": stats.repaired,
+            "fp64_attempt_fraction": stats.fp64_fraction,
+            "options": {"band_px": opts.band_px, "horizon_margin": opts.horizon_margin, "axis_sin": opts.axis_sin,
+                        "turn_rmax": opts.turn_rmax, "turn_drdl": opts.turn_drdl,
+                        "refill_interval": opts.refill_interval, "block_size": opts.block_size}}
+
+
 def ragged_batch(hip, log) -> dict:
     """64 ragged images in one persistent launch: autotuned shape, hipGraph replay, checked against the plain kernel."""
     import ctypes as _ct
@@ -271,6 +327,10 @@ def main() -> int:
                                     "roofline_frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
                                     "mask_identical_to_reference": mask_identical}}
     line["forms"]["fp64_persistent"] = persistent_single(hip, ref)
+    log("planned FP64 (longest-first schedule)")
+    line["forms"]["fp64_planned"] = planned_image(hip, ref, fp32_bulk=False)
+    log("planned FP32 bulk + FP64 repair")
+    line["forms"]["fp32_fp64_repair"] = planned_image(hip, ref, fp32_bulk=True)
     if not args.no_batch:
         log("ragged batch (64 images, autotuned, hipGraph)")
         line["forms"]["batched_ragged64"] = ragged_batch(hip, log)

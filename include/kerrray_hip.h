@@ -22,6 +22,7 @@
 #ifndef KERRRAY_HIP_H
 #define KERRRAY_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -201,6 +202,60 @@ kr_status kr_batch_graph_launch(kr_batch b, void* stream);
 /* KR_FLAG_* bits of the last completed run (synchronous read). */
 kr_status kr_batch_flags(kr_batch b, int32_t* flags);
 kr_status kr_batch_destroy(kr_batch b);
+
+/* ---------------------------------------------------------------------
+ * Planned image traces: longest-first scheduling and the FP32 bulk with
+ * FP64 repair.  No counterpart in the reference (whose float32 mode is a
+ * separate working dtype, numba_backend.py:16-20); the FP64 form gives
+ * exactly kr_trace_image's results.
+ * ------------------------------------------------------------------- */
+
+/* Options of a planned trace.
+ *  band_px: pixels within band_px (Chebyshev) of the capture boundary are
+ *    integrated in FP64: those within band_px + 1 pixels of the analytic
+ *    edge from the start, and after the trace every pixel within band_px of
+ *    a captured/not-captured transition of the mask, or whose photon ended
+ *    neither captured nor escaped, that was not.
+ *  fp32_bulk: 0 = every photon in FP64 (the longest-first schedule only);
+ *    1 = other photons take FP32 right-hand sides with the FP64 update,
+ *    error norm and step control, switching an attempt to FP64 when
+ *    r < r_+ + horizon_margin, |sin theta| < axis_sin, or r < turn_rmax and
+ *    |dr/dlambda| < turn_drdl.
+ *  refill_interval, block_size: launch shape of the persistent kernel. */
+typedef struct kr_planned_opts {
+    double horizon_margin;
+    double axis_sin;
+    double turn_rmax;
+    double turn_drdl;
+    int32_t band_px;
+    int32_t fp32_bulk;
+    int32_t refill_interval;
+    int32_t block_size;
+} kr_planned_opts;
+
+typedef struct kr_planned_stats {
+    int64_t planned_band;    /* pixels scheduled first and integrated in FP64 */
+    int64_t planned_inside;  /* pixels inside the analytic edge */
+    int64_t planned_outside;
+    int64_t repaired;        /* pixels the mask check sent back to FP64 */
+    int64_t fp64_attempts;   /* integration attempts taken in FP64 */
+} kr_planned_stats;
+
+/* Device workspace of kr_trace_image_planned_dev for an image of `pixels`
+ * pixels and an edge table of n_angles radii. */
+size_t kr_planned_workspace_bytes(int64_t pixels, int32_t n_angles);
+
+/* The analytic edge is given as radii[k] from (centre_alpha, centre_beta)
+ * along the angle 2 pi k / n_angles (e.g. Bardeen's curve).  Host arrays;
+ * stats and kernel_ms may be NULL. */
+kr_status kr_trace_image_planned(kr_handle h, const kr_params* p, const kr_image* img, const kr_planned_opts* o,
+                                 double centre_alpha, double centre_beta, const double* radii, int32_t n_angles,
+                                 const kr_outputs* out, kr_planned_stats* stats, double* kernel_ms);
+/* Same on device pointers (radii, outputs, workspace), enqueued on `stream`. */
+kr_status kr_trace_image_planned_dev(kr_handle h, const kr_params* p, const kr_image* img, const kr_planned_opts* o,
+                                     double centre_alpha, double centre_beta, const double* d_radii,
+                                     int32_t n_angles, const kr_outputs* d_out, void* d_work, size_t work_bytes,
+                                     void* stream);
 
 #ifdef __cplusplus
 }

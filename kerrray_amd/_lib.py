@@ -38,6 +38,9 @@ EXPORTED = (
     "kr_batch_graph_launch",
     "kr_batch_flags",
     "kr_batch_destroy",
+    "kr_planned_workspace_bytes",
+    "kr_trace_image_planned",
+    "kr_trace_image_planned_dev",
 )
 
 
@@ -103,6 +106,29 @@ class KrJob(ctypes.Structure):
     _fields_ = [("params", KrParams), ("image", KrImage)]
 
 
+class KrPlannedOpts(ctypes.Structure):
+    _fields_ = [
+        ("horizon_margin", ctypes.c_double),
+        ("axis_sin", ctypes.c_double),
+        ("turn_rmax", ctypes.c_double),
+        ("turn_drdl", ctypes.c_double),
+        ("band_px", ctypes.c_int32),
+        ("fp32_bulk", ctypes.c_int32),
+        ("refill_interval", ctypes.c_int32),
+        ("block_size", ctypes.c_int32),
+    ]
+
+
+class KrPlannedStats(ctypes.Structure):
+    _fields_ = [
+        ("planned_band", ctypes.c_int64),
+        ("planned_inside", ctypes.c_int64),
+        ("planned_outside", ctypes.c_int64),
+        ("repaired", ctypes.c_int64),
+        ("fp64_attempts", ctypes.c_int64),
+    ]
+
+
 class KrTune(ctypes.Structure):
     _fields_ = [
         ("block_size", ctypes.c_int32),
@@ -156,6 +182,32 @@ def _declare(L: ctypes.CDLL) -> None:
     L.kr_batch_flags.argtypes = [vp, P(ctypes.c_int32)]
     L.kr_batch_destroy.restype = ctypes.c_int
     L.kr_batch_destroy.argtypes = [vp]
+    L.kr_planned_workspace_bytes.restype = ctypes.c_size_t
+    L.kr_planned_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.c_int32]
+    L.kr_trace_image_planned.restype = ctypes.c_int
+    L.kr_trace_image_planned.argtypes = [vp, P(KrParams), P(KrImage), P(KrPlannedOpts), ctypes.c_double,
+                                         ctypes.c_double, vp, ctypes.c_int32, P(KrOutputs), P(KrPlannedStats),
+                                         c_double_p]
+    L.kr_trace_image_planned_dev.restype = ctypes.c_int
+    L.kr_trace_image_planned_dev.argtypes = [vp, P(KrParams), P(KrImage), P(KrPlannedOpts), ctypes.c_double,
+                                             ctypes.c_double, vp, ctypes.c_int32, P(KrOutputs), vp, ctypes.c_size_t,
+                                             vp]
+
+
+def _share_torch_runtime() -> None:
+    """Load PyTorch's HIP runtime first, when PyTorch is installed.
+
+    PyTorch-ROCm ships its own ``libamdhip64.so`` with the soname
+    ``libamdhip64.so.7``, the library this one needs. Loaded first, it is the
+    one the dynamic linker then binds ``libkrhip.so`` to, so the process has
+    a single HIP runtime and PyTorch's streams, events and allocations are
+    valid arguments of the ``_dev`` entry points. Loaded the other way round,
+    two runtimes would coexist and PyTorch would not see the GPU.
+    """
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
 
 
 def lib() -> ctypes.CDLL:
@@ -163,6 +215,7 @@ def lib() -> ctypes.CDLL:
     global _lib
     with _lock:
         if _lib is None:
+            _share_torch_runtime()
             if not os.path.exists(LIB_PATH):
                 raise LibraryMissing(
                     f"{LIB_PATH} is not built; run `python -m kerrray_amd.build` (needs ROCm hipcc)"

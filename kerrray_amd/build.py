@@ -28,7 +28,7 @@ LIB_NAME = "libkrhip.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 ARCH = "gfx950"
 
-SOURCES = ["kr_trace.hip", "kr_batch.hip"]
+SOURCES = ["kr_trace.hip", "kr_batch.hip", "kr_mixed.hip"]
 
 
 def hipcc() -> str:

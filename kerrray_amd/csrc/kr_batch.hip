@@ -38,33 +38,7 @@ struct JobDev {
 constexpr int MAX_LDS_JOBS = 128;
 
 template <typename T>
-struct BatchOut {
-    T* Y;
-    double* lam;
-    int64_t* n_steps;
-    int64_t* n_rej;
-    int64_t* state;
-    double* m_null;
-    double* m_e;
-    double* m_lz;
-    double* m_q;
-};
-
-template <typename T>
-__device__ __forceinline__ void batch_store(const BatchOut<T>& o, int64_t idx, const RayState<T>& R) {
-    RayResult<T> r;
-    ray_result<T, false>(R, r);
-#pragma unroll
-    for (int c = 0; c < 8; ++c) o.Y[idx * 8 + c] = r.y[c];
-    o.lam[idx] = r.lam;
-    o.n_steps[idx] = r.n_steps;
-    o.n_rej[idx] = r.n_rej;
-    o.state[idx] = r.state;
-    o.m_null[idx] = r.m_null;
-    o.m_e[idx] = r.m_e;
-    o.m_lz[idx] = r.m_lz;
-    o.m_q[idx] = r.m_q;
-}
+using BatchOut = RowsOut<T>;
 
 __device__ __forceinline__ int find_job(const JobDev* jobs, int n_jobs, int64_t idx) {
     int lo = 0, hi = n_jobs - 1;
@@ -137,7 +111,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, 8)
                         if (R.state == RUNNING)
                             busy = true;
                         else
-                            batch_store(out, idx, R);
+                            store_row(out, idx, R);
                     }
                 }
             }
@@ -148,7 +122,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, 8)
             const Params& P = UNIFORM ? uniform_job.P : lds_jobs[job].P;
             ray_attempt<T, true, false>(P, R);
             if (R.state != RUNNING) {
-                batch_store(out, idx, R);
+                store_row(out, idx, R);
                 busy = false;
             }
         }

@@ -394,11 +394,26 @@ KR_HD void ray_init(const Params& P, const double (&y64)[8], RayState<T>& R) {
     R.state = classify(P, R.y, R.p_ph, R.f, R.lam, R.n_steps, R.h, false, working_sin<T>(R.s_cur, R.y[2]));
 }
 
+// The exact step of the reference: Dormand-Prince 5(4) or RK4 in the
+// working type.
+template <typename T, bool ADAPTIVE>
+struct ExactStepper {
+    KR_HD void operator()(const Params& P, const RayState<T>& R, T h_eff, T (&y_new)[NA], T (&err)[NA],
+                          T (&k6)[NA], double& s_new, double& c_new) const {
+        if (ADAPTIVE)
+            dp5_step(P, R.y, R.p_t, R.p_ph, h_eff, R.f, y_new, err, k6, s_new, c_new);
+        else
+            rk4_step(P, R.y, R.p_t, R.p_ph, h_eff, R.f, y_new, k6, s_new, c_new);
+    }
+};
+
 // One iteration of the while loop of numba_kernel.py:376-455 (a `break`
 // there is a return here with R.state set; `continue` is a return with
-// R.state still RUNNING).
-template <typename T, bool ADAPTIVE, bool DISK>
-KR_HD void ray_attempt(const Params& P, RayState<T>& R) {
+// R.state still RUNNING).  `step` computes y_new, the error estimate and
+// f(y_new); everything else (error norm, acceptance, step control,
+// bookkeeping, classification) is the reference's.
+template <typename T, bool ADAPTIVE, bool DISK, class Stepper>
+KR_HD void ray_attempt_with(const Params& P, RayState<T>& R, const Stepper& step) {
     const double lambda_max = P.params[P_LAMMAX];
     const double h_cap = P.params[P_HMAX];
     R.attempts += 1;
@@ -412,10 +427,7 @@ KR_HD void ray_attempt(const Params& P, RayState<T>& R) {
     if (last) h_eff = static_cast<T>(remaining);
     T y_new[NA], k6[NA], err[NA];
     double s_new, c_new;
-    if (ADAPTIVE)
-        dp5_step(P, R.y, R.p_t, R.p_ph, h_eff, R.f, y_new, err, k6, s_new, c_new);
-    else
-        rk4_step(P, R.y, R.p_t, R.p_ph, h_eff, R.f, y_new, k6, s_new, c_new);
+    step(P, R, h_eff, y_new, err, k6, s_new, c_new);
     bool finite = all_finite(y_new, k6);
     bool accept = finite;
     T h_new = R.h;
@@ -511,6 +523,11 @@ KR_HD void ray_attempt(const Params& P, RayState<T>& R) {
     R.state = code;
 }
 
+template <typename T, bool ADAPTIVE, bool DISK>
+KR_HD void ray_attempt(const Params& P, RayState<T>& R) {
+    ray_attempt_with<T, ADAPTIVE, DISK>(P, R, ExactStepper<T, ADAPTIVE>{});
+}
+
 template <typename T>
 struct RayResult {
     T y[8];
@@ -542,6 +559,37 @@ KR_HD void ray_result(const RayState<T>& R, RayResult<T>& out) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) out.ev[i] = R.ev_hit ? R.ev[i] : T(0);
     }
+}
+
+// Output rows of a batch (the BatchResult arrays, kr_outputs without the
+// disk event) and the store of one finished photon into row `idx`.
+template <typename T>
+struct RowsOut {
+    T* Y;
+    double* lam;
+    int64_t* n_steps;
+    int64_t* n_rej;
+    int64_t* state;
+    double* m_null;
+    double* m_e;
+    double* m_lz;
+    double* m_q;
+};
+
+template <typename T>
+KR_HD void store_row(const RowsOut<T>& o, int64_t idx, const RayState<T>& R) {
+    RayResult<T> r;
+    ray_result<T, false>(R, r);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) o.Y[idx * 8 + c] = r.y[c];
+    o.lam[idx] = r.lam;
+    o.n_steps[idx] = r.n_steps;
+    o.n_rej[idx] = r.n_rej;
+    o.state[idx] = r.state;
+    o.m_null[idx] = r.m_null;
+    o.m_e[idx] = r.m_e;
+    o.m_lz[idx] = r.m_lz;
+    o.m_q[idx] = r.m_q;
 }
 
 // numba_kernel.py:340-463 for one photon whose float64 initial state is y64.

@@ -1,0 +1,424 @@
+// Planned image traces: longest-first scheduling, FP32 bulk with FP64 repair.
+//
+// One image, four kernels on one stream (no host synchronisation between):
+//  1. plan: every pixel is put in one of three classes with the analytic
+//     shadow edge (polar radii of Bardeen's curve about a centre, given by
+//     the caller): within `band_px` (+1 for the curve's sampling) pixels of
+//     the edge, inside, outside.  A work list is written in that order --
+//     the edge band holds the longest photons (near-critical orbits) and the
+//     inside the next longest (captured photons slow down at the horizon),
+//     so the slowest photons start first.  Band photons are flagged FP64.
+//  2. trace: a persistent kernel walks the work list with lane refill (as
+//     kr_batch.hip).  Flagged photons, and all photons when the FP32 bulk is
+//     off, take the exact FP64 attempt of the reference; the others take the
+//     mixed attempt of kr_mixed.h.  Each pixel records the precision used.
+//  3. verify: any pixel within `band_px` pixels (Chebyshev) of a
+//     captured/not-captured transition of the traced mask, or whose photon
+//     ended neither captured nor escaped, that was not traced in FP64 goes
+//     on a repair list.
+//  4. repair: the listed pixels are traced again with the exact FP64 kernel
+//     and their outputs overwritten.
+// With the FP32 bulk off this is the FP64 trace with a better schedule: the
+// results equal kr_trace_image's bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "kerrray_hip.h"
+#include "kr_device.h"
+#include "kr_host.h"
+#include "kr_mixed.h"
+
+namespace kr {
+
+constexpr int64_t FP64_FLAG = int64_t(1) << 62;
+
+struct PlanDev {
+    double ca, cb;          // centre of the polar radii table
+    const double* radii;    // n_angles radii at angles 2 pi k / n_angles
+    int32_t n_angles;
+    double band;            // band half-width in units of M
+};
+
+struct PlanCounters {
+    unsigned long long seg[3];     // band, inside, outside counts
+    unsigned long long cursor[3];  // scatter cursors
+    unsigned long long work;       // work-list counter of the persistent kernel
+    unsigned long long repair;     // repair-list length
+    unsigned long long fp64_attempts;
+    unsigned long long pad;        // pixels repaired over all rounds
+};
+
+// A repaired pixel may change class and so move the boundary; a second
+// verify/repair round covers its new neighbours.
+constexpr int REPAIR_ROUNDS = 2;
+
+__device__ __forceinline__ int pixel_class(const PlanDev& pl, const BardeenImage& B, int64_t idx) {
+    const int64_t i = idx / B.n, j = idx - (idx / B.n) * B.n;
+    const double da = pixel_centre(B.fov, j, B.n) - pl.ca;
+    const double db = -pixel_centre(B.fov, i, B.n) - pl.cb;
+    const double rho = sqrt(da * da + db * db);
+    double phi = atan2(db, da);
+    if (phi < 0.0) phi += 2.0 * M_PI;
+    const double u = phi * (pl.n_angles / (2.0 * M_PI));
+    int k0 = static_cast<int>(floor(u));
+    const double w = u - k0;
+    k0 = ((k0 % pl.n_angles) + pl.n_angles) % pl.n_angles;
+    const int k1 = (k0 + 1) % pl.n_angles;
+    const double rc = (1.0 - w) * pl.radii[k0] + w * pl.radii[k1];
+    if (fabs(rho - rc) <= pl.band) return 0;
+    return rho < rc ? 1 : 2;
+}
+
+__global__ void plan_count_kernel(PlanDev pl, BardeenImage B, PlanCounters* c) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= B.n * B.n) return;
+    atomicAdd(&c->seg[pixel_class(pl, B, idx)], 1ull);
+}
+
+__global__ void plan_scatter_kernel(PlanDev pl, BardeenImage B, PlanCounters* c, int64_t* work, int fp32_bulk) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= B.n * B.n) return;
+    const int k = pixel_class(pl, B, idx);
+    const unsigned long long base = k == 0 ? 0ull : (k == 1 ? c->seg[0] : c->seg[0] + c->seg[1]);
+    const unsigned long long pos = base + atomicAdd(&c->cursor[k], 1ull);
+    work[pos] = (k == 0 || !fp32_bulk) ? (idx | FP64_FLAG) : idx;
+}
+
+template <int BLOCK, bool MIXED>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) void planned_trace_kernel(
+    Params P, BardeenImage B, MixedCfg M, const int64_t* work, PlanCounters* c, int refill_interval,
+    uint8_t* used_fp64, RowsOut<double> out, int32_t* flags) {
+    const int lane = threadIdx.x & 63;
+    const int64_t total = B.n * B.n;
+    const Geom32 G = geom32(P);
+    RayState<double> R;
+    R.state = RUNNING;
+    bool busy = false, exhausted = false, exact = true;
+    int64_t idx = -1;
+    int64_t fp64_att = 0;
+    int since = refill_interval;
+    while (true) {
+        const bool any_busy = __any(busy);
+        if (!exhausted && (since >= refill_interval || !any_busy)) {
+            since = 0;
+            const uint64_t idle = __ballot(!busy);
+            const int n_idle = __popcll(idle);
+            if (n_idle > 0) {
+                const int leader = __ffsll(static_cast<unsigned long long>(idle)) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(&c->work, static_cast<unsigned long long>(n_idle));
+                base = __shfl(base, leader);
+                if (base + static_cast<unsigned long long>(n_idle) >= static_cast<unsigned long long>(total))
+                    exhausted = true;
+                if (!busy) {
+                    const uint64_t below = (lane == 0) ? 0ull : (idle & (~0ull >> (64 - lane)));
+                    const int64_t pos = static_cast<int64_t>(base) + __popcll(below);
+                    if (pos < total) {
+                        const int64_t w = work[pos];
+                        idx = w & (FP64_FLAG - 1);
+                        exact = !MIXED || (w & FP64_FLAG) != 0;
+                        const int64_t i = idx / B.n, j = idx - i * B.n;
+                        double y0[8];
+                        if (!bardeen_state(B, pixel_centre(B.fov, j, B.n), -pixel_centre(B.fov, i, B.n), y0) && flags)
+                            atomicOr(flags, KR_FLAG_RADICAND);
+                        ray_init<double, true>(P, y0, R);
+                        fp64_att = 0;
+                        if (R.state == RUNNING) {
+                            busy = true;
+                        } else {
+                            store_row(out, idx, R);
+                            used_fp64[idx] = 1;
+                        }
+                    }
+                }
+            }
+            continue;
+        }
+        if (!any_busy) break;
+        if (busy) {
+            if (MIXED && !exact) {
+                ray_attempt_with<double, true, false>(P, R, MixedStepper{G, M, &fp64_att});
+            } else {
+                ray_attempt<double, true, false>(P, R);
+            }
+            if (R.state != RUNNING) {
+                store_row(out, idx, R);
+                used_fp64[idx] = exact ? 1 : 0;
+                const int64_t n_fp64 = exact ? R.n_steps + R.n_rej : fp64_att;
+                if (n_fp64) atomicAdd(&c->fp64_attempts, static_cast<unsigned long long>(n_fp64));
+                busy = false;
+            }
+        }
+        ++since;
+    }
+}
+
+// Pixels that must be (and were not) traced in FP64.
+__global__ void verify_kernel(const int64_t* state, const uint8_t* used_fp64, int64_t n, int band_px,
+                              int64_t* repair, PlanCounters* c) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= n * n) return;
+    if (used_fp64[idx]) return;
+    const int64_t s = state[idx];
+    const bool cap = s == CAPTURED;
+    bool need = s != CAPTURED && s != ESCAPED;
+    const int64_t i = idx / n, j = idx - (idx / n) * n;
+    for (int di = -band_px; di <= band_px && !need; ++di) {
+        const int64_t ii = i + di;
+        if (ii < 0 || ii >= n) continue;
+        for (int dj = -band_px; dj <= band_px; ++dj) {
+            const int64_t jj = j + dj;
+            if (jj < 0 || jj >= n) continue;
+            if ((state[ii * n + jj] == CAPTURED) != cap) {
+                need = true;
+                break;
+            }
+        }
+    }
+    if (need) {
+        repair[atomicAdd(&c->repair, 1ull)] = idx;
+        atomicAdd(&c->pad, 1ull);
+    }
+}
+
+__global__ void repair_reset_kernel(PlanCounters* c) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) c->repair = 0ull;
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void repair_kernel(
+    Params P, BardeenImage B, const int64_t* repair, const PlanCounters* c, uint8_t* used_fp64, RowsOut<double> out) {
+    const int64_t count = static_cast<int64_t>(c->repair);
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < count; p += stride) {
+        const int64_t idx = repair[p];
+        const int64_t i = idx / B.n, j = idx - i * B.n;
+        double y0[8];
+        bardeen_state(B, pixel_centre(B.fov, j, B.n), -pixel_centre(B.fov, i, B.n), y0);
+        RayState<double> R;
+        ray_init<double, true>(P, y0, R);
+        while (R.state == RUNNING) ray_attempt<double, true, false>(P, R);
+        store_row(out, idx, R);
+        used_fp64[idx] = 1;
+    }
+}
+
+__global__ void planned_reset_kernel(PlanCounters* c, int32_t* flags) {
+    if (blockIdx.x == 0 && threadIdx.x < sizeof(PlanCounters) / 8)
+        reinterpret_cast<unsigned long long*>(c)[threadIdx.x] = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && flags) *flags = 0;
+}
+
+}  // namespace kr
+
+using namespace kr;
+
+namespace {
+
+struct Work {
+    PlanCounters* counters;
+    int32_t* flags;
+    double* radii;
+    int64_t* work;
+    int64_t* repair;
+    uint8_t* used_fp64;
+};
+
+size_t workspace_bytes(int64_t pixels, int32_t n_angles) {
+    host::Carver c;
+    c.take(sizeof(PlanCounters));
+    c.take(64);
+    c.take(static_cast<int64_t>(n_angles) * 8);
+    c.take(pixels * 8);
+    c.take(pixels * 8);
+    c.take(pixels);
+    return c.total;
+}
+
+Work carve_work(void* base, int64_t pixels, int32_t n_angles) {
+    host::Carver c;
+    char* b = static_cast<char*>(base);
+    Work w;
+    w.counters = reinterpret_cast<PlanCounters*>(b + c.take(sizeof(PlanCounters)));
+    w.flags = reinterpret_cast<int32_t*>(b + c.take(64));
+    w.radii = reinterpret_cast<double*>(b + c.take(static_cast<int64_t>(n_angles) * 8));
+    w.work = reinterpret_cast<int64_t*>(b + c.take(pixels * 8));
+    w.repair = reinterpret_cast<int64_t*>(b + c.take(pixels * 8));
+    w.used_fp64 = reinterpret_cast<uint8_t*>(b + c.take(pixels));
+    return w;
+}
+
+kr_status check_opts(const kr_planned_opts* o) {
+    if (!o) return host::fail(KR_ERR_INVALID, "options are NULL");
+    if (o->band_px < 0 || o->band_px > 64) return host::fail(KR_ERR_INVALID, "band_px must be in [0, 64]");
+    if (o->refill_interval < 1 || o->refill_interval > 4096)
+        return host::fail(KR_ERR_INVALID, "refill_interval must be in [1, 4096]");
+    if (o->block_size != 64 && o->block_size != 128 && o->block_size != 256)
+        return host::fail(KR_ERR_INVALID, "block_size must be 64, 128 or 256");
+    if (!(o->horizon_margin >= 0.0) || !(o->axis_sin >= 0.0) || !(o->turn_drdl >= 0.0) || !(o->turn_rmax >= 0.0))
+        return host::fail(KR_ERR_INVALID, "switch thresholds must be >= 0");
+    return KR_OK;
+}
+
+template <int BLOCK, bool MIXED>
+void launch_planned_t(int grid, hipStream_t s, const Params& P, const BardeenImage& B, const MixedCfg& M,
+                      const Work& w, int refill, const RowsOut<double>& o) {
+    hipLaunchKernelGGL((planned_trace_kernel<BLOCK, MIXED>), dim3(grid), dim3(BLOCK), 0, s, P, B, M, w.work,
+                       w.counters, refill, w.used_fp64, o, w.flags);
+}
+
+kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B, const kr_planned_opts* o,
+                          double ca, double cb, int32_t n_angles, const Work& w, const kr_outputs& d_out,
+                          hipStream_t s) {
+    const int64_t pixels = B.n * B.n;
+    int cus = 0;
+    KR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+    const unsigned pblocks = static_cast<unsigned>((pixels + 255) / 256);
+    PlanDev pl{ca, cb, w.radii, n_angles, (o->band_px + 1.0) * (2.0 * B.fov / static_cast<double>(B.n))};
+    hipLaunchKernelGGL(planned_reset_kernel, dim3(1), dim3(64), 0, s, w.counters, w.flags);
+    KR_HIP(hipGetLastError());
+    hipLaunchKernelGGL(plan_count_kernel, dim3(pblocks), dim3(256), 0, s, pl, B, w.counters);
+    KR_HIP(hipGetLastError());
+    hipLaunchKernelGGL(plan_scatter_kernel, dim3(pblocks), dim3(256), 0, s, pl, B, w.counters, w.work,
+                       o->fp32_bulk ? 1 : 0);
+    KR_HIP(hipGetLastError());
+    const double r_plus = P.geom[G_MASS] + std::sqrt(P.geom[G_MASS] * P.geom[G_MASS] - P.geom[G_A2]);
+    MixedCfg M{r_plus + o->horizon_margin, o->axis_sin, o->turn_rmax, o->turn_drdl};
+    RowsOut<double> bo{static_cast<double*>(d_out.Y), d_out.lam, d_out.n_steps, d_out.n_rejected, d_out.state,
+                   d_out.max_null_error, d_out.max_energy_drift, d_out.max_lz_drift, d_out.max_carter_drift};
+    const int blocks_per_cu = (4 * 2 * 64 + o->block_size - 1) / o->block_size;
+    const int grid = cus * blocks_per_cu;
+    const bool mixed = o->fp32_bulk != 0;
+    switch (o->block_size) {
+        case 64:
+            mixed ? launch_planned_t<64, true>(grid, s, P, B, M, w, o->refill_interval, bo)
+                  : launch_planned_t<64, false>(grid, s, P, B, M, w, o->refill_interval, bo);
+            break;
+        case 128:
+            mixed ? launch_planned_t<128, true>(grid, s, P, B, M, w, o->refill_interval, bo)
+                  : launch_planned_t<128, false>(grid, s, P, B, M, w, o->refill_interval, bo);
+            break;
+        default:
+            mixed ? launch_planned_t<256, true>(grid, s, P, B, M, w, o->refill_interval, bo)
+                  : launch_planned_t<256, false>(grid, s, P, B, M, w, o->refill_interval, bo);
+    }
+    KR_HIP(hipGetLastError());
+    for (int round = 0; round < REPAIR_ROUNDS; ++round) {
+        hipLaunchKernelGGL(repair_reset_kernel, dim3(1), dim3(64), 0, s, w.counters);
+        KR_HIP(hipGetLastError());
+        hipLaunchKernelGGL(verify_kernel, dim3(pblocks), dim3(256), 0, s, d_out.state, w.used_fp64, B.n, o->band_px,
+                           w.repair, w.counters);
+        KR_HIP(hipGetLastError());
+        hipLaunchKernelGGL(repair_kernel, dim3(static_cast<unsigned>(cus * 2)), dim3(256), 0, s, P, B, w.repair,
+                           w.counters, w.used_fp64, bo);
+        KR_HIP(hipGetLastError());
+    }
+    return KR_OK;
+}
+
+kr_status check_common(kr_handle h, const kr_params* p, const kr_image* img, const kr_planned_opts* o,
+                       int32_t n_angles, Params* P, BardeenImage* B) {
+    kr_status st = host::bardeen_image(img, true, B);
+    if (st != KR_OK) return st;
+    if ((st = host::validate(h, p, B->n * B->n, P)) != KR_OK) return st;
+    if (p->dtype != KR_FLOAT64 || P->ipar[I_ADAPTIVE] != 1 || P->ipar[I_DISK] != 0)
+        return host::fail(KR_ERR_UNSUPPORTED, "planned traces are float64 rk45 without a disk event");
+    if ((st = check_opts(o)) != KR_OK) return st;
+    if (n_angles < 8 || n_angles > (1 << 20)) return host::fail(KR_ERR_INVALID, "n_angles must be in [8, 2^20]");
+    return KR_OK;
+}
+
+}  // namespace
+
+extern "C" size_t kr_planned_workspace_bytes(int64_t pixels, int32_t n_angles) {
+    return workspace_bytes(pixels, n_angles);
+}
+
+extern "C" kr_status kr_trace_image_planned_dev(kr_handle h, const kr_params* p, const kr_image* img,
+                                                const kr_planned_opts* o, double centre_alpha, double centre_beta,
+                                                const double* d_radii, int32_t n_angles, const kr_outputs* d_out,
+                                                void* d_work, size_t work_bytes, void* stream) {
+    host::clear_error();
+    Params P;
+    BardeenImage B;
+    kr_status st = check_common(h, p, img, o, n_angles, &P, &B);
+    if (st != KR_OK) return st;
+    if (!d_out || !d_out->Y || !d_out->state || !d_radii || !d_work)
+        return host::fail(KR_ERR_INVALID, "an output, radii or workspace pointer is NULL");
+    if (work_bytes < workspace_bytes(B.n * B.n, n_angles)) return host::fail(KR_ERR_INVALID, "workspace too small");
+    KR_HIP(hipSetDevice(h->device));
+    Work w = carve_work(d_work, B.n * B.n, n_angles);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    KR_HIP(hipMemcpyAsync(w.radii, d_radii, static_cast<size_t>(n_angles) * 8, hipMemcpyDeviceToDevice, s));
+    return enqueue_planned(h, P, B, o, centre_alpha, centre_beta, n_angles, w, *d_out, s);
+}
+
+extern "C" kr_status kr_trace_image_planned(kr_handle h, const kr_params* p, const kr_image* img,
+                                            const kr_planned_opts* o, double centre_alpha, double centre_beta,
+                                            const double* radii, int32_t n_angles, const kr_outputs* out,
+                                            kr_planned_stats* stats, double* kernel_ms) {
+    host::clear_error();
+    Params P;
+    BardeenImage B;
+    kr_status st = check_common(h, p, img, o, n_angles, &P, &B);
+    if (st != KR_OK) return st;
+    if (!out || !out->Y || !out->lam || !out->n_steps || !out->n_rejected || !out->state || !out->max_null_error ||
+        !out->max_energy_drift || !out->max_lz_drift || !out->max_carter_drift || !radii)
+        return host::fail(KR_ERR_INVALID, "an output or radii array is NULL");
+    KR_HIP(hipSetDevice(h->device));
+    const int64_t n = B.n * B.n;
+    host::Carver c;
+    const size_t oY = c.take(n * 64), olam = c.take(n * 8), ost = c.take(n * 8), orej = c.take(n * 8),
+                 ostate = c.take(n * 8), om0 = c.take(n * 8), om1 = c.take(n * 8), om2 = c.take(n * 8),
+                 om3 = c.take(n * 8), owork = c.take(static_cast<int64_t>(workspace_bytes(n, n_angles)));
+    if ((st = host::reserve(h, c.total)) != KR_OK) return st;
+    char* base = static_cast<char*>(h->buf);
+    kr_outputs d{};
+    d.Y = base + oY;
+    d.lam = reinterpret_cast<double*>(base + olam);
+    d.n_steps = reinterpret_cast<int64_t*>(base + ost);
+    d.n_rejected = reinterpret_cast<int64_t*>(base + orej);
+    d.state = reinterpret_cast<int64_t*>(base + ostate);
+    d.max_null_error = reinterpret_cast<double*>(base + om0);
+    d.max_energy_drift = reinterpret_cast<double*>(base + om1);
+    d.max_lz_drift = reinterpret_cast<double*>(base + om2);
+    d.max_carter_drift = reinterpret_cast<double*>(base + om3);
+    Work w = carve_work(base + owork, n, n_angles);
+    hipStream_t s = h->stream;
+    KR_HIP(hipMemcpyAsync(w.radii, radii, static_cast<size_t>(n_angles) * 8, hipMemcpyHostToDevice, s));
+    KR_HIP(hipEventRecord(h->ev0, s));
+    if ((st = enqueue_planned(h, P, B, o, centre_alpha, centre_beta, n_angles, w, d, s)) != KR_OK) return st;
+    KR_HIP(hipEventRecord(h->ev1, s));
+    PlanCounters pc;
+    int32_t flags = 0;
+    KR_HIP(hipMemcpyAsync(&pc, w.counters, sizeof(pc), hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(&flags, w.flags, 4, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(out->Y, d.Y, n * 64, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(out->lam, d.lam, n * 8, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(out->n_steps, d.n_steps, n * 8, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(out->n_rejected, d.n_rejected, n * 8, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(out->state, d.state, n * 8, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(out->max_null_error, d.max_null_error, n * 8, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(out->max_energy_drift, d.max_energy_drift, n * 8, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(out->max_lz_drift, d.max_lz_drift, n * 8, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(out->max_carter_drift, d.max_carter_drift, n * 8, hipMemcpyDeviceToHost, s));
+    KR_HIP(hipStreamSynchronize(s));
+    if (flags & KR_FLAG_RADICAND)
+        return host::fail(KR_ERR_RADICAND, "a Carter radicand is negative beyond round-off at the observer");
+    if (stats) {
+        stats->planned_band = static_cast<int64_t>(pc.seg[0]);
+        stats->planned_inside = static_cast<int64_t>(pc.seg[1]);
+        stats->planned_outside = static_cast<int64_t>(pc.seg[2]);
+        stats->repaired = static_cast<int64_t>(pc.pad);
+        stats->fp64_attempts = static_cast<int64_t>(pc.fp64_attempts);
+    }
+    if (kernel_ms) {
+        float ms = 0.f;
+        KR_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        *kernel_ms = ms;
+    }
+    return KR_OK;
+}

@@ -74,8 +74,10 @@ __device__ __forceinline__ bool image_pixel(int64_t n, int64_t& i, int64_t& j) {
     return i < n && j < n;
 }
 
+// Two wavefronts per SIMD: the register budget (256 VGPRs) the per-photon
+// state and the seven Dormand-Prince stages fit in without spilling.
 template <typename T, bool ADAPTIVE, bool DISK, int IC>
-__global__ __launch_bounds__(256) void trace_kernel(Params P, ICArgs ic, Outputs<T> out, int64_t n) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void trace_kernel(Params P, ICArgs ic, Outputs<T> out, int64_t n) {
     int64_t idx;
     double y0[8];
     if (IC == IC_IMAGE) {
@@ -108,7 +110,7 @@ __device__ __forceinline__ double probe_radius(double lo, double hi, int j, int 
 }
 
 template <typename T, bool ADAPTIVE>
-__global__ __launch_bounds__(256) void bisect_probe_kernel(Params P, BardeenImage B, double ca, double cb,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void bisect_probe_kernel(Params P, BardeenImage B, double ca, double cb,
                                                           const double* cosk, const double* sink, const double* lo,
                                                           const double* hi, int64_t n_angles, int n_probe,
                                                           uint8_t* escaped, unsigned long long* failures,

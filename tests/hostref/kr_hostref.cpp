@@ -9,6 +9,7 @@
 #include "kerrray_hip.h"
 #include "kr_device.h"
 #include "kr_image.h"
+#include <cmath>
 
 namespace {
 
@@ -92,4 +93,39 @@ extern "C" int krh_trace_image(const kr_params* p, const kr_image* img, const kr
     if (p->dtype == KR_FLOAT64) ad ? run_image<double, true>(P, B, o, nt, &bad) : run_image<double, false>(P, B, o, nt, &bad);
     else ad ? run_image<float, true>(P, B, o, nt, &bad) : run_image<float, false>(P, B, o, nt, &bad);
     return bad ? 3 : 0;
+}
+
+#include "kr_mixed.h"
+
+// Every photon of an image with the FP32-bulk stepper (no band, no repair):
+// the CPU view of what the mixed attempt does to accuracy.
+extern "C" int krh_trace_image_mixed(const kr_params* p, const kr_image* img, double horizon_margin,
+                                     double axis_sin, double turn_rmax, double turn_drdl, const kr_outputs* o,
+                                     int64_t* fp64_attempts, int nt) {
+    kr::BardeenImage B;
+    if (kr::bardeen_image_scalars(img, true, &B)) return 1;
+    const kr::Params P = params_of(p);
+    if (nt <= 0) nt = omp_get_max_threads();
+    const double r_plus = P.geom[kr::G_MASS] + std::sqrt(P.geom[kr::G_MASS] * P.geom[kr::G_MASS] - P.geom[kr::G_A2]);
+    const kr::MixedCfg M{r_plus + horizon_margin, axis_sin, turn_rmax, turn_drdl};
+    const kr::Geom32 G = kr::geom32(P);
+    const int64_t n = B.n;
+    int64_t total_fp64 = 0;
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 16) reduction(+ : total_fp64)
+    for (int64_t idx = 0; idx < n * n; ++idx) {
+        const int64_t i = idx / n, j = idx % n;
+        double y0[8];
+        kr::bardeen_state(B, kr::pixel_centre(B.fov, j, n), -kr::pixel_centre(B.fov, i, n), y0);
+        kr::RayState<double> R;
+        kr::ray_init<double, true>(P, y0, R);
+        int64_t f64 = 0;
+        const kr::MixedStepper step{G, M, &f64};
+        while (R.state == kr::RUNNING) kr::ray_attempt_with<double, true, false>(P, R, step);
+        kr::RayResult<double> r;
+        kr::ray_result<double, false>(R, r);
+        store<double, true, false>(r, o, idx);
+        total_fp64 += f64;
+    }
+    *fp64_attempts = total_fp64;
+    return 0;
 }

@@ -1,0 +1,57 @@
+"""Planned traces: the FP64 schedule changes nothing; the FP32 bulk with FP64 repair keeps the FP64 mask."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+from conftest import options
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("Y", "state", "n_steps", "n_rejected", "lam", "max_null_error", "max_energy_drift", "max_lz_drift",
+          "max_carter_drift")
+
+
+def _images(hip, spin, incl, res, rtol, opts):
+    from kerrray_amd.camera import BardeenCamera
+    from kerrray_amd.geometry import Spacetime
+    from kerrray_amd.planned import compute_shadow_planned
+    from kerrray_amd.shadow import compute_shadow
+
+    st = Spacetime(1.0, spin)
+    integ, term = options(rtol=rtol, atol=1e-2 * rtol)
+    cam = BardeenCamera(radius=1000.0, inclination_deg=incl, fov=10.0, resolution=res)
+    plain = compute_shadow(st, cam, integ, term, backend=hip)
+    plain_ms = hip.last_kernel_ms
+    planned, stats = compute_shadow_planned(st, cam, integ, term, backend=hip, options=opts)
+    return plain, plain_ms, planned, stats
+
+
+def test_fp64_planned_schedule_is_bitwise_the_plain_trace(hip, golden2) -> None:
+    from kerrray_amd.planned import PlannedOptions
+
+    plain, plain_ms, planned, stats = _images(hip, 0.9, 60.0, 512, 1e-6, PlannedOptions(fp32_bulk=False))
+    print(f"[fp64 planned 512^2] {stats.kernel_ms:.3f} ms (plain {plain_ms:.3f} ms), band {stats.planned_band}, "
+          f"repaired {stats.repaired}", flush=True)
+    for name in FIELDS:
+        assert np.array_equal(getattr(planned.result, name), getattr(plain.result, name)), name
+    assert stats.repaired == 0 and stats.fp64_fraction == 1.0
+    assert np.array_equal(planned.result.state, golden2["numpy_state"].astype(np.int64))
+
+
+@pytest.mark.parametrize("case", [(0.9, 60.0, 512, 1e-6), (0.99, 85.0, 1024, 1e-6), (0.5, 17.0, 256, 1e-7)])
+def test_fp32_bulk_with_fp64_repair_keeps_the_fp64_mask(hip, case) -> None:
+    from kerrray_amd.planned import PlannedOptions
+
+    spin, incl, res, rtol = case
+    plain, plain_ms, mixed, stats = _images(hip, spin, incl, res, rtol, PlannedOptions(fp32_bulk=True))
+    diff = int(np.count_nonzero(mixed.captured != plain.captured))
+    esc = (plain.result.state == 1) & (mixed.result.state == 1)
+    theta_err = np.abs(mixed.result.Y[esc, 2] - plain.result.Y[esc, 2])
+    print(f"[fp32+fp64 a={spin} i={incl} {res}^2 rtol {rtol}] {stats.kernel_ms:.3f} ms (fp64 plain {plain_ms:.3f} ms) "
+          f"mask diff {diff}, band {stats.planned_band}, repaired {stats.repaired}, "
+          f"fp64 attempts {stats.fp64_fraction:.3f}, escaped theta |err| median {np.median(theta_err):.2e} "
+          f"max {theta_err.max():.2e}", flush=True)
+    assert diff == 0
+    assert np.array_equal(mixed.result.state, plain.result.state)
+    assert mixed.n_failed == 0
