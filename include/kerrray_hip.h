@@ -221,7 +221,11 @@ kr_status kr_batch_destroy(kr_batch b);
  *    error norm and step control, switching an attempt to FP64 when
  *    r < r_+ + horizon_margin, |sin theta| < axis_sin, or r < turn_rmax and
  *    |dr/dlambda| < turn_drdl.
- *  refill_interval, block_size: launch shape of the persistent kernel. */
+ *  refill_interval, block_size: launch shape of the persistent kernel.
+ *  longest_first: order of the work list (the classes still decide the
+ *    precision).
+ *  band_priority: raise the issue priority of wavefronts holding an edge-band
+ *    photon (the longest ones), which otherwise finish last. */
 typedef struct kr_planned_opts {
     double horizon_margin;
     double axis_sin;
@@ -231,6 +235,8 @@ typedef struct kr_planned_opts {
     int32_t fp32_bulk;
     int32_t refill_interval;
     int32_t block_size;
+    int32_t longest_first; /* 1: work list edge band -> inside -> outside; 0: row-major */
+    int32_t band_priority; /* 1: wavefronts tracing an edge-band photon issue first on their SIMD */
 } kr_planned_opts;
 
 typedef struct kr_planned_stats {

@@ -116,6 +116,8 @@ class KrPlannedOpts(ctypes.Structure):
         ("fp32_bulk", ctypes.c_int32),
         ("refill_interval", ctypes.c_int32),
         ("block_size", ctypes.c_int32),
+        ("longest_first", ctypes.c_int32),
+        ("band_priority", ctypes.c_int32),
     ]
 
 

@@ -34,7 +34,9 @@
 
 namespace kr {
 
-constexpr int64_t FP64_FLAG = int64_t(1) << 62;
+constexpr int64_t FP64_FLAG = int64_t(1) << 62;  // trace this photon with the exact FP64 attempt
+constexpr int64_t BAND_FLAG = int64_t(1) << 61;  // photon of the predicted edge band
+constexpr int64_t INDEX_MASK = BAND_FLAG - 1;
 
 struct PlanDev {
     double ca, cb;          // centre of the polar radii table
@@ -73,31 +75,53 @@ __device__ __forceinline__ int pixel_class(const PlanDev& pl, const BardeenImage
     return rho < rc ? 1 : 2;
 }
 
-__global__ void plan_count_kernel(PlanDev pl, BardeenImage B, PlanCounters* c) {
-    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= B.n * B.n) return;
-    atomicAdd(&c->seg[pixel_class(pl, B, idx)], 1ull);
+// Wavefront-aggregated append: one atomic per wavefront and class instead of
+// one per pixel (a same-address atomic from every lane serialises at the
+// L2); returns the lane's slot among the class's lanes of this wavefront.
+__device__ __forceinline__ unsigned long long wave_append(unsigned long long* counter, bool mine) {
+    const uint64_t m = __ballot(mine);
+    if (!m) return 0ull;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(counter, static_cast<unsigned long long>(__popcll(m)));
+    base = __shfl(base, leader);
+    const uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
+    return base + static_cast<unsigned long long>(__popcll(below));
 }
 
-__global__ void plan_scatter_kernel(PlanDev pl, BardeenImage B, PlanCounters* c, int64_t* work, int fp32_bulk) {
+__global__ void plan_count_kernel(PlanDev pl, BardeenImage B, PlanCounters* c) {
     const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= B.n * B.n) return;
-    const int k = pixel_class(pl, B, idx);
-    const unsigned long long base = k == 0 ? 0ull : (k == 1 ? c->seg[0] : c->seg[0] + c->seg[1]);
-    const unsigned long long pos = base + atomicAdd(&c->cursor[k], 1ull);
-    work[pos] = (k == 0 || !fp32_bulk) ? (idx | FP64_FLAG) : idx;
+    const bool valid = idx < B.n * B.n;
+    const int k = valid ? pixel_class(pl, B, idx) : -1;
+    for (int cls = 0; cls < 3; ++cls) wave_append(&c->seg[cls], k == cls);
+}
+
+__global__ void plan_scatter_kernel(PlanDev pl, BardeenImage B, PlanCounters* c, int64_t* work, int fp32_bulk,
+                                    int longest_first) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool valid = idx < B.n * B.n;
+    const int k = valid ? pixel_class(pl, B, idx) : -1;
+    unsigned long long pos = static_cast<unsigned long long>(idx);
+    if (longest_first) {
+        for (int cls = 0; cls < 3; ++cls) {
+            const unsigned long long slot = wave_append(&c->cursor[cls], k == cls);
+            if (k == cls) pos = (cls == 0 ? 0ull : (cls == 1 ? c->seg[0] : c->seg[0] + c->seg[1])) + slot;
+        }
+    }
+    if (valid) work[pos] = idx | ((k == 0 || !fp32_bulk) ? FP64_FLAG : 0) | (k == 0 ? BAND_FLAG : 0);
 }
 
 template <int BLOCK, bool MIXED>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) void planned_trace_kernel(
     Params P, BardeenImage B, MixedCfg M, const int64_t* work, PlanCounters* c, int refill_interval,
-    uint8_t* used_fp64, RowsOut<double> out, int32_t* flags) {
+    int band_priority, uint8_t* used_fp64, RowsOut<double> out, int32_t* flags) {
     const int lane = threadIdx.x & 63;
     const int64_t total = B.n * B.n;
     const Geom32 G = geom32(P);
     RayState<double> R;
     R.state = RUNNING;
-    bool busy = false, exhausted = false, exact = true;
+    bool busy = false, exhausted = false, exact = true, band = false;
     int64_t idx = -1;
     int64_t fp64_att = 0;
     int since = refill_interval;
@@ -119,8 +143,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) v
                     const int64_t pos = static_cast<int64_t>(base) + __popcll(below);
                     if (pos < total) {
                         const int64_t w = work[pos];
-                        idx = w & (FP64_FLAG - 1);
+                        idx = w & INDEX_MASK;
                         exact = !MIXED || (w & FP64_FLAG) != 0;
+                        band = (w & BAND_FLAG) != 0;
                         const int64_t i = idx / B.n, j = idx - i * B.n;
                         double y0[8];
                         if (!bardeen_state(B, pixel_centre(B.fov, j, B.n), -pixel_centre(B.fov, i, B.n), y0) && flags)
@@ -135,6 +160,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) v
                         }
                     }
                 }
+            }
+            // Edge-band photons are the longest and set the end of the launch:
+            // a wavefront holding one issues ahead of the others on its SIMD.
+            if (band_priority) {
+                if (__any(busy && band))
+                    __builtin_amdgcn_s_setprio(2);
+                else
+                    __builtin_amdgcn_s_setprio(0);
             }
             continue;
         }
@@ -265,9 +298,9 @@ kr_status check_opts(const kr_planned_opts* o) {
 
 template <int BLOCK, bool MIXED>
 void launch_planned_t(int grid, hipStream_t s, const Params& P, const BardeenImage& B, const MixedCfg& M,
-                      const Work& w, int refill, const RowsOut<double>& o) {
+                      const Work& w, int refill, int band_priority, const RowsOut<double>& o) {
     hipLaunchKernelGGL((planned_trace_kernel<BLOCK, MIXED>), dim3(grid), dim3(BLOCK), 0, s, P, B, M, w.work,
-                       w.counters, refill, w.used_fp64, o, w.flags);
+                       w.counters, refill, band_priority, w.used_fp64, o, w.flags);
 }
 
 kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B, const kr_planned_opts* o,
@@ -283,7 +316,7 @@ kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B,
     hipLaunchKernelGGL(plan_count_kernel, dim3(pblocks), dim3(256), 0, s, pl, B, w.counters);
     KR_HIP(hipGetLastError());
     hipLaunchKernelGGL(plan_scatter_kernel, dim3(pblocks), dim3(256), 0, s, pl, B, w.counters, w.work,
-                       o->fp32_bulk ? 1 : 0);
+                       o->fp32_bulk ? 1 : 0, o->longest_first ? 1 : 0);
     KR_HIP(hipGetLastError());
     const double r_plus = P.geom[G_MASS] + std::sqrt(P.geom[G_MASS] * P.geom[G_MASS] - P.geom[G_A2]);
     MixedCfg M{r_plus + o->horizon_margin, o->axis_sin, o->turn_rmax, o->turn_drdl};
@@ -292,18 +325,19 @@ kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B,
     const int blocks_per_cu = (4 * 2 * 64 + o->block_size - 1) / o->block_size;
     const int grid = cus * blocks_per_cu;
     const bool mixed = o->fp32_bulk != 0;
+    const int refill = o->refill_interval, prio = o->band_priority ? 1 : 0;
     switch (o->block_size) {
         case 64:
-            mixed ? launch_planned_t<64, true>(grid, s, P, B, M, w, o->refill_interval, bo)
-                  : launch_planned_t<64, false>(grid, s, P, B, M, w, o->refill_interval, bo);
+            mixed ? launch_planned_t<64, true>(grid, s, P, B, M, w, refill, prio, bo)
+                  : launch_planned_t<64, false>(grid, s, P, B, M, w, refill, prio, bo);
             break;
         case 128:
-            mixed ? launch_planned_t<128, true>(grid, s, P, B, M, w, o->refill_interval, bo)
-                  : launch_planned_t<128, false>(grid, s, P, B, M, w, o->refill_interval, bo);
+            mixed ? launch_planned_t<128, true>(grid, s, P, B, M, w, refill, prio, bo)
+                  : launch_planned_t<128, false>(grid, s, P, B, M, w, refill, prio, bo);
             break;
         default:
-            mixed ? launch_planned_t<256, true>(grid, s, P, B, M, w, o->refill_interval, bo)
-                  : launch_planned_t<256, false>(grid, s, P, B, M, w, o->refill_interval, bo);
+            mixed ? launch_planned_t<256, true>(grid, s, P, B, M, w, refill, prio, bo)
+                  : launch_planned_t<256, false>(grid, s, P, B, M, w, refill, prio, bo);
     }
     KR_HIP(hipGetLastError());
     for (int round = 0; round < REPAIR_ROUNDS; ++round) {

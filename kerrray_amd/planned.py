@@ -47,6 +47,8 @@ class PlannedOptions:
     turn_drdl: float = 1e-3
     refill_interval: int = 8
     block_size: int = 128
+    longest_first: bool = True
+    band_priority: bool = False
 
     def kr(self) -> _lib.KrPlannedOpts:
         o = _lib.KrPlannedOpts()
@@ -58,6 +60,8 @@ class PlannedOptions:
         o.fp32_bulk = 1 if self.fp32_bulk else 0
         o.refill_interval = self.refill_interval
         o.block_size = self.block_size
+        o.longest_first = 1 if self.longest_first else 0
+        o.band_priority = 1 if self.band_priority else 0
         return o
 
 
