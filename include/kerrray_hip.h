@@ -263,6 +263,45 @@ kr_status kr_trace_image_planned_dev(kr_handle h, const kr_params* p, const kr_i
                                      int32_t n_angles, const kr_outputs* d_out, void* d_work, size_t work_bytes,
                                      void* stream);
 
+/* ---------------------------------------------------------------------
+ * Differentiation form: escape-direction gradients by the discrete adjoint
+ * of Dormand-Prince 5(4).  No counterpart in the reference (which has no
+ * derivatives of the ray map); inputs are image-plane points as for
+ * kr_trace_points_dev.
+ * ------------------------------------------------------------------- */
+typedef struct kr_tape_s* kr_tape;
+
+/* Traces the n points (alpha, beta) of image `img` in FP64 (adaptive DP5,
+ * no events) into d_out, exactly as kr_trace_points_dev, and records for
+ * every escaped photon its accepted step sizes and a checkpoint of its state
+ * every `segment` accepted steps.  d_theta_inf / d_phi_inf (n each) receive
+ * the escape direction at r = infinity: (theta, phi) carried from the final
+ * state to u = 1/r = 0 along the geodesic (NaN for photons that did not
+ * escape).  The map differentiated later is the sequence of accepted steps
+ * with those step sizes frozen, followed by that tail.  p's geometry must be
+ * the image's (mass, a = spin * mass).  Synchronous; the tape owns copies of
+ * alpha, beta and the final states. */
+kr_status kr_tape_create(kr_handle h, const kr_params* p, const kr_image* img, const double* d_alpha,
+                         const double* d_beta, int64_t n, int32_t segment, const kr_outputs* d_out,
+                         double* d_theta_inf, double* d_phi_inf, kr_tape* tape);
+/* Photons on the tape (the escaped ones), accepted steps stored, and the
+ * recording kernel's time; -1 for a NULL tape. */
+int64_t kr_tape_recorded(kr_tape t);
+int64_t kr_tape_steps(kr_tape t);
+double kr_tape_record_ms(kr_tape t);
+/* Vector-Jacobian product: for every point i, d_grad[4 i + j] =
+ * g_theta[i] d theta_inf / d x_j + g_phi[i] d phi_inf / d x_j with
+ * x = (alpha, beta, spin, theta_o); 0 for photons that did not escape.
+ * Device arrays, enqueued on `stream` (NULL is the default stream, as for
+ * the other _dev calls).  Calls on one tape must not overlap (they share its
+ * segment buffer). */
+kr_status kr_tape_vjp(kr_tape t, const double* d_g_theta, const double* d_g_phi, double* d_grad, void* stream);
+/* Flags of the tape's runs (synchronises the device): bit 8 = a
+ * re-integration differed from the trace, bit 9 = a recomputed final state
+ * differed from the traced one. */
+kr_status kr_tape_flags(kr_tape t, int32_t* flags);
+kr_status kr_tape_destroy(kr_tape t);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,13 @@ EXPORTED = (
     "kr_planned_workspace_bytes",
     "kr_trace_image_planned",
     "kr_trace_image_planned_dev",
+    "kr_tape_create",
+    "kr_tape_recorded",
+    "kr_tape_steps",
+    "kr_tape_record_ms",
+    "kr_tape_vjp",
+    "kr_tape_flags",
+    "kr_tape_destroy",
 )
 
 
@@ -194,6 +201,21 @@ def _declare(L: ctypes.CDLL) -> None:
     L.kr_trace_image_planned_dev.argtypes = [vp, P(KrParams), P(KrImage), P(KrPlannedOpts), ctypes.c_double,
                                              ctypes.c_double, vp, ctypes.c_int32, P(KrOutputs), vp, ctypes.c_size_t,
                                              vp]
+    L.kr_tape_create.restype = ctypes.c_int
+    L.kr_tape_create.argtypes = [vp, P(KrParams), P(KrImage), vp, vp, ctypes.c_int64, ctypes.c_int32, P(KrOutputs),
+                                 vp, vp, P(vp)]
+    L.kr_tape_recorded.restype = ctypes.c_int64
+    L.kr_tape_recorded.argtypes = [vp]
+    L.kr_tape_steps.restype = ctypes.c_int64
+    L.kr_tape_steps.argtypes = [vp]
+    L.kr_tape_record_ms.restype = ctypes.c_double
+    L.kr_tape_record_ms.argtypes = [vp]
+    L.kr_tape_vjp.restype = ctypes.c_int
+    L.kr_tape_vjp.argtypes = [vp, vp, vp, vp, vp]
+    L.kr_tape_flags.restype = ctypes.c_int
+    L.kr_tape_flags.argtypes = [vp, P(ctypes.c_int32)]
+    L.kr_tape_destroy.restype = ctypes.c_int
+    L.kr_tape_destroy.argtypes = [vp]
 
 
 def _share_torch_runtime() -> None:

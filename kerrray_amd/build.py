@@ -28,7 +28,7 @@ LIB_NAME = "libkrhip.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 ARCH = "gfx950"
 
-SOURCES = ["kr_trace.hip", "kr_batch.hip", "kr_mixed.hip"]
+SOURCES = ["kr_trace.hip", "kr_batch.hip", "kr_mixed.hip", "kr_adjoint.hip"]
 
 
 def hipcc() -> str:

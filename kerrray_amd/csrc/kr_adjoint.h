@@ -1,0 +1,428 @@
+// Discrete adjoint of the FP64 Dormand-Prince 5(4) photon path.
+//
+// The map differentiated is the one the adaptive integrator actually
+// applied: the sequence of accepted steps y_{k+1} = Phi(y_k; h_k, a) with
+// every step size h_k frozen at the value it had (rejected attempts leave no
+// trace in it), started from the Bardeen image-plane state
+// y_0(alpha, beta, a, theta_o) (benchmarks/rayset.py:158-165,
+// geodesics/initial_conditions.py:50-112).  For the escape direction
+// (theta_N, phi_N) of an escaping photon it returns the gradient of
+// g_theta theta_N + g_phi phi_N with respect to (alpha, beta, spin, theta_o).
+//
+// Reverse sweep of one step (explicit Runge-Kutta, stages Y_i =
+// y + h sum_{j<i} a_ij k_j, k_i = f(Y_i), y_new = y + h sum_i b_i k_i):
+//   kbar_i = h b_i ybar_new,  ybar = ybar_new,
+//   for i = 5 .. 0:  Ybar_i = J_f(Y_i)^T kbar_i;  ybar += Ybar_i;
+//                    kbar_j += h a_ij Ybar_i (j < i);
+//                    (p_t, p_phi, a)bar += (df/d(p_t, p_phi, a))^T kbar_i.
+// J_f^T v is formed from forward-mode dual numbers in (r, theta, a) through
+// the inverse metric (the same 15 terms as numba_kernel.py:111-154, with the
+// spin a variable) and the closed-form momentum derivatives of the
+// Hamiltonian right-hand side (numba_kernel.py:158-197).
+//
+// HIP-free: the kernels include it, and so does the host build of the
+// kernel source used by the tests.
+#pragma once
+
+#include "kr_device.h"
+
+namespace kr {
+namespace ad {
+
+// Forward-mode dual number with N tangent directions.
+template <int N>
+struct Dual {
+    double v;
+    double d[N];
+};
+
+template <int N>
+KR_HD Dual<N> seed(double v, int k) {
+    Dual<N> x;
+    x.v = v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) x.d[i] = (i == k) ? 1.0 : 0.0;
+    return x;
+}
+
+template <int N>
+KR_HD Dual<N> constant(double v) {
+    return seed<N>(v, -1);
+}
+
+template <int N>
+KR_HD Dual<N> operator+(const Dual<N>& x, const Dual<N>& y) {
+    Dual<N> z;
+    z.v = x.v + y.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) z.d[i] = x.d[i] + y.d[i];
+    return z;
+}
+
+template <int N>
+KR_HD Dual<N> operator-(const Dual<N>& x, const Dual<N>& y) {
+    Dual<N> z;
+    z.v = x.v - y.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) z.d[i] = x.d[i] - y.d[i];
+    return z;
+}
+
+template <int N>
+KR_HD Dual<N> operator-(const Dual<N>& x) {
+    Dual<N> z;
+    z.v = -x.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) z.d[i] = -x.d[i];
+    return z;
+}
+
+template <int N>
+KR_HD Dual<N> operator*(const Dual<N>& x, const Dual<N>& y) {
+    Dual<N> z;
+    z.v = x.v * y.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) z.d[i] = x.d[i] * y.v + x.v * y.d[i];
+    return z;
+}
+
+template <int N>
+KR_HD Dual<N> operator*(double c, const Dual<N>& x) {
+    Dual<N> z;
+    z.v = c * x.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) z.d[i] = c * x.d[i];
+    return z;
+}
+
+template <int N>
+KR_HD Dual<N> operator+(double c, const Dual<N>& x) {
+    Dual<N> z = x;
+    z.v = c + x.v;
+    return z;
+}
+
+template <int N>
+KR_HD Dual<N> operator-(const Dual<N>& x, double c) {
+    Dual<N> z = x;
+    z.v = x.v - c;
+    return z;
+}
+
+template <int N>
+KR_HD Dual<N> operator/(const Dual<N>& x, const Dual<N>& y) {
+    Dual<N> z;
+    const double inv = 1.0 / y.v;
+    z.v = x.v * inv;
+#pragma unroll
+    for (int i = 0; i < N; ++i) z.d[i] = (x.d[i] - z.v * y.d[i]) * inv;
+    return z;
+}
+
+template <int N>
+KR_HD Dual<N> operator/(double c, const Dual<N>& y) {
+    return constant<N>(c) / y;
+}
+
+template <int N>
+KR_HD Dual<N> dsin(const Dual<N>& x) {
+    Dual<N> z;
+    const double s = sin(x.v), c = cos(x.v);
+    z.v = s;
+#pragma unroll
+    for (int i = 0; i < N; ++i) z.d[i] = c * x.d[i];
+    return z;
+}
+
+template <int N>
+KR_HD Dual<N> dcos(const Dual<N>& x) {
+    Dual<N> z;
+    const double s = sin(x.v), c = cos(x.v);
+    z.v = c;
+#pragma unroll
+    for (int i = 0; i < N; ++i) z.d[i] = -s * x.d[i];
+    return z;
+}
+
+template <int N>
+KR_HD Dual<N> dsqrt(const Dual<N>& x) {
+    Dual<N> z;
+    z.v = sqrt(x.v);
+    const double k = z.v > 0.0 ? 0.5 / z.v : 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) z.d[i] = k * x.d[i];
+    return z;
+}
+
+KR_HD double dsin(double x) { return sin(x); }
+KR_HD double dcos(double x) { return cos(x); }
+KR_HD double dsqrt(double x) { return sqrt(x); }
+
+// The inverse Kerr metric and its r and theta derivatives (the 15 terms of
+// numba_kernel.py:111-154, in the same order) with the spin a a variable.
+template <class S>
+KR_HD void metric_terms(const S& r, const S& th, const S& a, double m, S (&g)[15]) {
+    const S s = dsin(th);
+    const S c = dcos(th);
+    const S s2 = s * s;
+    const S sin2th = dsin(2.0 * th);
+    const S r2 = r * r;
+    const S a2 = a * a;
+    const S sig = r2 + a2 * (c * c);
+    const S dlt = r2 - (2.0 * m) * r + a2;
+    const S r2a2 = r2 + a2;
+    const S big_a = r2a2 * r2a2 - a2 * dlt * s2;
+    const S sd = sig * dlt;
+    const S ma2 = (2.0 * m) * a;
+    g[0] = -big_a / sd;
+    g[1] = -(ma2 * r) / sd;
+    g[2] = dlt / sig;
+    g[3] = 1.0 / sig;
+    g[4] = (dlt - a2 * s2) / (sd * s2);
+    const S sig_r = 2.0 * r;
+    const S sig_th = -(a2 * sin2th);
+    const S dlt_r = 2.0 * (r - m);
+    const S a_r = 4.0 * (r * r2a2) - 2.0 * (a2 * (r - m) * s2);
+    const S a_th = -(a2 * dlt * sin2th);
+    const S sd_r = sig_r * dlt + sig * dlt_r;
+    const S sd_th = sig_th * dlt;
+    const S sd2 = sd * sd;
+    const S sig2 = sig * sig;
+    g[5] = -(a_r * sd - big_a * sd_r) / sd2;
+    g[6] = -(ma2 * (sd - r * sd_r)) / sd2;
+    g[7] = (dlt_r * sig - dlt * sig_r) / sig2;
+    g[8] = -sig_r / sig2;
+    g[9] = -sig_r / (sig2 * s2) + a2 * sd_r / sd2;
+    g[10] = -(a_th * sd - big_a * sd_th) / sd2;
+    g[11] = ma2 * r * sd_th / sd2;
+    g[12] = -(dlt * sig_th) / sig2;
+    g[13] = -sig_th / sig2;
+    g[14] = -(sig_th * s2 + sig * sin2th) / (sig2 * (s2 * s2)) + a2 * sd_th / sd2;
+}
+
+// v^T df, for the right-hand side f(y; p_t, p_phi, a) of the six integrated
+// components (t, r, theta, phi, p_r, p_theta) at the state Y.  Accumulates
+// into ybar (six components), ptbar, pphbar and abar.
+KR_HD void rhs_vjp(double m, double a, const double (&Y)[NA], double pt, double pph, const double (&v)[NA],
+                   double (&ybar)[NA], double& ptbar, double& pphbar, double& abar) {
+    Dual<3> g[15];
+    metric_terms(seed<3>(Y[1], 0), seed<3>(Y[2], 1), seed<3>(a, 2), m, g);
+    if (pph == 0.0) {  // polar-axis guard of the primal right-hand side
+        g[4] = constant<3>(0.0);
+        g[9] = constant<3>(0.0);
+        g[14] = constant<3>(0.0);
+    }
+    const double pr = Y[4], pth = Y[5];
+    const double pt2 = pt * pt, ptpph = pt * pph, pr2 = pr * pr, pth2 = pth * pth, pph2 = pph * pph;
+    double dir[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double f0 = g[0].d[k] * pt + g[1].d[k] * pph;
+        const double f1 = g[2].d[k] * pr;
+        const double f2 = g[3].d[k] * pth;
+        const double f3 = g[1].d[k] * pt + g[4].d[k] * pph;
+        const double f4 = -0.5 * (g[5].d[k] * pt2 + 2.0 * g[6].d[k] * ptpph + g[7].d[k] * pr2 + g[8].d[k] * pth2 +
+                                  g[9].d[k] * pph2);
+        const double f5 = -0.5 * (g[10].d[k] * pt2 + 2.0 * g[11].d[k] * ptpph + g[12].d[k] * pr2 +
+                                  g[13].d[k] * pth2 + g[14].d[k] * pph2);
+        dir[k] = v[0] * f0 + v[1] * f1 + v[2] * f2 + v[3] * f3 + v[4] * f4 + v[5] * f5;
+    }
+    ybar[1] += dir[0];
+    ybar[2] += dir[1];
+    abar += dir[2];
+    ybar[4] += v[1] * g[2].v - v[4] * g[7].v * pr - v[5] * g[12].v * pr;
+    ybar[5] += v[2] * g[3].v - v[4] * g[8].v * pth - v[5] * g[13].v * pth;
+    ptbar += v[0] * g[0].v + v[3] * g[1].v - v[4] * (g[5].v * pt + g[6].v * pph) -
+             v[5] * (g[10].v * pt + g[11].v * pph);
+    pphbar += v[0] * g[1].v + v[3] * g[4].v - v[4] * (g[6].v * pt + g[9].v * pph) -
+              v[5] * (g[11].v * pt + g[14].v * pph);
+}
+
+// One accepted step of the primal, recomputed: y <- Phi(y; h).  Uses the
+// primal's own FP64 stage code, so a recomputed trajectory reproduces the
+// recorded one bit for bit.
+KR_HD void step_forward(const Params& P, double (&y)[NA], double pt, double pph, double h) {
+    double k0[NA], y_new[NA], err[NA], k6[NA], s, c;
+    rhs(P, y, pt, pph, k0, s, c);
+    dp5_step(P, y, pt, pph, h, k0, y_new, err, k6, s, c);
+#pragma unroll
+    for (int i = 0; i < NA; ++i) y[i] = y_new[i];
+}
+
+// Reverse sweep of one accepted step at the state y with step h: ybar (the
+// cotangent of y_new on entry) becomes the cotangent of y.
+KR_HD void step_adjoint(const Params& P, double m, double a, const double (&y)[NA], double pt, double pph, double h,
+                        double (&ybar)[NA], double& ptbar, double& pphbar, double& abar) {
+    using namespace dp;
+    // Stage slopes k_0 .. k_5 of the step (k_6 does not enter y_new).
+    double k[6][NA], ys[NA], s, c;
+    rhs(P, y, pt, pph, k[0], s, c);
+    const double A[6][5] = {{0, 0, 0, 0, 0},         {a10, 0, 0, 0, 0},     {a20, a21, 0, 0, 0},
+                            {a30, a31, a32, 0, 0},   {a40, a41, a42, a43, 0}, {a50, a51, a52, a53, a54}};
+    const double b[6] = {a60, 0.0, a62, a63, a64, a65};
+#pragma unroll
+    for (int st = 1; st < 6; ++st) {
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < st; ++j) acc += A[st][j] * k[j][i];
+            ys[i] = y[i] + h * acc;
+        }
+        rhs(P, ys, pt, pph, k[st], s, c);
+    }
+    double kbar[6][NA];
+#pragma unroll
+    for (int st = 0; st < 6; ++st)
+#pragma unroll
+        for (int i = 0; i < NA; ++i) kbar[st][i] = h * b[st] * ybar[i];
+#pragma unroll
+    for (int st = 5; st >= 0; --st) {
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < st; ++j) acc += A[st][j] * k[j][i];
+            ys[i] = y[i] + h * acc;
+        }
+        double Ybar[NA] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        rhs_vjp(m, a, ys, pt, pph, kbar[st], Ybar, ptbar, pphbar, abar);
+#pragma unroll
+        for (int i = 0; i < NA; ++i) ybar[i] += Ybar[i];
+#pragma unroll
+        for (int j = 0; j < st; ++j)
+#pragma unroll
+            for (int i = 0; i < NA; ++i) kbar[j][i] += h * A[st][j] * Ybar[i];
+    }
+}
+
+// Bardeen image-plane state (benchmarks/rayset.py:158-165 with E = 1,
+// inward) as a function of (alpha, beta, a, theta_o).  p_theta is written
+// as -beta: with Theta(theta_o) = beta^2 and sign_theta = -1 for beta >= 0,
+// +1 otherwise, sign_theta sqrt(Theta) = -beta exactly.
+template <int N>
+KR_HD void bardeen_state_of(const Dual<N>& alpha, const Dual<N>& beta, const Dual<N>& a, const Dual<N>& th_o,
+                            double m, double r_o, Dual<N> (&y)[8]) {
+    using S = Dual<N>;
+    const S sin_o = dsin(th_o), cos_o = dcos(th_o);
+    const S xi = -(alpha * sin_o);
+    const S cot = cos_o / sin_o;
+    const S a2 = a * a;
+    const S eta = beta * beta - a2 * (cos_o * cos_o) + xi * xi * (cot * cot);
+    const S term1 = (r_o * r_o) + a2 - a * xi;
+    const S d = xi - a;
+    const S dlt = (r_o * r_o - 2.0 * m * r_o) + a2;
+    const S big_r = term1 * term1 - dlt * (eta + d * d);
+    y[0] = constant<N>(0.0);
+    y[1] = constant<N>(r_o);
+    y[2] = th_o;
+    y[3] = constant<N>(0.0);
+    y[4] = constant<N>(-1.0);
+    y[5] = -(dsqrt(big_r) / dlt);
+    y[6] = -beta;
+    y[7] = xi;
+}
+
+
+// ---------------------------------------------------------------------------
+// Escape direction at infinity.  An escaping photon ends at some r_N >=
+// r_esc, where theta and phi are still changing (d theta / dr ~ p_theta /
+// r^2), so the final state's angles depend on where the last step landed.
+// The escape direction is therefore taken at r = infinity: from the final
+// state, (theta, phi, p_theta) are integrated in u = 1/r from u_N = 1/r_N to
+// 0 with the geodesic equations divided by du/dlambda, written in terms of
+//   Sigma^ = u^2 Sigma = 1 + a^2 u^2 cos^2,  Delta^ = u^2 Delta = 1 - 2 M u + a^2 u^2,
+//   A^ = u^4 A = (1 + a^2 u^2)^2 - a^2 u^2 Delta^ sin^2,
+// in which every term is finite at u = 0:
+//   d theta/du = -p_theta / (Delta^ p_r),
+//   d phi/du   = (2 M a u p_t - (Delta^ - a^2 u^2 sin^2) p_phi / sin^2) / (Delta^2 p_r),
+//   d p_theta/du = Sigma^ (T1 p_t^2 + 2 T2 p_t p_phi + T3 p_r^2 + T4 p_theta^2 + T5 p_phi^2) / (2 Delta^ p_r),
+// with T_i = d_theta g^{..} / u^2 and p_r > 0 from the null condition
+//   p_r^2 = (A^ p_t^2 + 4 M a u^3 p_t p_phi - u^2 Delta^ p_theta^2
+//            - u^2 (Delta^ - a^2 u^2 sin^2) p_phi^2 / sin^2) / Delta^2.
+// Classical RK4 with TAIL_STEPS steps in u.
+constexpr int TAIL_STEPS = 16;
+
+template <class S>
+KR_HD void tail_rhs(const S& u, const S& th, const S& pth, const S& pt, const S& pph, const S& a, double m,
+                    S& dth, S& dph, S& dpth) {
+    const S s = dsin(th), c = dcos(th);
+    const S s2 = s * s, sc = s * c;
+    const S u2 = u * u, a2 = a * a;
+    const S a2u2 = a2 * u2;
+    const S sg = 1.0 + a2u2 * (c * c);
+    const S dl = 1.0 + (a2u2 - (2.0 * m) * u);
+    const S one_a2u2 = 1.0 + a2u2;
+    const S ah = one_a2u2 * one_a2u2 - a2u2 * dl * s2;
+    const S dphi_core = dl - a2u2 * s2;
+    const S mau3 = (4.0 * m) * (a * u2 * u);
+    const S pr2 = (ah * pt * pt + mau3 * pt * pph - u2 * dl * pth * pth - u2 * dphi_core * pph * pph / s2) / (dl * dl);
+    const S pr = dsqrt(pr2);
+    dth = -(pth / (dl * pr));
+    dph = ((2.0 * m) * (a * u * pt) - dphi_core * pph / s2) / (dl * dl * pr);
+    const S sg2 = sg * sg;
+    const S t1 = 2.0 * (a2 * sc * (dl * sg - ah)) / (sg2 * dl);
+    const S t2 = -((4.0 * m) * (a2 * a * u2 * u * sc)) / (sg2 * dl);
+    const S t3 = 2.0 * (a2 * sc * dl) / sg2;
+    const S t4 = 2.0 * (a2u2 * sc) / sg2;
+    const S t5 = 2.0 * (sc * (a2u2 * s2 - sg)) / (sg2 * (s2 * s2)) - 2.0 * (a2u2 * a2u2 * sc) / (sg2 * dl);
+    dpth = 0.5 * (sg * (t1 * pt * pt + 2.0 * (t2 * pt * pph) + t3 * pr2 + t4 * pth * pth + t5 * pph * pph)) / (dl * pr);
+}
+
+// (theta, phi) at r = infinity from the state (r, theta, phi, p_theta) with
+// constants p_t, p_phi and spin parameter a.
+template <class S>
+KR_HD void escape_tail(const S& r, const S& th0, const S& ph0, const S& pth0, const S& pt, const S& pph, const S& a,
+                       double m, S& th_inf, S& ph_inf) {
+    S u = 1.0 / r;
+    const S h = (-1.0 / static_cast<double>(TAIL_STEPS)) * u;
+    S th = th0, ph = ph0, pth = pth0;
+    for (int k = 0; k < TAIL_STEPS; ++k) {
+        S k1t, k1p, k1q, k2t, k2p, k2q, k3t, k3p, k3q, k4t, k4p, k4q;
+        tail_rhs(u, th, pth, pt, pph, a, m, k1t, k1p, k1q);
+        const S hh = 0.5 * h;
+        tail_rhs(u + hh, th + hh * k1t, pth + hh * k1q, pt, pph, a, m, k2t, k2p, k2q);
+        tail_rhs(u + hh, th + hh * k2t, pth + hh * k2q, pt, pph, a, m, k3t, k3p, k3q);
+        const S u_next = (k == TAIL_STEPS - 1) ? 0.0 * u : u + h;
+        tail_rhs(u_next, th + h * k3t, pth + h * k3q, pt, pph, a, m, k4t, k4p, k4q);
+        const S w = (1.0 / 6.0) * h;
+        th = th + w * (k1t + 2.0 * k2t + 2.0 * k3t + k4t);
+        ph = ph + w * (k1p + 2.0 * k2p + 2.0 * k3p + k4p);
+        pth = pth + w * (k1q + 2.0 * k2q + 2.0 * k3q + k4q);
+        u = u_next;
+    }
+    th_inf = th;
+    ph_inf = ph;
+}
+
+// Cotangent of the final state for the cotangent (g_theta, g_phi) of the
+// escape direction at infinity: fills ybar (r, theta, phi, p_theta; p_r and
+// t do not enter) and the p_t, p_phi and a cotangents.
+KR_HD void escape_tail_vjp(double m, double a, const double (&y)[NA], double pt, double pph, double g_theta,
+                           double g_phi, double (&ybar)[NA], double& ptbar, double& pphbar, double& abar) {
+    using D = Dual<5>;
+    const D r = seed<5>(y[1], 0), th = seed<5>(y[2], 1), pth = seed<5>(y[5], 2), dpt = seed<5>(pt, 3),
+            dpph = seed<5>(pph, 4), da = constant<5>(a);
+    D th_inf, ph_inf;
+    escape_tail(r, th, constant<5>(y[3]), pth, dpt, dpph, da, m, th_inf, ph_inf);
+    double t[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) t[k] = g_theta * th_inf.d[k] + g_phi * ph_inf.d[k];
+    ybar[0] = 0.0;
+    ybar[1] = t[0];
+    ybar[2] = t[1];
+    ybar[3] = g_phi;  // phi enters additively
+    ybar[4] = 0.0;
+    ybar[5] = t[2];
+    ptbar = t[3];
+    pphbar = t[4];
+    // d/da through the tail, by a separate one-direction pass.
+    using E = Dual<1>;
+    E th1, ph1;
+    escape_tail(constant<1>(y[1]), constant<1>(y[2]), constant<1>(y[3]), constant<1>(y[5]), constant<1>(pt),
+                constant<1>(pph), seed<1>(a, 0), m, th1, ph1);
+    abar = g_theta * th1.d[0] + g_phi * ph1.d[0];
+}
+
+}  // namespace ad
+}  // namespace kr

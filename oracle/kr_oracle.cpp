@@ -30,6 +30,8 @@
 #include <cstring>
 #include <omp.h>
 
+#include <vector>
+
 namespace {
 
 enum { RUNNING = 0, ESCAPED = 1, CAPTURED = 2, MAX_AFFINE = 3, FAILURE = 4, OUT_OF_DOMAIN = 5, DISK_HIT = 6 };
@@ -260,6 +262,7 @@ struct RayArgs {
     const double* fq;
     const double* fe2;
     uint8_t* event_hit;
+    std::vector<double>* steps = nullptr;  // when set: accepted step sizes of ray n go to steps[n]
 };
 
 // numba_kernel.py:340-463
@@ -339,6 +342,7 @@ void integrate_ray(int64_t n, const RayArgs& a, const T* consts, const T* tab_a,
             y[i] = y_new[i];
             f[i] = k[6][i];
         }
+        if (a.steps) a.steps[n].push_back(static_cast<double>(h_eff));
         const double lam_old = lam;
         if (last)
             lam = lambda_max;
@@ -526,6 +530,321 @@ void kro_dp5_step(const double* geom, const double* consts64, const double* tab_
 int kro_classify(const double* consts64, const double* params, const int64_t* ipar, const double* y,
                  const double* f, double lam, int64_t n_steps, double h, int check_h) {
     return classify<double>(consts64, params, ipar, y, f, lam, n_steps, h, check_h != 0);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Forward-mode variational integration (checker of the discrete adjoint).
+//
+// For an escaping photon the oracle first integrates exactly as above and
+// keeps the accepted step sizes h_k.  It then integrates the variational
+// equations along the same steps: the Dormand-Prince 5(4) update applied to
+// the augmented state (y, dy/dx), which for an explicit Runge-Kutta method
+// with the steps held fixed is the derivative of the discrete map
+// y_{k+1} = Phi(y_k; h_k).  Here every quantity carries its derivatives with
+// respect to x = (alpha, beta, spin, theta_o) (forward-mode dual numbers),
+// from the Bardeen initial state (initial_conditions.py:50-112 with
+// Theta evaluated in full) through the inverse metric with the spin a
+// variable (numba_kernel.py:111-154) and the Hamiltonian right-hand side
+// (numba_kernel.py:158-197) to the final (theta, phi).
+namespace {
+
+constexpr int NX = 4;
+
+struct Dx {
+    double v;
+    double d[NX];
+};
+
+inline Dx dx_const(double v) {
+    Dx z{v, {0.0, 0.0, 0.0, 0.0}};
+    return z;
+}
+inline Dx dx_var(double v, int k) {
+    Dx z = dx_const(v);
+    z.d[k] = 1.0;
+    return z;
+}
+inline Dx operator+(Dx x, const Dx& y) {
+    x.v += y.v;
+    for (int i = 0; i < NX; ++i) x.d[i] += y.d[i];
+    return x;
+}
+inline Dx operator-(Dx x, const Dx& y) {
+    x.v -= y.v;
+    for (int i = 0; i < NX; ++i) x.d[i] -= y.d[i];
+    return x;
+}
+inline Dx operator-(Dx x) {
+    x.v = -x.v;
+    for (int i = 0; i < NX; ++i) x.d[i] = -x.d[i];
+    return x;
+}
+inline Dx operator*(const Dx& x, const Dx& y) {
+    Dx z;
+    z.v = x.v * y.v;
+    for (int i = 0; i < NX; ++i) z.d[i] = x.d[i] * y.v + x.v * y.d[i];
+    return z;
+}
+inline Dx operator*(double c, Dx x) {
+    x.v *= c;
+    for (int i = 0; i < NX; ++i) x.d[i] *= c;
+    return x;
+}
+inline Dx operator/(const Dx& x, const Dx& y) {
+    Dx z;
+    z.v = x.v / y.v;
+    for (int i = 0; i < NX; ++i) z.d[i] = (x.d[i] * y.v - x.v * y.d[i]) / (y.v * y.v);
+    return z;
+}
+inline Dx dx_sin(const Dx& x) {
+    Dx z;
+    z.v = std::sin(x.v);
+    const double c = std::cos(x.v);
+    for (int i = 0; i < NX; ++i) z.d[i] = c * x.d[i];
+    return z;
+}
+inline Dx dx_cos(const Dx& x) {
+    Dx z;
+    z.v = std::cos(x.v);
+    const double s = std::sin(x.v);
+    for (int i = 0; i < NX; ++i) z.d[i] = -s * x.d[i];
+    return z;
+}
+inline Dx dx_sqrt(const Dx& x) {
+    Dx z;
+    z.v = std::sqrt(x.v);
+    for (int i = 0; i < NX; ++i) z.d[i] = x.d[i] / (2.0 * z.v);
+    return z;
+}
+
+// numba_kernel.py:111-154 with m fixed and a = spin * m carried as a dual.
+void metric_dx(double m, const Dx& a, const Dx& r, const Dx& th, Dx* g) {
+    const Dx two_m = dx_const(2.0 * m);
+    const Dx a2 = a * a;
+    const Dx s = dx_sin(th), c = dx_cos(th);
+    const Dx s2 = s * s;
+    const Dx sin2th = dx_sin(2.0 * th);
+    const Dx r2 = r * r;
+    const Dx sig = r2 + a2 * (c * c);
+    const Dx dlt = r2 - two_m * r + a2;
+    const Dx r2a2 = r2 + a2;
+    const Dx big_a = r2a2 * r2a2 - a2 * dlt * s2;
+    const Dx sd = sig * dlt;
+    const Dx neg_2ma = -(two_m * a), pos_2ma = two_m * a;
+    g[0] = -big_a / sd;
+    g[1] = neg_2ma * r / sd;
+    g[2] = dlt / sig;
+    g[3] = dx_const(1.0) / sig;
+    g[4] = (dlt - a2 * s2) / (sd * s2);
+    const Dx sig_r = 2.0 * r;
+    const Dx sig_th = -a2 * sin2th;
+    const Dx dlt_r = 2.0 * (r - dx_const(m));
+    const Dx a_r = 4.0 * r * r2a2 - 2.0 * a2 * (r - dx_const(m)) * s2;
+    const Dx a_th = -a2 * dlt * sin2th;
+    const Dx sd_r = sig_r * dlt + sig * dlt_r;
+    const Dx sd_th = sig_th * dlt;
+    const Dx sd2 = sd * sd;
+    const Dx sig2 = sig * sig;
+    g[5] = -(a_r * sd - big_a * sd_r) / sd2;
+    g[6] = neg_2ma * (sd - r * sd_r) / sd2;
+    g[7] = (dlt_r * sig - dlt * sig_r) / sig2;
+    g[8] = -sig_r / sig2;
+    g[9] = -sig_r / (sig2 * s2) + a2 * sd_r / sd2;
+    g[10] = -(a_th * sd - big_a * sd_th) / sd2;
+    g[11] = pos_2ma * r * sd_th / sd2;
+    g[12] = -dlt * sig_th / sig2;
+    g[13] = -sig_th / sig2;
+    g[14] = -(sig_th * s2 + sig * sin2th) / (sig2 * (s2 * s2)) + a2 * sd_th / sd2;
+}
+
+// numba_kernel.py:158-197 on the full 8-component state.
+void rhs_dx(double m, const Dx& a, const Dx* y, Dx* f) {
+    Dx g[15];
+    metric_dx(m, a, y[IDX_R], y[IDX_TH], g);
+    const Dx p_t = y[IDX_PT], p_r = y[IDX_PR], p_th = y[IDX_PTH], p_ph = y[IDX_PPH];
+    Dx gphph = g[4], r_gphph = g[9], t_gphph = g[14];
+    if (p_ph.v == 0.0) gphph = r_gphph = t_gphph = dx_const(0.0);
+    f[IDX_T] = g[0] * p_t + g[1] * p_ph;
+    f[IDX_R] = g[2] * p_r;
+    f[IDX_TH] = g[3] * p_th;
+    f[IDX_PH] = g[1] * p_t + gphph * p_ph;
+    f[IDX_PT] = dx_const(0.0);
+    f[IDX_PR] = -0.5 * (g[5] * p_t * p_t + 2.0 * g[6] * p_t * p_ph + g[7] * p_r * p_r + g[8] * p_th * p_th +
+                        r_gphph * p_ph * p_ph);
+    f[IDX_PTH] = -0.5 * (g[10] * p_t * p_t + 2.0 * g[11] * p_t * p_ph + g[12] * p_r * p_r + g[13] * p_th * p_th +
+                         t_gphph * p_ph * p_ph);
+    f[IDX_PPH] = dx_const(0.0);
+}
+
+// photon_from_constants / carter_potentials (initial_conditions.py:50-112)
+// for the Bardeen constants of (alpha, beta) (rayset.py:158-165), E = 1.
+void bardeen_dx(double m, double r0, const Dx& alpha, const Dx& beta, const Dx& a, const Dx& th_o, Dx* y) {
+    const Dx sin_o = dx_sin(th_o), cos_o = dx_cos(th_o);
+    const Dx xi = -(alpha * sin_o);
+    const Dx cot = cos_o / sin_o;
+    const Dx eta = beta * beta - a * a * cos_o * cos_o + xi * xi * cot * cot;
+    const Dx rr = dx_const(r0 * r0);
+    const Dx dlt = rr - dx_const(2.0 * m * r0) + a * a;
+    const Dx term1 = (rr + a * a) - a * xi;
+    const Dx d = xi - a;
+    const Dx big_r = term1 * term1 - dlt * (eta + d * d);
+    const Dx big_theta = eta - cos_o * cos_o * (xi * xi / (sin_o * sin_o) - a * a);
+    const double sign_theta = beta.v >= 0.0 ? -1.0 : 1.0;
+    y[IDX_T] = dx_const(0.0);
+    y[IDX_R] = dx_const(r0);
+    y[IDX_TH] = th_o;
+    y[IDX_PH] = dx_const(0.0);
+    y[IDX_PT] = dx_const(-1.0);
+    y[IDX_PR] = -(dx_sqrt(big_r) / dlt);
+    y[IDX_PTH] = sign_theta * dx_sqrt(big_theta);
+    y[IDX_PPH] = xi;
+}
+
+// One Dormand-Prince 5(4) step (numba_kernel.py:201-238, y_new only) on
+// dual states with the tableau A (dense 7 x 7) and a fixed step h.
+void dp5_dx(double m, const Dx& a, const double* A, const Dx* y, double h, Dx* y_new) {
+    Dx k[6][S], ys[S];
+    rhs_dx(m, a, y, k[0]);
+    for (int st = 1; st < 6; ++st) {
+        for (int i = 0; i < S; ++i) {
+            Dx acc = dx_const(0.0);
+            for (int j = 0; j < st; ++j) acc = acc + A[st * 7 + j] * k[j][i];
+            ys[i] = y[i] + h * acc;
+        }
+        rhs_dx(m, a, ys, k[st]);
+    }
+    for (int i = 0; i < S; ++i) {
+        Dx acc = dx_const(0.0);
+        for (int j = 0; j < 6; ++j) acc = acc + A[6 * 7 + j] * k[j][i];
+        y_new[i] = y[i] + h * acc;
+    }
+}
+
+
+// Escape direction at r = infinity: (theta, phi, p_theta) integrated from
+// the final state in u = 1/r down to u = 0 (classical RK4, 16 steps), with
+// the geodesic equations divided by du/dlambda and written in
+// Sigma^ = u^2 Sigma, Delta^ = u^2 Delta, A^ = u^4 A so that they stay finite
+// at u = 0; p_r > 0 from the null condition.
+constexpr int TAIL_STEPS = 16;
+
+void tail_rhs_dx(double m, const Dx& a, const Dx& u, const Dx& th, const Dx& pth, const Dx& pt, const Dx& pph,
+                 Dx* d) {
+    const Dx one = dx_const(1.0);
+    const Dx s = dx_sin(th), c = dx_cos(th);
+    const Dx s2 = s * s, c2 = c * c;
+    const Dx a2 = a * a, u2 = u * u;
+    const Dx sig = one + a2 * u2 * c2;
+    const Dx dlt = one - 2.0 * m * u + a2 * u2;
+    const Dx big_a = (one + a2 * u2) * (one + a2 * u2) - a2 * u2 * dlt * s2;
+    const Dx gph = dlt - a2 * u2 * s2;
+    const Dx pr2 = (big_a * pt * pt + 4.0 * m * a * u * u2 * pt * pph - u2 * dlt * pth * pth - u2 * gph * pph * pph / s2) /
+                   (dlt * dlt);
+    const Dx pr = dx_sqrt(pr2);
+    d[0] = -(pth / (dlt * pr));
+    d[1] = (2.0 * m * a * u * pt - gph * pph / s2) / (dlt * dlt * pr);
+    const Dx sc = s * c, sig2 = sig * sig;
+    const Dx t1 = 2.0 * a2 * sc * (dlt * sig - big_a) / (sig2 * dlt);
+    const Dx t2 = -(4.0 * m * a * a2 * u * u2 * sc) / (sig2 * dlt);
+    const Dx t3 = 2.0 * a2 * sc * dlt / sig2;
+    const Dx t4 = 2.0 * a2 * u2 * sc / sig2;
+    const Dx t5 = 2.0 * sc * (a2 * u2 * s2 - sig) / (sig2 * s2 * s2) - 2.0 * a2 * a2 * u2 * u2 * sc / (sig2 * dlt);
+    d[2] = 0.5 * sig * (t1 * pt * pt + 2.0 * t2 * pt * pph + t3 * pr2 + t4 * pth * pth + t5 * pph * pph) / (dlt * pr);
+}
+
+void escape_tail_dx(double m, const Dx& a, const Dx* y, Dx& th_inf, Dx& ph_inf) {
+    Dx u = dx_const(1.0) / y[IDX_R];
+    const Dx h = (-1.0 / TAIL_STEPS) * u;
+    Dx z[3] = {y[IDX_TH], y[IDX_PH], y[IDX_PTH]};
+    const Dx pt = y[IDX_PT], pph = y[IDX_PPH];
+    for (int k = 0; k < TAIL_STEPS; ++k) {
+        Dx k1[3], k2[3], k3[3], k4[3], zz[3];
+        tail_rhs_dx(m, a, u, z[0], z[2], pt, pph, k1);
+        for (int i = 0; i < 3; ++i) zz[i] = z[i] + 0.5 * (h * k1[i]);
+        tail_rhs_dx(m, a, u + 0.5 * h, zz[0], zz[2], pt, pph, k2);
+        for (int i = 0; i < 3; ++i) zz[i] = z[i] + 0.5 * (h * k2[i]);
+        tail_rhs_dx(m, a, u + 0.5 * h, zz[0], zz[2], pt, pph, k3);
+        for (int i = 0; i < 3; ++i) zz[i] = z[i] + h * k3[i];
+        const Dx u_next = (k == TAIL_STEPS - 1) ? dx_const(0.0) : u + h;
+        tail_rhs_dx(m, a, u_next, zz[0], zz[2], pt, pph, k4);
+        for (int i = 0; i < 3; ++i) z[i] = z[i] + (1.0 / 6.0) * (h * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]));
+        u = u_next;
+    }
+    th_inf = z[0];
+    ph_inf = z[1];
+}
+
+}  // namespace
+
+extern "C" {
+
+// Escape directions at infinity of the points (alpha_i, beta_i) of one
+// image and their derivatives.  geom..tab_e are the packed kernel constants
+// (FP64 DP5, no events).  Outputs: state[n]; theta[n], phi[n], the escape
+// direction at r = infinity of the exact integration's final state (NaN
+// unless escaped); jac[8 n] = (d theta / d x, d phi / d x) for
+// x = (alpha, beta, spin, theta_o), NaN unless escaped.
+// replay_err[n] (may be NULL): max |final state of the variational pass -
+// exact final state| over the integrated components, a consistency check.
+int kro_escape_variational(double mass, double spin, double r0, double theta_o, const double* alpha,
+                           const double* beta, int64_t n, const double* geom, const double* consts,
+                           const double* params, const int64_t* ipar, const double* tab_a, const double* tab_e,
+                           const double* rk4_b, const double* rk4_c, int64_t* state, double* theta, double* phi,
+                           double* jac, double* replay_err, int n_threads) {
+    if (ipar[I_ADAPTIVE] != 1 || ipar[I_DISK] != 0) return 2;
+    if (n_threads <= 0) n_threads = omp_get_max_threads();
+    const double a_val = spin * mass;
+    int bad = 0;
+#pragma omp parallel for num_threads(n_threads) schedule(dynamic, 4) reduction(| : bad)
+    for (int64_t i = 0; i < n; ++i) {
+        double y[S], f[S];
+        const int sign = beta[i] >= 0.0 ? -1 : 1;
+        if (kro_bardeen_state(mass, a_val, r0, theta_o, alpha[i], beta[i], sign, y)) bad = 1;
+        double lam, mn, me, ml, mq, one = 1.0, zero = 0.0;
+        int64_t ns, nr, st;
+        uint8_t hit = 0;
+        std::vector<double> steps;
+        RayArgs args{geom, params, ipar, &lam, &ns, &nr, &st, &mn, &me, &ml, &mq,
+                     &one, &zero, &zero, &one, &one, &one, &one, &hit, &steps};
+        double cst[N_CONSTS];
+        for (int c = 0; c < N_CONSTS; ++c) cst[c] = consts[c];
+        integrate_ray<double>(0, args, cst, tab_a, tab_e, rk4_b, rk4_c, y, f, nullptr);
+        state[i] = st;
+        theta[i] = NAN;
+        phi[i] = NAN;
+        for (int c = 0; c < 8; ++c) jac[i * 8 + c] = NAN;
+        if (replay_err) replay_err[i] = NAN;
+        if (st != ESCAPED) continue;
+        Dx yd[S], yn[S];
+        const Dx ad = mass * dx_var(spin, 2);
+        bardeen_dx(mass, r0, dx_var(alpha[i], 0), dx_var(beta[i], 1), ad, dx_var(theta_o, 3), yd);
+        for (double h : steps) {
+            dp5_dx(mass, ad, tab_a, yd, h, yn);
+            for (int c = 0; c < S; ++c) yd[c] = yn[c];
+        }
+        Dx yexact[S], th_inf, ph_inf;
+        for (int c = 0; c < S; ++c) yexact[c] = yd[c];
+        escape_tail_dx(mass, ad, yexact, th_inf, ph_inf);
+        // The values come from the exact final state (the dual pass's values
+        // differ from it by rounding only).
+        Dx yv[S], thv, phv;
+        for (int c = 0; c < S; ++c) yv[c] = dx_const(y[c]);
+        escape_tail_dx(mass, dx_const(a_val), yv, thv, phv);
+        theta[i] = thv.v;
+        phi[i] = phv.v;
+        for (int j = 0; j < NX; ++j) {
+            jac[i * 8 + j] = th_inf.d[j];
+            jac[i * 8 + 4 + j] = ph_inf.d[j];
+        }
+        if (replay_err) {
+            double e = 0.0;
+            for (int c = 0; c < S; ++c) e = pymax(e, std::fabs(yd[c].v - y[c]) / pymax(std::fabs(y[c]), 1.0));
+            replay_err[i] = e;
+        }
+    }
+    return bad;
 }
 
 }  // extern "C"

@@ -86,6 +86,8 @@ def lib() -> ctypes.CDLL:
         L.kro_dp5_step.argtypes = [vp, vp, vp, vp, vp, ctypes.c_double, vp, vp, vp, vp]
         L.kro_classify.restype = ctypes.c_int
         L.kro_classify.argtypes = [vp, vp, vp, vp, vp, ctypes.c_double, ctypes.c_int64, ctypes.c_double, ctypes.c_int]
+        L.kro_escape_variational.restype = ctypes.c_int
+        L.kro_escape_variational.argtypes = [ctypes.c_double] * 4 + [vp, vp, ctypes.c_int64] + [vp] * 13 + [ctypes.c_int]
         del dp, ip
         _lib = L
     return _lib
@@ -238,3 +240,37 @@ def bardeen_points(mass: float, spin: float, r_o: float, inclination_deg: float,
             raise ValueError("negative radicand in the Bardeen initial conditions")
         Y[k] = yk
     return Y
+
+
+@dataclass
+class EscapeJacobian:
+    state: np.ndarray  # (n,) termination codes of the exact integration
+    theta: np.ndarray  # (n,) final theta
+    phi: np.ndarray  # (n,) final phi
+    jac: np.ndarray  # (n, 2, 4): d(theta, phi)/d(alpha, beta, spin, theta_o); NaN unless escaped
+    replay_err: np.ndarray  # (n,) relative difference of the variational pass's values from the exact states
+
+
+def escape_variational(st, theta_o: float, r_o: float, alpha, beta, integ, term, *, n_threads: int = 0) -> EscapeJacobian:
+    """Forward-mode variational integration along the accepted steps (see kr_oracle.cpp).
+
+    ``theta_o`` in radians; derivatives are with respect to
+    ``(alpha, beta, spin, theta_o)``.
+    """
+    alpha = np.ascontiguousarray(alpha, np.float64).ravel()
+    beta = np.ascontiguousarray(beta, np.float64).ravel()
+    n = alpha.size
+    geom, consts, params, ipar = kernel_constants(st, integ, term, None)
+    tab_a, tab_e, rk4_b, rk4_c = tableaus()
+    state = np.empty(n, np.int64)
+    theta, phi, err = np.empty(n), np.empty(n), np.empty(n)
+    jac = np.empty((n, 2, 4))
+    rc = lib().kro_escape_variational(
+        float(st.mass), float(st.spin), float(r_o), float(theta_o), _ptr(alpha), _ptr(beta), n, _ptr(geom),
+        _ptr(consts), _ptr(params), _ptr(ipar), _ptr(tab_a), _ptr(tab_e), _ptr(rk4_b), _ptr(rk4_c), _ptr(state),
+        _ptr(theta), _ptr(phi), _ptr(jac), _ptr(err), n_threads)
+    if rc == 2:
+        raise ValueError("the variational integration needs adaptive DP5 without events")
+    if rc != 0:
+        raise ValueError("negative radicand in the Bardeen initial conditions")
+    return EscapeJacobian(state, theta, phi, jac, err)

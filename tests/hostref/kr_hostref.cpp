@@ -129,3 +129,125 @@ extern "C" int krh_trace_image_mixed(const kr_params* p, const kr_image* img, do
     *fp64_attempts = total_fp64;
     return 0;
 }
+
+#include <vector>
+
+#include "kr_adjoint.h"
+
+// The discrete adjoint on the CPU, with the kernels' algorithm: trace,
+// record accepted steps and checkpoints every `segment` steps, then per
+// photon sweep the segments backwards, recomputing each from its checkpoint.
+// theta/phi = escape direction at infinity (NaN unless escaped);
+// grad[4 i + j] = g_theta[i] d theta_inf/dx_j + g_phi[i] d phi_inf/dx_j,
+// x = (alpha, beta, spin, theta_o); 0 for photons that did not escape.
+extern "C" int krh_escape_vjp(const kr_params* p, const kr_image* img, const double* alpha, const double* beta,
+                              int64_t n, const double* g_theta, const double* g_phi, int segment, int64_t* state,
+                              double* theta, double* phi, double* grad, int nt) {
+    kr::BardeenImage B;
+    if (kr::bardeen_image_scalars(img, false, &B)) return 1;
+    const kr::Params P = params_of(p);
+    if (nt <= 0) nt = omp_get_max_threads();
+    const double mass = img->mass, a = img->spin * img->mass;
+    int bad = 0;
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 4) reduction(| : bad)
+    for (int64_t i = 0; i < n; ++i) {
+        double y0[8];
+        kr::bardeen_state(B, alpha[i], beta[i], y0);
+        kr::RayState<double> R;
+        kr::ray_init<double, true>(P, y0, R);
+        std::vector<double> hs;
+        std::vector<double> ck;
+        for (int c = 0; c < kr::NA; ++c) ck.push_back(R.y[c]);
+        while (R.state == kr::RUNNING) {
+            const double remaining = P.params[kr::P_LAMMAX] - R.lam;
+            const double h_eff = (R.h >= remaining) ? remaining : R.h;
+            const int64_t k = R.n_steps;
+            kr::ray_attempt<double, true, false>(P, R);
+            if (R.n_steps != k) {
+                hs.push_back(h_eff);
+                if (R.n_steps % segment == 0 && R.state == kr::RUNNING)
+                    for (int c = 0; c < kr::NA; ++c) ck.push_back(R.y[c]);
+            }
+        }
+        kr::RayResult<double> r;
+        kr::ray_result<double, false>(R, r);
+        state[i] = r.state;
+        theta[i] = NAN;
+        phi[i] = NAN;
+        for (int j = 0; j < 4; ++j) grad[i * 4 + j] = 0.0;
+        if (r.state != kr::ESCAPED) continue;
+        const double pt = y0[4], pph = y0[7];
+        kr::ad::escape_tail(R.y[1], R.y[2], R.y[3], R.y[5], pt, pph, a, mass, theta[i], phi[i]);
+        const int64_t N = static_cast<int64_t>(hs.size());
+        const int64_t nseg = (N + segment - 1) / segment;
+        double ybar[kr::NA], ptbar, pphbar, abar;
+        kr::ad::escape_tail_vjp(mass, a, R.y, pt, pph, g_theta[i], g_phi[i], ybar, ptbar, pphbar, abar);
+        std::vector<double> seg(static_cast<size_t>(segment) * kr::NA);
+        for (int64_t s = nseg - 1; s >= 0; --s) {
+            double y[kr::NA];
+            for (int c = 0; c < kr::NA; ++c) y[c] = ck[s * kr::NA + c];
+            const int64_t kb = s * segment, ke = std::min<int64_t>(N, kb + segment);
+            for (int64_t k = kb; k < ke; ++k) {
+                for (int c = 0; c < kr::NA; ++c) seg[(k - kb) * kr::NA + c] = y[c];
+                kr::ad::step_forward(P, y, pt, pph, hs[k]);
+            }
+            if (s == nseg - 1)
+                for (int c = 0; c < kr::NA; ++c)
+                    if (!(y[c] == r.y[kr::full_index(c)])) bad |= 2;
+            for (int64_t k = ke - 1; k >= kb; --k) {
+                for (int c = 0; c < kr::NA; ++c) y[c] = seg[(k - kb) * kr::NA + c];
+                kr::ad::step_adjoint(P, mass, a, y, pt, pph, hs[k], ybar, ptbar, pphbar, abar);
+            }
+        }
+        kr::ad::Dual<4> yd[8];
+        kr::ad::bardeen_state_of<4>(kr::ad::seed<4>(alpha[i], 0), kr::ad::seed<4>(beta[i], 1),
+                                    kr::ad::seed<4>(a, 2), kr::ad::seed<4>(B.theta_o, 3), mass, B.r_o, yd);
+        for (int j = 0; j < 4; ++j) {
+            double g = ptbar * yd[4].d[j] + pphbar * yd[7].d[j];
+            for (int c = 0; c < kr::NA; ++c) g += ybar[c] * yd[kr::full_index(c)].d[j];
+            if (j == 2) g = (g + abar) * mass;
+            grad[i * 4 + j] = g;
+        }
+    }
+    return bad;
+}
+
+// Building blocks for unit tests: v^T df at one state, and one step's
+// forward map and reverse sweep (out = ybar[6], ptbar, pphbar, abar).
+extern "C" void krh_rhs_vjp(double m, double a, const double* Y, double pt, double pph, const double* v, double* out) {
+    double y[kr::NA], vv[kr::NA], yb[kr::NA] = {0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < kr::NA; ++c) {
+        y[c] = Y[c];
+        vv[c] = v[c];
+    }
+    double ptb = 0.0, pphb = 0.0, ab = 0.0;
+    kr::ad::rhs_vjp(m, a, y, pt, pph, vv, yb, ptb, pphb, ab);
+    for (int c = 0; c < kr::NA; ++c) out[c] = yb[c];
+    out[6] = ptb;
+    out[7] = pphb;
+    out[8] = ab;
+}
+
+extern "C" void krh_step_forward(const kr_params* p, const double* Y, double pt, double pph, double h, double* out) {
+    const kr::Params P = params_of(p);
+    double y[kr::NA];
+    for (int c = 0; c < kr::NA; ++c) y[c] = Y[c];
+    kr::ad::step_forward(P, y, pt, pph, h);
+    for (int c = 0; c < kr::NA; ++c) out[c] = y[c];
+}
+
+extern "C" void krh_step_adjoint(const kr_params* p, double m, double a, const double* Y, double pt, double pph,
+                                 double h, const double* ybar_new, double* out) {
+    const kr::Params P = params_of(p);
+    double y[kr::NA], yb[kr::NA];
+    for (int c = 0; c < kr::NA; ++c) {
+        y[c] = Y[c];
+        yb[c] = ybar_new[c];
+    }
+    double ptb = 0.0, pphb = 0.0, ab = 0.0;
+    kr::ad::step_adjoint(P, m, a, y, pt, pph, h, yb, ptb, pphb, ab);
+    for (int c = 0; c < kr::NA; ++c) out[c] = yb[c];
+    out[6] = ptb;
+    out[7] = pphb;
+    out[8] = ab;
+}

@@ -1,0 +1,255 @@
+"""The discrete adjoint, checked on the CPU.
+
+tests/hostref builds csrc/kr_adjoint.h with g++: the same reverse sweep the
+GPU kernel runs (checkpoints, segment recomputation, per-step adjoints, the
+tail to infinity, the chain rule through the initial state). It is checked
+against:
+
+* the transposed Jacobian of the right-hand side by central differences of
+  the oracle's RHS (numba_kernel.py:111-197);
+* the dot-product identity <ybar_new, J dy> = <J^T ybar_new, dy> of one step
+  against central differences of the step map;
+* the oracle's forward-mode variational integration along the same steps
+  (independent code: dual numbers through the whole forward map), to
+  rounding;
+* central finite differences (h = 1e-6) of the adaptive integration itself.
+  These differ from the frozen-step derivative by the effect of the step
+  sizes moving with the inputs (a global-error-sized term) and by integration
+  noise / 2h where a step decision flips.
+
+The escape direction at infinity is pinned by integrating the same photons
+to r = 1e6 M instead of 1e3 M, and for Schwarzschild equatorial photons by
+the closed-form deflection integral.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+from conftest import ROOT, options
+
+HOSTREF = os.path.join(ROOT, "tests", "hostref")
+vp = ctypes.c_void_p
+
+
+@pytest.fixture(scope="module")
+def hostref():
+    subprocess.run(["make", "-s", "-C", HOSTREF], check=True)
+    L = ctypes.CDLL(os.path.join(HOSTREF, "libkr_hostref.so"))
+    L.krh_escape_vjp.restype = ctypes.c_int
+    L.krh_escape_vjp.argtypes = [vp, vp, vp, vp, ctypes.c_int64, vp, vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int]
+    L.krh_rhs_vjp.restype = None
+    L.krh_rhs_vjp.argtypes = [ctypes.c_double, ctypes.c_double, vp, ctypes.c_double, ctypes.c_double, vp, vp]
+    L.krh_step_forward.restype = None
+    L.krh_step_forward.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, vp]
+    L.krh_step_adjoint.restype = None
+    L.krh_step_adjoint.argtypes = [vp, ctypes.c_double, ctypes.c_double, vp, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, vp, vp]
+    return L
+
+
+def _p(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def host_jacobian(L, spin, theta_o_deg, alpha, beta, integ, term, segment=16):
+    from kerrray_amd.backends import kernel_params
+    from kerrray_amd.geometry import Spacetime
+    from kerrray_amd.shadow import kr_image
+
+    st = Spacetime(1.0, spin)
+    p = kernel_params(st, integ, term)
+    img = kr_image(st, 1000.0, theta_o_deg, 10.0, 1)
+    n = alpha.size
+    J = np.zeros((n, 2, 4))
+    for col in range(2):
+        gt = np.full(n, 1.0 if col == 0 else 0.0)
+        gp = np.full(n, 1.0 if col == 1 else 0.0)
+        state = np.empty(n, np.int64)
+        th, ph, grad = np.empty(n), np.empty(n), np.empty((n, 4))
+        rc = L.krh_escape_vjp(ctypes.addressof(p), ctypes.addressof(img), _p(alpha), _p(beta), n, _p(gt), _p(gp),
+                              segment, _p(state), _p(th), _p(ph), _p(grad), 0)
+        assert rc == 0, "recomputed final state differs from the traced one"
+        J[:, col] = grad
+    return state, th, ph, J
+
+
+def points(rng, n, spin, incl, near_edge=0):
+    """Random image points, plus some within 1e-3 M outside Bardeen's curve."""
+    from kerrray_amd.boundary import polygon_centroid, polygon_radii
+    from kerrray_amd.geometry import Spacetime
+    from kerrray_amd.orbits import shadow_curve
+
+    a = rng.uniform(-10, 10, n)
+    b = rng.uniform(-10, 10, n)
+    if near_edge:
+        curve = shadow_curve(Spacetime(1.0, spin), incl, 4000)
+        c = polygon_centroid(*curve)
+        ang = rng.uniform(0, 2 * np.pi, near_edge)
+        r = polygon_radii(curve[0], curve[1], c, ang) + rng.uniform(1e-3, 1e-2, near_edge)
+        a = np.concatenate([a, c[0] + r * np.cos(ang)])
+        b = np.concatenate([b, c[1] + r * np.sin(ang)])
+    return np.ascontiguousarray(a), np.ascontiguousarray(b)
+
+
+def test_rhs_vjp_is_the_transposed_jacobian(hostref, oracle) -> None:
+    rng = np.random.default_rng(0)
+    for spin in (0.0, 0.6, 0.95):
+        for _ in range(20):
+            Y = np.array([0.0, rng.uniform(3, 60), rng.uniform(0.2, 2.9), 0.0, rng.uniform(-2, 2), rng.uniform(-6, 6)])
+            pt, pph = -1.0, rng.uniform(-6, 6)
+            v = rng.normal(size=6)
+            out = np.empty(9)
+            hostref.krh_rhs_vjp(1.0, spin, _p(Y), pt, pph, _p(v), _p(out))
+
+            def f(Yv, ptv, pphv, a):
+                st = type("S", (), {"mass": 1.0, "spin": a})()
+                geom, consts, _, _ = oracle.kernel_constants(st, *options(), None)
+                y8 = np.array([Yv[0], Yv[1], Yv[2], Yv[3], ptv, Yv[4], Yv[5], pphv])
+                f8 = np.empty(8)
+                oracle.lib().kro_rhs(_p(geom), _p(consts), _p(y8), _p(f8))
+                return np.array([f8[0], f8[1], f8[2], f8[3], f8[5], f8[6]])
+
+            fd = np.zeros(9)
+            for k in range(9):
+                h = 1e-6 * (abs(Y[1]) if k == 1 else 1.0)
+                up, dn = [Y.copy(), pt, pph, spin], [Y.copy(), pt, pph, spin]
+                if k < 6:  # a state component
+                    up[0][k] += h
+                    dn[0][k] -= h
+                else:  # p_t, p_phi, a
+                    up[k - 5] += h
+                    dn[k - 5] -= h
+                fd[k] = v @ (f(*up) - f(*dn)) / (2 * h)
+            scale = np.abs(fd) + np.max(np.abs(fd)) * 1e-4  # floor: rounding noise of the differences
+            assert np.all(np.abs(out - fd) <= 1e-5 * scale), (spin, out, fd)
+
+
+def test_step_adjoint_satisfies_the_dot_product_identity(hostref) -> None:
+    from kerrray_amd.backends import kernel_params
+    from kerrray_amd.geometry import Spacetime
+
+    integ, term = options(rtol=1e-8, atol=1e-10)
+    rng = np.random.default_rng(1)
+    for spin in (0.3, 0.9):
+        for _ in range(10):
+            r = rng.uniform(4, 40)
+            Y = np.array([rng.uniform(0, 50), r, rng.uniform(0.4, 2.7), rng.uniform(-3, 3), rng.uniform(-1, 1),
+                          rng.uniform(-4, 4)])
+            pt, pph, h = -1.0, rng.uniform(-4, 4), 0.02 * r
+            ybar_new = rng.normal(size=6)
+            out = np.empty(9)
+            p = kernel_params(Spacetime(1.0, spin), integ, term)
+            hostref.krh_step_adjoint(ctypes.addressof(p), 1.0, spin, _p(Y), pt, pph, h, _p(ybar_new), _p(out))
+            d = rng.normal(size=9)  # (y[6], p_t, p_phi, a)
+            eps = 1e-6
+
+            def step(sgn):
+                y = Y + sgn * eps * d[:6]
+                ps = kernel_params(Spacetime(1.0, spin + sgn * eps * d[8]), integ, term)
+                o = np.empty(6)
+                hostref.krh_step_forward(ctypes.addressof(ps), _p(y), pt + sgn * eps * d[6], pph + sgn * eps * d[7],
+                                         h, _p(o))
+                return o
+
+            jvp = (step(+1) - step(-1)) / (2 * eps)
+            lhs, rhs = float(ybar_new @ jvp), float(out @ d)
+            assert abs(lhs - rhs) <= 1e-6 * (abs(lhs) + np.linalg.norm(out) * np.linalg.norm(d) * 1e-3), (lhs, rhs)
+
+
+@pytest.mark.parametrize("spin", [0.5, 0.9])
+def test_adjoint_equals_forward_variational_integration(hostref, oracle, spin) -> None:
+    integ, term = options(rtol=1e-8, atol=1e-10)
+    rng = np.random.default_rng(2)
+    alpha, beta = points(rng, 48, spin, 60.0, near_edge=16)
+    state, th, ph, J = host_jacobian(hostref, spin, 60.0, alpha, beta, integ, term)
+    V = oracle.escape_variational(type("S", (), {"mass": 1.0, "spin": spin})(), math.radians(60.0), 1000.0,
+                                  alpha, beta, integ, term)
+    assert np.array_equal(state, V.state)
+    esc = state == 1
+    assert esc.sum() >= 40
+    assert np.array_equal(th[esc], V.theta[esc]) or np.max(np.abs(th[esc] - V.theta[esc])) < 1e-13
+    assert np.max(np.abs(ph[esc] - V.phi[esc])) < 1e-12
+    assert np.all(np.isnan(th[~esc]))
+    rel = np.abs(J[esc] - V.jac[esc]) / np.maximum(np.abs(V.jac[esc]), 1e-12 * np.abs(V.jac[esc]).max())
+    assert rel.max() < 1e-8, rel.max()
+    assert np.all(J[~esc] == 0.0)
+
+
+def test_variational_derivative_agrees_with_central_differences(oracle) -> None:
+    """FD of the adaptive integration with h = 1e-6 on (alpha, beta, spin, theta_o)."""
+    integ, term = options(rtol=1e-8, atol=1e-10)
+    rng = np.random.default_rng(3)
+    spin, th_o = 0.9, math.radians(60.0)
+    alpha, beta = rng.uniform(-10, 10, 48), rng.uniform(-10, 10, 48)
+    S = type("S", (), {})
+
+    def run(a, b, sp, t):
+        st = S()
+        st.mass, st.spin = 1.0, sp
+        return oracle.escape_variational(st, t, 1000.0, a, b, integ, term)
+
+    V = run(alpha, beta, spin, th_o)
+    h = 1e-6
+    base = [alpha, beta, spin, th_o]
+    FD = np.empty((alpha.size, 2, 4))
+    ok = V.state == 1
+    for j in range(4):
+        up, dn = list(base), list(base)
+        up[j] = base[j] + h
+        dn[j] = base[j] - h
+        U, D = run(*up), run(*dn)
+        ok &= (U.state == 1) & (D.state == 1)
+        FD[:, 0, j] = (U.theta - D.theta) / (2 * h)
+        FD[:, 1, j] = (U.phi - D.phi) / (2 * h)
+    rel = np.abs(FD[ok] - V.jac[ok]) / np.maximum(np.abs(V.jac[ok]), 1e-3)
+    assert ok.sum() >= 30
+    assert np.median(rel) < 1e-6 and np.quantile(rel, 0.95) < 1e-4, (np.median(rel), np.quantile(rel, 0.95))
+
+
+@pytest.mark.parametrize("spin", [0.0, 0.9, -0.6])
+def test_direction_at_infinity_matches_integration_to_a_million_m(oracle, spin) -> None:
+    integ, term = options(rtol=1e-12, atol=1e-14)
+    far_integ = dataclasses.replace(integ, lambda_max=1e8)
+    far_term = dataclasses.replace(term, escape_radius=1e6)
+    rng = np.random.default_rng(4)
+    alpha, beta = rng.uniform(-10, 10, 24), rng.uniform(-10, 10, 24)
+    st = type("S", (), {"mass": 1.0, "spin": spin})()
+    A = oracle.escape_variational(st, math.radians(60.0), 1000.0, alpha, beta, integ, term)
+    B = oracle.escape_variational(st, math.radians(60.0), 1000.0, alpha, beta, far_integ, far_term)
+    e = (A.state == 1) & (B.state == 1)
+    assert e.sum() >= 12
+    assert np.max(np.abs(A.theta[e] - B.theta[e])) < 1e-9
+    assert np.max(np.abs(A.phi[e] - B.phi[e])) < 1e-9
+    assert np.max(np.abs(A.jac[e] - B.jac[e]) / np.maximum(np.abs(A.jac[e]), 1.0)) < 1e-7
+
+
+def test_schwarzschild_equatorial_deflection_closed_form(oracle) -> None:
+    """a = 0, observer on the equator, beta = 0: phi_inf is the textbook integral.
+
+    The photon leaves r_o inward with impact parameter b = |alpha|, turns at
+    the largest root r_t of r^3 - b^2 r + 2 M b^2 = 0 and escapes, so
+    |phi_inf| = int_{u_o}^{u_t} + int_0^{u_t} b du / sqrt(1 - b^2 u^2 + 2 M b^2 u^3).
+    """
+    import mpmath as mp
+
+    mp.mp.dps = 30
+    integ, term = options(rtol=1e-11, atol=1e-13)
+    st = type("S", (), {"mass": 1.0, "spin": 0.0})()
+    bs = np.array([5.5, 6.5, 8.0, 10.0])
+    V = oracle.escape_variational(st, math.pi / 2, 1000.0, bs, np.zeros_like(bs), integ, term)
+    assert np.all(V.state == 1)
+    for k, b in enumerate(bs):
+        b = mp.mpf(b)
+        roots = mp.polyroots([2 * b * b, -b * b, 0, 1])  # 2 b^2 u^3 - b^2 u^2 + 1 = 0 in u = 1/r
+        u_t = min(mp.re(x) for x in roots if abs(mp.im(x)) < 1e-20 and mp.re(x) > 0)
+        f = lambda u: b / mp.sqrt(1 - b * b * u * u + 2 * b * b * u ** 3)  # noqa: E731
+        total = mp.re(mp.quad(f, [0, u_t]) + mp.quad(f, [mp.mpf(1) / 1000, u_t]))  # root rounding: imaginary dust
+        assert abs(abs(V.phi[k]) - float(total)) < 1e-8, (float(b), V.phi[k], float(total))
+        assert abs(V.theta[k] - math.pi / 2) < 1e-12

@@ -134,7 +134,7 @@ def test_trace_rays_chunking_and_progress() -> None:
 
 def _header_symbols() -> list[str]:
     text = open(os.path.join(ROOT, "include", "kerrray_hip.h")).read()
-    return sorted(set(re.findall(r"^(?:kr_status|int|int64_t|size_t|const char\*)\s+(kr_\w+)\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^(?:kr_status|int|int64_t|size_t|double|const char\*)\s+(kr_\w+)\(", text, flags=re.M)))
 
 
 def test_library_loads_and_exports_every_declared_symbol() -> None:
