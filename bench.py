@@ -124,60 +124,183 @@ def persistent_single(hip, ref_state) -> dict:
             "tune": {"block_size": 256, "waves_per_simd": 2, "refill_interval": 4}}
 
 
-def planned_image(hip, ref_state, fp32_bulk: bool, repeats: int = 10) -> dict:
-    """512^2 image through the planned kernels (device buffers, HIP events on the launch stream)."""
-    import ctypes as _ct
+class PlannedRunner:
+    """One planned-trace form of the 512^2 image on device buffers (launch stream = torch's current stream)."""
 
+    def __init__(self, hip, fp32_bulk: bool, st, integ, term, res: int = RES, spin: float = SPIN, incl: float = INCL):
+        import torch
+
+        from kerrray_amd import _lib
+        from kerrray_amd.backends import kernel_params
+        from kerrray_amd.planned import PlannedOptions, edge_table
+        from kerrray_amd.shadow import kr_image
+
+        self.hip, self.n = hip, res * res
+        self.opts = PlannedOptions(fp32_bulk=fp32_bulk)
+        self.table = edge_table(st, incl)
+        self.bufs, self.outs = device_outputs(torch, self.n, torch.float64)
+        self.radii = torch.from_numpy(self.table.radii).cuda()
+        self.nbytes = _lib.lib().kr_planned_workspace_bytes(self.n, self.table.n_angles)
+        self.work = torch.empty(self.nbytes, dtype=torch.uint8, device="cuda")
+        self.params, self.img, self.ko = kernel_params(st, integ, term), kr_image(st, R_O, incl, FOV, res), self.opts.kr()
+        self.stream = torch.cuda.current_stream()
+
+    def __call__(self) -> None:
+        from kerrray_amd import _lib
+
+        _lib.check(_lib.lib().kr_trace_image_planned_dev(
+            self.hip.handle.ptr, ctypes.byref(self.params), ctypes.byref(self.img), ctypes.byref(self.ko),
+            self.table.centre[0], self.table.centre[1], self.radii.data_ptr(), self.table.n_angles,
+            ctypes.byref(self.outs), self.work.data_ptr(), self.nbytes, ctypes.c_void_p(self.stream.cuda_stream)),
+            "kr_trace_image_planned_dev")
+
+    def stats(self) -> dict:
+        """Counters of the last run."""
+        from kerrray_amd import _lib
+
+        st = _lib.KrPlannedStats()
+        _lib.check(_lib.lib().kr_planned_read_stats(self.work.data_ptr(), self.nbytes, ctypes.byref(st),
+                                                    ctypes.c_void_p(self.stream.cuda_stream)), "kr_planned_read_stats")
+        total = int((self.bufs["n_steps"] + self.bufs["n_rejected"]).sum())
+        return {"edge_band_pixels": st.planned_band, "repaired_pixels": st.repaired,
+                "fp64_attempt_fraction": st.fp64_attempts / max(1, total)}
+
+    def options(self) -> dict:
+        o = self.opts
+        return {"band_px": o.band_px, "horizon_margin": o.horizon_margin, "axis_sin": o.axis_sin,
+                "turn_rmax": o.turn_rmax, "turn_drdl": o.turn_drdl, "refill_interval": o.refill_interval,
+                "block_size": o.block_size, "longest_first": o.longest_first}
+
+
+class PlainRunner:
+    """The plain one-photon-per-lane FP64 kernel (kr_trace_image_dev)."""
+
+    def __init__(self, hip, st, integ, term):
+        import torch
+
+        from kerrray_amd.backends import kernel_params
+        from kerrray_amd.shadow import kr_image
+
+        self.hip, self.n = hip, RES * RES
+        self.bufs, self.outs = device_outputs(torch, self.n, torch.float64)
+        self.params, self.img = kernel_params(st, integ, term), kr_image(st, R_O, INCL, FOV, RES)
+        self.stream = torch.cuda.current_stream()
+
+    def __call__(self) -> None:
+        from kerrray_amd import _lib
+
+        _lib.check(_lib.lib().kr_trace_image_dev(self.hip.handle.ptr, ctypes.byref(self.params), ctypes.byref(self.img),
+                                                 ctypes.byref(self.outs), None, ctypes.c_void_p(self.stream.cuda_stream)),
+                   "kr_trace_image_dev")
+
+
+def timed(run, repeats: int) -> float:
+    """Average ms per call from HIP events on the launch stream (torch's current stream)."""
     import torch
 
-    from kerrray_amd import _lib, flops
-    from kerrray_amd.backends import kernel_params
-    from kerrray_amd.camera import BardeenCamera
-    from kerrray_amd.geodesics import IntegratorOptions, TerminationOptions
-    from kerrray_amd.geometry import Spacetime
-    from kerrray_amd.planned import PlannedOptions, compute_shadow_planned, edge_table
-    from kerrray_amd.shadow import kr_image
-
-    st = Spacetime(1.0, SPIN)
-    integ, term = IntegratorOptions(method="rk45", rtol=RTOL, atol=ATOL), TerminationOptions()
-    opts = PlannedOptions(fp32_bulk=fp32_bulk)
-    cam = BardeenCamera(radius=R_O, inclination_deg=INCL, fov=FOV, resolution=RES)
-    image, stats = compute_shadow_planned(st, cam, integ, term, backend=hip, options=opts)
-    table = edge_table(st, INCL)
-    n = RES * RES
-    bufs, outs = device_outputs(torch, n, torch.float64)
-    radii = torch.from_numpy(table.radii).cuda()
-    nbytes = _lib.lib().kr_planned_workspace_bytes(n, table.n_angles)
-    work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
-    params, img, ko = kernel_params(st, integ, term), kr_image(st, R_O, INCL, FOV, RES), opts.kr()
     stream = torch.cuda.current_stream()
-
-    def step():
-        _lib.check(_lib.lib().kr_trace_image_planned_dev(
-            hip.handle.ptr, _ct.byref(params), _ct.byref(img), _ct.byref(ko), table.centre[0], table.centre[1],
-            radii.data_ptr(), table.n_angles, _ct.byref(outs), work.data_ptr(), nbytes,
-            _ct.c_void_p(stream.cuda_stream)), "kr_trace_image_planned_dev")
-
-    step()
-    torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(repeats):
-        step()
+        run()
     e1.record(stream)
     torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / repeats
-    state = bufs["state"].cpu().numpy()
-    n_steps, n_rej = bufs["n_steps"].cpu().numpy(), bufs["n_rejected"].cpu().numpy()
-    fl = flops.batch_flops(n_steps, n_rej)
-    return {"photons_per_s": n / (ms * 1e-3), "kernel_ms": ms,
-            "roofline_frac": fl / (ms * 1e-3) / 1e12 / FP64_VECTOR_PEAK_TFLOPS,
-            "mask_identical_to_reference": None if ref_state is None else bool(np.array_equal(state, ref_state)),
-            "edge_band_pixels": stats.planned_band, "repaired_pixels": stats.repaired,
-            "fp64_attempt_fraction": stats.fp64_fraction,
-            "options": {"band_px": opts.band_px, "horizon_margin": opts.horizon_margin, "axis_sin": opts.axis_sin,
-                        "turn_rmax": opts.turn_rmax, "turn_drdl": opts.turn_drdl,
-                        "refill_interval": opts.refill_interval, "block_size": opts.block_size}}
+    return e0.elapsed_time(e1) / repeats
+
+
+def form_report(run, ref_state, repeats: int = 5) -> dict:
+    from kerrray_amd import flops
+
+    run()
+    ms = timed(run, repeats)
+    state = run.bufs["state"].cpu().numpy()
+    fl = flops.batch_flops(run.bufs["n_steps"].cpu().numpy(), run.bufs["n_rejected"].cpu().numpy())
+    out = {"photons_per_s": run.n / (ms * 1e-3), "kernel_ms": ms,
+           "roofline_frac": fl / (ms * 1e-3) / 1e12 / FP64_VECTOR_PEAK_TFLOPS,
+           "mask_identical_to_reference": None if ref_state is None else bool(np.array_equal(state, ref_state))}
+    if isinstance(run, PlannedRunner):
+        out.update(run.stats())
+        out["options"] = run.options()
+    return out
+
+
+def precision_report(hip, log) -> dict:
+    """1024^2, a=0.99, i=85 deg, rtol 1e-6: FP32 bulk + FP64 repair mask against the FP64 mask, and the
+    FP64 720-angle edge bisection at rtol 1e-8 against Bardeen's curve."""
+    import torch
+
+    from kerrray_amd.bisection import shadow_edge
+    from kerrray_amd.geodesics import IntegratorOptions, TerminationOptions
+    from kerrray_amd.geometry import Spacetime
+
+    spin, incl, res = 0.99, 85.0, 1024
+    st = Spacetime(1.0, spin)
+    integ, term = IntegratorOptions(method="rk45", rtol=1e-6, atol=1e-8), TerminationOptions()
+    mixed = PlannedRunner(hip, True, st, integ, term, res=res, spin=spin, incl=incl)
+    exact = PlannedRunner(hip, False, st, integ, term, res=res, spin=spin, incl=incl)
+    mixed()
+    exact()
+    ms_mixed, ms_exact = timed(mixed, 3), timed(exact, 3)
+    m_state = mixed.bufs["state"].cpu().numpy()
+    e_state = exact.bufs["state"].cpu().numpy()
+    mask_diff = int(np.count_nonzero((m_state == 2) != (e_state == 2)))
+    esc = (m_state == 1) & (e_state == 1)
+    dth = (mixed.bufs["Y"][:, 2] - exact.bufs["Y"][:, 2]).abs().cpu().numpy()[esc]
+    stats = mixed.stats()
+    del mixed, exact
+    torch.cuda.empty_cache()
+    log(f"precision form 1024^2: mixed {ms_mixed:.2f} ms, fp64 {ms_exact:.2f} ms, mask diff {mask_diff}")
+    t0 = time.perf_counter()
+    edge, _, _, err = shadow_edge(st, incl, R_O, 1e-8, 720, backend=hip, n_probe=63)
+    edge_s = time.perf_counter() - t0
+    log(f"720-angle FP64 edge at rtol 1e-8: {edge_s:.2f} s, max |r - Bardeen| {err['max']:.2e}")
+    return {"workload": "1024x1024 Bardeen image, a=0.99, i=85deg, rtol 1e-6, atol 1e-8",
+            "fp32_fp64_repair_ms": ms_mixed, "fp64_planned_ms": ms_exact,
+            "photons_per_s_fp32_fp64_repair": res * res / (ms_mixed * 1e-3),
+            "photons_per_s_fp64": res * res / (ms_exact * 1e-3),
+            "mask_diff_vs_fp64": mask_diff, "escaped_theta_abs_err_median": float(np.median(dth)),
+            "escaped_theta_abs_err_max": float(np.max(dth)), **stats,
+            "edge": {"angles": 720, "rtol": 1e-8, "n_probe": 63, "seconds": edge_s, "failures": edge.failures,
+                     "nonmonotone": edge.nonmonotone, "max_bracket_width": float(edge.width.max()),
+                     **{f"vs_bardeen_{k}": float(v) for k, v in err.items()}}}
+
+
+def adjoint_report(hip, log) -> dict:
+    """Escape-direction Jacobians of a 256^2 image, a=0.9, i=60 deg, rtol 1e-8: tape + two adjoint sweeps."""
+    import torch
+
+    from kerrray_amd.adjoint import Tape
+    from kerrray_amd.camera import BardeenCamera
+    from kerrray_amd.geodesics import IntegratorOptions, TerminationOptions
+
+    cam = BardeenCamera(radius=R_O, inclination_deg=60.0, fov=FOV, resolution=256)
+    a, b = cam.pixel_coordinates()
+    alpha = torch.as_tensor(np.ascontiguousarray(a.ravel()), device="cuda")
+    beta = torch.as_tensor(np.ascontiguousarray(b.ravel()), device="cuda")
+    integ, term = IntegratorOptions(method="rk45", rtol=1e-8, atol=1e-10), TerminationOptions()
+    t0 = time.perf_counter()
+    tape = Tape(hip, alpha, beta, 0.9, math.radians(60.0), integ=integ, term=term)
+    create_s = time.perf_counter() - t0
+    tape.jacobian()
+    one = torch.ones(tape.n, dtype=torch.float64, device="cuda")
+    ms = timed(lambda: tape.vjp(one, one), 3)
+    flags = tape.flags()
+    out = {"workload": "256x256 Bardeen image, a=0.9, i=60deg, rtol 1e-8: d(theta_inf, phi_inf)/d(alpha, beta, spin, theta_o)",
+           "escaped": int(tape.escaped.sum()), "accepted_steps_on_tape": tape.steps,
+           "tape_create_s": create_s, "record_kernel_ms": tape.record_ms, "vjp_ms": ms,
+           "gradients_per_s": int(tape.escaped.sum()) / (ms * 1e-3), "consistency_flags": flags}
+    tape.close()
+    log(f"adjoint 256^2: vjp {ms:.1f} ms for {out['escaped']} photons")
+    return out
+
+
+def traffic() -> dict | None:
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 def ragged_batch(hip, log) -> dict:
@@ -230,18 +353,31 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the precision and adjoint reports")
+    ap.add_argument("--form", choices=["auto", "fp64_plain", "fp64_planned", "fp32_fp64_repair"], default="auto")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 256 MiB device-to-device once (the reference kernel for the HBM counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        x = torch.ones(1 << 28, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copied_bytes": x.numel()}), flush=True)
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: the per-photon path has no cross-GPU step "
               "(photons are independent); run with --gpus 1", file=sys.stderr)
         return 2
     import torch
 
-    from kerrray_amd import _lib, flops
-    from kerrray_amd.backends import HipBackend, kernel_params
+    from kerrray_amd import flops
+    from kerrray_amd.backends import HipBackend
     from kerrray_amd.geodesics import IntegratorOptions, TerminationOptions
     from kerrray_amd.geometry import Spacetime
-    from kerrray_amd.shadow import kr_image
 
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
@@ -252,42 +388,56 @@ def main() -> int:
     st = Spacetime(1.0, SPIN)
     integ = IntegratorOptions(method="rk45", rtol=RTOL, atol=ATOL)
     term = TerminationOptions()
-    params = kernel_params(st, integ, term)
-    img = kr_image(st, R_O, INCL, FOV, RES)
+    ref = golden_mask()
     n = RES * RES
-    bufs, outs = device_outputs(torch, n, torch.float64)
-    stream = torch.cuda.current_stream()
-    L = _lib.lib()
 
-    def step() -> None:
-        _lib.check(L.kr_trace_image_dev(hip.handle.ptr, ctypes.byref(params), ctypes.byref(img), ctypes.byref(outs),
-                                        None, ctypes.c_void_p(stream.cuda_stream)), "kr_trace_image_dev")
+    runners = {"fp64_plain": PlainRunner(hip, st, integ, term),
+               "fp64_planned": PlannedRunner(hip, False, st, integ, term),
+               "fp32_fp64_repair": PlannedRunner(hip, True, st, integ, term)}
+    forms = {}
+    for name, run in runners.items():
+        forms[name] = form_report(run, ref)
+        log(f"{name}: {forms[name]['kernel_ms']:.3f} ms, mask identical {forms[name]['mask_identical_to_reference']}")
+    if args.form == "auto":
+        ok = [k for k, v in forms.items() if v["mask_identical_to_reference"] is not False]
+        chosen = min(ok, key=lambda k: forms[k]["kernel_ms"])
+    else:
+        chosen = args.form
+    run = runners[chosen]
+    log(f"timed form: {chosen}")
 
     for _ in range(args.warmup):
-        step()
+        run()
     torch.cuda.synchronize()
     log(f"warmup {args.warmup} done")
+    stream = torch.cuda.current_stream()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ev0.record(stream)
     for _ in range(args.steps):
-        step()
+        run()
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kernel_ms = ev0.elapsed_time(ev1) / args.steps
     ms_per_step = 1e3 * wall / args.steps
-    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step wall, {kernel_ms:.3f} ms/kernel (events)")
+    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step wall, {kernel_ms:.3f} ms/launch (events)")
 
-    state = bufs["state"].cpu().numpy()
-    n_steps = bufs["n_steps"].cpu().numpy()
-    n_rej = bufs["n_rejected"].cpu().numpy()
-    ref = golden_mask()
+    state = run.bufs["state"].cpu().numpy()
+    n_steps = run.bufs["n_steps"].cpu().numpy()
+    n_rej = run.bufs["n_rejected"].cpu().numpy()
     mask_identical = None if ref is None else bool(np.array_equal(state, ref))
     fl = flops.batch_flops(n_steps, n_rej)
     achieved = fl / (kernel_ms * 1e-3) / 1e12
     counts = np.bincount(state, minlength=7)
+    described = {
+        "fp64_plain": "FP64, plain one-photon-per-lane kernel",
+        "fp64_planned": "FP64, persistent kernel over a longest-first work list (edge band, inside, outside); "
+                        "results bitwise those of the plain kernel",
+        "fp32_fp64_repair": "FP32 right-hand sides with FP64 state/update/error norm/step control, FP64 near "
+                            "horizon, axis and turning points, FP64 edge band and repair",
+    }
     line = {
         "metric": METRIC,
         "value": n / (ms_per_step * 1e-3),
@@ -299,11 +449,12 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if chosen != "fp32_fp64_repair" else "f32+f64",
         "data": "synthetic: Bardeen image-plane initial conditions generated on the device",
         "config": {
             "workload": "512x512 Bardeen image, a=0.9, i=60deg, r_o=1000M, alpha,beta in [-10,10]M, "
-                        "DP5(4) rtol=1e-6 atol=1e-8, FP64, plain one-photon-per-lane kernel",
+                        "DP5(4) rtol=1e-6 atol=1e-8; " + described[chosen],
+            "form": chosen,
             "photons": n,
             "mean_attempts_per_photon": float(np.mean(n_steps + n_rej)),
             "captured": int(counts[2]),
@@ -320,20 +471,25 @@ def main() -> int:
             "frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
             "traffic": None,
             "algorithmic_flops_per_launch": fl,
+            "launch": "kr_trace_image_dev" if chosen == "fp64_plain" else
+                      "kr_trace_image_planned_dev (plan + persistent trace + verify + repair kernels)",
         },
     }
+    t = traffic()
+    if t is not None and chosen in t:
+        line["roofline"]["traffic"] = t[chosen]["bytes_per_launch"]
+        line["roofline"]["traffic_source"] = t[chosen]["source"]
+    line["forms"] = forms
     log("persistent kernel, single image (uniform constants)")
-    line["forms"] = {"fp64_plain": {"photons_per_s": n / (kernel_ms * 1e-3), "kernel_ms": kernel_ms,
-                                    "roofline_frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
-                                    "mask_identical_to_reference": mask_identical}}
-    line["forms"]["fp64_persistent"] = persistent_single(hip, ref)
-    log("planned FP64 (longest-first schedule)")
-    line["forms"]["fp64_planned"] = planned_image(hip, ref, fp32_bulk=False)
-    log("planned FP32 bulk + FP64 repair")
-    line["forms"]["fp32_fp64_repair"] = planned_image(hip, ref, fp32_bulk=True)
+    forms["fp64_persistent"] = persistent_single(hip, ref)
     if not args.no_batch:
         log("ragged batch (64 images, autotuned, hipGraph)")
-        line["forms"]["batched_ragged64"] = ragged_batch(hip, log)
+        forms["batched_ragged64"] = ragged_batch(hip, log)
+    if not args.no_extras:
+        del runners, run
+        torch.cuda.empty_cache()
+        line["precision_1024_a099_i85"] = precision_report(hip, log)
+        line["adjoint_256_a09_i60"] = adjoint_report(hip, log)
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
         threads = min(threads, len(os.sched_getaffinity(0)))

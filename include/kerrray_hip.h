@@ -262,6 +262,9 @@ kr_status kr_trace_image_planned_dev(kr_handle h, const kr_params* p, const kr_i
                                      double centre_alpha, double centre_beta, const double* d_radii,
                                      int32_t n_angles, const kr_outputs* d_out, void* d_work, size_t work_bytes,
                                      void* stream);
+/* Counters of the last planned trace that used the workspace d_work, read
+ * after the work enqueued on `stream` (synchronises that stream). */
+kr_status kr_planned_read_stats(const void* d_work, size_t work_bytes, kr_planned_stats* stats, void* stream);
 
 /* ---------------------------------------------------------------------
  * Differentiation form: escape-direction gradients by the discrete adjoint

@@ -41,6 +41,7 @@ EXPORTED = (
     "kr_planned_workspace_bytes",
     "kr_trace_image_planned",
     "kr_trace_image_planned_dev",
+    "kr_planned_read_stats",
     "kr_tape_create",
     "kr_tape_recorded",
     "kr_tape_steps",
@@ -201,6 +202,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.kr_trace_image_planned_dev.argtypes = [vp, P(KrParams), P(KrImage), P(KrPlannedOpts), ctypes.c_double,
                                              ctypes.c_double, vp, ctypes.c_int32, P(KrOutputs), vp, ctypes.c_size_t,
                                              vp]
+    L.kr_planned_read_stats.restype = ctypes.c_int
+    L.kr_planned_read_stats.argtypes = [vp, ctypes.c_size_t, P(KrPlannedStats), vp]
     L.kr_tape_create.restype = ctypes.c_int
     L.kr_tape_create.argtypes = [vp, P(KrParams), P(KrImage), vp, vp, ctypes.c_int64, ctypes.c_int32, P(KrOutputs),
                                  vp, vp, P(vp)]

@@ -456,3 +456,20 @@ extern "C" kr_status kr_trace_image_planned(kr_handle h, const kr_params* p, con
     }
     return KR_OK;
 }
+
+extern "C" kr_status kr_planned_read_stats(const void* d_work, size_t work_bytes, kr_planned_stats* stats,
+                                           void* stream) {
+    host::clear_error();
+    if (!d_work || !stats) return host::fail(KR_ERR_INVALID, "workspace or stats is NULL");
+    if (work_bytes < sizeof(PlanCounters)) return host::fail(KR_ERR_INVALID, "workspace is too small");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    PlanCounters pc;
+    KR_HIP(hipMemcpyAsync(&pc, d_work, sizeof(pc), hipMemcpyDeviceToHost, s));
+    KR_HIP(hipStreamSynchronize(s));
+    stats->planned_band = static_cast<int64_t>(pc.seg[0]);
+    stats->planned_inside = static_cast<int64_t>(pc.seg[1]);
+    stats->planned_outside = static_cast<int64_t>(pc.seg[2]);
+    stats->repaired = static_cast<int64_t>(pc.pad);
+    stats->fp64_attempts = static_cast<int64_t>(pc.fp64_attempts);
+    return KR_OK;
+}

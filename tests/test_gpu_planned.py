@@ -55,3 +55,22 @@ def test_fp32_bulk_with_fp64_repair_keeps_the_fp64_mask(hip, case) -> None:
     assert diff == 0
     assert np.array_equal(mixed.result.state, plain.result.state)
     assert mixed.n_failed == 0
+
+
+def test_precision_form_mask_equals_the_oracle_fp64_mask_1024(hip, oracle) -> None:
+    """1024^2, a=0.99, i=85 deg, rtol 1e-6: the FP32+FP64-repair mask, the GPU FP64 mask and the CPU oracle's."""
+    from kerrray_amd.camera import BardeenCamera
+    from kerrray_amd.geometry import Spacetime
+    from kerrray_amd.planned import PlannedOptions, compute_shadow_planned
+
+    st = Spacetime(1.0, 0.99)
+    integ, term = options(rtol=1e-6, atol=1e-8)
+    cam = BardeenCamera(radius=1000.0, inclination_deg=85.0, fov=10.0, resolution=1024)
+    mixed, stats = compute_shadow_planned(st, cam, integ, term, backend=hip, options=PlannedOptions(fp32_bulk=True))
+    Y0 = oracle.bardeen_image(1.0, 0.99, 1000.0, 85.0, 10.0, 1024)
+    ref = oracle.integrate_batch(st, Y0, integ, term, dynamic=True)
+    ref_captured = (ref.state == 2).reshape(1024, 1024)
+    diff = int(np.count_nonzero(mixed.captured != ref_captured))
+    print(f"[precision 1024^2 a=0.99 i=85] mask diff vs oracle FP64 {diff}, band {stats.planned_band}, "
+          f"repaired {stats.repaired}, fp64 attempts {stats.fp64_fraction:.3f}", flush=True)
+    assert diff == 0
