@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define KR_ABI_VERSION 1
+#define KR_ABI_VERSION 2
 
 typedef enum kr_status {
     KR_OK = 0,
@@ -225,7 +225,9 @@ kr_status kr_batch_destroy(kr_batch b);
  *  longest_first: order of the work list (the classes still decide the
  *    precision).
  *  band_priority: raise the issue priority of wavefronts holding an edge-band
- *    photon (the longest ones), which otherwise finish last. */
+ *    photon (the longest ones), which otherwise finish last.
+ *  waves_per_simd: resident wavefronts per SIMD the persistent kernel is
+ *    compiled for; more hide more FP64 latency, fewer registers each. */
 typedef struct kr_planned_opts {
     double horizon_margin;
     double axis_sin;
@@ -237,6 +239,8 @@ typedef struct kr_planned_opts {
     int32_t block_size;
     int32_t longest_first; /* 1: work list edge band -> inside -> outside; 0: row-major */
     int32_t band_priority; /* 1: wavefronts tracing an edge-band photon issue first on their SIMD */
+    int32_t waves_per_simd; /* 2, 3 or 4: occupancy the trace kernel's registers are allocated for */
+    int32_t reserved;
 } kr_planned_opts;
 
 typedef struct kr_planned_stats {

@@ -114,6 +114,23 @@ KR_HD float wabs(float x) { return fabsf(x); }
 KR_HD bool wfinite(double x) { return isfinite(x); }
 KR_HD bool wfinite(float x) { return isfinite(x); }
 
+// a / b rounded to nearest, from y = 1 / b rounded to nearest, by
+// Markstein's theorem (Markstein 1990; Muller et al., Handbook of
+// Floating-Point Arithmetic, section 4.7): if y = RN(1/b) and q is within
+// one ulp of a/b, then r = a - b q is exact under a fused multiply-add and
+// RN(q + r y) = RN(a/b).  q0 = RN(a y) can be up to 1.5 ulp off; one
+// correction brings it within 1/2 ulp plus a term of order 2^-52 ulp, and the
+// second correction, under the theorem, gives exactly the IEEE quotient.  A zero
+// numerator returns a * y, the zero with the quotient's sign.  Valid away
+// from overflow and underflow, which the metric's operands here never reach; a
+// zero or non-finite b gives a non-finite result, as the division does.
+KR_HD double div_by(double a, double b, double y) {
+    const double q0 = a * y;
+    const double q1 = fma(fma(-b, q0, a), y, q0);
+    const double q2 = fma(fma(-b, q1, a), y, q1);
+    return a == 0.0 ? q0 : q2;
+}
+
 // numba_kernel.py:111-154 (= geometry/metric.py:166-195,265-313). Also
 // returns sin(th), cos(th) for reuse by the classifier and diagnostics.
 KR_HD void inverse_metric_terms(const Params& P, double r, double th, double (&out)[15],
@@ -130,10 +147,15 @@ KR_HD void inverse_metric_terms(const Params& P, double r, double th, double (&o
     const double r2a2 = r2 + a2;
     const double big_a = r2a2 * r2a2 - a2 * dlt * s2;
     const double sd = sig * dlt;
-    out[0] = -big_a / sd;
-    out[1] = P.geom[G_NEG_2MA] * r / sd;
-    out[2] = dlt / sig;
-    out[3] = 1.0 / sig;
+    // The seventeen quotients of numba_kernel.py:122-154 share four
+    // denominators; each of those is inverted once (correctly rounded) and
+    // div_by gives every quotient bit for bit.
+    const double i_sd = 1.0 / sd;
+    const double i_sig = 1.0 / sig;
+    out[0] = div_by(-big_a, sd, i_sd);
+    out[1] = div_by(P.geom[G_NEG_2MA] * r, sd, i_sd);
+    out[2] = div_by(dlt, sig, i_sig);
+    out[3] = i_sig;
     out[4] = (dlt - a2 * s2) / (sd * s2);
     const double sig_r = 2.0 * r;
     const double sig_th = (-a2) * sin2th;
@@ -144,16 +166,18 @@ KR_HD void inverse_metric_terms(const Params& P, double r, double th, double (&o
     const double sd_th = sig_th * dlt;
     const double sd2 = sd * sd;
     const double sig2 = sig * sig;
-    out[5] = -(a_r * sd - big_a * sd_r) / sd2;
-    out[6] = P.geom[G_NEG_2MA] * (sd - r * sd_r) / sd2;
-    out[7] = (dlt_r * sig - dlt * sig_r) / sig2;
-    out[8] = -sig_r / sig2;
-    out[9] = -sig_r / (sig2 * s2) + a2 * sd_r / sd2;
-    out[10] = -(a_th * sd - big_a * sd_th) / sd2;
-    out[11] = P.geom[G_POS_2MA] * r * sd_th / sd2;
-    out[12] = -dlt * sig_th / sig2;
-    out[13] = -sig_th / sig2;
-    out[14] = -(sig_th * s2 + sig * sin2th) / (sig2 * (s2 * s2)) + a2 * sd_th / sd2;
+    const double i_sd2 = 1.0 / sd2;
+    const double i_sig2 = 1.0 / sig2;
+    out[5] = div_by(-(a_r * sd - big_a * sd_r), sd2, i_sd2);
+    out[6] = div_by(P.geom[G_NEG_2MA] * (sd - r * sd_r), sd2, i_sd2);
+    out[7] = div_by(dlt_r * sig - dlt * sig_r, sig2, i_sig2);
+    out[8] = div_by(-sig_r, sig2, i_sig2);
+    out[9] = -sig_r / (sig2 * s2) + div_by(a2 * sd_r, sd2, i_sd2);
+    out[10] = div_by(-(a_th * sd - big_a * sd_th), sd2, i_sd2);
+    out[11] = div_by(P.geom[G_POS_2MA] * r * sd_th, sd2, i_sd2);
+    out[12] = div_by(-dlt * sig_th, sig2, i_sig2);
+    out[13] = div_by(-sig_th, sig2, i_sig2);
+    out[14] = -(sig_th * s2 + sig * sin2th) / (sig2 * (s2 * s2)) + div_by(a2 * sd_th, sd2, i_sd2);
     s_out = s;
     c_out = c;
 }
@@ -225,6 +249,67 @@ KR_HD void dp5_step(const Params& P, const T (&y)[NA], T p_t, T p_ph, T h, const
     for (int i = 0; i < NA; ++i) {
         y_new[i] = y[i] + h * (T(a60) * k0[i] + T(a62) * k2[i] + T(a63) * k3[i] + T(a64) * k4[i] + T(a65) * k5[i]);
         esum[i] = T(e0) * k0[i] + T(e2) * k2[i] + T(e3) * k3[i] + T(e4) * k4[i] + T(e5) * k5[i];
+    }
+    rhs(P, y_new, pt, pph, k6, s_new, c_new);
+#pragma unroll
+    for (int i = 0; i < NA; ++i) err[i] = h * (esum[i] + T(e6) * k6[i]);
+}
+
+// Where dp5_step_stored keeps the stage slopes k1..k4 of one lane: slope s,
+// component c at base[(s * NA + c) * stride].  In the persistent kernels
+// base points into LDS at the lane's own column (stride = lanes per block),
+// which takes 24 values per lane out of the register file for the whole
+// attempt.
+template <typename T>
+struct SlopeStore {
+    T* base;
+    int stride;
+    KR_HD T& at(int s, int c) const { return base[(s * NA + c) * stride]; }
+};
+
+// dp5_step with k1..k4 held in a SlopeStore.  Every value and every
+// operation is the one dp5_step performs, in the same order, so the results
+// are identical bit for bit.
+template <typename T>
+KR_HD void dp5_step_stored(const Params& P, const T (&y)[NA], T p_t, T p_ph, T h, const T (&k0)[NA],
+                           const SlopeStore<T>& K, T (&y_new)[NA], T (&err)[NA], T (&k6)[NA], double& s_new,
+                           double& c_new) {
+    using namespace dp;
+    T k[NA], ys[NA];
+    double s, c;
+    const T pt = p_t + T(0), pph = p_ph + T(0);
+#pragma unroll
+    for (int i = 0; i < NA; ++i) ys[i] = y[i] + h * (T(a10) * k0[i]);
+    rhs(P, ys, pt, pph, k, s, c);
+#pragma unroll
+    for (int i = 0; i < NA; ++i) K.at(0, i) = k[i];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) ys[i] = y[i] + h * (T(a20) * k0[i] + T(a21) * K.at(0, i));
+    rhs(P, ys, pt, pph, k, s, c);
+#pragma unroll
+    for (int i = 0; i < NA; ++i) K.at(1, i) = k[i];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) ys[i] = y[i] + h * (T(a30) * k0[i] + T(a31) * K.at(0, i) + T(a32) * K.at(1, i));
+    rhs(P, ys, pt, pph, k, s, c);
+#pragma unroll
+    for (int i = 0; i < NA; ++i) K.at(2, i) = k[i];
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+        ys[i] = y[i] + h * (T(a40) * k0[i] + T(a41) * K.at(0, i) + T(a42) * K.at(1, i) + T(a43) * K.at(2, i));
+    rhs(P, ys, pt, pph, k, s, c);
+#pragma unroll
+    for (int i = 0; i < NA; ++i) K.at(3, i) = k[i];
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+        ys[i] = y[i] + h * (T(a50) * k0[i] + T(a51) * K.at(0, i) + T(a52) * K.at(1, i) + T(a53) * K.at(2, i) +
+                            T(a54) * K.at(3, i));
+    rhs(P, ys, pt, pph, k, s, c);  // k5
+    T esum[NA];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+        y_new[i] = y[i] + h * (T(a60) * k0[i] + T(a62) * K.at(1, i) + T(a63) * K.at(2, i) + T(a64) * K.at(3, i) +
+                               T(a65) * k[i]);
+        esum[i] = T(e0) * k0[i] + T(e2) * K.at(1, i) + T(e3) * K.at(2, i) + T(e4) * K.at(3, i) + T(e5) * k[i];
     }
     rhs(P, y_new, pt, pph, k6, s_new, c_new);
 #pragma unroll
@@ -402,6 +487,20 @@ struct ExactStepper {
                           T (&k6)[NA], double& s_new, double& c_new) const {
         if (ADAPTIVE)
             dp5_step(P, R.y, R.p_t, R.p_ph, h_eff, R.f, y_new, err, k6, s_new, c_new);
+        else
+            rk4_step(P, R.y, R.p_t, R.p_ph, h_eff, R.f, y_new, k6, s_new, c_new);
+    }
+};
+
+// The exact attempt with the stage slopes in a SlopeStore (RK4 attempts
+// are short and keep theirs in registers).
+template <typename T, bool ADAPTIVE>
+struct StoredStepper {
+    SlopeStore<T> K;
+    KR_HD void operator()(const Params& P, const RayState<T>& R, T h_eff, T (&y_new)[NA], T (&err)[NA],
+                          T (&k6)[NA], double& s_new, double& c_new) const {
+        if (ADAPTIVE)
+            dp5_step_stored(P, R.y, R.p_t, R.p_ph, h_eff, R.f, K, y_new, err, k6, s_new, c_new);
         else
             rk4_step(P, R.y, R.p_t, R.p_ph, h_eff, R.f, y_new, k6, s_new, c_new);
     }

@@ -112,10 +112,18 @@ __global__ void plan_scatter_kernel(PlanDev pl, BardeenImage B, PlanCounters* c,
     if (valid) work[pos] = idx | ((k == 0 || !fp32_bulk) ? FP64_FLAG : 0) | (k == 0 ? BAND_FLAG : 0);
 }
 
-template <int BLOCK, bool MIXED>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) void planned_trace_kernel(
+// WAVES: wavefronts per SIMD the register allocation is made for (2: 256
+// VGPRs, 3: 168, 4: 128).  Up to 3 the stage slopes k1..k4 live in LDS
+// (192 B per lane: 147 KB per CU at 3 waves/SIMD); at 4 they stay private.
+template <int BLOCK, bool MIXED, int WAVES>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void planned_trace_kernel(
     Params P, BardeenImage B, MixedCfg M, const int64_t* work, PlanCounters* c, int refill_interval,
     int band_priority, uint8_t* used_fp64, RowsOut<double> out, int32_t* flags) {
+    constexpr bool LDS_SLOPES = WAVES <= 3;
+    __shared__ double slopes[LDS_SLOPES ? 4 * NA * BLOCK : 1];
+    double private_slopes[LDS_SLOPES ? 1 : 4 * NA];
+    const SlopeStore<double> K = LDS_SLOPES ? SlopeStore<double>{&slopes[threadIdx.x], BLOCK}
+                                            : SlopeStore<double>{private_slopes, 1};
     const int lane = threadIdx.x & 63;
     const int64_t total = B.n * B.n;
     const Geom32 G = geom32(P);
@@ -174,9 +182,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) v
         if (!any_busy) break;
         if (busy) {
             if (MIXED && !exact) {
-                ray_attempt_with<double, true, false>(P, R, MixedStepper{G, M, &fp64_att});
+                ray_attempt_with<double, true, false>(P, R, MixedStepper{G, M, &fp64_att, K});
             } else {
-                ray_attempt<double, true, false>(P, R);
+                ray_attempt_with<double, true, false>(P, R, StoredStepper<double, true>{K});
             }
             if (R.state != RUNNING) {
                 store_row(out, idx, R);
@@ -291,16 +299,29 @@ kr_status check_opts(const kr_planned_opts* o) {
         return host::fail(KR_ERR_INVALID, "refill_interval must be in [1, 4096]");
     if (o->block_size != 64 && o->block_size != 128 && o->block_size != 256)
         return host::fail(KR_ERR_INVALID, "block_size must be 64, 128 or 256");
+    if (o->waves_per_simd < 2 || o->waves_per_simd > 4) return host::fail(KR_ERR_INVALID, "waves_per_simd must be 2, 3 or 4");
     if (!(o->horizon_margin >= 0.0) || !(o->axis_sin >= 0.0) || !(o->turn_drdl >= 0.0) || !(o->turn_rmax >= 0.0))
         return host::fail(KR_ERR_INVALID, "switch thresholds must be >= 0");
     return KR_OK;
 }
 
-template <int BLOCK, bool MIXED>
-void launch_planned_t(int grid, hipStream_t s, const Params& P, const BardeenImage& B, const MixedCfg& M,
+template <int BLOCK, bool MIXED, int WAVES>
+void launch_planned_t(int cus, hipStream_t s, const Params& P, const BardeenImage& B, const MixedCfg& M,
                       const Work& w, int refill, int band_priority, const RowsOut<double>& o) {
-    hipLaunchKernelGGL((planned_trace_kernel<BLOCK, MIXED>), dim3(grid), dim3(BLOCK), 0, s, P, B, M, w.work,
+    const int grid = cus * (4 * WAVES * 64 / BLOCK);  // one resident wave per SIMD slot
+    hipLaunchKernelGGL((planned_trace_kernel<BLOCK, MIXED, WAVES>), dim3(grid), dim3(BLOCK), 0, s, P, B, M, w.work,
                        w.counters, refill, band_priority, w.used_fp64, o, w.flags);
+}
+
+template <int BLOCK, bool MIXED>
+void launch_planned_w(int waves, int cus, hipStream_t s, const Params& P, const BardeenImage& B,
+                      const MixedCfg& M, const Work& w, int refill, int band_priority, const RowsOut<double>& o) {
+    if (waves == 3)
+        launch_planned_t<BLOCK, MIXED, 3>(cus, s, P, B, M, w, refill, band_priority, o);
+    else if (waves == 4)
+        launch_planned_t<BLOCK, MIXED, 4>(cus, s, P, B, M, w, refill, band_priority, o);
+    else
+        launch_planned_t<BLOCK, MIXED, 2>(cus, s, P, B, M, w, refill, band_priority, o);
 }
 
 kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B, const kr_planned_opts* o,
@@ -322,22 +343,20 @@ kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B,
     MixedCfg M{r_plus + o->horizon_margin, o->axis_sin, o->turn_rmax, o->turn_drdl};
     RowsOut<double> bo{static_cast<double*>(d_out.Y), d_out.lam, d_out.n_steps, d_out.n_rejected, d_out.state,
                    d_out.max_null_error, d_out.max_energy_drift, d_out.max_lz_drift, d_out.max_carter_drift};
-    const int blocks_per_cu = (4 * 2 * 64 + o->block_size - 1) / o->block_size;
-    const int grid = cus * blocks_per_cu;
     const bool mixed = o->fp32_bulk != 0;
-    const int refill = o->refill_interval, prio = o->band_priority ? 1 : 0;
+    const int refill = o->refill_interval, prio = o->band_priority ? 1 : 0, waves = o->waves_per_simd;
     switch (o->block_size) {
         case 64:
-            mixed ? launch_planned_t<64, true>(grid, s, P, B, M, w, refill, prio, bo)
-                  : launch_planned_t<64, false>(grid, s, P, B, M, w, refill, prio, bo);
+            mixed ? launch_planned_w<64, true>(waves, cus, s, P, B, M, w, refill, prio, bo)
+                  : launch_planned_w<64, false>(waves, cus, s, P, B, M, w, refill, prio, bo);
             break;
         case 128:
-            mixed ? launch_planned_t<128, true>(grid, s, P, B, M, w, refill, prio, bo)
-                  : launch_planned_t<128, false>(grid, s, P, B, M, w, refill, prio, bo);
+            mixed ? launch_planned_w<128, true>(waves, cus, s, P, B, M, w, refill, prio, bo)
+                  : launch_planned_w<128, false>(waves, cus, s, P, B, M, w, refill, prio, bo);
             break;
         default:
-            mixed ? launch_planned_t<256, true>(grid, s, P, B, M, w, refill, prio, bo)
-                  : launch_planned_t<256, false>(grid, s, P, B, M, w, refill, prio, bo);
+            mixed ? launch_planned_w<256, true>(waves, cus, s, P, B, M, w, refill, prio, bo)
+                  : launch_planned_w<256, false>(waves, cus, s, P, B, M, w, refill, prio, bo);
     }
     KR_HIP(hipGetLastError());
     for (int round = 0; round < REPAIR_ROUNDS; ++round) {

@@ -49,6 +49,7 @@ class PlannedOptions:
     block_size: int = 128
     longest_first: bool = True
     band_priority: bool = False
+    waves_per_simd: int = 2
 
     def kr(self) -> _lib.KrPlannedOpts:
         o = _lib.KrPlannedOpts()
@@ -62,6 +63,7 @@ class PlannedOptions:
         o.block_size = self.block_size
         o.longest_first = 1 if self.longest_first else 0
         o.band_priority = 1 if self.band_priority else 0
+        o.waves_per_simd = self.waves_per_simd
         return o
 
 

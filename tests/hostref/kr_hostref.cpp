@@ -10,6 +10,7 @@
 #include "kr_device.h"
 #include "kr_image.h"
 #include <cmath>
+#include <cstring>
 
 namespace {
 
@@ -97,6 +98,32 @@ extern "C" int krh_trace_image(const kr_params* p, const kr_image* img, const kr
 
 #include "kr_mixed.h"
 
+// Every photon of an image with the stage slopes k1..k4 kept in a
+// SlopeStore (an interleaved buffer, as in LDS) instead of registers.
+extern "C" int krh_trace_image_stored(const kr_params* p, const kr_image* img, const kr_outputs* o, int nt) {
+    kr::BardeenImage B;
+    if (kr::bardeen_image_scalars(img, true, &B)) return 1;
+    const kr::Params P = params_of(p);
+    if (nt <= 0) nt = omp_get_max_threads();
+    const int64_t n = B.n;
+    constexpr int LANES = 16;
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 16)
+    for (int64_t idx = 0; idx < n * n; ++idx) {
+        const int64_t i = idx / n, j = idx % n;
+        double shared[4 * kr::NA * LANES];  // this photon uses column idx % LANES
+        const kr::SlopeStore<double> K{&shared[idx % LANES], LANES};
+        double y0[8];
+        kr::bardeen_state(B, kr::pixel_centre(B.fov, j, n), -kr::pixel_centre(B.fov, i, n), y0);
+        kr::RayState<double> R;
+        kr::ray_init<double, true>(P, y0, R);
+        while (R.state == kr::RUNNING) kr::ray_attempt_with<double, true, false>(P, R, kr::StoredStepper<double, true>{K});
+        kr::RayResult<double> r;
+        kr::ray_result<double, false>(R, r);
+        store<double, true, false>(r, o, idx);
+    }
+    return 0;
+}
+
 // Every photon of an image with the FP32-bulk stepper (no band, no repair):
 // the CPU view of what the mixed attempt does to accuracy.
 extern "C" int krh_trace_image_mixed(const kr_params* p, const kr_image* img, double horizon_margin,
@@ -119,7 +146,8 @@ extern "C" int krh_trace_image_mixed(const kr_params* p, const kr_image* img, do
         kr::RayState<double> R;
         kr::ray_init<double, true>(P, y0, R);
         int64_t f64 = 0;
-        const kr::MixedStepper step{G, M, &f64};
+        double slopes[4 * kr::NA];
+        const kr::MixedStepper step{G, M, &f64, kr::SlopeStore<double>{slopes, 1}};
         while (R.state == kr::RUNNING) kr::ray_attempt_with<double, true, false>(P, R, step);
         kr::RayResult<double> r;
         kr::ray_result<double, false>(R, r);
@@ -250,4 +278,17 @@ extern "C" void krh_step_adjoint(const kr_params* p, double m, double a, const d
     out[6] = ptb;
     out[7] = pphb;
     out[8] = ab;
+}
+
+// div_by (kr_device.h) against the IEEE quotient on n operand pairs:
+// returns the number of results whose bits differ.
+extern "C" int64_t krh_div_by_mismatches(const double* a, const double* b, int64_t n) {
+    int64_t bad = 0;
+#pragma omp parallel for reduction(+ : bad)
+    for (int64_t i = 0; i < n; ++i) {
+        const double q = a[i] / b[i];
+        const double m = kr::div_by(a[i], b[i], 1.0 / b[i]);
+        bad += std::memcmp(&q, &m, sizeof(double)) != 0;
+    }
+    return bad;
 }

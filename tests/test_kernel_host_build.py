@@ -121,3 +121,48 @@ def test_schwarzschild_and_negative_spin_are_bitwise_the_oracle(hostref, oracle)
         integ, term = options(rtol=1e-9, atol=1e-11)
         Y0 = bardeen_initial_states(BardeenCamera(radius=1000.0, inclination_deg=incl, fov=8.0, resolution=8), st)
         same(run_y0(hostref, st, Y0, integ, term), oracle.integrate_batch(st, Y0, integ, term))
+
+
+def test_slopes_held_outside_registers_give_the_same_bits(hostref, golden2) -> None:
+    """dp5_step_stored (k1..k4 in an interleaved store, as the persistent kernels keep them in LDS)."""
+    from kerrray_amd.backends import HostBatch, kernel_params
+    from kerrray_amd.camera import BardeenCamera
+    from kerrray_amd.geometry import Spacetime
+    from kerrray_amd.shadow import kr_image
+
+    hostref.krh_trace_image_stored.restype = ctypes.c_int
+    hostref.krh_trace_image_stored.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    st = Spacetime(1.0, 0.9)
+    integ, term = options(rtol=1e-6, atol=1e-8)
+    cam = BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=10.0, resolution=128)
+    p = kernel_params(st, integ, term)
+    img = kr_image(st, cam.radius, cam.inclination_deg, cam.fov, cam.resolution)
+    out = HostBatch(cam.n_rays, np.dtype(np.float64), False)
+    assert hostref.krh_trace_image_stored(ctypes.addressof(p), ctypes.addressof(img), ctypes.addressof(out.outputs), 0) == 0
+    same(out.result(0.0), run_image(hostref, st, cam, integ, term))
+
+
+def test_shared_reciprocal_quotients_are_the_ieee_quotients(hostref) -> None:
+    """div_by (one reciprocal per denominator, two Markstein corrections) against a / b, bit for bit.
+
+    Random significands over a wide exponent range, signed zeros, powers of
+    two, all-ones significands and the operand ranges the metric sees.
+    """
+    hostref.krh_div_by_mismatches.restype = ctypes.c_int64
+    hostref.krh_div_by_mismatches.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+    rng = np.random.default_rng(11)
+    n = 4_000_000
+    cases = []
+    m = rng.uniform(1.0, 2.0, (2, n)) * rng.choice([-1.0, 1.0], (2, n))
+    e = rng.integers(-300, 300, (2, n))
+    cases.append((np.ldexp(m[0], e[0]), np.ldexp(m[1], e[1] // 2)))
+    cases.append((rng.uniform(-1e6, 1e6, n), rng.uniform(1e-12, 1e12, n)))
+    ones = np.nextafter(2.0, 0.0)  # significand of all ones
+    special_b = np.array([1.0, 2.0, 0.5, 3.0, ones, -ones, 1.0 + 2**-52, 7.0, -1.0])
+    special_a = np.array([0.0, -0.0, 1.0, -1.0, ones, 3.0, 1.0 + 2**-52, 1e-300, -5e300])
+    A, Bm = np.meshgrid(special_a, special_b)
+    cases.append((A.ravel(), Bm.ravel()))
+    cases.append((rng.uniform(1.0, 2.0, n), np.full(n, ones)))
+    for a, b in cases:
+        a, b = np.ascontiguousarray(a, dtype=np.float64), np.ascontiguousarray(b, dtype=np.float64)
+        assert hostref.krh_div_by_mismatches(a.ctypes.data, b.ctypes.data, a.size) == 0
