@@ -309,6 +309,13 @@ kr_status kr_tape_vjp(kr_tape t, const double* d_g_theta, const double* d_g_phi,
 kr_status kr_tape_flags(kr_tape t, int32_t* flags);
 kr_status kr_tape_destroy(kr_tape t);
 
+/* Escape direction at r = infinity of n FP64 result rows (d_Y (n, 8), d_state
+ * (n) as written by the trace entry points, for the geometry in p): theta
+ * and phi carried from each escaped photon's final state along its geodesic
+ * to u = 1/r = 0; NaN for rows that did not escape.  Enqueued on `stream`. */
+kr_status kr_escape_directions_dev(kr_handle h, const kr_params* p, const double* d_Y, const int64_t* d_state,
+                                   int64_t n, double* d_theta_inf, double* d_phi_inf, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

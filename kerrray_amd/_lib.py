@@ -49,6 +49,7 @@ EXPORTED = (
     "kr_tape_vjp",
     "kr_tape_flags",
     "kr_tape_destroy",
+    "kr_escape_directions_dev",
 )
 
 
@@ -221,6 +222,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.kr_tape_flags.argtypes = [vp, P(ctypes.c_int32)]
     L.kr_tape_destroy.restype = ctypes.c_int
     L.kr_tape_destroy.argtypes = [vp]
+    L.kr_escape_directions_dev.restype = ctypes.c_int
+    L.kr_escape_directions_dev.argtypes = [vp, P(KrParams), vp, vp, ctypes.c_int64, vp, vp, vp]
 
 
 def _share_torch_runtime() -> None:

@@ -46,8 +46,8 @@ from kerrray_amd.geometry import Spacetime
 from kerrray_amd.photons import TerminationState
 
 __all__ = [
-    "EdgeFit", "EdgeRadius", "EscapeDirection", "Tape", "edge_radii", "escape_directions",
-    "fit_spin_inclination", "lensing_jacobian",
+    "EdgeFit", "EdgeRadius", "EscapeDirection", "Tape", "directions_at_infinity", "edge_radii",
+    "escape_directions", "fit_spin_inclination", "lensing_jacobian",
 ]
 
 DEFAULT_SEGMENT = 32
@@ -75,6 +75,23 @@ def _outputs(n: int, device: torch.device) -> tuple[dict[str, torch.Tensor], _li
     o.max_null_error, o.max_energy_drift, o.max_lz_drift, o.max_carter_drift = (
         bufs[k].data_ptr() for k in ("m0", "m1", "m2", "m3"))
     return bufs, o
+
+
+def directions_at_infinity(backend, st: Spacetime, integ: IntegratorOptions, term: TerminationOptions,
+                           Y: torch.Tensor, state: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+    """(theta_inf, phi_inf) of FP64 result rows Y (n, 8) on the device; NaN where state is not ESCAPED."""
+    engine = _engine(backend)
+    Y = Y.to(torch.float64).contiguous()
+    state = state.to(torch.int64).contiguous()
+    n = int(state.numel())
+    th = torch.empty(n, dtype=torch.float64, device=Y.device)
+    ph = torch.empty_like(th)
+    params = kernel_params(st, integ, term)
+    stream = torch.cuda.current_stream(Y.device).cuda_stream
+    _lib.check(_lib.lib().kr_escape_directions_dev(engine.handle.ptr, ctypes.byref(params), Y.data_ptr(),
+                                                   state.data_ptr(), n, th.data_ptr(), ph.data_ptr(),
+                                                   ctypes.c_void_p(stream)), "kr_escape_directions_dev")
+    return th, ph
 
 
 class Tape:

@@ -152,6 +152,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     }
 }
 
+// Escape direction at infinity of every escaped row of a result (NaN for
+// the others).
+__global__ void escape_direction_kernel(const double* Y, const int64_t* state, int64_t n, double a, double m,
+                                        double* theta_inf, double* phi_inf) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double th = __builtin_nan(""), ph = __builtin_nan("");
+    if (state[i] == ESCAPED) {
+        const double* y = Y + i * 8;
+        ad::escape_tail(y[1], y[2], y[3], y[6], y[4], y[7], a, m, th, ph);
+    }
+    theta_inf[i] = th;
+    phi_inf[i] = ph;
+}
+
 __global__ void fill_f64_kernel(double* p, int64_t n) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
@@ -355,5 +370,24 @@ extern "C" kr_status kr_tape_destroy(kr_tape t) {
     (void)hipSetDevice(t->h->device);
     (void)hipStreamSynchronize(t->h->stream);
     free_tape(t);
+    return KR_OK;
+}
+
+extern "C" kr_status kr_escape_directions_dev(kr_handle h, const kr_params* p, const double* d_Y,
+                                              const int64_t* d_state, int64_t n, double* d_theta_inf,
+                                              double* d_phi_inf, void* stream) {
+    host::clear_error();
+    Params P;
+    kr_status st = host::validate(h, p, n, &P);
+    if (st != KR_OK) return st;
+    if (p->dtype != KR_FLOAT64) return host::fail(KR_ERR_UNSUPPORTED, "escape directions are computed from FP64 rows");
+    if (n > 0 && (!d_Y || !d_state || !d_theta_inf || !d_phi_inf)) return host::fail(KR_ERR_INVALID, "an array is NULL");
+    KR_HIP(hipSetDevice(h->device));
+    if (n > 0) {
+        hipLaunchKernelGGL(escape_direction_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+                           static_cast<hipStream_t>(stream), d_Y, d_state, n, P.geom[G_A], P.geom[G_MASS],
+                           d_theta_inf, d_phi_inf);
+        KR_HIP(hipGetLastError());
+    }
     return KR_OK;
 }

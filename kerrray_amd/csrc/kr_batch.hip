@@ -62,7 +62,7 @@ struct BatchArgs {
 };
 
 template <typename T, bool UNIFORM, int BLOCK, int WAVES>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void batch_kernel(
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void batch_kernel(
     BatchArgs a, JobDev uniform_job, BatchOut<T> out) {
     extern __shared__ JobDev lds_jobs[];
     if (!UNIFORM) {

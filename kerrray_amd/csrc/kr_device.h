@@ -59,6 +59,16 @@ enum {
 enum { P_LAMMAX, P_HALFPI, P_HMAX };
 enum { I_MAX_STEPS, I_GUARD, I_ADAPTIVE, I_DISK };
 
+// Constants the reference's host always packs with the same values (the
+// module constants behind numba_backend.py:100-115: step controller and
+// classification tolerances).  The C ABI rejects any other value, so the
+// kernels use them as literals rather than holding them in scalar registers.
+namespace fixed {
+constexpr double safety = 0.9, factor_min = 0.2, factor_max = 5.0, exponent = -1.0 / 5.0;
+constexpr double theta_lo = -1e-9, theta_hi = 3.141592653589793 + 1e-9, axis_sin = 1e-12;
+constexpr double half_pi = 0.5 * 3.141592653589793;
+}  // namespace fixed
+
 // Dormand-Prince 5(4) tableau (Dormand & Prince 1980; HNW 1993 Table 5.2),
 // the values of src/kerrray/geodesics/tableaus.py:63-86 evaluated in double.
 // The host entry points reject any other tableau, so these literals are what
@@ -120,15 +130,16 @@ KR_HD bool wfinite(float x) { return isfinite(x); }
 // one ulp of a/b, then r = a - b q is exact under a fused multiply-add and
 // RN(q + r y) = RN(a/b).  q0 = RN(a y) can be up to 1.5 ulp off; one
 // correction brings it within 1/2 ulp plus a term of order 2^-52 ulp, and the
-// second correction, under the theorem, gives exactly the IEEE quotient.  A zero
-// numerator returns a * y, the zero with the quotient's sign.  Valid away
+// second correction, under the theorem, gives exactly the IEEE quotient.  The
+// sign is taken from q0 = a y, which always carries the quotient's sign (this
+// also gives a zero numerator the signed zero the division gives).  Valid away
 // from overflow and underflow, which the metric's operands here never reach; a
 // zero or non-finite b gives a non-finite result, as the division does.
 KR_HD double div_by(double a, double b, double y) {
     const double q0 = a * y;
     const double q1 = fma(fma(-b, q0, a), y, q0);
     const double q2 = fma(fma(-b, q1, a), y, q1);
-    return a == 0.0 ? q0 : q2;
+    return copysign(q2, q0);
 }
 
 // numba_kernel.py:111-154 (= geometry/metric.py:166-195,265-313). Also
@@ -375,9 +386,9 @@ KR_HD int classify(const Params& P, const T (&v)[NA], T p_ph, const T (&f)[NA], 
     if (check_h && h < static_cast<T>(P.consts[C_HMIN])) return FAILURE;
     const T r = v[1];
     const T th = v[2];
-    if (r < T(0) || th < static_cast<T>(P.consts[C_THLO]) || th > static_cast<T>(P.consts[C_THHI]))
+    if (r < T(0) || th < static_cast<T>(fixed::theta_lo) || th > static_cast<T>(fixed::theta_hi))
         return OUT_OF_DOMAIN;
-    if (p_ph != T(0) && wabs(sin_th) < static_cast<T>(P.consts[C_AXIS])) return OUT_OF_DOMAIN;
+    if (p_ph != T(0) && wabs(sin_th) < static_cast<T>(fixed::axis_sin)) return OUT_OF_DOMAIN;
     if (r <= static_cast<T>(P.consts[C_RCAP])) return CAPTURED;
     if (r >= static_cast<T>(P.consts[C_RESC]) && f[1] > T(0)) return ESCAPED;
     if (lam >= P.params[P_LAMMAX] || n_steps >= P.ipar[I_MAX_STEPS]) return MAX_AFFINE;
@@ -537,8 +548,8 @@ KR_HD void ray_attempt_with(const Params& P, RayState<T>& R, const Stepper& step
             accept = false;
         } else {
             accept = en <= T(1);
-            const T factor = step_factor(static_cast<T>(P.consts[C_SAFETY]), static_cast<T>(P.consts[C_FMIN]),
-                                         static_cast<T>(P.consts[C_FMAX]), static_cast<T>(P.consts[C_EXP]), en,
+            const T factor = step_factor(static_cast<T>(fixed::safety), static_cast<T>(fixed::factor_min),
+                                         static_cast<T>(fixed::factor_max), static_cast<T>(fixed::exponent), en,
                                          R.rejected_before);
             if (!(accept && last)) {
                 h_new = h_eff * factor;
@@ -592,7 +603,7 @@ KR_HD void ray_attempt_with(const Params& P, RayState<T>& R, const Stepper& step
     int code = classify(P, R.y, R.p_ph, R.f, R.lam, R.n_steps, R.h, true, working_sin<T>(R.s_cur, R.y[2]));
     if (DISK) {
         // numba_kernel.py:439-454
-        const double half_pi = P.params[P_HALFPI];
+        const double half_pi = fixed::half_pi;
         const double th_old = static_cast<double>(y_prev[2]);
         const double th_new = static_cast<double>(R.y[2]);
         if ((th_old - half_pi) * (th_new - half_pi) < 0.0) {
@@ -605,7 +616,7 @@ KR_HD void ray_attempt_with(const Params& P, RayState<T>& R, const Stepper& step
                 const T cast = static_cast<T>(sfrac * static_cast<double>(full_new[i] - full_old[i]));
                 yc[i] = full_old[i] + cast;
             }
-            yc[2] = static_cast<T>(P.consts[C_HALFPI]);
+            yc[2] = static_cast<T>(fixed::half_pi);
             if (yc[1] >= static_cast<T>(P.consts[C_RIN]) && yc[1] <= static_cast<T>(P.consts[C_ROUT])) {
 #pragma unroll
                 for (int i = 0; i < 8; ++i) R.ev[i] = yc[i];

@@ -102,6 +102,17 @@ inline kr_status validate(kr_context* h, const kr_params* p, int64_t n, kr::Para
     for (int i = 0; i < 5; ++i)
         if (!same_bits(p->consts[i], fixed[i]))
             return fail(KR_ERR_INVALID, "consts[0:5] must be (0, 1, 2, -0.5, 8) as in numba_backend.py:100-108");
+    const int fixed_idx[8] = {kr::C_SAFETY, kr::C_FMIN, kr::C_FMAX, kr::C_EXP, kr::C_THLO, kr::C_THHI, kr::C_AXIS,
+                              kr::C_HALFPI};
+    const double fixed_val[8] = {kr::fixed::safety,   kr::fixed::factor_min, kr::fixed::factor_max,
+                                 kr::fixed::exponent, kr::fixed::theta_lo,   kr::fixed::theta_hi,
+                                 kr::fixed::axis_sin, kr::fixed::half_pi};
+    for (int i = 0; i < 8; ++i)
+        if (!same_bits(p->consts[fixed_idx[i]], fixed_val[i]))
+            return fail(KR_ERR_UNSUPPORTED,
+                        "consts[5:9], [14:17] and [20] must be the reference's controller and classification constants");
+    if (!same_bits(p->params[kr::P_HALFPI], kr::fixed::half_pi))
+        return fail(KR_ERR_UNSUPPORTED, "params[1] must be pi / 2");
     if (!tableau_ok(p))
         return fail(KR_ERR_UNSUPPORTED, "only the Dormand-Prince 5(4) / classical RK4 tableaus are implemented");
     if (p->ipar[kr::I_ADAPTIVE] != 0 && p->ipar[kr::I_ADAPTIVE] != 1)
