@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define KR_ABI_VERSION 2
+#define KR_ABI_VERSION 3
 
 typedef enum kr_status {
     KR_OK = 0,
@@ -213,19 +213,22 @@ kr_status kr_batch_destroy(kr_batch b);
 /* Options of a planned trace.
  *  band_px: pixels within band_px (Chebyshev) of the capture boundary are
  *    integrated in FP64: those within band_px + 1 pixels of the analytic
- *    edge from the start, and after the trace every pixel within band_px of
- *    a captured/not-captured transition of the mask, or whose photon ended
- *    neither captured nor escaped, that was not.
+ *    edge from the start (with those of the polar strip, see axis_lz), and
+ *    after the trace every pixel within band_px of a captured/not-captured
+ *    transition of the mask, or whose photon ended neither captured nor
+ *    escaped, that was not.
  *  fp32_bulk: 0 = every photon in FP64 (the longest-first schedule only);
  *    1 = other photons take FP32 right-hand sides with the FP64 update,
  *    error norm and step control, switching an attempt to FP64 when
  *    r < r_+ + horizon_margin, |sin theta| < axis_sin, or r < turn_rmax and
  *    |dr/dlambda| < turn_drdl.
  *  refill_interval, block_size: launch shape of the persistent kernel.
- *  longest_first: order of the work list (the classes still decide the
- *    precision).
- *  band_priority: raise the issue priority of wavefronts holding an edge-band
- *    photon (the longest ones), which otherwise finish last.
+ *  longest_first: order of the work list: the FP64 band and polar strip
+ *    first, then the other pixels inside the analytic edge, then those
+ *    outside (the longest photons start first); 0: row-major.
+ *  band_priority: raise the issue priority of wavefronts holding a photon of
+ *    the nearest distance classes (the longest ones), which otherwise finish
+ *    last.
  *  waves_per_simd: resident wavefronts per SIMD the persistent kernel is
  *    compiled for; more hide more FP64 latency, fewer registers each. */
 typedef struct kr_planned_opts {
@@ -237,16 +240,17 @@ typedef struct kr_planned_opts {
     int32_t fp32_bulk;
     int32_t refill_interval;
     int32_t block_size;
-    int32_t longest_first; /* 1: work list edge band -> inside -> outside; 0: row-major */
-    int32_t band_priority; /* 1: wavefronts tracing an edge-band photon issue first on their SIMD */
+    int32_t longest_first; /* 1: work list band/strip -> inside -> outside; 0: row-major */
+    int32_t band_priority; /* 1: wavefronts tracing the nearest photons issue first on their SIMD */
     int32_t waves_per_simd; /* 2, 3 or 4: occupancy the trace kernel's registers are allocated for */
     int32_t reserved;
+    double axis_lz; /* pixels with |alpha sin theta_o| < axis_lz are integrated in FP64 (near-polar photons) */
 } kr_planned_opts;
 
 typedef struct kr_planned_stats {
-    int64_t planned_band;    /* pixels scheduled first and integrated in FP64 */
-    int64_t planned_inside;  /* pixels inside the analytic edge */
-    int64_t planned_outside;
+    int64_t planned_band;    /* pixels scheduled first and integrated in FP64 (band and polar strip) */
+    int64_t planned_inside;  /* other pixels inside the analytic edge */
+    int64_t planned_outside; /* other pixels outside it */
     int64_t repaired;        /* pixels the mask check sent back to FP64 */
     int64_t fp64_attempts;   /* integration attempts taken in FP64 */
 } kr_planned_stats;

@@ -17,7 +17,7 @@ from kerrray_amd.build import LIB_PATH
 KR_OK, KR_ERR_INVALID, KR_ERR_DEVICE, KR_ERR_RADICAND, KR_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 KR_FLOAT64, KR_FLOAT32 = 0, 1
 KR_FLAG_RADICAND = 1
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # Every symbol the header declares (checked by the tests).
 EXPORTED = (
@@ -129,6 +129,7 @@ class KrPlannedOpts(ctypes.Structure):
         ("band_priority", ctypes.c_int32),
         ("waves_per_simd", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("axis_lz", ctypes.c_double),
     ]
 
 

@@ -34,33 +34,51 @@
 
 namespace kr {
 
+// Work-list entry: pixel index | bin << BIN_SHIFT | BAND_FLAG | FP64_FLAG.
 constexpr int64_t FP64_FLAG = int64_t(1) << 62;  // trace this photon with the exact FP64 attempt
-constexpr int64_t BAND_FLAG = int64_t(1) << 61;  // photon of the predicted edge band
-constexpr int64_t INDEX_MASK = BAND_FLAG - 1;
+constexpr int64_t BAND_FLAG = int64_t(1) << 61;  // photon of bin 0 (edge band or polar strip)
+constexpr int BIN_SHIFT = 57;                     // bins 0..15 in bits 57..60
+constexpr int64_t INDEX_MASK = (int64_t(1) << BIN_SHIFT) - 1;
+
+// Pixels are classed in NBINS bins by their distance d to the analytic edge,
+// which is what sets a photon's length (near the edge it orbits before it
+// leaves or falls in).  Bin 0: d <= band, or |xi| = |alpha sin theta_o| <
+// axis_lz (photons with L_z near 0 pass close to the poles); bin b >= 1:
+// band 2^(b-1) < d <= band 2^b, the last bin open-ended.  Bin 0 is traced in
+// FP64; the bin also sets a wavefront's issue priority (band_priority).
+constexpr int NBINS = 8;
 
 struct PlanDev {
     double ca, cb;          // centre of the polar radii table
     const double* radii;    // n_angles radii at angles 2 pi k / n_angles
     int32_t n_angles;
     double band;            // band half-width in units of M
+    double axis_lz;         // |xi| below which a pixel belongs to bin 0
 };
 
 struct PlanCounters {
-    unsigned long long seg[3];     // band, inside, outside counts
-    unsigned long long cursor[3];  // scatter cursors
-    unsigned long long work;       // work-list counter of the persistent kernel
-    unsigned long long repair;     // repair-list length
+    unsigned long long seg[NBINS];     // pixels per bin
+    unsigned long long cursor[NBINS];  // scatter cursors
+    unsigned long long inside;         // pixels inside the analytic edge, outside bin 0
+    unsigned long long work;           // work-list counter of the persistent kernel
+    unsigned long long repair;         // repair-list length
     unsigned long long fp64_attempts;
-    unsigned long long pad;        // pixels repaired over all rounds
+    unsigned long long repaired;       // pixels repaired over all rounds
 };
 
 // A repaired pixel may change class and so move the boundary; a second
 // verify/repair round covers its new neighbours.
 constexpr int REPAIR_ROUNDS = 2;
 
-__device__ __forceinline__ int pixel_class(const PlanDev& pl, const BardeenImage& B, int64_t idx) {
+struct PixelPlan {
+    int bin;
+    bool inside;
+};
+
+__device__ __forceinline__ PixelPlan plan_pixel(const PlanDev& pl, const BardeenImage& B, int64_t idx) {
     const int64_t i = idx / B.n, j = idx - (idx / B.n) * B.n;
-    const double da = pixel_centre(B.fov, j, B.n) - pl.ca;
+    const double alpha = pixel_centre(B.fov, j, B.n);
+    const double da = alpha - pl.ca;
     const double db = -pixel_centre(B.fov, i, B.n) - pl.cb;
     const double rho = sqrt(da * da + db * db);
     double phi = atan2(db, da);
@@ -71,8 +89,13 @@ __device__ __forceinline__ int pixel_class(const PlanDev& pl, const BardeenImage
     k0 = ((k0 % pl.n_angles) + pl.n_angles) % pl.n_angles;
     const int k1 = (k0 + 1) % pl.n_angles;
     const double rc = (1.0 - w) * pl.radii[k0] + w * pl.radii[k1];
-    if (fabs(rho - rc) <= pl.band) return 0;
-    return rho < rc ? 1 : 2;
+    const double d = fabs(rho - rc);
+    int bin = 0;
+    if (!(d <= pl.band || fabs(alpha * B.sin_o) < pl.axis_lz)) {
+        bin = 1;
+        for (double edge = 2.0 * pl.band; bin < NBINS - 1 && d > edge; edge *= 2.0) ++bin;
+    }
+    return PixelPlan{bin, rho < rc};
 }
 
 // Wavefront-aggregated append: one atomic per wavefront and class instead of
@@ -90,26 +113,38 @@ __device__ __forceinline__ unsigned long long wave_append(unsigned long long* co
     return base + static_cast<unsigned long long>(__popcll(below));
 }
 
+// Work-list order: bin 0 (FP64: edge band and polar strip), then the other
+// pixels inside the analytic edge, then those outside, each row-major.  This
+// starts the longest photons first and mixes long and short photons among
+// those in flight; ordering every pixel by its distance bin instead puts
+// only long photons in flight at once and measures slower.
+__device__ __forceinline__ int order_class(const PixelPlan& q) { return q.bin == 0 ? 0 : (q.inside ? 1 : 2); }
+
 __global__ void plan_count_kernel(PlanDev pl, BardeenImage B, PlanCounters* c) {
     const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool valid = idx < B.n * B.n;
-    const int k = valid ? pixel_class(pl, B, idx) : -1;
-    for (int cls = 0; cls < 3; ++cls) wave_append(&c->seg[cls], k == cls);
+    const PixelPlan q = valid ? plan_pixel(pl, B, idx) : PixelPlan{-1, false};
+    for (int b = 0; b < NBINS; ++b) wave_append(&c->seg[b], q.bin == b);
+    wave_append(&c->inside, valid && q.bin > 0 && q.inside);
 }
 
 __global__ void plan_scatter_kernel(PlanDev pl, BardeenImage B, PlanCounters* c, int64_t* work, int fp32_bulk,
                                     int longest_first) {
     const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool valid = idx < B.n * B.n;
-    const int k = valid ? pixel_class(pl, B, idx) : -1;
+    const PixelPlan q = valid ? plan_pixel(pl, B, idx) : PixelPlan{-1, false};
+    const int k = valid ? order_class(q) : -1;
     unsigned long long pos = static_cast<unsigned long long>(idx);
     if (longest_first) {
+        const unsigned long long first[3] = {0ull, c->seg[0], c->seg[0] + c->inside};
         for (int cls = 0; cls < 3; ++cls) {
             const unsigned long long slot = wave_append(&c->cursor[cls], k == cls);
-            if (k == cls) pos = (cls == 0 ? 0ull : (cls == 1 ? c->seg[0] : c->seg[0] + c->seg[1])) + slot;
+            if (k == cls) pos = first[cls] + slot;
         }
     }
-    if (valid) work[pos] = idx | ((k == 0 || !fp32_bulk) ? FP64_FLAG : 0) | (k == 0 ? BAND_FLAG : 0);
+    if (valid)
+        work[pos] = idx | (static_cast<int64_t>(q.bin) << BIN_SHIFT) | ((q.bin == 0 || !fp32_bulk) ? FP64_FLAG : 0) |
+                    (q.bin == 0 ? BAND_FLAG : 0);
 }
 
 // WAVES: wavefronts per SIMD the register allocation is made for (2: 256
@@ -129,7 +164,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
     const Geom32 G = geom32(P);
     RayState<double> R;
     R.state = RUNNING;
-    bool busy = false, exhausted = false, exact = true, band = false;
+    bool busy = false, exhausted = false, exact = true;
+    int bin = NBINS;
     int64_t idx = -1;
     int64_t fp64_att = 0;
     int since = refill_interval;
@@ -153,7 +189,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
                         const int64_t w = work[pos];
                         idx = w & INDEX_MASK;
                         exact = !MIXED || (w & FP64_FLAG) != 0;
-                        band = (w & BAND_FLAG) != 0;
+                        bin = static_cast<int>((w >> BIN_SHIFT) & 15);
                         const int64_t i = idx / B.n, j = idx - i * B.n;
                         double y0[8];
                         if (!bardeen_state(B, pixel_centre(B.fov, j, B.n), -pixel_centre(B.fov, i, B.n), y0) && flags)
@@ -169,11 +205,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
                     }
                 }
             }
-            // Edge-band photons are the longest and set the end of the launch:
-            // a wavefront holding one issues ahead of the others on its SIMD.
+            // Photons of the lowest bins are the longest and set the end of
+            // the launch: a wavefront holding one issues ahead of the others on
+            // its SIMD (priority 3 for bin 0, 2 for bin 1, 1 for bin 2).
             if (band_priority) {
-                if (__any(busy && band))
+                if (__any(busy && bin == 0))
+                    __builtin_amdgcn_s_setprio(3);
+                else if (__any(busy && bin == 1))
                     __builtin_amdgcn_s_setprio(2);
+                else if (__any(busy && bin == 2))
+                    __builtin_amdgcn_s_setprio(1);
                 else
                     __builtin_amdgcn_s_setprio(0);
             }
@@ -222,7 +263,7 @@ __global__ void verify_kernel(const int64_t* state, const uint8_t* used_fp64, in
     }
     if (need) {
         repair[atomicAdd(&c->repair, 1ull)] = idx;
-        atomicAdd(&c->pad, 1ull);
+        atomicAdd(&c->repaired, 1ull);
     }
 }
 
@@ -268,6 +309,18 @@ struct Work {
     uint8_t* used_fp64;
 };
 
+// pixels < 0: unknown here, the outside count is then left at -1.
+void fill_stats(const PlanCounters& pc, int64_t pixels, kr_planned_stats* stats) {
+    stats->planned_band = static_cast<int64_t>(pc.seg[0]);
+    stats->planned_inside = static_cast<int64_t>(pc.inside);
+    int64_t planned = 0;
+    for (int b = 0; b < NBINS; ++b) planned += static_cast<int64_t>(pc.seg[b]);
+    stats->planned_outside = planned - stats->planned_band - stats->planned_inside;
+    (void)pixels;
+    stats->repaired = static_cast<int64_t>(pc.repaired);
+    stats->fp64_attempts = static_cast<int64_t>(pc.fp64_attempts);
+}
+
 size_t workspace_bytes(int64_t pixels, int32_t n_angles) {
     host::Carver c;
     c.take(sizeof(PlanCounters));
@@ -300,6 +353,7 @@ kr_status check_opts(const kr_planned_opts* o) {
     if (o->block_size != 64 && o->block_size != 128 && o->block_size != 256)
         return host::fail(KR_ERR_INVALID, "block_size must be 64, 128 or 256");
     if (o->waves_per_simd < 2 || o->waves_per_simd > 4) return host::fail(KR_ERR_INVALID, "waves_per_simd must be 2, 3 or 4");
+    if (!(o->axis_lz >= 0.0)) return host::fail(KR_ERR_INVALID, "axis_lz must be >= 0");
     if (!(o->horizon_margin >= 0.0) || !(o->axis_sin >= 0.0) || !(o->turn_drdl >= 0.0) || !(o->turn_rmax >= 0.0))
         return host::fail(KR_ERR_INVALID, "switch thresholds must be >= 0");
     return KR_OK;
@@ -331,7 +385,7 @@ kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B,
     int cus = 0;
     KR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
     const unsigned pblocks = static_cast<unsigned>((pixels + 255) / 256);
-    PlanDev pl{ca, cb, w.radii, n_angles, (o->band_px + 1.0) * (2.0 * B.fov / static_cast<double>(B.n))};
+    PlanDev pl{ca, cb, w.radii, n_angles, (o->band_px + 1.0) * (2.0 * B.fov / static_cast<double>(B.n)), o->axis_lz};
     hipLaunchKernelGGL(planned_reset_kernel, dim3(1), dim3(64), 0, s, w.counters, w.flags);
     KR_HIP(hipGetLastError());
     hipLaunchKernelGGL(plan_count_kernel, dim3(pblocks), dim3(256), 0, s, pl, B, w.counters);
@@ -462,11 +516,7 @@ extern "C" kr_status kr_trace_image_planned(kr_handle h, const kr_params* p, con
     if (flags & KR_FLAG_RADICAND)
         return host::fail(KR_ERR_RADICAND, "a Carter radicand is negative beyond round-off at the observer");
     if (stats) {
-        stats->planned_band = static_cast<int64_t>(pc.seg[0]);
-        stats->planned_inside = static_cast<int64_t>(pc.seg[1]);
-        stats->planned_outside = static_cast<int64_t>(pc.seg[2]);
-        stats->repaired = static_cast<int64_t>(pc.pad);
-        stats->fp64_attempts = static_cast<int64_t>(pc.fp64_attempts);
+        fill_stats(pc, B.n * B.n, stats);
     }
     if (kernel_ms) {
         float ms = 0.f;
@@ -485,10 +535,6 @@ extern "C" kr_status kr_planned_read_stats(const void* d_work, size_t work_bytes
     PlanCounters pc;
     KR_HIP(hipMemcpyAsync(&pc, d_work, sizeof(pc), hipMemcpyDeviceToHost, s));
     KR_HIP(hipStreamSynchronize(s));
-    stats->planned_band = static_cast<int64_t>(pc.seg[0]);
-    stats->planned_inside = static_cast<int64_t>(pc.seg[1]);
-    stats->planned_outside = static_cast<int64_t>(pc.seg[2]);
-    stats->repaired = static_cast<int64_t>(pc.pad);
-    stats->fp64_attempts = static_cast<int64_t>(pc.fp64_attempts);
+    fill_stats(pc, -1, stats);
     return KR_OK;
 }

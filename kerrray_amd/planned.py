@@ -56,6 +56,10 @@ class PlannedOptions:
     longest_first: bool = True
     band_priority: bool = False
     waves_per_simd: int = 2
+    # Photons with |L_z| = |alpha sin theta_o| < axis_lz (M) pass close to a
+    # pole, where the FP32 error estimate can stall the step size: they are
+    # planned in FP64 from the start, with the edge band.
+    axis_lz: float = 0.1
 
     def kr(self) -> _lib.KrPlannedOpts:
         o = _lib.KrPlannedOpts()
@@ -70,6 +74,7 @@ class PlannedOptions:
         o.longest_first = 1 if self.longest_first else 0
         o.band_priority = 1 if self.band_priority else 0
         o.waves_per_simd = self.waves_per_simd
+        o.axis_lz = self.axis_lz
         return o
 
 
