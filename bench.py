@@ -244,8 +244,14 @@ def precision_report(hip, log) -> dict:
     m_state = mixed.bufs["state"].cpu().numpy()
     e_state = exact.bufs["state"].cpu().numpy()
     mask_diff = int(np.count_nonzero((m_state == 2) != (e_state == 2)))
+    # Escape directions at r = infinity: the final rows of the two forms end
+    # at different radii >= r_esc, so their angles there are not comparable.
+    from kerrray_amd.adjoint import directions_at_infinity
+
+    m_th, m_ph = (x.cpu().numpy() for x in directions_at_infinity(hip, st, integ, term, mixed.bufs["Y"], mixed.bufs["state"]))
+    e_th, e_ph = (x.cpu().numpy() for x in directions_at_infinity(hip, st, integ, term, exact.bufs["Y"], exact.bufs["state"]))
     esc = (m_state == 1) & (e_state == 1)
-    dth = (mixed.bufs["Y"][:, 2] - exact.bufs["Y"][:, 2]).abs().cpu().numpy()[esc]
+    dth, dph = np.abs(m_th - e_th)[esc], np.abs(m_ph - e_ph)[esc]
     stats = mixed.stats()
     del mixed, exact
     torch.cuda.empty_cache()
@@ -258,8 +264,12 @@ def precision_report(hip, log) -> dict:
             "fp32_fp64_repair_ms": ms_mixed, "fp64_planned_ms": ms_exact,
             "photons_per_s_fp32_fp64_repair": res * res / (ms_mixed * 1e-3),
             "photons_per_s_fp64": res * res / (ms_exact * 1e-3),
-            "mask_diff_vs_fp64": mask_diff, "escaped_theta_abs_err_median": float(np.median(dth)),
-            "escaped_theta_abs_err_max": float(np.max(dth)), **stats,
+            "mask_diff_vs_fp64": mask_diff, "states_identical_to_fp64": bool(np.array_equal(m_state, e_state)),
+            "escape_direction_error_vs_fp64": {
+                "theta_inf_median": float(np.median(dth)), "theta_inf_p99": float(np.quantile(dth, 0.99)),
+                "theta_inf_max": float(np.max(dth)), "phi_inf_median": float(np.median(dph)),
+                "phi_inf_p99": float(np.quantile(dph, 0.99)), "phi_inf_max": float(np.max(dph))},
+            **stats,
             "edge": {"angles": 720, "rtol": 1e-8, "n_probe": 63, "seconds": edge_s, "failures": edge.failures,
                      "nonmonotone": edge.nonmonotone, "max_bracket_width": float(edge.width.max()),
                      **{f"vs_bardeen_{k}": float(v) for k, v in err.items()}}}
