@@ -29,14 +29,15 @@
 extern "C" {
 #endif
 
-#define KR_ABI_VERSION 3
+#define KR_ABI_VERSION 4
 
 typedef enum kr_status {
     KR_OK = 0,
     KR_ERR_INVALID = 1,     /* bad argument: maps to ValueError */
     KR_ERR_DEVICE = 2,      /* HIP runtime failure / no device: BackendUnavailable */
     KR_ERR_RADICAND = 3,    /* negative Carter radicand in an initial condition: ValueError */
-    KR_ERR_UNSUPPORTED = 4  /* option the kernels do not implement: ValueError */
+    KR_ERR_UNSUPPORTED = 4, /* option the kernels do not implement: ValueError */
+    KR_ERR_NOT_NULL = 5     /* a ZAMO camera's photon is not null: RuntimeError (camera.py:290-291,366-371) */
 } kr_status;
 
 typedef struct kr_context* kr_handle;
@@ -82,11 +83,19 @@ typedef struct kr_outputs {
     uint8_t* event_hit;       /* (n) or NULL */
 } kr_outputs;
 
-/* A square image of Bardeen image-plane initial conditions at the observer:
- * pixel centres fov (2j+1-n)/n (src/kerrray/raytracing/camera.py:206-222),
- * alpha along columns, beta = -centre_i along rows (row 0 on top), and per
- * pixel the photon of src/kerrray/benchmarks/rayset.py:158-165 with E = 1,
- * inward, sign_theta = -1 for beta >= 0 and +1 otherwise. */
+/* A square image at the observer: pixel centres fov (2j+1-n)/n
+ * (src/kerrray/raytracing/camera.py:206-222), alpha along columns,
+ * beta = -centre_i along rows (row 0 on top).  The camera sets the photon of
+ * each image point (alpha, beta):
+ *  - KR_CAMERA_BARDEEN: the photon of src/kerrray/benchmarks/rayset.py:158-165
+ *    with E = 1, inward, sign_theta = -1 for beta >= 0 and +1 otherwise;
+ *  - KR_CAMERA_ZAMO: the photon of initial_states
+ *    (src/kerrray/raytracing/camera.py:298-381) seen by a zero-angular-
+ *    momentum observer at (r_o, theta_o, phi_o), reversed for backward
+ *    tracing.  theta_o is Camera.inclination_rad, i.e. already clamped off
+ *    the axis. */
+#define KR_CAMERA_BARDEEN 0
+#define KR_CAMERA_ZAMO 1
 typedef struct kr_image {
     double mass;
     double spin;        /* a / M */
@@ -94,6 +103,9 @@ typedef struct kr_image {
     double theta_o;     /* observer polar angle, radians */
     double fov;         /* half-width of the image plane */
     int64_t resolution; /* n: the image has n x n pixels, row-major */
+    int32_t camera;     /* KR_CAMERA_BARDEEN or KR_CAMERA_ZAMO */
+    int32_t reserved;   /* 0 */
+    double phi_o;       /* observer azimuth, radians (ZAMO) */
 } kr_image;
 
 /* ABI version (KR_ABI_VERSION). */
@@ -130,6 +142,26 @@ kr_status kr_integrate_batch_dev(kr_handle h, const kr_params* p, const double* 
  * (src/kerrray/geodesics/initial_conditions.py:71-74).  The caller zeroes
  * the flag word before the launch. */
 #define KR_FLAG_RADICAND 1
+/* ZAMO cameras: a photon whose tetrad projection is not null to 1e-10, or
+ * for which no null p_t exists (RuntimeError in camera.py:290-291,366-371),
+ * and an image point with alpha^2 + beta^2 >= r_o^2, which is no direction
+ * on the observer's sky (ValueError in camera.py:342-349). */
+#define KR_FLAG_NOT_NULL 2
+#define KR_FLAG_SKY 4
+
+/* The initial states of every pixel of `img`, row-major (n*n, 8) float64
+ * into d_Y0: initial_states(cam, st, reverse) (camera.py:298-381) for a ZAMO
+ * camera, the Bardeen photons for a Bardeen image (reverse must be 1 there:
+ * they are already the inward rays).  reverse = 0 gives the ZAMO camera's
+ * physical, future-directed photon.  KR_FLAG_* bits go to *d_flags. */
+kr_status kr_image_states_dev(kr_handle h, const kr_image* img, int32_t reverse, double* d_Y0, int32_t* d_flags,
+                              void* stream);
+
+/* The ZAMO frame at (r_o, theta_o) of `img` as 4 x 4 row-major float64
+ * arrays: the tetrad rows e_(a)^mu (zamo_tetrad, camera.py:225-262), the
+ * metric g_mu_nu and its inverse g^mu_nu (geometry/metric.py:147-219), in
+ * the values the ZAMO kernels use.  Host only. */
+kr_status kr_zamo_frame(const kr_image* img, double* tetrad, double* metric, double* inverse_metric);
 
 /* compute_shadow's trace (src/kerrray/raytracing/shadow.py:179-211 ->
  * rays.py:72-117) for a Bardeen image: initial conditions are generated

@@ -14,10 +14,11 @@ import threading
 
 from kerrray_amd.build import LIB_PATH
 
-KR_OK, KR_ERR_INVALID, KR_ERR_DEVICE, KR_ERR_RADICAND, KR_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
+KR_OK, KR_ERR_INVALID, KR_ERR_DEVICE, KR_ERR_RADICAND, KR_ERR_UNSUPPORTED, KR_ERR_NOT_NULL = 0, 1, 2, 3, 4, 5
 KR_FLOAT64, KR_FLOAT32 = 0, 1
-KR_FLAG_RADICAND = 1
-ABI_VERSION = 3
+KR_FLAG_RADICAND, KR_FLAG_NOT_NULL, KR_FLAG_SKY = 1, 2, 4
+KR_CAMERA_BARDEEN, KR_CAMERA_ZAMO = 0, 1
+ABI_VERSION = 4
 
 # Every symbol the header declares (checked by the tests).
 EXPORTED = (
@@ -31,6 +32,8 @@ EXPORTED = (
     "kr_trace_image",
     "kr_trace_image_dev",
     "kr_trace_points_dev",
+    "kr_image_states_dev",
+    "kr_zamo_frame",
     "kr_bisect_edge",
     "kr_batch_create",
     "kr_batch_photons",
@@ -108,6 +111,9 @@ class KrImage(ctypes.Structure):
         ("theta_o", ctypes.c_double),
         ("fov", ctypes.c_double),
         ("resolution", ctypes.c_int64),
+        ("camera", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("phi_o", ctypes.c_double),
     ]
 
 
@@ -223,6 +229,10 @@ def _declare(L: ctypes.CDLL) -> None:
     L.kr_tape_flags.argtypes = [vp, P(ctypes.c_int32)]
     L.kr_tape_destroy.restype = ctypes.c_int
     L.kr_tape_destroy.argtypes = [vp]
+    L.kr_image_states_dev.restype = ctypes.c_int
+    L.kr_image_states_dev.argtypes = [vp, P(KrImage), ctypes.c_int32, vp, vp, vp]
+    L.kr_zamo_frame.restype = ctypes.c_int
+    L.kr_zamo_frame.argtypes = [P(KrImage), c_double_p, c_double_p, c_double_p]
     L.kr_escape_directions_dev.restype = ctypes.c_int
     L.kr_escape_directions_dev.argtypes = [vp, P(KrParams), vp, vp, ctypes.c_int64, vp, vp, vp]
 
@@ -270,13 +280,31 @@ def last_error() -> str:
 
 
 def check(status: int, what: str) -> None:
-    """Raise the Python exception the reference raises for this kind of failure."""
+    """Raise the Python exception the reference raises for this kind of failure.
+
+    Invalid arguments and initial conditions raise ``ValueError``; the rest,
+    including a ZAMO photon that is not null (``RuntimeError`` in the
+    reference, camera.py:290-291,366-371), raise :class:`KrError`, a
+    ``RuntimeError``.
+    """
     if status == KR_OK:
         return
     msg = f"{what}: {last_error()}"
     if status in (KR_ERR_INVALID, KR_ERR_RADICAND, KR_ERR_UNSUPPORTED):
         raise ValueError(msg)
     raise KrError(status, msg)
+
+
+def raise_for_flags(flags: int) -> None:
+    """Raise what a device flag word (``KR_FLAG_*`` bits) stands for, as :func:`check` would."""
+    if flags & KR_FLAG_RADICAND:
+        raise ValueError("a Carter radicand is negative beyond round-off at the observer")
+    if flags & KR_FLAG_SKY:
+        raise ValueError("an image point has alpha^2 + beta^2 >= r_o^2: no direction on the observer's sky")
+    if flags & KR_FLAG_NOT_NULL:
+        raise KrError(KR_ERR_NOT_NULL, "the ZAMO tetrad projection is not null to 1e-10, or no null p_t exists")
+    if flags:
+        raise KrError(KR_ERR_DEVICE, f"unknown device flags {flags:#x}")
 
 
 class Handle:

@@ -32,6 +32,7 @@ import numpy as np
 
 from kerrray_amd import _lib
 from kerrray_amd.backends import HipBackend, kernel_params
+from kerrray_amd.camera import BardeenCamera, Camera, image_struct
 from kerrray_amd.geodesics import IntegratorOptions, TerminationOptions
 from kerrray_amd.geometry import Spacetime
 from kerrray_amd.photons import BatchResult
@@ -53,10 +54,21 @@ class ImageJob:
     fov: float = 10.0
     r_o: float = 1000.0
     mass: float = 1.0
+    camera: str = "bardeen"  # "bardeen" (image-plane photons) or "zamo" (the observer camera)
+    phi_deg: float = 0.0
+
+    def __post_init__(self) -> None:
+        if self.camera not in ("bardeen", "zamo"):
+            raise ValueError(f"camera must be 'bardeen' or 'zamo', got {self.camera!r}")
 
     @property
     def n_rays(self) -> int:
         return self.resolution * self.resolution
+
+    def make_camera(self) -> Camera | BardeenCamera:
+        kind = Camera if self.camera == "zamo" else BardeenCamera
+        return kind(radius=self.r_o, inclination_deg=self.inclination_deg, phi_deg=self.phi_deg, fov=self.fov,
+                    resolution=self.resolution)
 
     def options(self) -> tuple[Spacetime, IntegratorOptions, TerminationOptions]:
         return (Spacetime(self.mass, self.spin), IntegratorOptions(method="rk45", rtol=self.rtol, atol=self.atol),
@@ -80,12 +92,7 @@ def _kr_job(job: ImageJob) -> _lib.KrJob:
     st, integ, term = job.options()
     k = _lib.KrJob()
     k.params = kernel_params(st, integ, term)
-    k.image.mass = job.mass
-    k.image.spin = job.spin
-    k.image.r_o = job.r_o
-    k.image.theta_o = math.radians(job.inclination_deg)
-    k.image.fov = job.fov
-    k.image.resolution = job.resolution
+    k.image = image_struct(st, job.make_camera())
     return k
 
 
@@ -150,8 +157,7 @@ class DeviceBatch:
     def check_flags(self) -> None:
         flags = ctypes.c_int32(0)
         _lib.check(_lib.lib().kr_batch_flags(self._b, ctypes.byref(flags)), "kr_batch_flags")
-        if flags.value & _lib.KR_FLAG_RADICAND:
-            raise ValueError("a Carter radicand is negative beyond round-off at the observer")
+        _lib.raise_for_flags(flags.value)
 
     def results(self) -> list[BatchResult]:
         """Per-image results (host copies), in job order."""

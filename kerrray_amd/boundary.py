@@ -18,6 +18,8 @@ import math
 import numpy as np
 from numpy.typing import ArrayLike, NDArray
 
+from kerrray_amd.geometry import Spacetime
+
 __all__ = [
     "EDGE_TOLERANCE", "STEP_FRACTION", "edge_error", "extract_boundary_from_mask", "mask_centroid",
     "polygon_centroid", "polygon_radii",
@@ -110,6 +112,14 @@ def extract_boundary_from_mask(alpha: ArrayLike, beta: ArrayLike, captured: Arra
     rows = np.arange(n_angles)
     f_in, f_out = f[rows, k - 1], f[rows, k]
     return angles, s[k - 1] + ds * (f_in - 0.5) / (f_in - f_out)
+
+
+def finite_distance_scale(st: Spacetime, radius: float) -> float:
+    """``1 / sqrt(1 - 2M / r_o)``, which maps a ZAMO camera's image coordinates at ``r_o`` to
+    Bardeen's: exact for ``a = 0``, to leading order otherwise (boundary.py:247-252)."""
+    if not radius > 2.0 * st.mass:
+        raise ValueError("the observer radius must exceed 2M for the finite-distance factor")
+    return 1.0 / math.sqrt(1.0 - 2.0 * st.mass / radius)
 
 
 def edge_error(radii: ArrayLike, angles: ArrayLike, centre: tuple[float, float],

@@ -227,6 +227,8 @@ extern "C" kr_status kr_tape_create(kr_handle h, const kr_params* p, const kr_im
     BardeenImage B;
     kr_status st = host::bardeen_image(img, false, &B);
     if (st != KR_OK) return st;
+    if (B.camera != CAMERA_BARDEEN)
+        return host::fail(KR_ERR_UNSUPPORTED, "the adjoint differentiates the Bardeen image-plane initial state only");
     Params P;
     if ((st = host::validate(h, p, n, &P)) != KR_OK) return st;
     if (p->dtype != KR_FLOAT64 || P.ipar[I_ADAPTIVE] != 1 || P.ipar[I_DISK] != 0)

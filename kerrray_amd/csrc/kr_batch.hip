@@ -103,10 +103,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
                         const int64_t local = idx - J.first;
                         const int64_t i = local / J.B.n, j = local - i * J.B.n;
                         double y0[8];
-                        if (!bardeen_state(J.B, pixel_centre(J.B.fov, j, J.B.n), -pixel_centre(J.B.fov, i, J.B.n),
-                                           y0) &&
-                            a.flags)
-                            atomicOr(a.flags, KR_FLAG_RADICAND);
+                        const int32_t bad = image_state(J.B, pixel_centre(J.B.fov, j, J.B.n),
+                                                        -pixel_centre(J.B.fov, i, J.B.n), y0);
+                        if (bad && a.flags) atomicOr(a.flags, bad);
                         ray_init<T, true>(J.P, y0, R);
                         if (R.state == RUNNING)
                             busy = true;

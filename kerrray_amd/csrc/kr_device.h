@@ -47,6 +47,11 @@ using std::sqrt;
 
 namespace kr {
 
+// Flag bits a kernel ORs into the caller's flag word (the KR_FLAG_* values of
+// the C ABI), and the camera kinds of an image (KR_CAMERA_*).
+enum : int32_t { FLAG_RADICAND = 1, FLAG_NOT_NULL = 2, FLAG_SKY = 4 };
+enum : int64_t { CAMERA_BARDEEN = 0, CAMERA_ZAMO = 1 };
+
 // Termination codes (src/kerrray/photons/classification.py:60-69).
 enum : int { RUNNING = 0, ESCAPED = 1, CAPTURED = 2, MAX_AFFINE = 3, FAILURE = 4, OUT_OF_DOMAIN = 5, DISK_HIT = 6 };
 
@@ -711,9 +716,23 @@ KR_HD void integrate_ray(const Params& P, const double (&y64)[8], RayResult<T>& 
     ray_result<T, DISK>(R, out);
 }
 
+// Per-image scalars of the ZAMO camera (camera.py:225-381): the observer's
+// tetrad rows e_(a)^mu, the covariant metric g_mu_nu and the contravariant
+// g^mu_nu at (r_o, theta_o), computed on the host as geometry/metric.py:147-195
+// and camera.py:256-262 compute them.
+struct ZamoImage {
+    double r_o, theta_o, phi_o;
+    double r_o2;                              // r_o**2
+    double e00, e03, e11, e22, e33;           // 1/lapse, omega/lapse, 1/sqrt(g_rr), 1/sqrt(g_thth), 1/sqrt(g_phph)
+    double g_tt, g_tph, g_rr, g_thth, g_phph;
+    double gi_tt, gi_tph, gi_rr, gi_thth, gi_phph;
+    double sign;                              // -1: reversed for backward tracing, +1: the physical photon
+};
+
 // Per-image scalars of the Bardeen initial conditions, computed on the host
 // with the reference's own Python float expressions (rayset.py:158-165,
-// initial_conditions.py:50-112, horizons via delta()):
+// initial_conditions.py:50-112, horizons via delta()), and the camera whose
+// pixels an image traces:
 struct BardeenImage {
     double r_o, theta_o;     // observer radius and polar angle
     double sin_o;            // math.sin(theta_o)
@@ -726,6 +745,8 @@ struct BardeenImage {
     double aaEE;             // a * a * E * E
     double fov;              // half-width of the image plane (M)
     int64_t n;               // resolution (n x n pixels)
+    int64_t camera;          // KR_CAMERA_BARDEEN or KR_CAMERA_ZAMO
+    ZamoImage Z;             // KR_CAMERA_ZAMO only
 };
 
 // _off_equatorial_state + photon_from_constants for one image-plane point.
@@ -760,6 +781,70 @@ KR_HD bool bardeen_state(const BardeenImage& B, double alpha, double beta, doubl
 KR_HD double pixel_centre(double fov, int64_t j, int64_t n) {
     const double nd = static_cast<double>(n);
     return (fov * ((2.0 * static_cast<double>(j) + 1.0) - nd)) / nd;
+}
+
+// The ZAMO photon of initial_states (camera.py:298-381) for the image point
+// (alpha, beta), followed by _null_pt (camera.py:265-295).  Returns 0, or
+// KR_FLAG_SKY when alpha^2 + beta^2 >= r_o^2 (no such direction on the sky)
+// and KR_FLAG_NOT_NULL when the tetrad projection is not null to 1e-10 or no
+// null p_t exists.
+//
+// The tetrad and metric products are the reference's 4 x 4 matrix products,
+// p^mu = sum_a p^(a) e_(a)^mu and p_mu = sum_nu p^nu g_nu_mu.  NumPy hands
+// them to BLAS, whose double kernels accumulate the terms in index order with
+// fused multiply-adds from zero.  Products with the zero entries of the
+// tetrad and the metric leave the sum unchanged, so each component is one
+// product, or one fused multiply-add onto one product; the expressions below
+// are those, term for term.
+KR_HD int32_t zamo_state(const ZamoImage& Z, double alpha, double beta, double (&y)[8]) {
+    int32_t bad = 0;
+    const double rho2 = (alpha * alpha + beta * beta) / Z.r_o2;
+    if (!(rho2 < 1.0)) bad |= FLAG_SKY;
+    const double l_r = sqrt(1.0 - rho2);
+    const double l_th = beta / Z.r_o;
+    const double l_ph = -alpha / Z.r_o;
+    // p^mu; p^(t) = 1
+    const double u_t = Z.e00;
+    const double u_r = l_r * Z.e11;
+    const double u_th = l_th * Z.e22;
+    const double u_ph = fma(l_ph, Z.e33, Z.e03);
+    // p_mu
+    double p_t = fma(u_ph, Z.g_tph, u_t * Z.g_tt);
+    double p_r = u_r * Z.g_rr;
+    double p_th = u_th * Z.g_thth;
+    double p_ph = fma(u_ph, Z.g_phph, u_t * Z.g_tph);
+    const double residual = Z.gi_tt * (p_t * p_t) + 2.0 * Z.gi_tph * p_t * p_ph + Z.gi_rr * (p_r * p_r) +
+                            Z.gi_thth * (p_th * p_th) + Z.gi_phph * (p_ph * p_ph);
+    if (!(fabs(residual) < 1e-10)) bad |= FLAG_NOT_NULL;
+    p_t *= Z.sign;
+    p_r *= Z.sign;
+    p_th *= Z.sign;
+    p_ph *= Z.sign;
+    // g^{mu nu} p_mu p_nu = 0 solved for p_t: the root nearest the projected one.
+    const double qa = Z.gi_tt;
+    const double qb = 2.0 * Z.gi_tph * p_ph;
+    const double qc = Z.gi_rr * (p_r * p_r) + Z.gi_thth * (p_th * p_th) + Z.gi_phph * (p_ph * p_ph);
+    const double disc = qb * qb - 4.0 * qa * qc;
+    if (disc < -1e-10 * (qb * qb + fabs(4.0 * qa * qc))) bad |= FLAG_NOT_NULL;
+    const double q = -0.5 * (qb + copysign(sqrt(disc >= 0.0 ? disc : 0.0), qb));
+    const double root_1 = q / qa;
+    const double root_2 = q != 0.0 ? qc / q : root_1;
+    y[0] = 0.0;
+    y[1] = Z.r_o;
+    y[2] = Z.theta_o;
+    y[3] = Z.phi_o;
+    y[4] = fabs(root_1 - p_t) <= fabs(root_2 - p_t) ? root_1 : root_2;
+    y[5] = p_r;
+    y[6] = p_th;
+    y[7] = p_ph;
+    return bad;
+}
+
+// The initial state of the image point (alpha, beta) for the image's camera:
+// 0, or the KR_FLAG_* bits of what failed.
+KR_HD int32_t image_state(const BardeenImage& B, double alpha, double beta, double (&y)[8]) {
+    if (B.camera == CAMERA_ZAMO) return zamo_state(B.Z, alpha, beta, y);
+    return bardeen_state(B, alpha, beta, y) ? 0 : FLAG_RADICAND;
 }
 
 }  // namespace kr

@@ -97,6 +97,7 @@ inline kr_status validate(kr_context* h, const kr_params* p, int64_t n, kr::Para
     if (!h) return fail(KR_ERR_INVALID, "handle is NULL");
     if (!p) return fail(KR_ERR_INVALID, "params is NULL");
     if (n < 0) return fail(KR_ERR_INVALID, "negative photon count");
+    if (n > (int64_t(1) << 31)) return fail(KR_ERR_INVALID, "at most 2^31 photons per call");
     if (p->dtype != KR_FLOAT64 && p->dtype != KR_FLOAT32) return fail(KR_ERR_INVALID, "dtype must be 0 or 1");
     const double fixed[5] = {0.0, 1.0, 2.0, -0.5, 8.0};
     for (int i = 0; i < 5; ++i)
@@ -127,6 +128,20 @@ inline kr_status validate(kr_context* h, const kr_params* p, int64_t n, kr::Para
     std::memcpy(out->params, p->params, sizeof(out->params));
     std::memcpy(out->ipar, p->ipar, sizeof(out->ipar));
     return KR_OK;
+}
+
+// The error a device flag word stands for (KR_FLAG_* bits; 0 is KR_OK).
+inline kr_status flag_status(int32_t flags) {
+    static_assert(KR_FLAG_RADICAND == kr::FLAG_RADICAND && KR_FLAG_NOT_NULL == kr::FLAG_NOT_NULL &&
+                      KR_FLAG_SKY == kr::FLAG_SKY && KR_CAMERA_ZAMO == kr::CAMERA_ZAMO,
+                  "flag and camera values of the ABI and the kernels");
+    if (flags & KR_FLAG_RADICAND)
+        return fail(KR_ERR_RADICAND, "a Carter radicand is negative beyond round-off at the observer");
+    if (flags & KR_FLAG_SKY)
+        return fail(KR_ERR_INVALID, "an image point has alpha^2 + beta^2 >= r_o^2: no direction on the observer's sky");
+    if (flags & KR_FLAG_NOT_NULL)
+        return fail(KR_ERR_NOT_NULL, "the ZAMO tetrad projection is not null to 1e-10, or no null p_t exists");
+    return flags ? fail(KR_ERR_DEVICE, "unknown device flag") : KR_OK;
 }
 
 // Per-image scalars of the Bardeen initial conditions (kr_image.h).

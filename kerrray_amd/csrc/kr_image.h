@@ -26,11 +26,67 @@ inline double libm_pow(double x, double y) {
     return f(x, y);
 }
 
+// The ZAMO frame at (r, th): metric_components and inverse_metric_components
+// (geometry/metric.py:147-195) and zamo_tetrad (camera.py:225-262) as NumPy
+// evaluates them on float64 scalars (x ** 2 of an array is x * x, a ** 2 of
+// the Python float a is pow(a, 2.0) = a * a, np.sin/np.cos are glibc's).
+// sign is -1 for the reversed (backward-traced) photon, +1 for the physical
+// one.  Returns nullptr, or a message when the frame does not exist.
+inline const char* zamo_frame_scalars(double m, double a, double r, double th, double ph, double sign,
+                                      ZamoImage* Z) {
+    if (!std::isfinite(ph)) return "phi_o must be finite";
+    const double s = libm_sin(th), c = libm_cos(th);
+    const double s2 = s * s;
+    const double a2 = libm_pow(a, 2.0);
+    const double sig = r * r + a2 * (c * c);
+    const double dlt = r * r - 2.0 * m * r + a2;
+    // metric_components
+    const double g_tt = -(1.0 - 2.0 * m * r / sig);
+    const double g_tph = -2.0 * m * a * r * s2 / sig;
+    const double g_rr = sig / dlt;
+    const double g_thth = sig + 0.0 * dlt;
+    const double g_phph = (r * r + a2 + 2.0 * m * a2 * r * s2 / sig) * s2;
+    // zamo_tetrad
+    const double omega = -g_tph / g_phph;
+    const double lapse2 = -(g_tt - g_tph * g_tph / g_phph);
+    if (!(lapse2 > 0.0)) return "lapse^2 = Sigma Delta / A is not positive at the observer";
+    const double lapse = std::sqrt(lapse2);
+    // inverse_metric_components
+    const double r2a2 = r * r + a2;
+    const double big_a = r2a2 * r2a2 - a2 * dlt * s2;
+    const double sd = sig * dlt;
+    Z->r_o = r;
+    Z->theta_o = th;
+    Z->phi_o = ph;
+    Z->r_o2 = libm_pow(r, 2.0);
+    Z->e00 = 1.0 / lapse;
+    Z->e03 = omega / lapse;
+    Z->e11 = 1.0 / std::sqrt(g_rr);
+    Z->e22 = 1.0 / std::sqrt(g_thth);
+    Z->e33 = 1.0 / std::sqrt(g_phph);
+    Z->g_tt = g_tt;
+    Z->g_tph = g_tph;
+    Z->g_rr = g_rr;
+    Z->g_thth = g_thth;
+    Z->g_phph = g_phph;
+    Z->gi_tt = -big_a / sd;
+    Z->gi_tph = -2.0 * m * a * r / sd;
+    Z->gi_rr = dlt / sig;
+    Z->gi_thth = 1.0 / sig + 0.0 * dlt;
+    Z->gi_phph = (dlt - a2 * s2) / (sd * s2);
+    Z->sign = sign;
+    if (!(Z->gi_tt < 0.0)) return "null re-normalisation needs g^tt < 0 at the observer";
+    return nullptr;
+}
+
 // Fills *B with the Python float expressions of rayset.py:158-165 and
-// initial_conditions.py:50-68 (x ** 2 is pow(x, 2.0); math.sin/math.cos).
-// Returns nullptr, or a message for an invalid image.
+// initial_conditions.py:50-68 (x ** 2 is pow(x, 2.0); math.sin/math.cos),
+// and for a ZAMO camera its frame (reversed photons, as the tracer takes
+// them).  Returns nullptr, or a message for an invalid image.
 inline const char* bardeen_image_scalars(const kr_image* img, bool need_grid, BardeenImage* B) {
     if (!img) return "image is NULL";
+    if (img->camera != KR_CAMERA_BARDEEN && img->camera != KR_CAMERA_ZAMO)
+        return "camera must be KR_CAMERA_BARDEEN or KR_CAMERA_ZAMO";
     const double m = img->mass, spin = img->spin, r = img->r_o, th = img->theta_o;
     if (!(std::isfinite(m) && m > 0.0)) return "mass must be finite and > 0";
     if (!(std::isfinite(spin) && std::fabs(spin) < 1.0)) return "spin must satisfy |spin| < 1";
@@ -39,7 +95,8 @@ inline const char* bardeen_image_scalars(const kr_image* img, bool need_grid, Ba
     if (!(std::isfinite(r) && r > r_plus)) return "r_o must lie outside the outer horizon";
     if (!(th > 0.0 && th < M_PI)) return "theta_o must lie strictly between 0 and pi";
     if (need_grid) {
-        if (img->resolution < 1 || img->resolution > (1 << 20)) return "resolution must be in [1, 2^20]";
+        // n * n rows at most 2^31: one lane per pixel fits a launch grid.
+        if (img->resolution < 1 || img->resolution > 46340) return "resolution must be in [1, 46340]";
         if (!(std::isfinite(img->fov) && img->fov > 0.0)) return "fov must be finite and > 0";
     }
     const double sin_o = libm_sin(th), cos_o = libm_cos(th);
@@ -56,6 +113,18 @@ inline const char* bardeen_image_scalars(const kr_image* img, bool need_grid, Ba
     B->aaEE = a * a * 1.0 * 1.0;
     B->fov = img->fov;
     B->n = img->resolution;
+    B->camera = img->camera;
+    B->Z = ZamoImage{};
+    if (img->camera == KR_CAMERA_ZAMO) {
+        if (const char* err = zamo_frame_scalars(m, a, r, th, img->phi_o, -1.0, &B->Z)) return err;
+        if (need_grid) {
+            // The corner pixels are the farthest from the centre
+            // (camera.py:342-349 checks every pixel).
+            const double x = pixel_centre(img->fov, 0, img->resolution);
+            if (!((x * x + x * x) / B->Z.r_o2 < 1.0))
+                return "fov is too large for r_o: pixels with alpha^2 + beta^2 >= r_o^2 are not directions on the sky";
+        }
+    }
     return nullptr;
 }
 

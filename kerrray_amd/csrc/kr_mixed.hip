@@ -192,8 +192,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
                         bin = static_cast<int>((w >> BIN_SHIFT) & 15);
                         const int64_t i = idx / B.n, j = idx - i * B.n;
                         double y0[8];
-                        if (!bardeen_state(B, pixel_centre(B.fov, j, B.n), -pixel_centre(B.fov, i, B.n), y0) && flags)
-                            atomicOr(flags, KR_FLAG_RADICAND);
+                        const int32_t bad =
+                            image_state(B, pixel_centre(B.fov, j, B.n), -pixel_centre(B.fov, i, B.n), y0);
+                        if (bad && flags) atomicOr(flags, bad);
                         ray_init<double, true>(P, y0, R);
                         fp64_att = 0;
                         if (R.state == RUNNING) {
@@ -279,7 +280,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         const int64_t idx = repair[p];
         const int64_t i = idx / B.n, j = idx - i * B.n;
         double y0[8];
-        bardeen_state(B, pixel_centre(B.fov, j, B.n), -pixel_centre(B.fov, i, B.n), y0);
+        image_state(B, pixel_centre(B.fov, j, B.n), -pixel_centre(B.fov, i, B.n), y0);
         RayState<double> R;
         ray_init<double, true>(P, y0, R);
         while (R.state == RUNNING) ray_attempt<double, true, false>(P, R);
@@ -513,8 +514,7 @@ extern "C" kr_status kr_trace_image_planned(kr_handle h, const kr_params* p, con
     KR_HIP(hipMemcpyAsync(out->max_lz_drift, d.max_lz_drift, n * 8, hipMemcpyDeviceToHost, s));
     KR_HIP(hipMemcpyAsync(out->max_carter_drift, d.max_carter_drift, n * 8, hipMemcpyDeviceToHost, s));
     KR_HIP(hipStreamSynchronize(s));
-    if (flags & KR_FLAG_RADICAND)
-        return host::fail(KR_ERR_RADICAND, "a Carter radicand is negative beyond round-off at the observer");
+    if (flags) return host::flag_status(flags);
     if (stats) {
         fill_stats(pc, B.n * B.n, stats);
     }

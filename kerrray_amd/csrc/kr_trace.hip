@@ -86,7 +86,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         idx = i * ic.B.n + j;
         const double alpha = pixel_centre(ic.B.fov, j, ic.B.n);
         const double beta = -pixel_centre(ic.B.fov, i, ic.B.n);
-        if (!bardeen_state(ic.B, alpha, beta, y0) && ic.flags) atomicOr(ic.flags, KR_FLAG_RADICAND);
+        const int32_t bad = image_state(ic.B, alpha, beta, y0);
+        if (bad && ic.flags) atomicOr(ic.flags, bad);
     } else {
         idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
         if (idx >= n) return;
@@ -94,8 +95,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
             for (int c = 0; c < 8; ++c) y0[c] = ic.Y0[idx * 8 + c];
         } else {
-            if (!bardeen_state(ic.B, ic.alpha[idx], ic.beta[idx], y0) && ic.flags)
-                atomicOr(ic.flags, KR_FLAG_RADICAND);
+            const int32_t bad = image_state(ic.B, ic.alpha[idx], ic.beta[idx], y0);
+            if (bad && ic.flags) atomicOr(ic.flags, bad);
         }
     }
     RayResult<T> r;
@@ -123,7 +124,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const double alpha = ca + mid * cosk[k];
     const double beta = cb + mid * sink[k];
     double y0[8];
-    if (!bardeen_state(B, alpha, beta, y0)) atomicOr(flags, KR_FLAG_RADICAND);
+    if (const int32_t bad = image_state(B, alpha, beta, y0)) atomicOr(flags, bad);
     RayResult<T> r;
     integrate_ray<T, ADAPTIVE, false>(P, y0, r);
     escaped[t] = r.state == ESCAPED ? 1 : 0;
@@ -150,6 +151,19 @@ __global__ void bisect_update_kernel(double* lo, double* hi, const uint8_t* esca
         hi[k] = probe_radius(l, h, first, n_probe);
         if (first > 0) lo[k] = probe_radius(l, h, first - 1, n_probe);
     }
+}
+
+// Initial states of every pixel of an image, row-major (n*n, 8): one lane per
+// pixel, each lane writing its own 64-byte row.
+__global__ __launch_bounds__(256) void image_states_kernel(BardeenImage B, double* Y0, int32_t* flags) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= B.n * B.n) return;
+    const int64_t i = idx / B.n, j = idx - i * B.n;
+    double y[8];
+    const int32_t bad = image_state(B, pixel_centre(B.fov, j, B.n), -pixel_centre(B.fov, i, B.n), y);
+    if (bad && flags) atomicOr(flags, bad);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) Y0[idx * 8 + c] = y[c];
 }
 
 __global__ void fill_u64_kernel(unsigned long long* p, int64_t n) {
@@ -423,12 +437,60 @@ extern "C" kr_status kr_trace_image(kr_handle h, const kr_params* p, const kr_im
     int32_t flags = 0;
     KR_HIP(hipMemcpyAsync(&flags, b.flags, 4, hipMemcpyDeviceToHost, s));
     if ((st = copy_back(h, b, *out, n, p->dtype, false)) != KR_OK) return st;
-    if (flags & KR_FLAG_RADICAND)
-        return host::fail(KR_ERR_RADICAND, "a Carter radicand is negative beyond round-off at the observer");
+    if (flags) return host::flag_status(flags);
     if (kernel_ms) {
         float ms = 0.f;
         KR_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
         *kernel_ms = ms;
+    }
+    return KR_OK;
+}
+
+extern "C" kr_status kr_image_states_dev(kr_handle h, const kr_image* img, int32_t reverse, double* d_Y0,
+                                         int32_t* d_flags, void* stream) {
+    host::clear_error();
+    if (!h) return host::fail(KR_ERR_INVALID, "handle is NULL");
+    BardeenImage B;
+    kr_status st = host::bardeen_image(img, true, &B);
+    if (st != KR_OK) return st;
+    if (reverse != 0 && reverse != 1) return host::fail(KR_ERR_INVALID, "reverse must be 0 or 1");
+    if (B.camera == CAMERA_BARDEEN && reverse != 1)
+        return host::fail(KR_ERR_INVALID, "Bardeen image photons are the inward rays: reverse must be 1");
+    if (B.camera == CAMERA_ZAMO) B.Z.sign = reverse ? -1.0 : 1.0;
+    const int64_t n = B.n * B.n;
+    if (!d_Y0) return host::fail(KR_ERR_INVALID, "Y0 is NULL");
+    KR_HIP(hipSetDevice(h->device));
+    hipLaunchKernelGGL(image_states_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), B, d_Y0, d_flags);
+    KR_HIP(hipGetLastError());
+    return KR_OK;
+}
+
+extern "C" kr_status kr_zamo_frame(const kr_image* img, double* tetrad, double* metric, double* inverse_metric) {
+    host::clear_error();
+    if (!tetrad || !metric || !inverse_metric) return host::fail(KR_ERR_INVALID, "an output array is NULL");
+    kr_image zi = img ? *img : kr_image{};
+    zi.camera = KR_CAMERA_ZAMO;
+    BardeenImage B;
+    kr_status st = host::bardeen_image(img ? &zi : nullptr, false, &B);
+    if (st != KR_OK) return st;
+    const ZamoImage& Z = B.Z;
+    for (int k = 0; k < 16; ++k) tetrad[k] = metric[k] = inverse_metric[k] = 0.0;
+    tetrad[0] = Z.e00;
+    tetrad[3] = Z.e03;
+    tetrad[5] = Z.e11;
+    tetrad[10] = Z.e22;
+    tetrad[15] = Z.e33;
+    const double g[5] = {Z.g_tt, Z.g_tph, Z.g_rr, Z.g_thth, Z.g_phph};
+    const double gi[5] = {Z.gi_tt, Z.gi_tph, Z.gi_rr, Z.gi_thth, Z.gi_phph};
+    double* mats[2] = {metric, inverse_metric};
+    const double* comps[2] = {g, gi};
+    for (int m = 0; m < 2; ++m) {
+        mats[m][0] = comps[m][0];
+        mats[m][3] = mats[m][12] = comps[m][1];
+        mats[m][5] = comps[m][2];
+        mats[m][10] = comps[m][3];
+        mats[m][15] = comps[m][4];
     }
     return KR_OK;
 }
@@ -533,8 +595,7 @@ extern "C" kr_status kr_bisect_edge(kr_handle h, const kr_params* p, const kr_im
     KR_HIP(hipStreamSynchronize(s));
     if (failures) *failures = static_cast<int64_t>(cnt[0]);
     if (nonmonotone) *nonmonotone = static_cast<int64_t>(cnt[1]);
-    if (flags & KR_FLAG_RADICAND)
-        return host::fail(KR_ERR_RADICAND, "a Carter radicand is negative beyond round-off at the observer");
+    if (flags) return host::flag_status(flags);
     if (kernel_ms) {
         float ms = 0.f;
         KR_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));

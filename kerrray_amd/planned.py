@@ -26,13 +26,13 @@ import numpy as np
 
 from kerrray_amd import _lib
 from kerrray_amd.backends import HipBackend, HostBatch, get_backend, kernel_params
-from kerrray_amd.boundary import polygon_centroid, polygon_radii
-from kerrray_amd.camera import BardeenCamera
+from kerrray_amd.boundary import finite_distance_scale, polygon_centroid, polygon_radii
+from kerrray_amd.camera import BardeenCamera, Camera, image_struct
 from kerrray_amd.geodesics import IntegratorOptions, TerminationOptions
 from kerrray_amd.geometry import Spacetime
 from kerrray_amd.orbits import shadow_curve
 from kerrray_amd.photons import TerminationState
-from kerrray_amd.shadow import ShadowImage, kr_image, state_counts
+from kerrray_amd.shadow import ShadowImage, state_counts
 
 __all__ = ["EdgeTable", "PlannedOptions", "compute_shadow_planned", "edge_table"]
 
@@ -121,7 +121,7 @@ class PlannedStats:
 
 def compute_shadow_planned(
     st: Spacetime,
-    cam: BardeenCamera,
+    cam: Camera | BardeenCamera,
     integ: IntegratorOptions,
     term: TerminationOptions,
     *,
@@ -133,8 +133,13 @@ def compute_shadow_planned(
         raise TypeError("planned traces run on the hip backend")
     t0 = time.perf_counter()
     table = edge_table(st, cam.inclination_deg)
+    if isinstance(cam, Camera):
+        # The ZAMO camera sees Bardeen's curve shrunk by the finite-distance
+        # factor; the plan only needs it to a fraction of a pixel.
+        k = finite_distance_scale(st, cam.radius)
+        table = EdgeTable(centre=(table.centre[0] / k, table.centre[1] / k), radii=table.radii / k)
     params = kernel_params(st, integ, term)
-    img = kr_image(st, cam.radius, cam.inclination_deg, cam.fov, cam.resolution)
+    img = image_struct(st, cam)
     out = HostBatch(cam.n_rays, np.dtype(np.float64), False)
     stats = _lib.KrPlannedStats()
     ms = ctypes.c_double(0.0)

@@ -2,10 +2,12 @@
 
 Mirrors ``ShadowImage``, ``state_counts`` and ``compute_shadow`` of the
 reference (src/kerrray/raytracing/shadow.py:59-211), including its ``.npz``
-file layout (format version 1). For a :class:`~kerrray_amd.camera.BardeenCamera`
-on the ``hip`` backend the initial states are generated on the device
-(``kr_trace_image``: one pixel per lane, 8 x 8 pixel tiles per wavefront)
-instead of being built on the host and copied over.
+file layout (format version 1). The camera is the reference's ZAMO
+:class:`~kerrray_amd.camera.Camera` or a
+:class:`~kerrray_amd.camera.BardeenCamera`. On the ``hip`` backend the initial
+states of either are generated on the device (``kr_trace_image``: one pixel
+per lane, 8 x 8 pixel tiles per wavefront) instead of being built on the host
+and copied over.
 """
 
 from __future__ import annotations
@@ -21,7 +23,7 @@ from numpy.typing import NDArray
 
 from kerrray_amd import _lib
 from kerrray_amd.backends import Backend, HipBackend, HostBatch, get_backend, kernel_params
-from kerrray_amd.camera import BardeenCamera, bardeen_initial_states
+from kerrray_amd.camera import BardeenCamera, Camera, bardeen_initial_states, image_struct, initial_states
 from kerrray_amd.geodesics import IntegratorOptions, TerminationOptions
 from kerrray_amd.geometry import Spacetime
 from kerrray_amd.photons import FAILURE_STATES, BatchResult, TerminationState
@@ -51,7 +53,7 @@ class ShadowImage:
     captured: NDArray[np.bool_]
     counts: dict[str, int]
     result: BatchResult
-    camera: BardeenCamera
+    camera: Camera | BardeenCamera
     spacetime: Spacetime
 
     @property
@@ -94,6 +96,8 @@ class ShadowImage:
         }
         for name in _CAMERA_FIELDS:
             payload[f"camera_{name}"] = np.asarray(getattr(self.camera, name))
+        if isinstance(self.camera, BardeenCamera):
+            payload["camera_model"] = np.asarray("bardeen")
         for name in _RESULT_ARRAYS:
             payload[f"result_{name}"] = np.asarray(getattr(self.result, name))
         for name in _OPTIONAL_RESULT_ARRAYS:
@@ -110,7 +114,9 @@ class ShadowImage:
             version = int(data["format_version"])
             if version != NPZ_FORMAT_VERSION:
                 raise ValueError(f"unsupported shadow file version {version} (expected {NPZ_FORMAT_VERSION})")
-            camera = BardeenCamera(
+            # Files without a camera model are the reference's, whose camera is the ZAMO one.
+            model = str(data["camera_model"]) if "camera_model" in data.files else "zamo"
+            camera = (BardeenCamera if model == "bardeen" else Camera)(
                 radius=float(data["camera_radius"]),
                 inclination_deg=float(data["camera_inclination_deg"]),
                 phi_deg=float(data["camera_phi_deg"]),
@@ -137,6 +143,7 @@ class ShadowImage:
 
 
 def kr_image(st: Spacetime, r_o: float, inclination_deg: float, fov: float = 1.0, resolution: int = 2) -> _lib.KrImage:
+    """The ``kr_image`` of a Bardeen image plane (image points only: fov and resolution are placeholders)."""
     import math
 
     img = _lib.KrImage()
@@ -149,14 +156,14 @@ def kr_image(st: Spacetime, r_o: float, inclination_deg: float, fov: float = 1.0
     return img
 
 
-def trace_image(engine: HipBackend, st: Spacetime, cam: BardeenCamera, integ: IntegratorOptions,
+def trace_image(engine: HipBackend, st: Spacetime, cam: Camera | BardeenCamera, integ: IntegratorOptions,
                 term: TerminationOptions) -> BatchResult:
     """All pixels of ``cam`` on the device in one launch (``kr_trace_image``), row-major."""
     if integ.method == "dop853":
         raise ValueError("method 'dop853' is available in integrate() only; the hip backend supports 'rk4' and 'rk45'")
     t0 = time.perf_counter()
     params = kernel_params(st, integ, term, None)
-    img = kr_image(st, cam.radius, cam.inclination_deg, cam.fov, cam.resolution)
+    img = image_struct(st, cam)
     out = HostBatch(cam.n_rays, integ.np_dtype, False)
     ms = ctypes.c_double(0.0)
     with engine.handle._lock:
@@ -171,7 +178,7 @@ def trace_image(engine: HipBackend, st: Spacetime, cam: BardeenCamera, integ: In
 
 def compute_shadow(
     st: Spacetime,
-    cam: BardeenCamera,
+    cam: Camera | BardeenCamera,
     integ: IntegratorOptions,
     term: TerminationOptions,
     *,
@@ -179,17 +186,22 @@ def compute_shadow(
     progress: ProgressCallback | None = None,
     chunk_size: int = DEFAULT_CHUNK_SIZE,
 ) -> ShadowImage:
-    """Trace every pixel of ``cam`` and classify it (shadow.py:179-211)."""
-    if not isinstance(cam, BardeenCamera):
-        raise TypeError("compute_shadow needs a BardeenCamera (image-plane initial conditions)")
+    """Trace every pixel of ``cam`` and classify it (shadow.py:179-211).
+
+    On the ``hip`` backend one launch generates and traces every pixel's
+    photon. Another :class:`Backend` gets the initial states as an (N, 8)
+    array and traces them in chunks through :func:`trace_rays`.
+    """
+    if not isinstance(cam, (Camera, BardeenCamera)):
+        raise TypeError("compute_shadow needs a Camera or a BardeenCamera")
     engine = get_backend(backend) if isinstance(backend, str) else backend
     if isinstance(engine, HipBackend):
         result = trace_image(engine, st, cam, integ, term)
         if progress is not None:
             progress(1.0)
     else:
-        result = trace_rays(st, bardeen_initial_states(cam, st), integ, term, backend=engine, progress=progress,
-                            chunk_size=chunk_size)
+        Y0 = initial_states(cam, st) if isinstance(cam, Camera) else bardeen_initial_states(cam, st)
+        result = trace_rays(st, Y0, integ, term, backend=engine, progress=progress, chunk_size=chunk_size)
     alpha, beta = cam.pixel_coordinates()
     state = np.asarray(result.state, dtype=np.int64).reshape(cam.shape)
     return ShadowImage(

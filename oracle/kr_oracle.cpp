@@ -848,3 +848,130 @@ int kro_escape_variational(double mass, double spin, double r0, double theta_o, 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// ZAMO camera (src/kerrray/raytracing/camera.py:225-381), restated with the
+// full 4 x 4 matrices of the reference:
+//   metric_components / inverse_metric_components  geometry/metric.py:147-195
+//   _assemble                                      geometry/metric.py:198-209
+//   zamo_tetrad                                    camera.py:225-262
+//   _null_pt                                       camera.py:265-295
+//   initial_states                                 camera.py:298-381
+// The two matrix products p_local @ e and p_up @ g are evaluated as the BLAS
+// double kernel NumPy calls evaluates them: for each output component, an
+// accumulator starting at zero receives fma(x_k, m_kj, acc) for k = 0..3 in
+// order.  Per-image quantities follow NumPy's float64 rules: x ** 2 of an
+// array is x * x; a ** 2 of the Python float a is glibc pow(a, 2.0).
+namespace {
+
+struct ZamoFrame {
+    double e[4][4], g[4][4], gi[4][4];
+};
+
+// Returns 0, or 1 where zamo_tetrad raises.
+int zamo_frame(double m, double a, double r, double th, ZamoFrame* F) {
+    std::memset(F, 0, sizeof(*F));
+    const double s2 = std::sin(th) * std::sin(th);
+    const double c = std::cos(th);
+    const double a_sq = std::pow(a, 2.0);
+    const double sig = r * r + a_sq * (c * c);
+    const double dlt = r * r - 2.0 * m * r + a_sq;
+    const double g_tt = -(1.0 - 2.0 * m * r / sig);
+    const double g_tphi = -2.0 * m * a * r * s2 / sig;
+    const double g_rr = sig / dlt;
+    const double g_thth = sig + 0.0 * dlt;
+    const double g_phph = (r * r + a_sq + 2.0 * m * a_sq * r * s2 / sig) * s2;
+    const double big_a = (r * r + a_sq) * (r * r + a_sq) - a_sq * dlt * s2;
+    const double sd = sig * dlt;
+    const double gtt = -big_a / sd, gtphi = -2.0 * m * a * r / sd, grr = dlt / sig, gthth = 1.0 / sig + 0.0 * dlt,
+                 gphph = (dlt - a_sq * s2) / (sd * s2);
+    const double comps[2][5] = {{g_tt, g_tphi, g_rr, g_thth, g_phph}, {gtt, gtphi, grr, gthth, gphph}};
+    double(*mats[2])[4] = {F->g, F->gi};
+    for (int k = 0; k < 2; ++k) {
+        mats[k][0][0] = comps[k][0];
+        mats[k][0][3] = mats[k][3][0] = comps[k][1];
+        mats[k][1][1] = comps[k][2];
+        mats[k][2][2] = comps[k][3];
+        mats[k][3][3] = comps[k][4];
+    }
+    const double omega = -g_tphi / g_phph;
+    const double lapse2 = -(g_tt - g_tphi * g_tphi / g_phph);
+    if (!(lapse2 > 0.0)) return 1;
+    const double lapse = std::sqrt(lapse2);
+    F->e[0][0] = 1.0 / lapse;
+    F->e[0][3] = omega / lapse;
+    F->e[1][1] = 1.0 / std::sqrt(g_rr);
+    F->e[2][2] = 1.0 / std::sqrt(g_thth);
+    F->e[3][3] = 1.0 / std::sqrt(g_phph);
+    return 0;
+}
+
+// out_j = sum_k x_k M_kj, accumulated as BLAS does (see above).
+void row_times(const double* x, const double (*M)[4], double* out) {
+    for (int j = 0; j < 4; ++j) {
+        double acc = 0.0;
+        for (int k = 0; k < 4; ++k) acc = std::fma(x[k], M[k][j], acc);
+        out[j] = acc;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// The ZAMO frame at (r, theta) as 4 x 4 row-major arrays; 0, or 1 where the
+// tetrad does not exist.
+int kro_zamo_frame(double mass, double spin, double r, double theta, double* tetrad, double* metric,
+                   double* inverse_metric) {
+    ZamoFrame F;
+    const int bad = zamo_frame(mass, spin * mass, r, theta, &F);
+    std::memcpy(tetrad, F.e, sizeof(F.e));
+    std::memcpy(metric, F.g, sizeof(F.g));
+    std::memcpy(inverse_metric, F.gi, sizeof(F.gi));
+    return bad;
+}
+
+// initial_states(cam, st, reverse) for n image points.  Returns a bit set:
+// 1 the frame does not exist, 2 a projection is not null to 1e-10 or no null
+// p_t exists, 4 a point has alpha^2 + beta^2 >= r_o^2.
+int kro_zamo_states(double mass, double spin, double r_o, double theta_o, double phi_o, const double* alpha,
+                    const double* beta, int64_t n, int reverse, double* Y) {
+    ZamoFrame F;
+    int bad = zamo_frame(mass, spin * mass, r_o, theta_o, &F);
+    const double r_o_sq = std::pow(r_o, 2.0);
+    for (int64_t i = 0; i < n; ++i) {
+        const double rho2 = (alpha[i] * alpha[i] + beta[i] * beta[i]) / r_o_sq;
+        if (!(rho2 < 1.0)) bad |= 4;
+        const double local[4] = {1.0, std::sqrt(1.0 - rho2), beta[i] / r_o, -alpha[i] / r_o};
+        double up[4], p[4];
+        row_times(local, F.e, up);
+        row_times(up, F.g, p);
+        const double residual = F.gi[0][0] * (p[0] * p[0]) + 2.0 * F.gi[0][3] * p[0] * p[3] +
+                                F.gi[1][1] * (p[1] * p[1]) + F.gi[2][2] * (p[2] * p[2]) + F.gi[3][3] * (p[3] * p[3]);
+        if (!(std::fabs(residual) < 1e-10)) bad |= 2;
+        if (reverse)
+            for (double& v : p) v = -v;
+        // _null_pt
+        const double qa = F.gi[0][0];
+        const double qb = 2.0 * F.gi[0][3] * p[3];
+        const double qc = F.gi[1][1] * (p[1] * p[1]) + F.gi[2][2] * (p[2] * p[2]) + F.gi[3][3] * (p[3] * p[3]);
+        if (!(qa < 0.0)) bad |= 2;
+        const double disc = qb * qb - 4.0 * qa * qc;
+        if (disc < -1e-10 * (qb * qb + std::fabs(4.0 * qa * qc))) bad |= 2;
+        const double q = -0.5 * (qb + std::copysign(std::sqrt(pymax(disc, 0.0)), qb));
+        const double root_1 = q / qa;
+        const double root_2 = q != 0.0 ? qc / q : root_1;
+        double* y = Y + i * S;
+        y[IDX_T] = 0.0;
+        y[IDX_R] = r_o;
+        y[IDX_TH] = theta_o;
+        y[IDX_PH] = phi_o;
+        y[IDX_PT] = std::fabs(root_1 - p[0]) <= std::fabs(root_2 - p[0]) ? root_1 : root_2;
+        y[IDX_PR] = p[1];
+        y[IDX_PTH] = p[2];
+        y[IDX_PPH] = p[3];
+    }
+    return bad;
+}
+
+}  // extern "C"

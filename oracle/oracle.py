@@ -86,6 +86,10 @@ def lib() -> ctypes.CDLL:
         L.kro_dp5_step.argtypes = [vp, vp, vp, vp, vp, ctypes.c_double, vp, vp, vp, vp]
         L.kro_classify.restype = ctypes.c_int
         L.kro_classify.argtypes = [vp, vp, vp, vp, vp, ctypes.c_double, ctypes.c_int64, ctypes.c_double, ctypes.c_int]
+        L.kro_zamo_frame.restype = ctypes.c_int
+        L.kro_zamo_frame.argtypes = [ctypes.c_double] * 4 + [vp] * 3
+        L.kro_zamo_states.restype = ctypes.c_int
+        L.kro_zamo_states.argtypes = [ctypes.c_double] * 5 + [vp, vp, ctypes.c_int64, ctypes.c_int, vp]
         L.kro_escape_variational.restype = ctypes.c_int
         L.kro_escape_variational.argtypes = [ctypes.c_double] * 4 + [vp, vp, ctypes.c_int64] + [vp] * 13 + [ctypes.c_int]
         del dp, ip
@@ -274,3 +278,30 @@ def escape_variational(st, theta_o: float, r_o: float, alpha, beta, integ, term,
     if rc != 0:
         raise ValueError("negative radicand in the Bardeen initial conditions")
     return EscapeJacobian(state, theta, phi, jac, err)
+
+
+@dataclass
+class ZamoFrame:
+    tetrad: np.ndarray
+    metric: np.ndarray
+    inverse_metric: np.ndarray
+
+
+def zamo_frame(mass: float, spin: float, r: float, theta: float) -> ZamoFrame:
+    """ZAMO tetrad rows, metric and inverse metric at (r, theta) (camera.py:225-262, metric.py:147-219)."""
+    e, g, gi = np.empty((4, 4)), np.empty((4, 4)), np.empty((4, 4))
+    if lib().kro_zamo_frame(mass, spin, r, theta, _ptr(e), _ptr(g), _ptr(gi)):
+        raise ValueError("the ZAMO tetrad does not exist here")
+    return ZamoFrame(e, g, gi)
+
+
+def zamo_states(mass: float, spin: float, r_o: float, theta_o: float, phi_o: float, alpha, beta,
+                reverse: bool = True) -> tuple[np.ndarray, int]:
+    """``initial_states`` of a ZAMO camera at image points (camera.py:298-381): (N, 8) states and the flag bits
+    (1 no frame, 2 not null, 4 outside the sky)."""
+    a = np.ascontiguousarray(np.asarray(alpha, dtype=np.float64).ravel())
+    b = np.ascontiguousarray(np.asarray(beta, dtype=np.float64).ravel())
+    Y = np.empty((a.size, 8))
+    bad = lib().kro_zamo_states(mass, spin, r_o, theta_o, phi_o, _ptr(a), _ptr(b), a.size, 1 if reverse else 0,
+                                _ptr(Y))
+    return Y, int(bad)

@@ -220,12 +220,68 @@ def bisection(workers: int) -> None:
     )
 
 
+# ZAMO cameras: (spin, inclination_deg, phi_deg, fov, resolution, r_o).  The
+# first two are the 64 x 64 images of docs/shadow.md section 5; the rest cover
+# retrograde spin, the clamped axis inclinations, large radii, the equatorial
+# plane and an observer inside the ergosphere.
+ZAMO_CAMERAS = (
+    (0.9, 60.0, 0.0, 12.0, 64, 1000.0),
+    (0.0, 60.0, 0.0, 12.0, 64, 1000.0),
+    (-0.7, 17.0, 20.0, 12.0, 8, 1000.0),
+    (0.998, 85.0, 0.0, 12.0, 8, 1000.0),
+    (0.9, 0.0, 0.0, 12.0, 4, 1000.0),
+    (0.9, 180.0, 30.0, 12.0, 4, 1000.0),
+    (0.5, 150.0, 20.0, 5.5, 10, 1.0e4),
+    (0.0, 45.0, 0.0, 12.0, 4, 1.0e6),
+    (0.9, 90.0, 0.0, 12.0, 8, 1000.0),
+    (0.9, 90.0, 0.0, 1.2, 8, 1.9),
+)
+
+
+def zamo(workers: int) -> None:
+    """ZAMO camera states (camera.py:225-381) and the two 64 x 64 shadow images at rtol 1e-9.
+
+    For every camera: the tetrad, metric and inverse metric at the observer
+    and ``initial_states`` reversed and physical.  For the first two cameras
+    the reference's NumPy integrator traces the reversed states with rk45,
+    rtol 1e-9, atol 1e-11 and default termination (the settings of
+    docs/shadow.md section 5, whose table gives 558 / 3538 and 608 / 3488
+    captured / escaped pixels).
+    """
+    if REF_SRC not in sys.path:
+        sys.path.insert(0, REF_SRC)
+    from kerrray.geometry import Spacetime, inverse_metric, metric
+    from kerrray.raytracing.camera import Camera, initial_states, zamo_tetrad
+
+    out: dict[str, np.ndarray] = {"cameras": np.asarray(ZAMO_CAMERAS, dtype=np.float64)}
+    t0 = time.time()
+    for k, (spin, incl, phi, fov, n, r_o) in enumerate(ZAMO_CAMERAS):
+        st = Spacetime(1.0, spin)
+        cam = Camera(radius=r_o, inclination_deg=incl, phi_deg=phi, fov=fov, resolution=int(n))
+        th = cam.inclination_rad
+        out[f"c{k}_theta_o"] = np.float64(th)
+        out[f"c{k}_tetrad"] = zamo_tetrad(st, r_o, th)
+        out[f"c{k}_metric"] = metric(st, r_o, th)
+        out[f"c{k}_inverse_metric"] = inverse_metric(st, r_o, th)
+        out[f"c{k}_Y0"] = initial_states(cam, st)
+        out[f"c{k}_Y0_physical"] = initial_states(cam, st, reverse=False)
+    t1 = time.time()
+    for k in (0, 1):
+        spin = ZAMO_CAMERAS[k][0]
+        res = run_parallel(_numpy_chunk, spin, out[f"c{k}_Y0"], 1e-9, 1e-11, workers, 256)
+        out[f"c{k}_numpy_state"] = res["state"].astype(np.int8)
+        out[f"c{k}_numpy_n_steps"] = res["n_steps"].astype(np.int32)
+        out[f"c{k}_numpy_n_rejected"] = res["n_rejected"].astype(np.int32)
+    print(f"zamo: states {t1 - t0:.1f}s, two 64x64 traces at rtol 1e-9 {time.time() - t1:.1f}s", flush=True)
+    np.savez_compressed(os.path.join(HERE, "zamo_cameras.npz"), **out)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--workers", type=int, default=8)
-    ap.add_argument("--only", choices=["config1", "config2", "bisection"], default=None)
+    ap.add_argument("--only", choices=["config1", "config2", "bisection", "zamo"], default=None)
     args = ap.parse_args()
-    for name, fn in (("config1", config1), ("config2", config2), ("bisection", bisection)):
+    for name, fn in (("config1", config1), ("config2", config2), ("bisection", bisection), ("zamo", zamo)):
         if args.only in (None, name):
             fn(args.workers)
 

@@ -52,7 +52,10 @@ void run_image(const kr::Params& P, const kr::BardeenImage& B, const kr_outputs*
     for (int64_t idx = 0; idx < n * n; ++idx) {
         const int64_t i = idx / n, j = idx % n;
         double y0[8];
-        if (!kr::bardeen_state(B, kr::pixel_centre(B.fov, j, n), -kr::pixel_centre(B.fov, i, n), y0)) *bad = 1;
+        if (const int f = kr::image_state(B, kr::pixel_centre(B.fov, j, n), -kr::pixel_centre(B.fov, i, n), y0)) {
+#pragma omp atomic
+            *bad |= f;
+        }
         kr::RayResult<T> r;
         kr::integrate_ray<T, ADAPTIVE, false>(P, y0, r);
         store<T, ADAPTIVE, false>(r, o, idx);
@@ -94,6 +97,22 @@ extern "C" int krh_trace_image(const kr_params* p, const kr_image* img, const kr
     if (p->dtype == KR_FLOAT64) ad ? run_image<double, true>(P, B, o, nt, &bad) : run_image<double, false>(P, B, o, nt, &bad);
     else ad ? run_image<float, true>(P, B, o, nt, &bad) : run_image<float, false>(P, B, o, nt, &bad);
     return bad ? 3 : 0;
+}
+
+// Initial states of every pixel of an image (the device's image_states_kernel
+// on the CPU); returns the KR_FLAG_* bits, or -1 for an invalid image.
+extern "C" int krh_image_states(const kr_image* img, int reverse, double* Y0) {
+    kr::BardeenImage B;
+    if (kr::bardeen_image_scalars(img, true, &B)) return -1;
+    if (B.camera == kr::CAMERA_ZAMO) B.Z.sign = reverse ? -1.0 : 1.0;
+    int bad = 0;
+    for (int64_t idx = 0; idx < B.n * B.n; ++idx) {
+        const int64_t i = idx / B.n, j = idx % B.n;
+        double y[8];
+        bad |= kr::image_state(B, kr::pixel_centre(B.fov, j, B.n), -kr::pixel_centre(B.fov, i, B.n), y);
+        for (int c = 0; c < 8; ++c) Y0[idx * 8 + c] = y[c];
+    }
+    return bad;
 }
 
 #include "kr_mixed.h"

@@ -169,7 +169,9 @@ def test_ragged_workload_is_the_committed_table() -> None:
 
     spec = json.load(open(os.path.join(ROOT, "tests", "golden", "ragged64_spec.json")))
     jobs = ragged_workload(0, 64)
-    assert [asdict(j) for j in jobs] == spec["images"]
+    keys = set(spec["images"][0])
+    assert [{k: v for k, v in asdict(j).items() if k in keys} for j in jobs] == spec["images"]
+    assert all(j.camera == "bardeen" and j.phi_deg == 0.0 for j in jobs)
     assert sum(j.n_rays for j in jobs) == 5_828_608
     assert {j.inclination_deg for j in jobs} == {17.0, 60.0, 85.0}
     assert {j.resolution for j in jobs} == {64, 128, 256, 512}
