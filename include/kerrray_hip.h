@@ -148,6 +148,10 @@ kr_status kr_integrate_batch_dev(kr_handle h, const kr_params* p, const double* 
  * on the observer's sky (ValueError in camera.py:342-349). */
 #define KR_FLAG_NOT_NULL 2
 #define KR_FLAG_SKY 4
+/* Disk shading: a crossing state that is not finite or not equatorial, with
+ * no timelike circular orbit at its radius, or with a non-positive frequency
+ * ratio (ValueError in physics/redshift.py:108-151). */
+#define KR_FLAG_DISK 8
 
 /* The initial states of every pixel of `img`, row-major (n*n, 8) float64
  * into d_Y0: initial_states(cam, st, reverse) (camera.py:298-381) for a ZAMO
@@ -156,6 +160,41 @@ kr_status kr_integrate_batch_dev(kr_handle h, const kr_params* p, const double* 
  * physical, future-directed photon.  KR_FLAG_* bits go to *d_flags. */
 kr_status kr_image_states_dev(kr_handle h, const kr_image* img, int32_t reverse, double* d_Y0, int32_t* d_flags,
                               void* stream);
+
+/* ---------------------------------------------------------------------
+ * Accretion-disk images: render_disk (src/kerrray/raytracing/renderer.py:
+ * 163-216).  The trace stops photons at the disk plane (DISK_HIT, the event
+ * of numba_kernel.py:439-454, enabled by ipar[3] = 1 with consts[18:20] =
+ * r_in, r_out); each crossing is then shaded with the redshift factor of
+ * the Keplerian gas (physics/redshift.py:63-151) and I_obs = g^n I_em(r)
+ * (physics/accretion.py:180-196).
+ * ------------------------------------------------------------------- */
+typedef struct kr_disk {
+    double r_in, r_out;       /* emitting annulus (M); must equal consts[18], consts[19] */
+    double emissivity_index;  /* p of I_em = (r / r_in)^(-p) */
+    int32_t redshift_power;   /* n of I_obs = g^n I_em: 3 or 4 */
+    int32_t prograde;         /* 1: gas orbits with a L_z > 0; 0: retrograde */
+} kr_disk;
+
+/* Shades n crossing rows: event_Y (n, 8) in the params' working dtype
+ * (float64 or float32 values are read as float64), event_hit (n) uint8.
+ * Writes intensity (0 off the disk), g and r_hit (NaN off the disk), float64
+ * device arrays; KR_FLAG_DISK into *d_flags where the reference raises. */
+kr_status kr_disk_shade_dev(kr_handle h, const kr_disk* disk, double mass, double spin, int32_t dtype,
+                            const void* d_event_Y, const uint8_t* d_event_hit, int64_t n, double* d_intensity,
+                            double* d_g, double* d_r_hit, int32_t* d_flags, void* stream);
+
+/* render_disk for every pixel of `img` (ZAMO or Bardeen camera): the trace
+ * of kr_trace_image with the disk event, then the shading above.  Host
+ * arrays: the BatchResult fields in `out` (event_Y, event_hit included) and
+ * intensity, g, r_hit (n*n each).  kernel_ms (may be NULL) receives the
+ * device time of trace and shading. */
+kr_status kr_render_disk(kr_handle h, const kr_params* p, const kr_image* img, const kr_disk* disk,
+                         const kr_outputs* out, double* intensity, double* g, double* r_hit, double* kernel_ms);
+/* Same on device arrays, enqueued on `stream`. */
+kr_status kr_render_disk_dev(kr_handle h, const kr_params* p, const kr_image* img, const kr_disk* disk,
+                             const kr_outputs* d_out, double* d_intensity, double* d_g, double* d_r_hit,
+                             int32_t* d_flags, void* stream);
 
 /* The ZAMO frame at (r_o, theta_o) of `img` as 4 x 4 row-major float64
  * arrays: the tetrad rows e_(a)^mu (zamo_tetrad, camera.py:225-262), the

@@ -16,7 +16,7 @@ from kerrray_amd.build import LIB_PATH
 
 KR_OK, KR_ERR_INVALID, KR_ERR_DEVICE, KR_ERR_RADICAND, KR_ERR_UNSUPPORTED, KR_ERR_NOT_NULL = 0, 1, 2, 3, 4, 5
 KR_FLOAT64, KR_FLOAT32 = 0, 1
-KR_FLAG_RADICAND, KR_FLAG_NOT_NULL, KR_FLAG_SKY = 1, 2, 4
+KR_FLAG_RADICAND, KR_FLAG_NOT_NULL, KR_FLAG_SKY, KR_FLAG_DISK = 1, 2, 4, 8
 KR_CAMERA_BARDEEN, KR_CAMERA_ZAMO = 0, 1
 ABI_VERSION = 4
 
@@ -34,6 +34,9 @@ EXPORTED = (
     "kr_trace_points_dev",
     "kr_image_states_dev",
     "kr_zamo_frame",
+    "kr_disk_shade_dev",
+    "kr_render_disk",
+    "kr_render_disk_dev",
     "kr_bisect_edge",
     "kr_batch_create",
     "kr_batch_photons",
@@ -114,6 +117,16 @@ class KrImage(ctypes.Structure):
         ("camera", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
         ("phi_o", ctypes.c_double),
+    ]
+
+
+class KrDisk(ctypes.Structure):
+    _fields_ = [
+        ("r_in", ctypes.c_double),
+        ("r_out", ctypes.c_double),
+        ("emissivity_index", ctypes.c_double),
+        ("redshift_power", ctypes.c_int32),
+        ("prograde", ctypes.c_int32),
     ]
 
 
@@ -231,6 +244,13 @@ def _declare(L: ctypes.CDLL) -> None:
     L.kr_tape_destroy.argtypes = [vp]
     L.kr_image_states_dev.restype = ctypes.c_int
     L.kr_image_states_dev.argtypes = [vp, P(KrImage), ctypes.c_int32, vp, vp, vp]
+    L.kr_disk_shade_dev.restype = ctypes.c_int
+    L.kr_disk_shade_dev.argtypes = [vp, P(KrDisk), ctypes.c_double, ctypes.c_double, ctypes.c_int32, vp, vp,
+                                    ctypes.c_int64, vp, vp, vp, vp, vp]
+    L.kr_render_disk.restype = ctypes.c_int
+    L.kr_render_disk.argtypes = [vp, P(KrParams), P(KrImage), P(KrDisk), P(KrOutputs), vp, vp, vp, c_double_p]
+    L.kr_render_disk_dev.restype = ctypes.c_int
+    L.kr_render_disk_dev.argtypes = [vp, P(KrParams), P(KrImage), P(KrDisk), P(KrOutputs), vp, vp, vp, vp, vp]
     L.kr_zamo_frame.restype = ctypes.c_int
     L.kr_zamo_frame.argtypes = [P(KrImage), c_double_p, c_double_p, c_double_p]
     L.kr_escape_directions_dev.restype = ctypes.c_int
@@ -301,6 +321,8 @@ def raise_for_flags(flags: int) -> None:
         raise ValueError("a Carter radicand is negative beyond round-off at the observer")
     if flags & KR_FLAG_SKY:
         raise ValueError("an image point has alpha^2 + beta^2 >= r_o^2: no direction on the observer's sky")
+    if flags & KR_FLAG_DISK:
+        raise ValueError("a disk crossing cannot be shaded: not equatorial, no circular orbit at its radius, or g <= 0")
     if flags & KR_FLAG_NOT_NULL:
         raise KrError(KR_ERR_NOT_NULL, "the ZAMO tetrad projection is not null to 1e-10, or no null p_t exists")
     if flags:

@@ -49,7 +49,7 @@ namespace kr {
 
 // Flag bits a kernel ORs into the caller's flag word (the KR_FLAG_* values of
 // the C ABI), and the camera kinds of an image (KR_CAMERA_*).
-enum : int32_t { FLAG_RADICAND = 1, FLAG_NOT_NULL = 2, FLAG_SKY = 4 };
+enum : int32_t { FLAG_RADICAND = 1, FLAG_NOT_NULL = 2, FLAG_SKY = 4, FLAG_DISK = 8 };
 enum : int64_t { CAMERA_BARDEEN = 0, CAMERA_ZAMO = 1 };
 
 // Termination codes (src/kerrray/photons/classification.py:60-69).
@@ -838,6 +838,52 @@ KR_HD int32_t zamo_state(const ZamoImage& Z, double alpha, double beta, double (
     y[6] = p_th;
     y[7] = p_ph;
     return bad;
+}
+
+// Per-image scalars of the disk shading (physics/accretion.py:130-205,
+// physics/redshift.py:63-151, photons/orbits.py:281-292), computed on the
+// host.  The emitter is the Keplerian gas at the crossing radius r,
+// Omega = omega_num / (r^1.5 + omega_c), in the equatorial metric at
+// theta = 0.5 pi (s2, c2 = sin^2, cos^2 of that angle).
+struct DiskShade {
+    double m, a, a2;          // mass, spin * mass, a ** 2
+    double s2, c2;            // np.sin(0.5 pi) ** 2, np.cos(0.5 pi) ** 2
+    double omega_num;         // s sign(a) sqrt(M)  (s = +1 prograde, -1 retrograde)
+    double omega_c;           // s |a| sqrt(M)
+    double r_in, r_out;       // emitting annulus
+    double neg_p;             // -emissivity_index
+    double n_pow;             // 3 or 4: I_obs = g^n I_em
+};
+
+// g, r_hit and I_obs of one disk crossing y (camera.py convention: reversed
+// momentum; g is invariant under the reversal).  Returns 0, or FLAG_DISK when
+// the reference raises: a non-finite or non-equatorial state, no timelike
+// circular orbit at r, or a non-positive frequency ratio.
+KR_HD int32_t disk_shade(const DiskShade& D, const double (&y)[8], double& intensity, double& g, double& r_hit) {
+    bool finite = true;
+    for (int c = 0; c < 8; ++c) finite = finite && isfinite(y[c]);
+    const double r = y[1];
+    r_hit = r;
+    intensity = 0.0;
+    g = 0.0;
+    if (!finite || fabs(y[2] - 0.5 * 3.141592653589793) > 1e-6) return FLAG_DISK;
+    // keplerian_angular_velocity and metric_components at theta = pi/2
+    const double omega = D.omega_num / (pow(r, 1.5) + D.omega_c);
+    const double sig = r * r + D.a2 * D.c2;
+    const double g_tt = -(1.0 - 2.0 * D.m * r / sig);
+    const double g_tph = -2.0 * D.m * D.a * r * D.s2 / sig;
+    const double g_phph = (r * r + D.a2 + 2.0 * D.m * D.a2 * r * D.s2 / sig) * D.s2;
+    const double radicand = -(g_tt + 2.0 * omega * g_tph + omega * omega * g_phph);
+    if (!isfinite(radicand) || radicand <= 0.0) return FLAG_DISK;
+    const double u_t = 1.0 / sqrt(radicand);
+    const double u_ph = omega * u_t;
+    // -p_mu u^mu over the four components; the r and theta terms are zeros.
+    const double p_dot_u = y[4] * u_t + y[5] * 0.0 + y[6] * 0.0 + y[7] * u_ph;
+    g = y[4] / p_dot_u;
+    if (!isfinite(g) || g <= 0.0) return FLAG_DISK;
+    const double emitted = (r >= D.r_in && r <= D.r_out) ? pow(r / D.r_in, D.neg_p) : 0.0;
+    intensity = emitted * pow(g, D.n_pow);
+    return 0;
 }
 
 // The initial state of the image point (alpha, beta) for the image's camera:

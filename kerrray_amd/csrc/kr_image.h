@@ -128,4 +128,37 @@ inline const char* bardeen_image_scalars(const kr_image* img, bool need_grid, Ba
     return nullptr;
 }
 
+// The disk shading scalars of DiskModel (physics/accretion.py:130-178) in
+// spacetime (mass, spin): keplerian_angular_velocity's sign and offset
+// (photons/orbits.py:281-292, Python int and float arithmetic) and the
+// equatorial sin^2, cos^2 of metric_components(st, r, 0.5 pi).  Returns
+// nullptr, or a message for an invalid disk.
+inline const char* disk_shade_scalars(const kr_disk* disk, double mass, double spin, DiskShade* D) {
+    if (!disk) return "disk is NULL";
+    if (!(std::isfinite(mass) && mass > 0.0)) return "mass must be finite and > 0";
+    if (!(std::isfinite(spin) && std::fabs(spin) < 1.0)) return "spin must satisfy |spin| < 1";
+    if (!(std::isfinite(disk->r_in) && std::isfinite(disk->r_out) && 0.0 < disk->r_in && disk->r_in < disk->r_out))
+        return "need finite 0 < r_in < r_out";
+    if (!std::isfinite(disk->emissivity_index)) return "emissivity_index must be finite";
+    if (disk->redshift_power != 3 && disk->redshift_power != 4) return "redshift_power must be 3 or 4";
+    if (disk->prograde != 0 && disk->prograde != 1) return "prograde must be 0 or 1";
+    const double a = spin * mass;
+    const double th = 0.5 * M_PI;
+    const double sq_m = std::sqrt(mass);
+    const double s = disk->prograde ? 1.0 : -1.0;
+    const double sign_a = a < 0.0 ? -1.0 : 1.0;
+    D->m = mass;
+    D->a = a;
+    D->a2 = libm_pow(a, 2.0);
+    D->s2 = libm_sin(th) * libm_sin(th);
+    D->c2 = libm_cos(th) * libm_cos(th);
+    D->omega_num = (s * sign_a) * sq_m;
+    D->omega_c = s * std::fabs(a) * sq_m;
+    D->r_in = disk->r_in;
+    D->r_out = disk->r_out;
+    D->neg_p = -disk->emissivity_index;
+    D->n_pow = static_cast<double>(disk->redshift_power);
+    return nullptr;
+}
+
 }  // namespace kr

@@ -166,6 +166,30 @@ __global__ __launch_bounds__(256) void image_states_kernel(BardeenImage B, doubl
     for (int c = 0; c < 8; ++c) Y0[idx * 8 + c] = y[c];
 }
 
+// Shading of disk crossings, one lane per row: g, r_hit and I_obs of every
+// DISK_HIT row; 0 intensity and NaN elsewhere (renderer.py:190-205).
+template <typename T>
+__global__ __launch_bounds__(256) void disk_shade_kernel(DiskShade D, const T* ev_Y, const uint8_t* ev_hit, int64_t n,
+                                                         double* intensity, double* g, double* r_hit, int32_t* flags) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const double nan = __builtin_nan("");
+    double I = 0.0, gg = nan, rr = nan;
+    if (ev_hit[idx]) {
+        double y[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) y[c] = static_cast<double>(ev_Y[idx * 8 + c]);
+        if (const int32_t bad = disk_shade(D, y, I, gg, rr)) {
+            if (flags) atomicOr(flags, bad);
+            I = 0.0;
+            gg = nan;
+        }
+    }
+    intensity[idx] = I;
+    g[idx] = gg;
+    r_hit[idx] = rr;
+}
+
 __global__ void fill_u64_kernel(unsigned long long* p, int64_t n) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
@@ -287,9 +311,11 @@ struct BatchBuffers {
     double* Y0;
     kr_outputs d;
     int32_t* flags;
+    char* extra;
 };
 
-kr_status carve(kr_handle h, int64_t n, int dtype, bool need_y0, bool disk, BatchBuffers& b) {
+// `extra` more bytes (for the caller's own arrays) start at b.extra.
+kr_status carve(kr_handle h, int64_t n, int dtype, bool need_y0, bool disk, BatchBuffers& b, size_t extra = 0) {
     const size_t ts = dtype == KR_FLOAT64 ? 8 : 4;
     host::Carver c;
     const size_t oY0 = need_y0 ? c.take(n * 64) : 0;
@@ -299,6 +325,7 @@ kr_status carve(kr_handle h, int64_t n, int dtype, bool need_y0, bool disk, Batc
     const size_t oev = disk ? c.take(n * 8 * ts) : 0;
     const size_t ohit = disk ? c.take(n) : 0;
     const size_t oflags = c.take(256);
+    const size_t oextra = c.take(static_cast<int64_t>(extra));
     kr_status st = host::reserve(h, c.total);
     if (st != KR_OK) return st;
     char* base = static_cast<char*>(h->buf);
@@ -315,6 +342,7 @@ kr_status carve(kr_handle h, int64_t n, int dtype, bool need_y0, bool disk, Batc
     b.d.event_Y = disk ? base + oev : nullptr;
     b.d.event_hit = disk ? reinterpret_cast<uint8_t*>(base + ohit) : nullptr;
     b.flags = reinterpret_cast<int32_t*>(base + oflags);
+    b.extra = base + oextra;
     return KR_OK;
 }
 
@@ -402,8 +430,7 @@ extern "C" kr_status kr_trace_image_dev(kr_handle h, const kr_params* p, const k
     if (st != KR_OK) return st;
     Params P;
     if ((st = host::validate(h, p, B.n * B.n, &P)) != KR_OK) return st;
-    if (P.ipar[I_DISK] != 0) return host::fail(KR_ERR_UNSUPPORTED, "the image entry points have no disk event");
-    if ((st = check_outputs(d_out, false)) != KR_OK) return st;
+    if ((st = check_outputs(d_out, P.ipar[I_DISK] != 0)) != KR_OK) return st;
     KR_HIP(hipSetDevice(h->device));
     ICArgs ic{};
     ic.B = B;
@@ -412,8 +439,32 @@ extern "C" kr_status kr_trace_image_dev(kr_handle h, const kr_params* p, const k
     return KR_OK;
 }
 
-extern "C" kr_status kr_trace_image(kr_handle h, const kr_params* p, const kr_image* img, const kr_outputs* out,
-                                    double* kernel_ms) {
+namespace {
+
+hipError_t launch_shade(const DiskShade& D, int dtype, const void* ev_Y, const uint8_t* ev_hit, int64_t n,
+                        double* intensity, double* g, double* r_hit, int32_t* flags, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const dim3 grid(static_cast<unsigned>((n + 255) / 256)), block(256);
+    if (dtype == KR_FLOAT64)
+        hipLaunchKernelGGL(disk_shade_kernel<double>, grid, block, 0, s, D, static_cast<const double*>(ev_Y), ev_hit,
+                           n, intensity, g, r_hit, flags);
+    else
+        hipLaunchKernelGGL(disk_shade_kernel<float>, grid, block, 0, s, D, static_cast<const float*>(ev_Y), ev_hit, n,
+                           intensity, g, r_hit, flags);
+    return hipGetLastError();
+}
+
+// The disk's shading scalars, checked against the event constants the trace uses.
+kr_status disk_setup(const kr_disk* disk, const kr_image* img, const Params& P, DiskShade* D) {
+    if (const char* err = disk_shade_scalars(disk, img->mass, img->spin, D)) return host::fail(KR_ERR_INVALID, err);
+    if (P.ipar[I_DISK] != 1 || !(P.consts[C_RIN] == disk->r_in) || !(P.consts[C_ROUT] == disk->r_out))
+        return host::fail(KR_ERR_INVALID, "params must enable the disk event (ipar[3] = 1) with the disk's r_in, r_out");
+    return KR_OK;
+}
+
+// kr_trace_image, and with `disk` kr_render_disk: host arrays in and out.
+kr_status image_host(kr_handle h, const kr_params* p, const kr_image* img, const kr_disk* disk, const kr_outputs* out,
+                     double* intensity, double* g, double* r_hit, double* kernel_ms) {
     host::clear_error();
     BardeenImage B;
     kr_status st = host::bardeen_image(img, true, &B);
@@ -421,11 +472,19 @@ extern "C" kr_status kr_trace_image(kr_handle h, const kr_params* p, const kr_im
     const int64_t n = B.n * B.n;
     Params P;
     if ((st = host::validate(h, p, n, &P)) != KR_OK) return st;
-    if (P.ipar[I_DISK] != 0) return host::fail(KR_ERR_UNSUPPORTED, "the image entry points have no disk event");
-    if ((st = check_outputs(out, false)) != KR_OK) return st;
+    const bool events = P.ipar[I_DISK] != 0;
+    if ((st = check_outputs(out, events)) != KR_OK) return st;
+    DiskShade D{};
+    if (disk) {
+        if ((st = disk_setup(disk, img, P, &D)) != KR_OK) return st;
+        if (!out->event_Y || !out->event_hit || !intensity || !g || !r_hit)
+            return host::fail(KR_ERR_INVALID, "render_disk needs event_Y, event_hit, intensity, g and r_hit");
+    }
     KR_HIP(hipSetDevice(h->device));
     BatchBuffers b;
-    if ((st = carve(h, n, p->dtype, false, false, b)) != KR_OK) return st;
+    // intensity, g, r_hit follow the batch buffers
+    if ((st = carve(h, n, p->dtype, false, events, b, disk ? static_cast<size_t>(n) * 24 : 0)) != KR_OK) return st;
+    double* shade = reinterpret_cast<double*>(b.extra);
     hipStream_t s = h->stream;
     KR_HIP(fill_i32(b.flags, 1, s));
     ICArgs ic{};
@@ -433,16 +492,82 @@ extern "C" kr_status kr_trace_image(kr_handle h, const kr_params* p, const kr_im
     ic.flags = b.flags;
     KR_HIP(hipEventRecord(h->ev0, s));
     KR_HIP(launch_trace<IC_IMAGE>(P, p->dtype, ic, b.d, n, s));
+    if (disk)
+        KR_HIP(launch_shade(D, p->dtype, b.d.event_Y, b.d.event_hit, n, shade, shade + n, shade + 2 * n, b.flags, s));
     KR_HIP(hipEventRecord(h->ev1, s));
     int32_t flags = 0;
     KR_HIP(hipMemcpyAsync(&flags, b.flags, 4, hipMemcpyDeviceToHost, s));
-    if ((st = copy_back(h, b, *out, n, p->dtype, false)) != KR_OK) return st;
+    if (disk) {
+        KR_HIP(hipMemcpyAsync(intensity, shade, n * 8, hipMemcpyDeviceToHost, s));
+        KR_HIP(hipMemcpyAsync(g, shade + n, n * 8, hipMemcpyDeviceToHost, s));
+        KR_HIP(hipMemcpyAsync(r_hit, shade + 2 * n, n * 8, hipMemcpyDeviceToHost, s));
+    }
+    if ((st = copy_back(h, b, *out, n, p->dtype, events)) != KR_OK) return st;
     if (flags) return host::flag_status(flags);
     if (kernel_ms) {
         float ms = 0.f;
         KR_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
         *kernel_ms = ms;
     }
+    return KR_OK;
+}
+
+}  // namespace
+
+extern "C" kr_status kr_trace_image(kr_handle h, const kr_params* p, const kr_image* img, const kr_outputs* out,
+                                    double* kernel_ms) {
+    return image_host(h, p, img, nullptr, out, nullptr, nullptr, nullptr, kernel_ms);
+}
+
+extern "C" kr_status kr_render_disk(kr_handle h, const kr_params* p, const kr_image* img, const kr_disk* disk,
+                                    const kr_outputs* out, double* intensity, double* g, double* r_hit,
+                                    double* kernel_ms) {
+    if (!disk) {
+        host::clear_error();
+        return host::fail(KR_ERR_INVALID, "disk is NULL");
+    }
+    return image_host(h, p, img, disk, out, intensity, g, r_hit, kernel_ms);
+}
+
+extern "C" kr_status kr_render_disk_dev(kr_handle h, const kr_params* p, const kr_image* img, const kr_disk* disk,
+                                        const kr_outputs* d_out, double* d_intensity, double* d_g, double* d_r_hit,
+                                        int32_t* d_flags, void* stream) {
+    host::clear_error();
+    BardeenImage B;
+    kr_status st = host::bardeen_image(img, true, &B);
+    if (st != KR_OK) return st;
+    const int64_t n = B.n * B.n;
+    Params P;
+    if ((st = host::validate(h, p, n, &P)) != KR_OK) return st;
+    if ((st = check_outputs(d_out, true)) != KR_OK) return st;
+    DiskShade D{};
+    if ((st = disk_setup(disk, img, P, &D)) != KR_OK) return st;
+    if (!d_intensity || !d_g || !d_r_hit) return host::fail(KR_ERR_INVALID, "intensity, g or r_hit is NULL");
+    KR_HIP(hipSetDevice(h->device));
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    ICArgs ic{};
+    ic.B = B;
+    ic.flags = d_flags;
+    KR_HIP(launch_trace<IC_IMAGE>(P, p->dtype, ic, *d_out, n, s));
+    KR_HIP(launch_shade(D, p->dtype, d_out->event_Y, d_out->event_hit, n, d_intensity, d_g, d_r_hit, d_flags, s));
+    return KR_OK;
+}
+
+extern "C" kr_status kr_disk_shade_dev(kr_handle h, const kr_disk* disk, double mass, double spin, int32_t dtype,
+                                       const void* d_event_Y, const uint8_t* d_event_hit, int64_t n,
+                                       double* d_intensity, double* d_g, double* d_r_hit, int32_t* d_flags,
+                                       void* stream) {
+    host::clear_error();
+    if (!h) return host::fail(KR_ERR_INVALID, "handle is NULL");
+    DiskShade D{};
+    if (const char* err = disk_shade_scalars(disk, mass, spin, &D)) return host::fail(KR_ERR_INVALID, err);
+    if (dtype != KR_FLOAT64 && dtype != KR_FLOAT32) return host::fail(KR_ERR_INVALID, "dtype must be float64 or float32");
+    if (n < 0 || n > (int64_t(1) << 31)) return host::fail(KR_ERR_INVALID, "n must be in [0, 2^31]");
+    if (n > 0 && (!d_event_Y || !d_event_hit || !d_intensity || !d_g || !d_r_hit))
+        return host::fail(KR_ERR_INVALID, "an array is NULL");
+    KR_HIP(hipSetDevice(h->device));
+    KR_HIP(launch_shade(D, dtype, d_event_Y, d_event_hit, n, d_intensity, d_g, d_r_hit, d_flags,
+                        static_cast<hipStream_t>(stream)));
     return KR_OK;
 }
 

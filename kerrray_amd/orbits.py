@@ -128,3 +128,17 @@ def shadow_curve(st: Spacetime, inclination_deg: float, n: int = 720) -> tuple[n
     beta = np.concatenate([beta_up, -beta_up[-2:0:-1]])
     order = np.argsort(np.arctan2(beta, alpha), kind="stable")
     return alpha[order], beta[order]
+
+
+def isco_radius(st: Spacetime, prograde: bool = True) -> float:
+    """Marginally stable circular orbit radius, BPT 1972 eq. 2.21 (orbits.py:336-350).
+
+    Z1 = 1 + (1 - chi^2)^(1/3) [(1 + chi)^(1/3) + (1 - chi)^(1/3)], Z2 = (3 chi^2 + Z1^2)^(1/2),
+    r_ms = M {3 + Z2 -+ [(3 - Z1)(3 + Z1 + 2 Z2)]^(1/2)}, chi = |a| / M, upper sign prograde.
+    """
+    m = st.mass
+    s = 1.0 if prograde else -1.0
+    chi = abs(st.a) / m
+    z1 = 1.0 + (1.0 - chi * chi) ** (1.0 / 3.0) * ((1.0 + chi) ** (1.0 / 3.0) + (1.0 - chi) ** (1.0 / 3.0))
+    z2 = math.sqrt(3.0 * chi * chi + z1 * z1)
+    return m * (3.0 + z2 - s * math.sqrt((3.0 - z1) * (3.0 + z1 + 2.0 * z2)))

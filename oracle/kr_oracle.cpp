@@ -975,3 +975,46 @@ int kro_zamo_states(double mass, double spin, double r_o, double theta_o, double
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Disk shading (render_disk's post-processing, renderer.py:190-205):
+//   keplerian_angular_velocity  photons/orbits.py:281-292
+//   emitter_four_velocity       physics/redshift.py:63-90 (metric at theta = pi/2)
+//   _frequency_ratio            physics/redshift.py:117-128 (sum over the four
+//                               components of p_mu u^mu, in index order)
+//   emissivity, observed_intensity  physics/accretion.py:180-196
+// Returns 0, or 1 where the reference raises ValueError.
+extern "C" int kro_disk_shade(double mass, double spin, double r_in, double r_out, double p_index, int n_pow,
+                              int prograde, const double* Y, int64_t n, double* g_out, double* intensity_out) {
+    const double a = spin * mass;
+    const int s = prograde ? 1 : -1;
+    const int sign_a = a < 0.0 ? -1 : 1;
+    const double th = 0.5 * M_PI;
+    const double s2 = std::sin(th) * std::sin(th), c2 = std::cos(th) * std::cos(th);
+    const double a_sq = std::pow(a, 2.0);
+    int bad = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double* y = Y + i * S;
+        for (int c = 0; c < S; ++c)
+            if (!std::isfinite(y[c])) bad = 1;
+        if (std::fabs(y[IDX_TH] - 0.5 * M_PI) > 1e-6) bad = 1;
+        const double r = y[IDX_R];
+        const double omega = (s * sign_a) * std::sqrt(mass) / (std::pow(r, 1.5) + s * std::fabs(a) * std::sqrt(mass));
+        const double sig = r * r + a_sq * c2;
+        const double g_tt = -(1.0 - 2.0 * mass * r / sig);
+        const double g_tphi = -2.0 * mass * a * r * s2 / sig;
+        const double g_phph = (r * r + a_sq + 2.0 * mass * a_sq * r * s2 / sig) * s2;
+        const double radicand = -(g_tt + 2.0 * omega * g_tphi + omega * omega * g_phph);
+        if (!std::isfinite(radicand) || radicand <= 0.0) bad = 1;
+        const double u_t = 1.0 / std::sqrt(radicand);
+        const double u[4] = {u_t, 0.0, 0.0, omega * u_t};
+        double p_dot_u = 0.0;
+        for (int k = 0; k < 4; ++k) p_dot_u += y[IDX_PT + k] * u[k];
+        const double g = y[IDX_PT] / p_dot_u;
+        if (!std::isfinite(g) || g <= 0.0) bad = 1;
+        const double emitted = (r >= r_in && r <= r_out) ? std::pow(r / r_in, -p_index) : 0.0;
+        g_out[i] = g;
+        intensity_out[i] = emitted * std::pow(g, static_cast<double>(n_pow));
+    }
+    return bad;
+}

@@ -86,6 +86,8 @@ def lib() -> ctypes.CDLL:
         L.kro_dp5_step.argtypes = [vp, vp, vp, vp, vp, ctypes.c_double, vp, vp, vp, vp]
         L.kro_classify.restype = ctypes.c_int
         L.kro_classify.argtypes = [vp, vp, vp, vp, vp, ctypes.c_double, ctypes.c_int64, ctypes.c_double, ctypes.c_int]
+        L.kro_disk_shade.restype = ctypes.c_int
+        L.kro_disk_shade.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int64, vp, vp]
         L.kro_zamo_frame.restype = ctypes.c_int
         L.kro_zamo_frame.argtypes = [ctypes.c_double] * 4 + [vp] * 3
         L.kro_zamo_states.restype = ctypes.c_int
@@ -305,3 +307,14 @@ def zamo_states(mass: float, spin: float, r_o: float, theta_o: float, phi_o: flo
     bad = lib().kro_zamo_states(mass, spin, r_o, theta_o, phi_o, _ptr(a), _ptr(b), a.size, 1 if reverse else 0,
                                 _ptr(Y))
     return Y, int(bad)
+
+
+def disk_shade(mass: float, spin: float, r_in: float, r_out: float, emissivity_index: float, redshift_power: int,
+               prograde: bool, y_hit) -> tuple[np.ndarray, np.ndarray]:
+    """(g, I_obs) of disk crossings (redshift.py:63-151, accretion.py:180-196); ValueError where the reference raises."""
+    y = np.ascontiguousarray(np.asarray(y_hit, dtype=np.float64).reshape(-1, 8))
+    g, intensity = np.empty(len(y)), np.empty(len(y))
+    if lib().kro_disk_shade(mass, spin, r_in, r_out, emissivity_index, int(redshift_power), 1 if prograde else 0,
+                            _ptr(y), len(y), _ptr(g), _ptr(intensity)):
+        raise ValueError("a disk crossing cannot be shaded")
+    return g, intensity

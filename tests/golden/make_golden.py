@@ -276,12 +276,90 @@ def zamo(workers: int) -> None:
     np.savez_compressed(os.path.join(HERE, "zamo_cameras.npz"), **out)
 
 
+def _numba_disk_chunk(args):
+    spin, Y0, rtol, atol, r_in, r_out = args
+    if REF_SRC not in sys.path:
+        sys.path.insert(0, REF_SRC)
+    from kerrray.geodesics import EventOptions
+
+    _, IntegratorOptions, TerminationOptions, _, Spacetime, integrate_batch_numba, _ = _ref()
+    res = integrate_batch_numba(
+        Spacetime(1.0, spin), Y0, IntegratorOptions(method="rk45", rtol=rtol, atol=atol), TerminationOptions(),
+        events=EventOptions(disk_plane=True, r_in=r_in, r_out=r_out), interpreted=True,
+    )
+    out = _fields(res)
+    out["event_Y"], out["event_hit"] = res.event_Y, res.event_hit
+    return out
+
+
+def disk(workers: int) -> None:
+    """render_disk (renderer.py:163-216) at the measured run of docs/rendering.md section 6.
+
+    a = 0.9, i = 75 deg, 64 x 64, fov 25, r_o 1000, rk45 rtol 1e-9 / atol 1e-11,
+    disk defaults (r_in = prograde ISCO, r_out 20, p 3, g4): the table there
+    gives 3182 / 54 / 860 ESCAPED / CAPTURED / DISK_HIT pixels.  The reference's
+    NumPy backend traces the rays, and its Numba kernel, interpreted, traces
+    them again (glibc pow: the bitwise answer for the compiled kernel); the crossing states, g, I_obs and r_hit of
+    every pixel are kept, and g and I_obs of crossings outside the retrograde
+    photon orbit for a retrograde g3 disk, and g of crossings in a
+    Schwarzschild frame.
+    """
+    if REF_SRC not in sys.path:
+        sys.path.insert(0, REF_SRC)
+    from kerrray.geometry import Spacetime
+    from kerrray.physics.accretion import DiskModel, DiskParams, observed_intensity
+    from kerrray.physics.redshift import redshift_factor
+    from kerrray.photons.orbits import isco_radius
+    from kerrray.raytracing.camera import Camera, initial_states
+    from kerrray.raytracing.renderer import render_disk
+
+    _, IntegratorOptions, TerminationOptions, _, _, _, _ = _ref()
+    spin, incl, n, fov = 0.9, 75.0, 64, 25.0
+    st = Spacetime(1.0, spin)
+    model = DiskModel.from_params(st, DiskParams())
+    cam = Camera(radius=1000.0, inclination_deg=incl, fov=fov, resolution=n)
+    t0 = time.time()
+    img = render_disk(st, cam, IntegratorOptions(method="rk45", rtol=1e-9, atol=1e-11), TerminationOptions(), model,
+                      backend="numpy")
+    hit = img.hit.ravel()
+    y_hit = img.hit_states.reshape(-1, 8)[hit]
+    retro = DiskModel(r_in=model.r_in, r_out=model.r_out, emissivity_index=2.5, intensity_law="g3", prograde=False)
+    y_retro = y_hit[y_hit[:, 1] > 4.5]  # retrograde gas orbits exist only outside r_ph ~ 3.9 M
+    g_retro = redshift_factor(st, y_retro, prograde=False)
+    y_schw = y_hit[y_hit[:, 1] > 6.5]
+    g_schw = redshift_factor(Spacetime(1.0, 0.0), y_schw)
+    t1 = time.time()
+    Y0 = initial_states(cam, st)
+    jobs = [(spin, Y0[i:i + 64], 1e-9, 1e-11, model.r_in, model.r_out) for i in range(0, len(Y0), 64)]
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        parts = list(ex.map(_numba_disk_chunk, jobs))
+    nb = {k: np.concatenate([p_[k] for p_ in parts]) for k in parts[0]}
+    print(f"disk: numpy render {t1 - t0:.1f}s, numba-interpreted trace {time.time() - t1:.1f}s, "
+          f"counts {img.counts()}", flush=True)
+    np.savez_compressed(
+        os.path.join(HERE, "disk_a09_i75_64.npz"),
+        spin=spin, inclination_deg=incl, resolution=n, fov=fov, r_o=1000.0, rtol=1e-9, atol=1e-11,
+        r_in=model.r_in, r_out=model.r_out, emissivity_index=model.emissivity_index,
+        redshift_power=model.redshift_power,
+        isco=np.array([isco_radius(Spacetime(1.0, a), prograde=p) for a in (0.0, 0.5, 0.9, -0.7) for p in (True, False)]),
+        state=img.state.ravel().astype(np.int8), hit=hit, y_hit=y_hit,
+        g=img.g.ravel(), intensity=img.intensity.ravel(), r_hit=img.r_hit.ravel(),
+        n_steps=img.result.n_steps.astype(np.int32),
+        y_retro=y_retro, g_retro=g_retro, intensity_retro=observed_intensity(retro, y_retro[:, 1], g_retro),
+        y_schw=y_schw, g_schw=g_schw,
+        numba_state=nb["state"].astype(np.int8), numba_n_steps=nb["n_steps"].astype(np.int32),
+        numba_n_rejected=nb["n_rejected"].astype(np.int32), numba_event_hit=nb["event_hit"],
+        numba_event_Y_hit=nb["event_Y"][nb["event_hit"].astype(bool)],
+    )
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--workers", type=int, default=8)
-    ap.add_argument("--only", choices=["config1", "config2", "bisection", "zamo"], default=None)
+    ap.add_argument("--only", choices=["config1", "config2", "bisection", "zamo", "disk"], default=None)
     args = ap.parse_args()
-    for name, fn in (("config1", config1), ("config2", config2), ("bisection", bisection), ("zamo", zamo)):
+    for name, fn in (("config1", config1), ("config2", config2), ("bisection", bisection), ("zamo", zamo),
+                     ("disk", disk)):
         if args.only in (None, name):
             fn(args.workers)
 

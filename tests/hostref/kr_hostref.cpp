@@ -311,3 +311,18 @@ extern "C" int64_t krh_div_by_mismatches(const double* a, const double* b, int64
     }
     return bad;
 }
+
+// Disk shading of n crossing rows by the kernel source (disk_shade); returns
+// the KR_FLAG_* bits, or -1 for an invalid disk.
+extern "C" int krh_disk_shade(const kr_disk* disk, double mass, double spin, const double* Y, int64_t n,
+                              double* intensity, double* g, double* r_hit) {
+    kr::DiskShade D;
+    if (kr::disk_shade_scalars(disk, mass, spin, &D)) return -1;
+    int bad = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        double y[8];
+        for (int c = 0; c < 8; ++c) y[c] = Y[i * 8 + c];
+        bad |= kr::disk_shade(D, y, intensity[i], g[i], r_hit[i]);
+    }
+    return bad;
+}

@@ -1,0 +1,103 @@
+"""Accretion-disk rendering on the GPU (render_disk, renderer.py:163-216).
+
+* The measured run of docs/rendering.md section 6 (a = 0.9, i = 75 deg,
+  64 x 64, fov 25, rtol 1e-9, disk defaults): 3182 / 54 / 860 ESCAPED /
+  CAPTURED / DISK_HIT, every pixel's state and hit the reference's, the
+  crossing states within the class tolerance of the Numba kernel's (device
+  sin/cos/pow differ from glibc's by an ulp in a few percent of calls), and
+  g, I_obs against the reference within the integration tolerance; the
+  documented g and I_obs statistics.
+* ``redshift_factor`` on the device against the oracle to a few ulp, and the
+  errors the reference raises.
+* The FP32 working dtype renders the same hit set as the oracle's FP32 mode.
+"""
+
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+from conftest import GOLDEN, options
+
+from kerrray_amd.camera import Camera
+from kerrray_amd.geometry import Spacetime
+from kerrray_amd.renderer import (DiskModel, event_options, load_disk_image_arrays, redshift_factor, render_disk,
+                                  save_disk_image)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gold() -> dict:
+    with np.load(os.path.join(GOLDEN, "disk_a09_i75_64.npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def _rel(a, b):
+    return np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+
+
+def test_measured_run(hip, gold, tmp_path) -> None:
+    st = Spacetime(1.0, 0.9)
+    cam = Camera(radius=1000.0, inclination_deg=75.0, fov=25.0, resolution=64)
+    disk = DiskModel(r_in=float(gold["r_in"]), r_out=float(gold["r_out"]))
+    integ, term = options(rtol=1e-9, atol=1e-11)
+    img = render_disk(st, cam, integ, term, disk, backend=hip)
+    c = img.counts()
+    hit = img.hit.ravel()
+    g, I = img.g.ravel()[hit], img.intensity.ravel()[hit]
+    print(f"[disk 64^2] counts {c}, g {g.min():.6f}/{g.max():.5f}/{g.mean():.5f}, I max {I.max():.5f}, "
+          f"{img.kernel_ms:.2f} ms", flush=True)
+    assert (c["ESCAPED"], c["CAPTURED"], c["DISK_HIT"]) == (3182, 54, 860)
+    assert np.array_equal(img.state.ravel(), gold["state"]) and np.array_equal(hit, gold["hit"])
+    same = img.result.n_steps[hit] == gold["numba_n_steps"][hit]
+    ev = img.hit_states.reshape(-1, 8)[hit]
+    err = np.abs(ev - gold["numba_event_Y_hit"]) / np.maximum(1.0, np.abs(gold["numba_event_Y_hit"]))
+    print(f"  same step count as the Numba kernel {same.mean():.4f}; crossing rows max rel "
+          f"{err[same].max():.2e}; g vs reference max rel {_rel(g, gold['g'][hit]).max():.2e}", flush=True)
+    # Same step sequence: rounding-level differences that grow along the
+    # path; the bound is that of rtol 1e-7, the median that of 1e-9.
+    row = err.max(axis=1)
+    assert same.mean() > 0.98 and row[same].max() < 1e-6 and np.median(row[same]) < 1e-9
+    assert _rel(g, gold["g"][hit]).max() < 1e-7 and np.median(_rel(g, gold["g"][hit])) < 1e-10
+    assert _rel(I, gold["intensity"][hit]).max() < 1e-6
+    assert np.all(np.isnan(img.g.ravel()[~hit])) and np.all(img.intensity.ravel()[~hit] == 0.0)
+    assert (round(g.min(), 6), round(g.max(), 5), round(g.mean(), 5)) == (0.238877, 1.34898, 0.88578)
+    assert round(float(I.max()), 5) == 2.18889
+    path = save_disk_image(tmp_path / "disk.npz", img)
+    back = load_disk_image_arrays(path)
+    assert np.array_equal(back["intensity"], img.intensity, equal_nan=True) and back["redshift_power"] == 4
+
+
+def test_device_redshift_factor_equals_the_oracle(hip, oracle, gold) -> None:
+    for spin, pro, y in ((0.9, True, gold["y_hit"]), (0.9, False, gold["y_retro"]), (0.0, True, gold["y_schw"])):
+        g = redshift_factor(Spacetime(1.0, spin), y, prograde=pro, backend=hip)
+        g_o, _ = oracle.disk_shade(1.0, spin, 1.0, 2.0, 0.0, 4, pro, y)
+        print(f"[redshift a={spin} prograde={pro}] bitwise {np.mean(g == g_o):.4f} max rel {_rel(g, g_o).max():.2e}",
+              flush=True)
+        assert _rel(g, g_o).max() < 2e-15
+    bad = gold["y_hit"][:3].copy()
+    bad[1, 2] = 1.0
+    with pytest.raises(ValueError):
+        redshift_factor(Spacetime(1.0, 0.9), bad, backend=hip)
+    with pytest.raises(ValueError):
+        redshift_factor(Spacetime(1.0, 0.9), gold["y_hit"][gold["y_hit"][:, 1] < 3.0], prograde=False, backend=hip)
+
+
+def test_float32_render_matches_the_oracle_float32_mode(hip, oracle) -> None:
+    from kerrray_amd.camera import initial_states
+
+    st = Spacetime(1.0, 0.5)
+    cam = Camera(radius=1000.0, inclination_deg=80.0, fov=25.0, resolution=64)
+    disk = DiskModel(r_in=6.0, r_out=20.0)
+    integ, term = options(rtol=1e-5, atol=1e-7, dtype="float32", horizon_epsilon=1e-3)
+    img = render_disk(st, cam, integ, term, disk, backend=hip)
+    ref = oracle.integrate_batch(st, initial_states(cam, st, backend=hip), integ, term, event_options(disk))
+    hit = img.hit.ravel()
+    agree = np.mean(hit == ref.event_hit.astype(bool))
+    print(f"[disk fp32 64^2] hits {hit.sum()} vs oracle {int(ref.event_hit.sum())}, agreement {agree:.4f}", flush=True)
+    assert agree > 0.995 and hit.sum() > 500
+    both = hit & ref.event_hit.astype(bool)
+    g_o, _ = oracle.disk_shade(1.0, 0.5, 6.0, 20.0, 3.0, 4, True, ref.event_Y[both].astype(np.float64))
+    assert np.median(_rel(img.g.ravel()[both], g_o)) < 1e-5
