@@ -127,22 +127,31 @@ def persistent_single(hip, ref_state) -> dict:
 class PlannedRunner:
     """One planned-trace form of the 512^2 image on device buffers (launch stream = torch's current stream)."""
 
-    def __init__(self, hip, fp32_bulk: bool, st, integ, term, res: int = RES, spin: float = SPIN, incl: float = INCL):
+    def __init__(self, hip, fp32_bulk: bool, st, integ, term, res: int = RES, spin: float = SPIN, incl: float = INCL,
+                 camera=None):
         import torch
 
         from kerrray_amd import _lib
         from kerrray_amd.backends import kernel_params
-        from kerrray_amd.planned import PlannedOptions, edge_table
+        from kerrray_amd.boundary import finite_distance_scale
+        from kerrray_amd.camera import image_struct
+        from kerrray_amd.planned import EdgeTable, PlannedOptions, edge_table
         from kerrray_amd.shadow import kr_image
 
         self.hip, self.n = hip, res * res
         self.opts = PlannedOptions(fp32_bulk=fp32_bulk)
         self.table = edge_table(st, incl)
+        img = kr_image(st, R_O, incl, FOV, res)
+        if camera is not None:  # a ZAMO camera sees Bardeen's curve shrunk by the finite-distance factor
+            k = finite_distance_scale(st, camera.radius)
+            self.table = EdgeTable(centre=(self.table.centre[0] / k, self.table.centre[1] / k),
+                                   radii=self.table.radii / k)
+            img = image_struct(st, camera)
         self.bufs, self.outs = device_outputs(torch, self.n, torch.float64)
         self.radii = torch.from_numpy(self.table.radii).cuda()
         self.nbytes = _lib.lib().kr_planned_workspace_bytes(self.n, self.table.n_angles)
         self.work = torch.empty(self.nbytes, dtype=torch.uint8, device="cuda")
-        self.params, self.img, self.ko = kernel_params(st, integ, term), kr_image(st, R_O, incl, FOV, res), self.opts.kr()
+        self.params, self.img, self.ko = kernel_params(st, integ, term), img, self.opts.kr()
         self.stream = torch.cuda.current_stream()
 
     def __call__(self) -> None:
@@ -301,6 +310,72 @@ def adjoint_report(hip, log) -> dict:
            "gradients_per_s": int(tape.escaped.sum()) / (ms * 1e-3), "consistency_flags": flags}
     tape.close()
     log(f"adjoint 256^2: vjp {ms:.1f} ms for {out['escaped']} photons")
+    return out
+
+
+def camera_report(hip, log) -> dict:
+    """The reference's own image path: a ZAMO camera at 512^2 (a=0.9, i=60 deg, fov 10, rtol 1e-6) through
+    the planned FP64 and FP32+FP64-repair forms; a 512^2 disk render (a=0.9, i=75 deg, fov 25, rtol 1e-6,
+    disk r_in = ISCO, r_out = 20); the 360-direction mask edge of the ZAMO image."""
+    import torch
+
+    from kerrray_amd import _lib
+    from kerrray_amd.backends import kernel_params
+    from kerrray_amd.boundary import extract_boundary_from_mask
+    from kerrray_amd.camera import Camera, image_struct
+    from kerrray_amd.geodesics import IntegratorOptions, TerminationOptions
+    from kerrray_amd.geometry import Spacetime
+    from kerrray_amd.renderer import DiskModel, DiskParams, event_options
+
+    st = Spacetime(1.0, SPIN)
+    integ, term = IntegratorOptions(method="rk45", rtol=1e-6, atol=1e-8), TerminationOptions()
+    cam = Camera(radius=R_O, inclination_deg=INCL, fov=FOV, resolution=RES)
+    out: dict = {"workload": "ZAMO camera (camera.py initial_states) 512x512, a=0.9, i=60deg, fov 10, rtol 1e-6; "
+                             "disk render 512x512, a=0.9, i=75deg, fov 25; mask edge 360 directions"}
+    masks = {}
+    for fp32 in (False, True):
+        run = PlannedRunner(hip, fp32, st, integ, term, camera=cam)
+        run()
+        ms = timed(run, 5)
+        masks[fp32] = run.bufs["state"].cpu().numpy() == 2
+        out["fp32_fp64_repair" if fp32 else "fp64_planned"] = {
+            "kernel_ms": ms, "photons_per_s": run.n / (ms * 1e-3), **run.stats()}
+        del run
+    out["fp32_fp64_repair"]["mask_identical_to_fp64"] = bool(np.array_equal(masks[False], masks[True]))
+    a, b = cam.pixel_coordinates()
+    edge = lambda: extract_boundary_from_mask(a, b, masks[False].reshape(RES, RES), n_angles=360, backend=hip)  # noqa: E731
+    edge()
+    t0 = time.perf_counter()
+    _, radii = edge()
+    out["mask_edge_360_s"] = time.perf_counter() - t0
+    out["mask_edge_mean_radius"] = float(radii.mean())
+    # Disk render on device buffers.
+    dcam = Camera(radius=R_O, inclination_deg=75.0, fov=25.0, resolution=RES)
+    disk = DiskModel.from_params(st, DiskParams())
+    params, img, kd = kernel_params(st, integ, term, event_options(disk)), image_struct(st, dcam), disk.kr()
+    n = dcam.n_rays
+    bufs, outs = device_outputs(torch, n, torch.float64)
+    ev_Y = torch.empty((n, 8), dtype=torch.float64, device="cuda")
+    ev_hit = torch.empty(n, dtype=torch.uint8, device="cuda")
+    outs.event_Y, outs.event_hit = ev_Y.data_ptr(), ev_hit.data_ptr()
+    shade = torch.empty((3, n), dtype=torch.float64, device="cuda")
+    flags = torch.zeros(1, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def render() -> None:
+        _lib.check(_lib.lib().kr_render_disk_dev(hip.handle.ptr, ctypes.byref(params), ctypes.byref(img),
+                                                 ctypes.byref(kd), ctypes.byref(outs), shade[0].data_ptr(),
+                                                 shade[1].data_ptr(), shade[2].data_ptr(), flags.data_ptr(),
+                                                 ctypes.c_void_p(stream.cuda_stream)), "kr_render_disk_dev")
+
+    render()
+    ms = timed(render, 5)
+    hit = ev_hit.cpu().numpy().astype(bool)
+    g = shade[1].cpu().numpy()[hit]
+    out["disk_render"] = {"kernel_ms": ms, "photons_per_s": n / (ms * 1e-3), "disk_hits": int(hit.sum()),
+                          "g_min_max": [float(g.min()), float(g.max())], "flags": int(flags.item())}
+    log(f"ZAMO 512^2: planned FP64 {out['fp64_planned']['kernel_ms']:.2f} ms, FP32+repair "
+        f"{out['fp32_fp64_repair']['kernel_ms']:.2f} ms; disk render {ms:.2f} ms")
     return out
 
 
@@ -500,6 +575,7 @@ def main() -> int:
         torch.cuda.empty_cache()
         line["precision_1024_a099_i85"] = precision_report(hip, log)
         line["adjoint_256_a09_i60"] = adjoint_report(hip, log)
+        line["camera_zamo_512_and_disk"] = camera_report(hip, log)
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
         threads = min(threads, len(os.sched_getaffinity(0)))

@@ -152,6 +152,9 @@ kr_status kr_integrate_batch_dev(kr_handle h, const kr_params* p, const double* 
  * no timelike circular orbit at its radius, or with a non-positive frequency
  * ratio (ValueError in physics/redshift.py:108-151). */
 #define KR_FLAG_DISK 8
+/* Mask edge: a direction whose starting point is outside the captured region
+ * (ValueError in raytracing/boundary.py:163-164). */
+#define KR_FLAG_EDGE 16
 
 /* The initial states of every pixel of `img`, row-major (n*n, 8) float64
  * into d_Y0: initial_states(cam, st, reverse) (camera.py:298-381) for a ZAMO
@@ -195,6 +198,20 @@ kr_status kr_render_disk(kr_handle h, const kr_params* p, const kr_image* img, c
 kr_status kr_render_disk_dev(kr_handle h, const kr_params* p, const kr_image* img, const kr_disk* disk,
                              const kr_outputs* d_out, double* d_intensity, double* d_g, double* d_r_hit,
                              int32_t* d_flags, void* stream);
+
+/* Sub-pixel shadow edge of a captured mask: extract_boundary_from_mask
+ * (src/kerrray/raytracing/boundary.py:116-170) for the directions
+ * (cos[k], sin[k]) from (centre_alpha, centre_beta).  d_mask is rows x cols
+ * bytes (1 = captured) of a regular grid whose first pixel centre is
+ * (alpha0, beta0), pixel size d (alpha along columns, beta down the rows);
+ * the mask's bilinear interpolant (zero outside the image) is sampled every
+ * d / 8 up to `reach` and radii[k] is the linear 0.5 crossing after the last
+ * sample at or above 0.5.  A direction whose first sample is below 0.5 gets
+ * NaN and KR_FLAG_EDGE in *d_flags.  One lane per direction. */
+kr_status kr_mask_edge_dev(kr_handle h, const uint8_t* d_mask, int64_t rows, int64_t cols, double alpha0,
+                           double beta0, double d, double centre_alpha, double centre_beta, const double* d_cos,
+                           const double* d_sin, int64_t n_angles, double reach, double* d_radii, int32_t* d_flags,
+                           void* stream);
 
 /* The ZAMO frame at (r_o, theta_o) of `img` as 4 x 4 row-major float64
  * arrays: the tetrad rows e_(a)^mu (zamo_tetrad, camera.py:225-262), the

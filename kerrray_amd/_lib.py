@@ -16,7 +16,7 @@ from kerrray_amd.build import LIB_PATH
 
 KR_OK, KR_ERR_INVALID, KR_ERR_DEVICE, KR_ERR_RADICAND, KR_ERR_UNSUPPORTED, KR_ERR_NOT_NULL = 0, 1, 2, 3, 4, 5
 KR_FLOAT64, KR_FLOAT32 = 0, 1
-KR_FLAG_RADICAND, KR_FLAG_NOT_NULL, KR_FLAG_SKY, KR_FLAG_DISK = 1, 2, 4, 8
+KR_FLAG_RADICAND, KR_FLAG_NOT_NULL, KR_FLAG_SKY, KR_FLAG_DISK, KR_FLAG_EDGE = 1, 2, 4, 8, 16
 KR_CAMERA_BARDEEN, KR_CAMERA_ZAMO = 0, 1
 ABI_VERSION = 4
 
@@ -35,6 +35,7 @@ EXPORTED = (
     "kr_image_states_dev",
     "kr_zamo_frame",
     "kr_disk_shade_dev",
+    "kr_mask_edge_dev",
     "kr_render_disk",
     "kr_render_disk_dev",
     "kr_bisect_edge",
@@ -247,6 +248,9 @@ def _declare(L: ctypes.CDLL) -> None:
     L.kr_disk_shade_dev.restype = ctypes.c_int
     L.kr_disk_shade_dev.argtypes = [vp, P(KrDisk), ctypes.c_double, ctypes.c_double, ctypes.c_int32, vp, vp,
                                     ctypes.c_int64, vp, vp, vp, vp, vp]
+    L.kr_mask_edge_dev.restype = ctypes.c_int
+    L.kr_mask_edge_dev.argtypes = [vp, vp, ctypes.c_int64, ctypes.c_int64] + [ctypes.c_double] * 5 + [
+        vp, vp, ctypes.c_int64, ctypes.c_double, vp, vp, vp]
     L.kr_render_disk.restype = ctypes.c_int
     L.kr_render_disk.argtypes = [vp, P(KrParams), P(KrImage), P(KrDisk), P(KrOutputs), vp, vp, vp, c_double_p]
     L.kr_render_disk_dev.restype = ctypes.c_int
@@ -321,6 +325,8 @@ def raise_for_flags(flags: int) -> None:
         raise ValueError("a Carter radicand is negative beyond round-off at the observer")
     if flags & KR_FLAG_SKY:
         raise ValueError("an image point has alpha^2 + beta^2 >= r_o^2: no direction on the observer's sky")
+    if flags & KR_FLAG_EDGE:
+        raise ValueError("the centre lies outside the captured region")
     if flags & KR_FLAG_DISK:
         raise ValueError("a disk crossing cannot be shaded: not equatorial, no circular orbit at its radius, or g <= 0")
     if flags & KR_FLAG_NOT_NULL:

@@ -1,4 +1,4 @@
-"""Shadow-edge geometry on the image plane (host side).
+"""Shadow-edge geometry on the image plane.
 
 Polygon radii and centroid, the sub-pixel edge of a captured mask and the
 comparison with an analytic curve, as in src/kerrray/raytracing/boundary.py:74-307:
@@ -7,13 +7,16 @@ comparison with an analytic curve, as in src/kerrray/raytracing/boundary.py:74-3
   0 <= s <= 1 (widened by 1e-9 so a ray through a vertex is not lost);
 * ``polygon_centroid``: shoelace area centroid;
 * ``extract_boundary_from_mask``: bilinear 0.5 level set of the mask (zero
-  outside the image) marched along polar rays in steps of 1/8 pixel;
+  outside the image) marched along polar rays in steps of 1/8 pixel, on the
+  device (one lane per direction);
 * ``edge_error``: radial differences between traced edge radii and the curve.
 """
 
 from __future__ import annotations
 
+import ctypes
 import math
+from typing import Any
 
 import numpy as np
 from numpy.typing import ArrayLike, NDArray
@@ -69,49 +72,60 @@ def mask_centroid(alpha: ArrayLike, beta: ArrayLike, captured: ArrayLike) -> tup
     return float(a[mask].mean()), float(b[mask].mean())
 
 
-def _bilinear(field: Float, u: Float, v: Float) -> Float:
-    n_rows, n_cols = field.shape
-    padded = np.pad(field, 1)
-    up, vp = u + 1.0, v + 1.0
-    inside = (up >= 0.0) & (up <= n_cols) & (vp >= 0.0) & (vp <= n_rows)
-    j0 = np.clip(np.floor(up).astype(np.intp), 0, n_cols)
-    i0 = np.clip(np.floor(vp).astype(np.intp), 0, n_rows)
-    fu = np.clip(up - j0, 0.0, 1.0)
-    fv = np.clip(vp - i0, 0.0, 1.0)
-    j1 = np.minimum(j0 + 1, n_cols + 1)
-    i1 = np.minimum(i0 + 1, n_rows + 1)
-    value = (padded[i0, j0] * (1.0 - fu) * (1.0 - fv) + padded[i0, j1] * fu * (1.0 - fv)
-             + padded[i1, j0] * (1.0 - fu) * fv + padded[i1, j1] * fu * fv)
-    return np.where(inside, value, 0.0)
+def _grid_geometry(alpha: Float, beta: Float) -> tuple[float, float, float]:
+    """``(alpha_0, beta_0, d)`` of a regular camera grid (boundary.py:84-92)."""
+    if alpha.ndim != 2 or alpha.shape != beta.shape or min(alpha.shape) < 2:
+        raise ValueError("alpha and beta must be 2-D arrays of the same shape with >= 2 pixels per side")
+    d = float(alpha[0, 1] - alpha[0, 0])
+    if not d > 0.0:
+        raise ValueError("alpha must increase along the columns")
+    if not np.allclose(np.diff(alpha, axis=1), d) or not np.allclose(np.diff(beta, axis=0), -d):
+        raise ValueError("the pixel grid must be regular with alpha increasing rightward and beta downward")
+    return float(alpha[0, 0]), float(beta[0, 0]), d
 
 
 def extract_boundary_from_mask(alpha: ArrayLike, beta: ArrayLike, captured: ArrayLike, *, n_angles: int = 360,
-                               centre: tuple[float, float] | None = None) -> tuple[Float, Float]:
-    """Sub-pixel edge radii ``r(phi_k)``, ``phi_k = 2 pi k / n_angles`` (boundary.py:116-170)."""
+                               centre: tuple[float, float] | None = None, backend: Any = "hip") -> tuple[Float, Float]:
+    """Sub-pixel edge radii ``r(phi_k)``, ``phi_k = 2 pi k / n_angles`` (boundary.py:116-170).
+
+    The marching runs on the device (``kr_mask_edge_dev``, one lane per
+    direction); the checks, the captured centroid and the directions'
+    cosines and sines are the host's, as the reference computes them.
+    """
+    import torch
+
+    from kerrray_amd import _lib
+    from kerrray_amd.backends import get_backend
+
     a = np.asarray(alpha, dtype=np.float64)
     b = np.asarray(beta, dtype=np.float64)
     mask = np.asarray(captured, dtype=bool)
     if mask.shape != a.shape:
         raise ValueError(f"captured has shape {mask.shape}, expected {a.shape}")
+    if n_angles < 1:
+        raise ValueError("n_angles must be >= 1")
     if mask[0, :].any() or mask[-1, :].any() or mask[:, 0].any() or mask[:, -1].any():
         raise ValueError("the captured region touches the image border; increase fov")
-    d = float(a[0, 1] - a[0, 0])
-    a0, b0 = float(a[0, 0]), float(b[0, 0])
+    a0, b0, d = _grid_geometry(a, b)
     ca, cb = mask_centroid(a, b, mask) if centre is None else (float(centre[0]), float(centre[1]))
-    field = mask.astype(np.float64)
     angles = 2.0 * np.pi * np.arange(n_angles, dtype=np.float64) / n_angles
-    ds = STEP_FRACTION * d
     reach = math.hypot(a[0, -1] - a[0, 0], b[0, 0] - b[-1, 0]) + 2.0 * d
-    s = ds * np.arange(int(math.ceil(reach / ds)) + 1, dtype=np.float64)
-    x = ca + np.cos(angles)[:, None] * s[None, :]
-    y = cb + np.sin(angles)[:, None] * s[None, :]
-    f = _bilinear(field, (x - a0) / d, (b0 - y) / d)
-    if np.any(f[:, 0] < 0.5):
-        raise ValueError("the centre lies outside the captured region")
-    k = np.argmax(f < 0.5, axis=1)
-    rows = np.arange(n_angles)
-    f_in, f_out = f[rows, k - 1], f[rows, k]
-    return angles, s[k - 1] + ds * (f_in - 0.5) / (f_in - f_out)
+    engine = get_backend(backend) if isinstance(backend, str) else backend
+    dev = torch.device("cuda", engine.handle.device)
+    d_mask = torch.as_tensor(np.ascontiguousarray(mask.astype(np.uint8)), device=dev)
+    d_cos = torch.as_tensor(np.cos(angles), device=dev)
+    d_sin = torch.as_tensor(np.sin(angles), device=dev)
+    radii = torch.empty(n_angles, dtype=torch.float64, device=dev)
+    flags = torch.zeros(1, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    with engine.handle._lock:
+        _lib.check(_lib.lib().kr_mask_edge_dev(engine.handle.ptr, d_mask.data_ptr(), mask.shape[0], mask.shape[1], a0,
+                                               b0, d, ca, cb, d_cos.data_ptr(), d_sin.data_ptr(), n_angles, reach,
+                                               radii.data_ptr(), flags.data_ptr(), ctypes.c_void_p(stream)),
+                   "kr_mask_edge_dev")
+    out = radii.cpu().numpy()
+    _lib.raise_for_flags(int(flags.item()))
+    return angles, out
 
 
 def finite_distance_scale(st: Spacetime, radius: float) -> float:

@@ -49,7 +49,7 @@ namespace kr {
 
 // Flag bits a kernel ORs into the caller's flag word (the KR_FLAG_* values of
 // the C ABI), and the camera kinds of an image (KR_CAMERA_*).
-enum : int32_t { FLAG_RADICAND = 1, FLAG_NOT_NULL = 2, FLAG_SKY = 4, FLAG_DISK = 8 };
+enum : int32_t { FLAG_RADICAND = 1, FLAG_NOT_NULL = 2, FLAG_SKY = 4, FLAG_DISK = 8, FLAG_EDGE = 16 };
 enum : int64_t { CAMERA_BARDEEN = 0, CAMERA_ZAMO = 1 };
 
 // Termination codes (src/kerrray/photons/classification.py:60-69).
@@ -884,6 +884,53 @@ KR_HD int32_t disk_shade(const DiskShade& D, const double (&y)[8], double& inten
     const double emitted = (r >= D.r_in && r <= D.r_out) ? pow(r / D.r_in, D.neg_p) : 0.0;
     intensity = emitted * pow(g, D.n_pow);
     return 0;
+}
+
+// Bilinear value of a 0/1 mask (rows x cols, row-major bytes) at fractional
+// (row v, column u), with a ring of zeros around it: boundary.py:95-112 on one
+// point, operation for operation.
+KR_HD double mask_bilinear(const uint8_t* mask, int64_t rows, int64_t cols, double u, double v) {
+    const double up = u + 1.0, vp = v + 1.0;
+    const bool inside = up >= 0.0 && up <= static_cast<double>(cols) && vp >= 0.0 && vp <= static_cast<double>(rows);
+    if (!inside) return 0.0;
+    int64_t j0 = static_cast<int64_t>(floor(up)), i0 = static_cast<int64_t>(floor(vp));
+    j0 = j0 < 0 ? 0 : (j0 > cols ? cols : j0);
+    i0 = i0 < 0 ? 0 : (i0 > rows ? rows : i0);
+    const double fu = pymin(pymax(up - static_cast<double>(j0), 0.0), 1.0);
+    const double fv = pymin(pymax(vp - static_cast<double>(i0), 0.0), 1.0);
+    const int64_t j1 = j0 + 1 < cols + 1 ? j0 + 1 : cols + 1;
+    const int64_t i1 = i0 + 1 < rows + 1 ? i0 + 1 : rows + 1;
+    auto at = [&](int64_t i, int64_t j) -> double {
+        return (i >= 1 && i <= rows && j >= 1 && j <= cols && mask[(i - 1) * cols + (j - 1)]) ? 1.0 : 0.0;
+    };
+    return at(i0, j0) * (1.0 - fu) * (1.0 - fv) + at(i0, j1) * fu * (1.0 - fv) + at(i1, j0) * (1.0 - fu) * fv +
+           at(i1, j1) * fu * fv;
+}
+
+// Sub-pixel edge radius of a mask along one direction (cos_k, sin_k) from
+// (ca, cb): samples s_m = ds m, m = 0 .. n_samples - 1, the first one whose
+// bilinear value is below 0.5, and the linear crossing between it and the
+// sample before (boundary.py:156-170).  Grid: first pixel centre (a0, b0),
+// pixel size d, alpha along columns, beta down the rows.  Returns false when
+// the centre itself is below 0.5 or no sample is.
+KR_HD bool mask_edge_radius(const uint8_t* mask, int64_t rows, int64_t cols, double a0, double b0, double d,
+                            double ca, double cb, double cos_k, double sin_k, int64_t n_samples, double& radius) {
+    const double ds = 0.125 * d;
+    double f_prev = 0.0, s_prev = 0.0;
+    for (int64_t m = 0; m < n_samples; ++m) {
+        const double s = ds * static_cast<double>(m);
+        const double x = ca + cos_k * s;
+        const double y = cb + sin_k * s;
+        const double f = mask_bilinear(mask, rows, cols, (x - a0) / d, (b0 - y) / d);
+        if (f < 0.5) {
+            if (m == 0) return false;
+            radius = s_prev + ds * (f_prev - 0.5) / (f_prev - f);
+            return true;
+        }
+        f_prev = f;
+        s_prev = s;
+    }
+    return false;
 }
 
 // The initial state of the image point (alpha, beta) for the image's camera:

@@ -133,13 +133,14 @@ inline kr_status validate(kr_context* h, const kr_params* p, int64_t n, kr::Para
 // The error a device flag word stands for (KR_FLAG_* bits; 0 is KR_OK).
 inline kr_status flag_status(int32_t flags) {
     static_assert(KR_FLAG_RADICAND == kr::FLAG_RADICAND && KR_FLAG_NOT_NULL == kr::FLAG_NOT_NULL &&
-                      KR_FLAG_SKY == kr::FLAG_SKY && KR_FLAG_DISK == kr::FLAG_DISK &&
+                      KR_FLAG_SKY == kr::FLAG_SKY && KR_FLAG_DISK == kr::FLAG_DISK && KR_FLAG_EDGE == kr::FLAG_EDGE &&
                       KR_CAMERA_ZAMO == kr::CAMERA_ZAMO,
                   "flag and camera values of the ABI and the kernels");
     if (flags & KR_FLAG_RADICAND)
         return fail(KR_ERR_RADICAND, "a Carter radicand is negative beyond round-off at the observer");
     if (flags & KR_FLAG_SKY)
         return fail(KR_ERR_INVALID, "an image point has alpha^2 + beta^2 >= r_o^2: no direction on the observer's sky");
+    if (flags & KR_FLAG_EDGE) return fail(KR_ERR_INVALID, "the centre lies outside the captured region");
     if (flags & KR_FLAG_DISK)
         return fail(KR_ERR_INVALID,
                     "a disk crossing cannot be shaded: not equatorial, no circular orbit at its radius, or g <= 0");

@@ -190,6 +190,20 @@ __global__ __launch_bounds__(256) void disk_shade_kernel(DiskShade D, const T* e
     r_hit[idx] = rr;
 }
 
+// Mask edge radii, one lane per direction; the mask (at most a few MB) is
+// read through the caches by every lane.
+__global__ __launch_bounds__(256) void mask_edge_kernel(const uint8_t* mask, int64_t rows, int64_t cols, double a0,
+                                                        double b0, double d, double ca, double cb, const double* cosk,
+                                                        const double* sink, int64_t n_angles, int64_t n_samples,
+                                                        double* radii, int32_t* flags) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n_angles) return;
+    double r = __builtin_nan("");
+    if (!mask_edge_radius(mask, rows, cols, a0, b0, d, ca, cb, cosk[k], sink[k], n_samples, r) && flags)
+        atomicOr(flags, FLAG_EDGE);
+    radii[k] = r;
+}
+
 __global__ void fill_u64_kernel(unsigned long long* p, int64_t n) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
@@ -550,6 +564,31 @@ extern "C" kr_status kr_render_disk_dev(kr_handle h, const kr_params* p, const k
     ic.flags = d_flags;
     KR_HIP(launch_trace<IC_IMAGE>(P, p->dtype, ic, *d_out, n, s));
     KR_HIP(launch_shade(D, p->dtype, d_out->event_Y, d_out->event_hit, n, d_intensity, d_g, d_r_hit, d_flags, s));
+    return KR_OK;
+}
+
+extern "C" kr_status kr_mask_edge_dev(kr_handle h, const uint8_t* d_mask, int64_t rows, int64_t cols, double alpha0,
+                                      double beta0, double d, double centre_alpha, double centre_beta,
+                                      const double* d_cos, const double* d_sin, int64_t n_angles, double reach,
+                                      double* d_radii, int32_t* d_flags, void* stream) {
+    host::clear_error();
+    if (!h) return host::fail(KR_ERR_INVALID, "handle is NULL");
+    if (rows < 2 || cols < 2 || rows > 65536 || cols > 65536) return host::fail(KR_ERR_INVALID, "need 2 <= rows, cols <= 65536");
+    if (!(d > 0.0) || !std::isfinite(d) || !std::isfinite(alpha0) || !std::isfinite(beta0))
+        return host::fail(KR_ERR_INVALID, "the grid needs a finite first centre and pixel size d > 0");
+    if (!std::isfinite(centre_alpha) || !std::isfinite(centre_beta) || !(reach > 0.0) || !std::isfinite(reach))
+        return host::fail(KR_ERR_INVALID, "centre and reach must be finite, reach > 0");
+    if (n_angles < 0 || n_angles > (int64_t(1) << 24)) return host::fail(KR_ERR_INVALID, "n_angles must be in [0, 2^24]");
+    if (n_angles > 0 && (!d_mask || !d_cos || !d_sin || !d_radii)) return host::fail(KR_ERR_INVALID, "an array is NULL");
+    const double ds = 0.125 * d;
+    const double samples = std::ceil(reach / ds) + 1.0;
+    if (!(samples <= 1e8)) return host::fail(KR_ERR_INVALID, "reach / (d / 8) must stay below 1e8 samples");
+    if (n_angles == 0) return KR_OK;
+    KR_HIP(hipSetDevice(h->device));
+    hipLaunchKernelGGL(mask_edge_kernel, dim3(static_cast<unsigned>((n_angles + 255) / 256)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), d_mask, rows, cols, alpha0, beta0, d, centre_alpha,
+                       centre_beta, d_cos, d_sin, n_angles, static_cast<int64_t>(samples), d_radii, d_flags);
+    KR_HIP(hipGetLastError());
     return KR_OK;
 }
 

@@ -318,3 +318,49 @@ def disk_shade(mass: float, spin: float, r_in: float, r_out: float, emissivity_i
                             _ptr(y), len(y), _ptr(g), _ptr(intensity)):
         raise ValueError("a disk crossing cannot be shaded")
     return g, intensity
+
+
+def _bilinear(field: np.ndarray, u: np.ndarray, v: np.ndarray) -> np.ndarray:
+    """Bilinear value of ``field`` at fractional (row v, column u), zero ring outside (boundary.py:95-112)."""
+    n_rows, n_cols = field.shape
+    padded = np.pad(field, 1)
+    up, vp = u + 1.0, v + 1.0
+    inside = (up >= 0.0) & (up <= n_cols) & (vp >= 0.0) & (vp <= n_rows)
+    j0 = np.clip(np.floor(up).astype(np.intp), 0, n_cols)
+    i0 = np.clip(np.floor(vp).astype(np.intp), 0, n_rows)
+    fu = np.clip(up - j0, 0.0, 1.0)
+    fv = np.clip(vp - i0, 0.0, 1.0)
+    j1 = np.minimum(j0 + 1, n_cols + 1)
+    i1 = np.minimum(i0 + 1, n_rows + 1)
+    value = (padded[i0, j0] * (1.0 - fu) * (1.0 - fv) + padded[i0, j1] * fu * (1.0 - fv)
+             + padded[i1, j0] * (1.0 - fu) * fv + padded[i1, j1] * fu * fv)
+    return np.where(inside, value, 0.0)
+
+
+def mask_edge(alpha, beta, captured, n_angles: int = 360, centre=None) -> tuple[np.ndarray, np.ndarray]:
+    """``extract_boundary_from_mask`` (boundary.py:74-170): (angles, radii) of the 0.5 level set of the
+    bilinearly interpolated mask, marched from ``centre`` (the captured centroid by default) in steps of
+    1/8 pixel."""
+    a = np.asarray(alpha, dtype=np.float64)
+    b = np.asarray(beta, dtype=np.float64)
+    mask = np.asarray(captured, dtype=bool)
+    d = float(a[0, 1] - a[0, 0])
+    a0, b0 = float(a[0, 0]), float(b[0, 0])
+    if centre is None:
+        ca, cb = float(a[mask].mean()), float(b[mask].mean())
+    else:
+        ca, cb = float(centre[0]), float(centre[1])
+    field = mask.astype(np.float64)
+    angles = 2.0 * np.pi * np.arange(n_angles, dtype=np.float64) / n_angles
+    ds = 0.125 * d
+    reach = math.hypot(a[0, -1] - a[0, 0], b[0, 0] - b[-1, 0]) + 2.0 * d
+    s = ds * np.arange(int(math.ceil(reach / ds)) + 1, dtype=np.float64)
+    x = ca + np.cos(angles)[:, None] * s[None, :]
+    y = cb + np.sin(angles)[:, None] * s[None, :]
+    f = _bilinear(field, (x - a0) / d, (b0 - y) / d)
+    if np.any(f[:, 0] < 0.5):
+        raise ValueError("the centre lies outside the captured region")
+    k = np.argmax(f < 0.5, axis=1)
+    rows = np.arange(n_angles)
+    f_in, f_out = f[rows, k - 1], f[rows, k]
+    return angles, s[k - 1] + ds * (f_in - 0.5) / (f_in - f_out)

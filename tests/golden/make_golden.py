@@ -353,13 +353,45 @@ def disk(workers: int) -> None:
     )
 
 
+def edges(workers: int) -> None:
+    """extract_boundary_from_mask (boundary.py:116-170) on reference masks.
+
+    The config-2 mask (512 x 512, fov 10) and the two 64 x 64 ZAMO masks of
+    docs/shadow.md section 5 (fov 12) about their captured centroids, 360
+    directions; a 128 x 128 synthetic disk about a given centre, 90 directions.
+    """
+    if REF_SRC not in sys.path:
+        sys.path.insert(0, REF_SRC)
+    from kerrray.raytracing.boundary import extract_boundary_from_mask, mask_centroid
+
+    with np.load(os.path.join(HERE, "config2_a09_i60_512.npz")) as z:
+        mask2 = z["numpy_state"].reshape(512, 512) == 2
+    with np.load(os.path.join(HERE, "zamo_cameras.npz")) as z:
+        zamo = [z[f"c{k}_numpy_state"].reshape(64, 64) == 2 for k in (0, 1)]
+    out = {}
+    cases = [("c2", 10.0, mask2, None, 360), ("zamo0", 12.0, zamo[0], None, 360), ("zamo1", 12.0, zamo[1], None, 360)]
+    a128, b128 = pixel_grid(10.0, 128)
+    cases.append(("disk", 10.0, (a128 - 0.5) ** 2 + b128**2 < 16.0, (0.5, 0.0), 90))
+    for name, fov, mask, centre, n_angles in cases:
+        a, b = pixel_grid(fov, mask.shape[0])
+        ang, rad = extract_boundary_from_mask(a, b, mask, n_angles=n_angles, centre=centre)
+        out[f"{name}_mask"] = np.packbits(mask)
+        out[f"{name}_fov"] = np.float64(fov)
+        out[f"{name}_n"] = np.int64(mask.shape[0])
+        out[f"{name}_centre"] = np.asarray(mask_centroid(a, b, mask) if centre is None else centre, dtype=np.float64)
+        out[f"{name}_angles"] = ang
+        out[f"{name}_radii"] = rad
+    np.savez_compressed(os.path.join(HERE, "mask_edges.npz"), **out)
+    print("edges: written", flush=True)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--workers", type=int, default=8)
-    ap.add_argument("--only", choices=["config1", "config2", "bisection", "zamo", "disk"], default=None)
+    ap.add_argument("--only", choices=["config1", "config2", "bisection", "zamo", "disk", "edges"], default=None)
     args = ap.parse_args()
     for name, fn in (("config1", config1), ("config2", config2), ("bisection", bisection), ("zamo", zamo),
-                     ("disk", disk)):
+                     ("disk", disk), ("edges", edges)):
         if args.only in (None, name):
             fn(args.workers)
 

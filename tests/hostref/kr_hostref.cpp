@@ -326,3 +326,15 @@ extern "C" int krh_disk_shade(const kr_disk* disk, double mass, double spin, con
     }
     return bad;
 }
+
+// Mask edge radii by the kernel source (mask_edge_radius), one direction at a
+// time; returns the number of directions that failed.
+extern "C" int krh_mask_edge(const uint8_t* mask, int64_t rows, int64_t cols, double a0, double b0, double d,
+                             double ca, double cb, const double* cosk, const double* sink, int64_t n_angles,
+                             double reach, double* radii) {
+    const int64_t n_samples = static_cast<int64_t>(std::ceil(reach / (0.125 * d)) + 1.0);
+    int bad = 0;
+    for (int64_t k = 0; k < n_angles; ++k)
+        if (!kr::mask_edge_radius(mask, rows, cols, a0, b0, d, ca, cb, cosk[k], sink[k], n_samples, radii[k])) ++bad;
+    return bad;
+}

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from kerrray_amd.bisection import rounds_for_width
-from kerrray_amd.boundary import extract_boundary_from_mask, polygon_centroid, polygon_radii
+from kerrray_amd.boundary import polygon_centroid, polygon_radii
 from kerrray_amd.camera import BardeenCamera
 from kerrray_amd.geometry import Spacetime
 from kerrray_amd.orbits import (
@@ -66,11 +66,11 @@ def test_polygon_radii_of_a_circle() -> None:
     assert abs(cx - 2.0) < 1e-12 and abs(cy + 1.0) < 1e-12
 
 
-def test_mask_boundary_of_a_synthetic_disk() -> None:
+def test_oracle_mask_boundary_of_a_synthetic_disk(oracle) -> None:
     cam = BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=10.0, resolution=128)
     alpha, beta = cam.pixel_coordinates()
     captured = (alpha - 0.5) ** 2 + beta**2 < 4.0**2
-    ang, rad = extract_boundary_from_mask(alpha, beta, captured, n_angles=90, centre=(0.5, 0.0))
+    ang, rad = oracle.mask_edge(alpha, beta, captured, 90, (0.5, 0.0))
     assert np.max(np.abs(rad - 4.0)) < 0.5 * cam.pixel_size
 
 
