@@ -148,7 +148,7 @@ class KrPlannedOpts(ctypes.Structure):
         ("longest_first", ctypes.c_int32),
         ("band_priority", ctypes.c_int32),
         ("waves_per_simd", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("split_roles", ctypes.c_int32),
         ("axis_lz", ctypes.c_double),
     ]
 

@@ -61,6 +61,7 @@ struct PlanCounters {
     unsigned long long cursor[NBINS];  // scatter cursors
     unsigned long long inside;         // pixels inside the analytic edge, outside bin 0
     unsigned long long work;           // work-list counter of the persistent kernel
+    unsigned long long work_fp32;      // second counter: the FP32 part of the list when roles are split
     unsigned long long repair;         // repair-list length
     unsigned long long fp64_attempts;
     unsigned long long repaired;       // pixels repaired over all rounds
@@ -150,17 +151,40 @@ __global__ void plan_scatter_kernel(PlanDev pl, BardeenImage B, PlanCounters* c,
 // WAVES: wavefronts per SIMD the register allocation is made for (2: 256
 // VGPRs, 3: 168, 4: 128).  Up to 3 the stage slopes k1..k4 live in LDS
 // (192 B per lane: 147 KB per CU at 3 waves/SIMD); at 4 they stay private.
+//
+// split (mixed form only): the first ceil(bin-0 pixels / 64) wavefronts of
+// the grid take work only from the bin-0 part of the list (the FP64
+// photons), the others only from the rest (FP32); a wavefront of the first
+// kind joins the FP32 part once the FP64 part is handed out and its own
+// photons are done.  A wavefront then never holds lanes of both precisions,
+// which would run both attempts one after the other and slow the longest
+// photons, the FP64 ones, down.
 template <int BLOCK, bool MIXED, int WAVES>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void planned_trace_kernel(
     Params P, BardeenImage B, MixedCfg M, const int64_t* work, PlanCounters* c, int refill_interval,
-    int band_priority, uint8_t* used_fp64, RowsOut<double> out, int32_t* flags) {
+    int band_priority, int split, uint8_t* used_fp64, RowsOut<double> out, int32_t* flags) {
     constexpr bool LDS_SLOPES = WAVES <= 3;
     __shared__ double slopes[LDS_SLOPES ? 4 * NA * BLOCK : 1];
     double private_slopes[LDS_SLOPES ? 1 : 4 * NA];
     const SlopeStore<double> K = LDS_SLOPES ? SlopeStore<double>{&slopes[threadIdx.x], BLOCK}
                                             : SlopeStore<double>{private_slopes, 1};
     const int lane = threadIdx.x & 63;
-    const int64_t total = B.n * B.n;
+    // The part of the work list this wavefront draws from, [lo, lo + count).
+    int64_t lo = 0, count = B.n * B.n;
+    unsigned long long* cursor = &c->work;
+    const int64_t seg0 = MIXED && split ? static_cast<int64_t>(c->seg[0]) : 0;
+    bool fp64_role = false;
+    if (MIXED && split) {
+        const int64_t wave = (static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x) >> 6;
+        fp64_role = wave < (seg0 + 63) / 64;
+        if (fp64_role) {
+            count = seg0;
+        } else {
+            lo = seg0;
+            count -= seg0;
+            cursor = &c->work_fp32;
+        }
+    }
     const Geom32 G = geom32(P);
     RayState<double> R;
     R.state = RUNNING;
@@ -171,6 +195,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
     int since = refill_interval;
     while (true) {
         const bool any_busy = __any(busy);
+        if (MIXED && fp64_role && exhausted && !any_busy) {
+            // The FP64 part is handed out and this wavefront's photons are
+            // done: it joins the FP32 part instead of leaving its slot idle.
+            fp64_role = false;
+            exhausted = false;
+            lo = seg0;
+            count = B.n * B.n - seg0;
+            cursor = &c->work_fp32;
+        }
         if (!exhausted && (since >= refill_interval || !any_busy)) {
             since = 0;
             const uint64_t idle = __ballot(!busy);
@@ -178,15 +211,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
             if (n_idle > 0) {
                 const int leader = __ffsll(static_cast<unsigned long long>(idle)) - 1;
                 unsigned long long base = 0;
-                if (lane == leader) base = atomicAdd(&c->work, static_cast<unsigned long long>(n_idle));
+                if (lane == leader) base = atomicAdd(cursor, static_cast<unsigned long long>(n_idle));
                 base = __shfl(base, leader);
-                if (base + static_cast<unsigned long long>(n_idle) >= static_cast<unsigned long long>(total))
+                if (base + static_cast<unsigned long long>(n_idle) >= static_cast<unsigned long long>(count))
                     exhausted = true;
                 if (!busy) {
                     const uint64_t below = (lane == 0) ? 0ull : (idle & (~0ull >> (64 - lane)));
                     const int64_t pos = static_cast<int64_t>(base) + __popcll(below);
-                    if (pos < total) {
-                        const int64_t w = work[pos];
+                    if (pos < count) {
+                        const int64_t w = work[lo + pos];
                         idx = w & INDEX_MASK;
                         exact = !MIXED || (w & FP64_FLAG) != 0;
                         bin = static_cast<int>((w >> BIN_SHIFT) & 15);
@@ -355,6 +388,7 @@ kr_status check_opts(const kr_planned_opts* o) {
         return host::fail(KR_ERR_INVALID, "block_size must be 64, 128 or 256");
     if (o->waves_per_simd < 2 || o->waves_per_simd > 4) return host::fail(KR_ERR_INVALID, "waves_per_simd must be 2, 3 or 4");
     if (!(o->axis_lz >= 0.0)) return host::fail(KR_ERR_INVALID, "axis_lz must be >= 0");
+    if (o->split_roles != 0 && o->split_roles != 1) return host::fail(KR_ERR_INVALID, "split_roles must be 0 or 1");
     if (!(o->horizon_margin >= 0.0) || !(o->axis_sin >= 0.0) || !(o->turn_drdl >= 0.0) || !(o->turn_rmax >= 0.0))
         return host::fail(KR_ERR_INVALID, "switch thresholds must be >= 0");
     return KR_OK;
@@ -362,21 +396,22 @@ kr_status check_opts(const kr_planned_opts* o) {
 
 template <int BLOCK, bool MIXED, int WAVES>
 void launch_planned_t(int cus, hipStream_t s, const Params& P, const BardeenImage& B, const MixedCfg& M,
-                      const Work& w, int refill, int band_priority, const RowsOut<double>& o) {
+                      const Work& w, int refill, int band_priority, int split, const RowsOut<double>& o) {
     const int grid = cus * (4 * WAVES * 64 / BLOCK);  // one resident wave per SIMD slot
     hipLaunchKernelGGL((planned_trace_kernel<BLOCK, MIXED, WAVES>), dim3(grid), dim3(BLOCK), 0, s, P, B, M, w.work,
-                       w.counters, refill, band_priority, w.used_fp64, o, w.flags);
+                       w.counters, refill, band_priority, split, w.used_fp64, o, w.flags);
 }
 
 template <int BLOCK, bool MIXED>
 void launch_planned_w(int waves, int cus, hipStream_t s, const Params& P, const BardeenImage& B,
-                      const MixedCfg& M, const Work& w, int refill, int band_priority, const RowsOut<double>& o) {
+                      const MixedCfg& M, const Work& w, int refill, int band_priority, int split,
+                      const RowsOut<double>& o) {
     if (waves == 3)
-        launch_planned_t<BLOCK, MIXED, 3>(cus, s, P, B, M, w, refill, band_priority, o);
+        launch_planned_t<BLOCK, MIXED, 3>(cus, s, P, B, M, w, refill, band_priority, split, o);
     else if (waves == 4)
-        launch_planned_t<BLOCK, MIXED, 4>(cus, s, P, B, M, w, refill, band_priority, o);
+        launch_planned_t<BLOCK, MIXED, 4>(cus, s, P, B, M, w, refill, band_priority, split, o);
     else
-        launch_planned_t<BLOCK, MIXED, 2>(cus, s, P, B, M, w, refill, band_priority, o);
+        launch_planned_t<BLOCK, MIXED, 2>(cus, s, P, B, M, w, refill, band_priority, split, o);
 }
 
 kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B, const kr_planned_opts* o,
@@ -400,18 +435,19 @@ kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B,
                    d_out.max_null_error, d_out.max_energy_drift, d_out.max_lz_drift, d_out.max_carter_drift};
     const bool mixed = o->fp32_bulk != 0;
     const int refill = o->refill_interval, prio = o->band_priority ? 1 : 0, waves = o->waves_per_simd;
+    const int split = o->split_roles ? 1 : 0;
     switch (o->block_size) {
         case 64:
-            mixed ? launch_planned_w<64, true>(waves, cus, s, P, B, M, w, refill, prio, bo)
-                  : launch_planned_w<64, false>(waves, cus, s, P, B, M, w, refill, prio, bo);
+            mixed ? launch_planned_w<64, true>(waves, cus, s, P, B, M, w, refill, prio, split, bo)
+                  : launch_planned_w<64, false>(waves, cus, s, P, B, M, w, refill, prio, split, bo);
             break;
         case 128:
-            mixed ? launch_planned_w<128, true>(waves, cus, s, P, B, M, w, refill, prio, bo)
-                  : launch_planned_w<128, false>(waves, cus, s, P, B, M, w, refill, prio, bo);
+            mixed ? launch_planned_w<128, true>(waves, cus, s, P, B, M, w, refill, prio, split, bo)
+                  : launch_planned_w<128, false>(waves, cus, s, P, B, M, w, refill, prio, split, bo);
             break;
         default:
-            mixed ? launch_planned_w<256, true>(waves, cus, s, P, B, M, w, refill, prio, bo)
-                  : launch_planned_w<256, false>(waves, cus, s, P, B, M, w, refill, prio, bo);
+            mixed ? launch_planned_w<256, true>(waves, cus, s, P, B, M, w, refill, prio, split, bo)
+                  : launch_planned_w<256, false>(waves, cus, s, P, B, M, w, refill, prio, split, bo);
     }
     KR_HIP(hipGetLastError());
     for (int round = 0; round < REPAIR_ROUNDS; ++round) {

@@ -60,6 +60,9 @@ class PlannedOptions:
     # pole, where the FP32 error estimate can stall the step size: they are
     # planned in FP64 from the start, with the edge band.
     axis_lz: float = 0.1
+    # FP32 bulk only: wavefronts trace either the FP64 band and strip or FP32
+    # photons, never both in one wavefront.
+    split_roles: bool = False
 
     def kr(self) -> _lib.KrPlannedOpts:
         o = _lib.KrPlannedOpts()
@@ -75,6 +78,7 @@ class PlannedOptions:
         o.band_priority = 1 if self.band_priority else 0
         o.waves_per_simd = self.waves_per_simd
         o.axis_lz = self.axis_lz
+        o.split_roles = 1 if self.split_roles else 0
         return o
 
 

@@ -57,6 +57,28 @@ def test_fp32_bulk_with_fp64_repair_keeps_the_fp64_mask(hip, case) -> None:
     assert mixed.n_failed == 0
 
 
+@pytest.mark.parametrize("case", [(0.9, 60.0, 512, 1e-6), (0.99, 85.0, 1024, 1e-6)])
+def test_split_roles_give_the_same_rows(hip, case) -> None:
+    """Wavefronts of one precision each (split_roles) trace every photon exactly as mixed wavefronts do:
+    a photon's arithmetic does not depend on its neighbours."""
+    from kerrray_amd.camera import BardeenCamera
+    from kerrray_amd.geometry import Spacetime
+    from kerrray_amd.planned import PlannedOptions, compute_shadow_planned
+
+    spin, incl, res, rtol = case
+    st = Spacetime(1.0, spin)
+    integ, term = options(rtol=rtol, atol=1e-2 * rtol)
+    cam = BardeenCamera(radius=1000.0, inclination_deg=incl, fov=10.0, resolution=res)
+    mixed, s0 = compute_shadow_planned(st, cam, integ, term, backend=hip, options=PlannedOptions(fp32_bulk=True))
+    split, s1 = compute_shadow_planned(st, cam, integ, term, backend=hip,
+                                       options=PlannedOptions(fp32_bulk=True, split_roles=True))
+    print(f"[split roles a={spin} {res}^2] {s1.kernel_ms:.3f} ms (mixed wavefronts {s0.kernel_ms:.3f} ms), "
+          f"repaired {s1.repaired}", flush=True)
+    for name in FIELDS:
+        assert np.array_equal(getattr(split.result, name), getattr(mixed.result, name)), name
+    assert s1.fp64_attempts == s0.fp64_attempts and s1.repaired == s0.repaired
+
+
 def test_precision_form_mask_equals_the_oracle_fp64_mask_1024(hip, oracle) -> None:
     """1024^2, a=0.99, i=85 deg, rtol 1e-6: the FP32+FP64-repair mask, the GPU FP64 mask and the CPU oracle's."""
     from kerrray_amd.camera import BardeenCamera
