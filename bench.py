@@ -178,7 +178,8 @@ class PlannedRunner:
         o = self.opts
         return {"band_px": o.band_px, "horizon_margin": o.horizon_margin, "axis_sin": o.axis_sin,
                 "turn_rmax": o.turn_rmax, "turn_drdl": o.turn_drdl, "refill_interval": o.refill_interval,
-                "block_size": o.block_size, "longest_first": o.longest_first}
+                "block_size": o.block_size, "longest_first": o.longest_first, "split_roles": o.split_roles,
+                "waves_per_simd": o.waves_per_simd or "auto"}
 
 
 class PlainRunner:
@@ -520,8 +521,9 @@ def main() -> int:
         "fp64_plain": "FP64, plain one-photon-per-lane kernel",
         "fp64_planned": "FP64, persistent kernel over a longest-first work list (edge band, inside, outside); "
                         "results bitwise those of the plain kernel",
-        "fp32_fp64_repair": "FP32 right-hand sides with FP64 state/update/error norm/step control, FP64 near "
-                            "horizon, axis and turning points, FP64 edge band and repair",
+        "fp32_fp64_repair": "FP32 right-hand sides with FP64 state/update/error norm/step control; the edge band "
+                            "and polar strip traced in exact FP64 by wavefronts of their own (one per SIMD at "
+                            "this size); FP64 verify and repair; mask identical to the FP64 mask",
     }
     line = {
         "metric": METRIC,

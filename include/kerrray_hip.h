@@ -318,7 +318,9 @@ kr_status kr_batch_destroy(kr_batch b);
  *    the nearest distance classes (the longest ones), which otherwise finish
  *    last.
  *  waves_per_simd: resident wavefronts per SIMD the persistent kernel is
- *    compiled for; more hide more FP64 latency, fewer registers each.
+ *    compiled for; more hide more FP64 latency, fewer registers each.  0:
+ *    one for the split FP32 form up to 2^19 pixels (its longest FP64 photons
+ *    then run alone on their SIMD), two otherwise.
  *  split_roles (fp32_bulk only): the first ceil(band / 64) wavefronts trace
  *    only the FP64 band and strip, the others only FP32 photons, so no
  *    wavefront runs both attempts. */
@@ -333,7 +335,7 @@ typedef struct kr_planned_opts {
     int32_t block_size;
     int32_t longest_first; /* 1: work list band/strip -> inside -> outside; 0: row-major */
     int32_t band_priority; /* 1: wavefronts tracing the nearest photons issue first on their SIMD */
-    int32_t waves_per_simd; /* 2, 3 or 4: occupancy the trace kernel's registers are allocated for */
+    int32_t waves_per_simd; /* 0 automatic, or 1-4: occupancy the trace kernel's registers are allocated for */
     int32_t split_roles;    /* 1: wavefronts trace FP64 or FP32 photons, never both (fp32_bulk only) */
     double axis_lz; /* pixels with |alpha sin theta_o| < axis_lz are integrated in FP64 (near-polar photons) */
 } kr_planned_opts;

@@ -148,9 +148,11 @@ __global__ void plan_scatter_kernel(PlanDev pl, BardeenImage B, PlanCounters* c,
                     (q.bin == 0 ? BAND_FLAG : 0);
 }
 
-// WAVES: wavefronts per SIMD the register allocation is made for (2: 256
-// VGPRs, 3: 168, 4: 128).  Up to 3 the stage slopes k1..k4 live in LDS
-// (192 B per lane: 147 KB per CU at 3 waves/SIMD); at 4 they stay private.
+// WAVES: wavefronts per SIMD the register allocation is made for (1: 512
+// VGPRs, 2: 256, 3: 168, 4: 128).  At 2 and 3 the stage slopes k1..k4 live in
+// LDS (192 B per lane: 147 KB per CU at 3 waves/SIMD); at 1 they fit in
+// registers, at 4 they stay private (scratch).  With one wavefront per SIMD
+// the longest photons never share their SIMD's issue slots.
 //
 // split (mixed form only): the first ceil(bin-0 pixels / 64) wavefronts of
 // the grid take work only from the bin-0 part of the list (the FP64
@@ -163,7 +165,7 @@ template <int BLOCK, bool MIXED, int WAVES>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void planned_trace_kernel(
     Params P, BardeenImage B, MixedCfg M, const int64_t* work, PlanCounters* c, int refill_interval,
     int band_priority, int split, uint8_t* used_fp64, RowsOut<double> out, int32_t* flags) {
-    constexpr bool LDS_SLOPES = WAVES <= 3;
+    constexpr bool LDS_SLOPES = WAVES == 2 || WAVES == 3;
     __shared__ double slopes[LDS_SLOPES ? 4 * NA * BLOCK : 1];
     double private_slopes[LDS_SLOPES ? 1 : 4 * NA];
     const SlopeStore<double> K = LDS_SLOPES ? SlopeStore<double>{&slopes[threadIdx.x], BLOCK}
@@ -386,7 +388,8 @@ kr_status check_opts(const kr_planned_opts* o) {
         return host::fail(KR_ERR_INVALID, "refill_interval must be in [1, 4096]");
     if (o->block_size != 64 && o->block_size != 128 && o->block_size != 256)
         return host::fail(KR_ERR_INVALID, "block_size must be 64, 128 or 256");
-    if (o->waves_per_simd < 2 || o->waves_per_simd > 4) return host::fail(KR_ERR_INVALID, "waves_per_simd must be 2, 3 or 4");
+    if (o->waves_per_simd < 0 || o->waves_per_simd > 4)
+        return host::fail(KR_ERR_INVALID, "waves_per_simd must be 0 (automatic), 1, 2, 3 or 4");
     if (!(o->axis_lz >= 0.0)) return host::fail(KR_ERR_INVALID, "axis_lz must be >= 0");
     if (o->split_roles != 0 && o->split_roles != 1) return host::fail(KR_ERR_INVALID, "split_roles must be 0 or 1");
     if (!(o->horizon_margin >= 0.0) || !(o->axis_sin >= 0.0) || !(o->turn_drdl >= 0.0) || !(o->turn_rmax >= 0.0))
@@ -406,7 +409,9 @@ template <int BLOCK, bool MIXED>
 void launch_planned_w(int waves, int cus, hipStream_t s, const Params& P, const BardeenImage& B,
                       const MixedCfg& M, const Work& w, int refill, int band_priority, int split,
                       const RowsOut<double>& o) {
-    if (waves == 3)
+    if (waves == 1)
+        launch_planned_t<BLOCK, MIXED, 1>(cus, s, P, B, M, w, refill, band_priority, split, o);
+    else if (waves == 3)
         launch_planned_t<BLOCK, MIXED, 3>(cus, s, P, B, M, w, refill, band_priority, split, o);
     else if (waves == 4)
         launch_planned_t<BLOCK, MIXED, 4>(cus, s, P, B, M, w, refill, band_priority, split, o);
@@ -434,8 +439,14 @@ kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B,
     RowsOut<double> bo{static_cast<double*>(d_out.Y), d_out.lam, d_out.n_steps, d_out.n_rejected, d_out.state,
                    d_out.max_null_error, d_out.max_energy_drift, d_out.max_lz_drift, d_out.max_carter_drift};
     const bool mixed = o->fp32_bulk != 0;
-    const int refill = o->refill_interval, prio = o->band_priority ? 1 : 0, waves = o->waves_per_simd;
+    const int refill = o->refill_interval, prio = o->band_priority ? 1 : 0;
     const int split = o->split_roles ? 1 : 0;
+    // Automatic occupancy: up to 2^19 pixels the split FP32 form is bounded
+    // by its longest FP64 photons, which run fastest alone on their SIMD (one
+    // wavefront per SIMD: 4.3 against 5.1 ms at 512^2); above, throughput
+    // decides and two wavefronts per SIMD are faster (8.9 against 10.4 ms at
+    // 1024^2).  The FP64 form is faster at two in both cases.
+    const int waves = o->waves_per_simd != 0 ? o->waves_per_simd : (mixed && split && pixels <= (int64_t(1) << 19) ? 1 : 2);
     switch (o->block_size) {
         case 64:
             mixed ? launch_planned_w<64, true>(waves, cus, s, P, B, M, w, refill, prio, split, bo)

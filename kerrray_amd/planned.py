@@ -55,14 +55,16 @@ class PlannedOptions:
     block_size: int = 128
     longest_first: bool = True
     band_priority: bool = False
-    waves_per_simd: int = 2
+    # 0: automatic (one wavefront per SIMD for the split FP32 form up to 2^19
+    # pixels, where its longest FP64 photons bound the time; two otherwise).
+    waves_per_simd: int = 0
     # Photons with |L_z| = |alpha sin theta_o| < axis_lz (M) pass close to a
     # pole, where the FP32 error estimate can stall the step size: they are
     # planned in FP64 from the start, with the edge band.
     axis_lz: float = 0.1
     # FP32 bulk only: wavefronts trace either the FP64 band and strip or FP32
     # photons, never both in one wavefront.
-    split_roles: bool = False
+    split_roles: bool = True
 
     def kr(self) -> _lib.KrPlannedOpts:
         o = _lib.KrPlannedOpts()

@@ -59,8 +59,9 @@ def test_fp32_bulk_with_fp64_repair_keeps_the_fp64_mask(hip, case) -> None:
 
 @pytest.mark.parametrize("case", [(0.9, 60.0, 512, 1e-6), (0.99, 85.0, 1024, 1e-6)])
 def test_split_roles_give_the_same_rows(hip, case) -> None:
-    """Wavefronts of one precision each (split_roles) trace every photon exactly as mixed wavefronts do:
-    a photon's arithmetic does not depend on its neighbours."""
+    """Wavefronts of one precision each (split_roles, with the automatic occupancy: one wavefront per SIMD
+    at 512^2, two at 1024^2) trace every photon exactly as mixed wavefronts at two per SIMD do: a photon's
+    arithmetic depends neither on its neighbours nor on the register allocation."""
     from kerrray_amd.camera import BardeenCamera
     from kerrray_amd.geometry import Spacetime
     from kerrray_amd.planned import PlannedOptions, compute_shadow_planned
@@ -69,7 +70,8 @@ def test_split_roles_give_the_same_rows(hip, case) -> None:
     st = Spacetime(1.0, spin)
     integ, term = options(rtol=rtol, atol=1e-2 * rtol)
     cam = BardeenCamera(radius=1000.0, inclination_deg=incl, fov=10.0, resolution=res)
-    mixed, s0 = compute_shadow_planned(st, cam, integ, term, backend=hip, options=PlannedOptions(fp32_bulk=True))
+    mixed, s0 = compute_shadow_planned(st, cam, integ, term, backend=hip,
+                                       options=PlannedOptions(fp32_bulk=True, split_roles=False, waves_per_simd=2))
     split, s1 = compute_shadow_planned(st, cam, integ, term, backend=hip,
                                        options=PlannedOptions(fp32_bulk=True, split_roles=True))
     print(f"[split roles a={spin} {res}^2] {s1.kernel_ms:.3f} ms (mixed wavefronts {s0.kernel_ms:.3f} ms), "
