@@ -99,53 +99,67 @@ __device__ __forceinline__ PixelPlan plan_pixel(const PlanDev& pl, const Bardeen
     return PixelPlan{bin, rho < rc};
 }
 
-// Wavefront-aggregated append: one atomic per wavefront and class instead of
-// one per pixel (a same-address atomic from every lane serialises at the
-// L2); returns the lane's slot among the class's lanes of this wavefront.
-__device__ __forceinline__ unsigned long long wave_append(unsigned long long* counter, bool mine) {
-    const uint64_t m = __ballot(mine);
-    if (!m) return 0ull;
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(counter, static_cast<unsigned long long>(__popcll(m)));
-    base = __shfl(base, leader);
-    const uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
-    return base + static_cast<unsigned long long>(__popcll(below));
-}
-
 // Work-list order: bin 0 (FP64: edge band and polar strip), then the other
-// pixels inside the analytic edge, then those outside, each row-major.  This
-// starts the longest photons first and mixes long and short photons among
-// those in flight; ordering every pixel by its distance bin instead puts
-// only long photons in flight at once and measures slower.
-__device__ __forceinline__ int order_class(const PixelPlan& q) { return q.bin == 0 ? 0 : (q.inside ? 1 : 2); }
+// pixels inside the analytic edge, then those outside, each in no particular
+// order.  This starts the longest photons first and mixes long and short
+// photons among those in flight; ordering every pixel by its distance bin
+// instead puts only long photons in flight at once and measures slower.
+__device__ __forceinline__ int order_class(int bin, bool inside) { return bin == 0 ? 0 : (inside ? 1 : 2); }
 
-__global__ void plan_count_kernel(PlanDev pl, BardeenImage B, PlanCounters* c) {
-    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool valid = idx < B.n * B.n;
-    const PixelPlan q = valid ? plan_pixel(pl, B, idx) : PixelPlan{-1, false};
-    for (int b = 0; b < NBINS; ++b) wave_append(&c->seg[b], q.bin == b);
-    wave_append(&c->inside, valid && q.bin > 0 && q.inside);
+// The plan kernels run 1024-thread workgroups and count in LDS, so the
+// global counters take one atomic per workgroup and class: same-address
+// atomics from every wavefront serialise at the L2 and cost more than the
+// classification itself.
+constexpr int PLAN_BLOCK = 1024;
+
+// Classifies every pixel (bin | inside << 4 into cls) and counts the bins and
+// the inside pixels outside bin 0.
+__global__ __launch_bounds__(PLAN_BLOCK) void plan_count_kernel(PlanDev pl, BardeenImage B, PlanCounters* c,
+                                                                uint8_t* cls) {
+    __shared__ unsigned int counts[NBINS + 1];
+    if (threadIdx.x < NBINS + 1) counts[threadIdx.x] = 0u;
+    __syncthreads();
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * PLAN_BLOCK + threadIdx.x;
+    if (idx < B.n * B.n) {
+        const PixelPlan q = plan_pixel(pl, B, idx);
+        cls[idx] = static_cast<uint8_t>(q.bin | (q.inside ? 0x10 : 0));
+        atomicAdd(&counts[q.bin], 1u);
+        if (q.bin > 0 && q.inside) atomicAdd(&counts[NBINS], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < NBINS + 1 && counts[threadIdx.x] != 0u) {
+        unsigned long long* target = threadIdx.x < NBINS ? &c->seg[threadIdx.x] : &c->inside;
+        atomicAdd(target, static_cast<unsigned long long>(counts[threadIdx.x]));
+    }
 }
 
-__global__ void plan_scatter_kernel(PlanDev pl, BardeenImage B, PlanCounters* c, int64_t* work, int fp32_bulk,
-                                    int longest_first) {
-    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool valid = idx < B.n * B.n;
-    const PixelPlan q = valid ? plan_pixel(pl, B, idx) : PixelPlan{-1, false};
-    const int k = valid ? order_class(q) : -1;
-    unsigned long long pos = static_cast<unsigned long long>(idx);
+// Writes the work list from the class bytes: each workgroup reserves one
+// contiguous run per order class.
+__global__ __launch_bounds__(PLAN_BLOCK) void plan_scatter_kernel(int64_t total, PlanCounters* c, const uint8_t* cls,
+                                                                  int64_t* work, int fp32_bulk, int longest_first) {
+    __shared__ unsigned int local[3];
+    __shared__ unsigned long long base[3];
+    if (threadIdx.x < 3) local[threadIdx.x] = 0u;
+    __syncthreads();
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * PLAN_BLOCK + threadIdx.x;
+    const bool valid = idx < total;
+    const int code = valid ? cls[idx] : 0;
+    const int bin = code & 15;
+    const int k = order_class(bin, (code & 0x10) != 0);
+    unsigned int slot = 0u;
+    if (valid && longest_first) slot = atomicAdd(&local[k], 1u);
+    __syncthreads();
+    if (longest_first && threadIdx.x < 3 && local[threadIdx.x] != 0u)
+        base[threadIdx.x] = atomicAdd(&c->cursor[threadIdx.x], static_cast<unsigned long long>(local[threadIdx.x]));
+    __syncthreads();
+    if (!valid) return;
+    int64_t pos = idx;
     if (longest_first) {
-        const unsigned long long first[3] = {0ull, c->seg[0], c->seg[0] + c->inside};
-        for (int cls = 0; cls < 3; ++cls) {
-            const unsigned long long slot = wave_append(&c->cursor[cls], k == cls);
-            if (k == cls) pos = first[cls] + slot;
-        }
+        const unsigned long long first = k == 0 ? 0ull : (k == 1 ? c->seg[0] : c->seg[0] + c->inside);
+        pos = static_cast<int64_t>(first + base[k] + slot);
     }
-    if (valid)
-        work[pos] = idx | (static_cast<int64_t>(q.bin) << BIN_SHIFT) | ((q.bin == 0 || !fp32_bulk) ? FP64_FLAG : 0) |
-                    (q.bin == 0 ? BAND_FLAG : 0);
+    work[pos] = idx | (static_cast<int64_t>(bin) << BIN_SHIFT) | ((bin == 0 || !fp32_bulk) ? FP64_FLAG : 0) |
+                (bin == 0 ? BAND_FLAG : 0);
 }
 
 // WAVES: wavefronts per SIMD the register allocation is made for (1: 512
@@ -426,13 +440,15 @@ kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B,
     int cus = 0;
     KR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
     const unsigned pblocks = static_cast<unsigned>((pixels + 255) / 256);
+    const unsigned plan_blocks = static_cast<unsigned>((pixels + PLAN_BLOCK - 1) / PLAN_BLOCK);
     PlanDev pl{ca, cb, w.radii, n_angles, (o->band_px + 1.0) * (2.0 * B.fov / static_cast<double>(B.n)), o->axis_lz};
     hipLaunchKernelGGL(planned_reset_kernel, dim3(1), dim3(64), 0, s, w.counters, w.flags);
     KR_HIP(hipGetLastError());
-    hipLaunchKernelGGL(plan_count_kernel, dim3(pblocks), dim3(256), 0, s, pl, B, w.counters);
+    // The class bytes live in used_fp64 until the trace overwrites it.
+    hipLaunchKernelGGL(plan_count_kernel, dim3(plan_blocks), dim3(PLAN_BLOCK), 0, s, pl, B, w.counters, w.used_fp64);
     KR_HIP(hipGetLastError());
-    hipLaunchKernelGGL(plan_scatter_kernel, dim3(pblocks), dim3(256), 0, s, pl, B, w.counters, w.work,
-                       o->fp32_bulk ? 1 : 0, o->longest_first ? 1 : 0);
+    hipLaunchKernelGGL(plan_scatter_kernel, dim3(plan_blocks), dim3(PLAN_BLOCK), 0, s, pixels, w.counters,
+                       w.used_fp64, w.work, o->fp32_bulk ? 1 : 0, o->longest_first ? 1 : 0);
     KR_HIP(hipGetLastError());
     const double r_plus = P.geom[G_MASS] + std::sqrt(P.geom[G_MASS] * P.geom[G_MASS] - P.geom[G_A2]);
     MixedCfg M{r_plus + o->horizon_margin, o->axis_sin, o->turn_rmax, o->turn_drdl};
