@@ -385,13 +385,56 @@ def edges(workers: int) -> None:
     print("edges: written", flush=True)
 
 
+# Critical-impact searches (validation/kerr_bisection.py): spin, r0, rtol,
+# bracket, tol, n_probe, directions.
+IMPACT_SEARCHES = (
+    (0.0, 50.0, 1e-8, 4.0, 7.0, 1e-7, 8, (True,)),
+    (0.9, 50.0, 1e-8, 2.0, 8.0, 1e-6, 8, (True, False)),
+    (-0.5, 1000.0, 1e-9, 2.0, 9.0, 1e-6, 5, (True, False)),
+    (0.998, 50.0, 1e-8, 1.5, 8.0, 1e-5, 1, (True,)),
+)
+
+
+def impacts(workers: int) -> None:
+    """photon_from_constants (initial_conditions.py:83-112) on equatorial inward rays, and
+    bisect_critical_impacts (kerr_bisection.py:177-300) on the searches above (atol = 1e-2 rtol,
+    escape radius = r0)."""
+    if REF_SRC not in sys.path:
+        sys.path.insert(0, REF_SRC)
+    from kerrray.geodesics import IntegratorOptions, TerminationOptions, photon_from_constants
+    from kerrray.geometry import Spacetime
+    from kerrray.validation.kerr_bisection import bisect_critical_impacts
+
+    out: dict[str, np.ndarray] = {}
+    rows, states = [], []
+    for spin in (0.0, 0.9, -0.5, 0.998):
+        st = Spacetime(1.0, spin)
+        for r0 in (50.0, 1000.0):
+            for b in (1.5, 3.1, 5.196152422706632, 7.25):
+                for lz in (b, -b):
+                    rows.append((spin, r0, lz))
+                    states.append(photon_from_constants(st, r0, 0.5 * math.pi, 1.0, lz, 0.0, sign_r=-1, sign_theta=1))
+    out["constants_rows"] = np.asarray(rows)
+    out["constants_Y0"] = np.asarray(states)
+    t0 = time.time()
+    for k, (spin, r0, rtol, lo, hi, tol, n_probe, dirs) in enumerate(IMPACT_SEARCHES):
+        res = bisect_critical_impacts(
+            Spacetime(1.0, spin), IntegratorOptions(method="rk45", rtol=rtol, atol=1e-2 * rtol),
+            TerminationOptions(escape_radius=r0), r0=r0, b_lo=lo, b_hi=hi, tol=tol, max_iterations=60,
+            n_probe=n_probe, directions=dirs)
+        out[f"s{k}_brackets"] = np.asarray([[r.b_lo, r.b_hi, r.iterations, r.n_rays, r.max_steps] for r in res])
+    print(f"impacts: {time.time() - t0:.1f}s", flush=True)
+    np.savez_compressed(os.path.join(HERE, "critical_impacts.npz"), **out)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--workers", type=int, default=8)
-    ap.add_argument("--only", choices=["config1", "config2", "bisection", "zamo", "disk", "edges"], default=None)
+    ap.add_argument("--only", choices=["config1", "config2", "bisection", "zamo", "disk", "edges", "impacts"],
+                    default=None)
     args = ap.parse_args()
     for name, fn in (("config1", config1), ("config2", config2), ("bisection", bisection), ("zamo", zamo),
-                     ("disk", disk), ("edges", edges)):
+                     ("disk", disk), ("edges", edges), ("impacts", impacts)):
         if args.only in (None, name):
             fn(args.workers)
 
