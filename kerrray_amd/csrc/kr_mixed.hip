@@ -3,15 +3,17 @@
 // One image, four kernels on one stream (no host synchronisation between):
 //  1. plan: every pixel is put in one of three classes with the analytic
 //     shadow edge (polar radii of Bardeen's curve about a centre, given by
-//     the caller): within `band_px` (+1 for the curve's sampling) pixels of
-//     the edge, inside, outside.  A work list is written in that order --
-//     the edge band holds the longest photons (near-critical orbits) and the
-//     inside the next longest (captured photons slow down at the horizon),
-//     so the slowest photons start first.  Band photons are flagged FP64.
+//     the caller): the edge band (within `band_px` + 1 pixels of the edge)
+//     with the polar strip, inside, outside.  A work list is written in that
+//     order -- the edge band holds the longest photons (near-critical
+//     orbits) and the inside the next longest (captured photons slow down at
+//     the horizon), so the slowest photons start first.  Band and strip
+//     photons are flagged FP64.
 //  2. trace: a persistent kernel walks the work list with lane refill (as
 //     kr_batch.hip).  Flagged photons, and all photons when the FP32 bulk is
 //     off, take the exact FP64 attempt of the reference; the others take the
-//     mixed attempt of kr_mixed.h.  Each pixel records the precision used.
+//     mixed attempt of kr_mixed.h.  With split roles a wavefront traces
+//     photons of one precision only.  Each pixel records the precision used.
 //  3. verify: any pixel within `band_px` pixels (Chebyshev) of a
 //     captured/not-captured transition of the traced mask, or whose photon
 //     ended neither captured nor escaped, that was not traced in FP64 goes
