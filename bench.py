@@ -409,6 +409,10 @@ def ragged_batch(hip, log) -> dict:
     res = batch.results()
     ms = batch.timed(repeats=5, graph=True)
     batch.close()
+    in_job_order = DeviceBatch(hip, jobs, _lib.KrTune(tune.block_size, tune.waves_per_simd, tune.refill_interval, 0))
+    in_job_order.run(graph=True)
+    ms_job_order = in_job_order.timed(repeats=3, graph=True)
+    in_job_order.close()
     total = sum(j.n_rays for j in jobs)
     fl = sum(flops.batch_flops(r.n_steps, r.n_rejected) for r in res)
     ach = fl / (ms * 1e-3) / 1e12
@@ -427,6 +431,7 @@ def ragged_batch(hip, log) -> dict:
         identical &= bool(np.array_equal(out.state, r.state) and np.array_equal(out.Y, r.Y))
     return {"photons": total, "photons_per_s": total / (ms * 1e-3), "kernel_ms": ms,
             "plain_kernel_ms_sum": plain_ms, "speedup_vs_plain_per_image": plain_ms / ms,
+            "kernel_ms_images_in_job_order": ms_job_order,
             "roofline_frac": ach / FP64_VECTOR_PEAK_TFLOPS, "masks_identical_to_plain_kernel": identical,
             "tune": {"block_size": tune.block_size, "waves_per_simd": tune.waves_per_simd,
                      "refill_interval": tune.refill_interval, "candidates": len(table), "seconds": tune_s}}

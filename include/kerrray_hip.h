@@ -272,7 +272,7 @@ typedef struct kr_tune {
     int32_t block_size;
     int32_t waves_per_simd;
     int32_t refill_interval;
-    int32_t reserved;
+    int32_t order; /* 0: images in job order; 1: smallest rtol first, then most pixels (outputs stay in job order) */
 } kr_tune;
 
 typedef struct kr_batch_s* kr_batch;

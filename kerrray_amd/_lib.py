@@ -168,7 +168,7 @@ class KrTune(ctypes.Structure):
         ("block_size", ctypes.c_int32),
         ("waves_per_simd", ctypes.c_int32),
         ("refill_interval", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("order", ctypes.c_int32),
     ]
 
 

@@ -128,7 +128,7 @@ class DeviceBatch:
         o.max_lz_drift, o.max_carter_drift = self.bufs["m2"].data_ptr(), self.bufs["m3"].data_ptr()
         self.outputs = o
         arr = (_lib.KrJob * len(self.jobs))(*[_kr_job(j) for j in self.jobs])
-        self.tune = tune or _lib.KrTune(256, 2, 4, 0)
+        self.tune = tune or _lib.KrTune(256, 2, 4, 1)
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().kr_batch_create(backend.handle.ptr, arr, len(self.jobs), ctypes.byref(self.tune),
                                               ctypes.byref(o), ctypes.byref(h)), "kr_batch_create")
@@ -194,7 +194,8 @@ def _cache_path() -> str:
 def autotune(backend: HipBackend, jobs: list[ImageJob], *, block_sizes=BLOCK_SIZES, waves=WAVES_PER_SIMD,
              intervals=REFILL_INTERVALS, repeats: int = 2, cache: bool = True,
              log=None) -> tuple[_lib.KrTune, list[dict]]:
-    """Time every (block size, waves per SIMD, refill interval) on ``jobs``; return the fastest and the table."""
+    """Time every (block size, waves per SIMD, refill interval) on ``jobs``, images in cost order (most
+    expensive photons first); return the fastest and the table."""
     import torch
 
     key = None
@@ -202,19 +203,19 @@ def autotune(backend: HipBackend, jobs: list[ImageJob], *, block_sizes=BLOCK_SIZ
         import hashlib
 
         sig = json.dumps([asdict(j) for j in jobs], sort_keys=True)
-        digest = hashlib.sha256(f"{sig}|{block_sizes}|{waves}|{intervals}".encode()).hexdigest()[:16]
+        digest = hashlib.sha256(f"{sig}|{block_sizes}|{waves}|{intervals}|order1".encode()).hexdigest()[:16]
         key = f"{torch.cuda.get_device_name(backend.device)}|{digest}"
         try:
             with open(_cache_path()) as f:
                 hit = json.load(f).get(key)
             if hit:
-                return _lib.KrTune(hit["block_size"], hit["waves_per_simd"], hit["refill_interval"], 0), hit["table"]
+                return _lib.KrTune(hit["block_size"], hit["waves_per_simd"], hit["refill_interval"], 1), hit["table"]
         except (OSError, ValueError):
             pass
     table = []
     best = None
     for bs, w, k in itertools.product(block_sizes, waves, intervals):
-        tune = _lib.KrTune(bs, w, k, 0)
+        tune = _lib.KrTune(bs, w, k, 1)
         batch = DeviceBatch(backend, jobs, tune)
         try:
             batch.run(graph=False)
@@ -227,7 +228,7 @@ def autotune(backend: HipBackend, jobs: list[ImageJob], *, block_sizes=BLOCK_SIZ
             log(f"autotune block={bs} waves={w} refill={k}: {ms:.3f} ms")
         if best is None or ms < best["ms"]:
             best = row
-    tune = _lib.KrTune(best["block_size"], best["waves_per_simd"], best["refill_interval"], 0)
+    tune = _lib.KrTune(best["block_size"], best["waves_per_simd"], best["refill_interval"], 1)
     if cache and key:
         try:
             path = _cache_path()

@@ -15,8 +15,15 @@
 // is copied into LDS at kernel start; a lane reads its own job's constants
 // from there.  A batch of one image uses the uniform-constant variant
 // (constants in kernel arguments, scalar registers).
+//
+// Photons are handed out in a processing order of the images that need not
+// be the job order: with kr_tune.order = 1 the images whose photons take the
+// most attempts (smallest rtol, then most pixels) go first, so their longest
+// photons do not start at the end of the launch.  Each image writes its rows
+// where the job order puts them either way.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <string>
@@ -31,8 +38,9 @@ namespace kr {
 struct JobDev {
     Params P;
     BardeenImage B;
-    int64_t first;  // first photon of this image in the batch numbering (= its first output row)
-    int64_t count;  // n * n
+    int64_t first;      // first photon of this image in the processing order
+    int64_t count;      // n * n
+    int64_t out_first;  // first output row of this image (images in job order)
 };
 
 constexpr int MAX_LDS_JOBS = 128;
@@ -102,6 +110,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
                         const JobDev& J = UNIFORM ? uniform_job : lds_jobs[job];
                         const int64_t local = idx - J.first;
                         const int64_t i = local / J.B.n, j = local - i * J.B.n;
+                        idx = J.out_first + local;  // from here on: the output row
                         double y0[8];
                         const int32_t bad = image_state(J.B, pixel_centre(J.B.fov, j, J.B.n),
                                                         -pixel_centre(J.B.fov, i, J.B.n), y0);
@@ -173,10 +182,12 @@ kr_status check_tune(const kr_tune* t) {
     if (t->waves_per_simd < 1 || t->waves_per_simd > 8) return host::fail(KR_ERR_INVALID, "waves_per_simd in [1, 8]");
     if (t->refill_interval < 1 || t->refill_interval > 4096)
         return host::fail(KR_ERR_INVALID, "refill_interval in [1, 4096]");
+    if (t->order != 0 && t->order != 1) return host::fail(KR_ERR_INVALID, "order must be 0 or 1");
     return KR_OK;
 }
 
-kr_status build_jobs(kr_context* h, const kr_job* jobs, int32_t n_jobs, std::vector<JobDev>& out, int64_t& total) {
+kr_status build_jobs(kr_context* h, const kr_job* jobs, int32_t n_jobs, int order, std::vector<JobDev>& out,
+                     int64_t& total) {
     if (!jobs || n_jobs < 1) return host::fail(KR_ERR_INVALID, "need at least one job");
     if (n_jobs > MAX_LDS_JOBS) return host::fail(KR_ERR_UNSUPPORTED, "at most 128 images per batch");
     out.resize(n_jobs);
@@ -190,9 +201,19 @@ kr_status build_jobs(kr_context* h, const kr_job* jobs, int32_t n_jobs, std::vec
         if (jobs[k].params.dtype != KR_FLOAT64 || J.P.ipar[I_ADAPTIVE] != 1 || J.P.ipar[I_DISK] != 0)
             return host::fail(KR_ERR_UNSUPPORTED, "batched images must be float64 rk45 without a disk event");
         if ((st = host::bardeen_image(&jobs[k].image, true, &J.B)) != KR_OK) return st;
-        J.first = total;
+        J.out_first = total;
         J.count = n * n;
         total += n * n;
+    }
+    if (order == 1)  // smallest rtol first, then most pixels; ties keep the job order
+        std::stable_sort(out.begin(), out.end(), [](const JobDev& x, const JobDev& y) {
+            if (x.P.consts[C_RTOL] != y.P.consts[C_RTOL]) return x.P.consts[C_RTOL] < y.P.consts[C_RTOL];
+            return x.count > y.count;
+        });
+    int64_t first = 0;
+    for (JobDev& J : out) {
+        J.first = first;
+        first += J.count;
     }
     return KR_OK;
 }
@@ -233,7 +254,7 @@ extern "C" kr_status kr_batch_create(kr_handle h, const kr_job* jobs, int32_t n_
     kr_batch_s* b = new (std::nothrow) kr_batch_s();
     if (!b) return host::fail(KR_ERR_DEVICE, "out of host memory");
     b->h = h;
-    if ((st = build_jobs(h, jobs, n_jobs, b->jobs, b->total)) != KR_OK) {
+    if ((st = build_jobs(h, jobs, n_jobs, tune->order, b->jobs, b->total)) != KR_OK) {
         delete b;
         return st;
     }

@@ -33,13 +33,14 @@ def _plain(hip, job):
     return compute_shadow(st, cam, integ, term, backend=hip).result
 
 
-@pytest.mark.parametrize("tune", [(256, 2, 4), (64, 3, 1), (128, 2, 32)])
+@pytest.mark.parametrize("tune", [(256, 2, 4, 0), (64, 3, 1, 1), (128, 2, 32, 0), (256, 2, 8, 1)])
 def test_ragged_batch_equals_plain_kernel_per_image(hip, tune) -> None:
+    """Every image's rows equal the plain kernel's, in job order, whatever the processing order (last field)."""
     from kerrray_amd import _lib
     from kerrray_amd.batch import DeviceBatch
 
     jobs = _small_ragged()
-    batch = DeviceBatch(hip, jobs, _lib.KrTune(*tune, 0))
+    batch = DeviceBatch(hip, jobs, _lib.KrTune(*tune))
     batch.run(graph=False)
     res = batch.results()
     batch.close()
