@@ -204,6 +204,38 @@ class PlainRunner:
                    "kr_trace_image_dev")
 
 
+def critical_path(hip, st, integ, term, n_steps, n_rej) -> dict:
+    """The photon with the most attempts in the timed image, traced alone (kr_trace_points_dev, one lane, exact
+    FP64): no launch of the image can finish before it."""
+    import torch
+
+    from kerrray_amd import _lib
+    from kerrray_amd.backends import kernel_params
+    from kerrray_amd.camera import BardeenCamera
+    from kerrray_amd.shadow import kr_image
+
+    attempts = n_steps + n_rej
+    k = int(np.argmax(attempts))
+    i, j = divmod(k, RES)
+    a, b = BardeenCamera(radius=R_O, inclination_deg=INCL, fov=FOV, resolution=RES).pixel_coordinates()
+    alpha = torch.tensor([a[i, j]], dtype=torch.float64, device="cuda")
+    beta = torch.tensor([b[i, j]], dtype=torch.float64, device="cuda")
+    bufs, outs = device_outputs(torch, 1, torch.float64)
+    params, img = kernel_params(st, integ, term), kr_image(st, R_O, INCL, FOV, RES)
+    stream = torch.cuda.current_stream()
+
+    def one() -> None:
+        _lib.check(_lib.lib().kr_trace_points_dev(hip.handle.ptr, ctypes.byref(params), ctypes.byref(img),
+                                                  alpha.data_ptr(), beta.data_ptr(), 1, ctypes.byref(outs), None,
+                                                  ctypes.c_void_p(stream.cuda_stream)), "kr_trace_points_dev")
+
+    one()
+    ms = timed(one, 3)
+    alone = int(bufs["n_steps"].item() + bufs["n_rejected"].item())
+    return {"pixel": [i, j], "attempts": int(attempts[k]), "attempts_alone_fp64": alone, "alone_ms": ms,
+            "us_per_attempt_alone": 1e3 * ms / max(1, alone)}
+
+
 def timed(run, repeats: int) -> float:
     """Average ms per call from HIP events on the launch stream (torch's current stream)."""
     import torch
@@ -514,6 +546,10 @@ def main() -> int:
     kernel_ms = ev0.elapsed_time(ev1) / args.steps
     ms_per_step = 1e3 * wall / args.steps
     log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step wall, {kernel_ms:.3f} ms/launch (events)")
+    crit = critical_path(hip, st, integ, term, run.bufs["n_steps"].cpu().numpy(), run.bufs["n_rejected"].cpu().numpy())
+    crit["frac_of_ms_per_launch"] = crit["alone_ms"] / kernel_ms
+    log(f"longest photon alone: {crit['attempts_alone_fp64']} attempts, {crit['alone_ms']:.3f} ms "
+        f"({crit['frac_of_ms_per_launch']:.0%} of a launch)")
 
     state = run.bufs["state"].cpu().numpy()
     n_steps = run.bufs["n_steps"].cpu().numpy()
@@ -555,6 +591,7 @@ def main() -> int:
         },
         "mask_identical_to_reference": mask_identical,
         "kernel_ms": kernel_ms,
+        "critical_path": crit,
         "roofline": {
             "bound": "valu-fp64",
             "achieved": achieved,
