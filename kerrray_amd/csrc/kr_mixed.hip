@@ -179,8 +179,33 @@ __global__ __launch_bounds__(PLAN_BLOCK) void plan_scatter_kernel(int64_t total,
 // photons, the FP64 ones, down.
 template <int BLOCK, bool MIXED, int WAVES>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void planned_trace_kernel(
-    Params P, BardeenImage B, MixedCfg M, const int64_t* work, PlanCounters* c, int refill_interval,
+    Params P_arg, BardeenImage B_arg, MixedCfg M, const int64_t* work, PlanCounters* c, int refill_interval,
     int band_priority, int split, uint8_t* used_fp64, RowsOut<double> out, int32_t* flags) {
+    // At one wavefront per SIMD there are registers to spare but scalar
+    // registers are short: the launch constants, uniform values the compiler
+    // would hold in scalar registers (and spill to vector lanes and reload,
+    // instructions a lone wavefront waits for), are moved to vector registers.
+    // The empty asm statements only pin where the values live.
+    Params P = P_arg;
+    BardeenImage B = B_arg;
+    if (WAVES == 1) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k) asm volatile("" : "+v"(P.geom[k]));
+#pragma unroll
+        for (int k = 0; k < 21; ++k) asm volatile("" : "+v"(P.consts[k]));
+#pragma unroll
+        for (int k = 0; k < 3; ++k) asm volatile("" : "+v"(P.params[k]));
+        // The image's initial-state scalars (read when a lane refills); the
+        // pixel count and camera kind stay scalar.
+        asm volatile("" : "+v"(B.r_o), "+v"(B.theta_o), "+v"(B.sin_o), "+v"(B.a), "+v"(B.a2c2), "+v"(B.cot2));
+        asm volatile("" : "+v"(B.r2a2), "+v"(B.dlt), "+v"(B.c2), "+v"(B.s2), "+v"(B.aaEE), "+v"(B.fov));
+        asm volatile("" : "+v"(B.Z.r_o), "+v"(B.Z.theta_o), "+v"(B.Z.phi_o), "+v"(B.Z.r_o2), "+v"(B.Z.e00),
+                     "+v"(B.Z.e03), "+v"(B.Z.e11));
+        asm volatile("" : "+v"(B.Z.e22), "+v"(B.Z.e33), "+v"(B.Z.g_tt), "+v"(B.Z.g_tph), "+v"(B.Z.g_rr),
+                     "+v"(B.Z.g_thth), "+v"(B.Z.g_phph));
+        asm volatile("" : "+v"(B.Z.gi_tt), "+v"(B.Z.gi_tph), "+v"(B.Z.gi_rr), "+v"(B.Z.gi_thth),
+                     "+v"(B.Z.gi_phph), "+v"(B.Z.sign));
+    }
     constexpr bool LDS_SLOPES = WAVES == 2 || WAVES == 3;
     __shared__ double slopes[LDS_SLOPES ? 4 * NA * BLOCK : 1];
     double private_slopes[LDS_SLOPES ? 1 : 4 * NA];
