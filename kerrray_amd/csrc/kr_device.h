@@ -285,8 +285,11 @@ struct SlopeStore {
 
 // dp5_step with k1..k4 held in a SlopeStore.  Every value and every
 // operation is the one dp5_step performs, in the same order, so the results
-// are identical bit for bit.
-template <typename T>
+// are identical bit for bit.  With PARK the partial error sums wait in the
+// store (k1's slot, no longer read) while the last right-hand side is
+// evaluated: for a store in LDS that takes six more values per lane out of
+// the register file across the deepest point of the attempt.
+template <typename T, bool PARK = false>
 KR_HD void dp5_step_stored(const Params& P, const T (&y)[NA], T p_t, T p_ph, T h, const T (&k0)[NA],
                            const SlopeStore<T>& K, T (&y_new)[NA], T (&err)[NA], T (&k6)[NA], double& s_new,
                            double& c_new) {
@@ -320,16 +323,17 @@ KR_HD void dp5_step_stored(const Params& P, const T (&y)[NA], T p_t, T p_ph, T h
         ys[i] = y[i] + h * (T(a50) * k0[i] + T(a51) * K.at(0, i) + T(a52) * K.at(1, i) + T(a53) * K.at(2, i) +
                             T(a54) * K.at(3, i));
     rhs(P, ys, pt, pph, k, s, c);  // k5
-    T esum[NA];
+    T esum[PARK ? 1 : NA];
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
         y_new[i] = y[i] + h * (T(a60) * k0[i] + T(a62) * K.at(1, i) + T(a63) * K.at(2, i) + T(a64) * K.at(3, i) +
                                T(a65) * k[i]);
-        esum[i] = T(e0) * k0[i] + T(e2) * K.at(1, i) + T(e3) * K.at(2, i) + T(e4) * K.at(3, i) + T(e5) * k[i];
+        T& e = PARK ? K.at(0, i) : esum[PARK ? 0 : i];
+        e = T(e0) * k0[i] + T(e2) * K.at(1, i) + T(e3) * K.at(2, i) + T(e4) * K.at(3, i) + T(e5) * k[i];
     }
     rhs(P, y_new, pt, pph, k6, s_new, c_new);
 #pragma unroll
-    for (int i = 0; i < NA; ++i) err[i] = h * (esum[i] + T(e6) * k6[i]);
+    for (int i = 0; i < NA; ++i) err[i] = h * ((PARK ? K.at(0, i) : esum[PARK ? 0 : i]) + T(e6) * k6[i]);
 }
 
 // numba_kernel.py:242-257.
@@ -509,14 +513,14 @@ struct ExactStepper {
 };
 
 // The exact attempt with the stage slopes in a SlopeStore (RK4 attempts
-// are short and keep theirs in registers).
-template <typename T, bool ADAPTIVE>
+// are short and keep theirs in registers); PARK as in dp5_step_stored.
+template <typename T, bool ADAPTIVE, bool PARK = false>
 struct StoredStepper {
     SlopeStore<T> K;
     KR_HD void operator()(const Params& P, const RayState<T>& R, T h_eff, T (&y_new)[NA], T (&err)[NA],
                           T (&k6)[NA], double& s_new, double& c_new) const {
         if (ADAPTIVE)
-            dp5_step_stored(P, R.y, R.p_t, R.p_ph, h_eff, R.f, K, y_new, err, k6, s_new, c_new);
+            dp5_step_stored<T, PARK>(P, R.y, R.p_t, R.p_ph, h_eff, R.f, K, y_new, err, k6, s_new, c_new);
         else
             rk4_step(P, R.y, R.p_t, R.p_ph, h_eff, R.f, y_new, k6, s_new, c_new);
     }

@@ -194,7 +194,8 @@ KR_HD bool needs_fp64(const MixedCfg& M, const RayState<double>& R) {
 }
 
 // Chooses the FP32-bulk or the exact FP64 step for each attempt (the FP64
-// one with its stage slopes in K).
+// one with its stage slopes in K; PARK as in dp5_step_stored).
+template <bool PARK = false>
 struct MixedStepper {
     Geom32 G;
     MixedCfg M;
@@ -204,7 +205,7 @@ struct MixedStepper {
                           double (&err)[NA], double (&k6)[NA], double& s_new, double& c_new) const {
         if (needs_fp64(M, R)) {
             if (fp64_attempts) *fp64_attempts += 1;
-            dp5_step_stored(P, R.y, R.p_t, R.p_ph, h_eff, R.f, K, y_new, err, k6, s_new, c_new);
+            dp5_step_stored<double, PARK>(P, R.y, R.p_t, R.p_ph, h_eff, R.f, K, y_new, err, k6, s_new, c_new);
         } else {
             dp5_step_mixed(G, R.y, R.p_t + 0.0, R.p_ph + 0.0, h_eff, R.f, y_new, err, k6, s_new, c_new);
         }

@@ -170,6 +170,12 @@ __global__ __launch_bounds__(PLAN_BLOCK) void plan_scatter_kernel(int64_t total,
 // registers, at 4 they stay private (scratch).  With one wavefront per SIMD
 // the longest photons never share their SIMD's issue slots.
 //
+// PARK (LDS slopes only): the DP5 error partial sums wait in LDS during the
+// attempt's last right-hand side (dp5_step_stored), which frees six more
+// values per lane: the FP64 kernel at two wavefronts per SIMD then spills
+// nothing.  It pays where throughput decides and costs where the longest
+// photon's latency does (one LDS round trip more per attempt).
+//
 // split (mixed form only): the first ceil(bin-0 pixels / 64) wavefronts of
 // the grid take work only from the bin-0 part of the list (the FP64
 // photons), the others only from the rest (FP32); a wavefront of the first
@@ -177,7 +183,7 @@ __global__ __launch_bounds__(PLAN_BLOCK) void plan_scatter_kernel(int64_t total,
 // photons are done.  A wavefront then never holds lanes of both precisions,
 // which would run both attempts one after the other and slow the longest
 // photons, the FP64 ones, down.
-template <int BLOCK, bool MIXED, int WAVES>
+template <int BLOCK, bool MIXED, int WAVES, bool PARK>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void planned_trace_kernel(
     Params P_arg, BardeenImage B_arg, MixedCfg M, const int64_t* work, PlanCounters* c, int refill_interval,
     int band_priority, int split, uint8_t* used_fp64, RowsOut<double> out, int32_t* flags) {
@@ -207,6 +213,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
                      "+v"(B.Z.gi_phph), "+v"(B.Z.sign));
     }
     constexpr bool LDS_SLOPES = WAVES == 2 || WAVES == 3;
+    static_assert(LDS_SLOPES || !PARK, "the error sums are parked only in LDS");
     __shared__ double slopes[LDS_SLOPES ? 4 * NA * BLOCK : 1];
     double private_slopes[LDS_SLOPES ? 1 : 4 * NA];
     const SlopeStore<double> K = LDS_SLOPES ? SlopeStore<double>{&slopes[threadIdx.x], BLOCK}
@@ -300,9 +307,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
         if (!any_busy) break;
         if (busy) {
             if (MIXED && !exact) {
-                ray_attempt_with<double, true, false>(P, R, MixedStepper{G, M, &fp64_att, K});
+                ray_attempt_with<double, true, false>(P, R, MixedStepper<PARK>{G, M, &fp64_att, K});
             } else {
-                ray_attempt_with<double, true, false>(P, R, StoredStepper<double, true>{K});
+                ray_attempt_with<double, true, false>(P, R, StoredStepper<double, true, PARK>{K});
             }
             if (R.state != RUNNING) {
                 store_row(out, idx, R);
@@ -438,16 +445,16 @@ kr_status check_opts(const kr_planned_opts* o) {
     return KR_OK;
 }
 
-template <int BLOCK, bool MIXED, int WAVES>
+template <int BLOCK, bool MIXED, int WAVES, bool PARK = false>
 void launch_planned_t(int cus, hipStream_t s, const Params& P, const BardeenImage& B, const MixedCfg& M,
                       const Work& w, int refill, int band_priority, int split, const RowsOut<double>& o) {
     const int grid = cus * (4 * WAVES * 64 / BLOCK);  // one resident wave per SIMD slot
-    hipLaunchKernelGGL((planned_trace_kernel<BLOCK, MIXED, WAVES>), dim3(grid), dim3(BLOCK), 0, s, P, B, M, w.work,
+    hipLaunchKernelGGL((planned_trace_kernel<BLOCK, MIXED, WAVES, PARK>), dim3(grid), dim3(BLOCK), 0, s, P, B, M, w.work,
                        w.counters, refill, band_priority, split, w.used_fp64, o, w.flags);
 }
 
 template <int BLOCK, bool MIXED>
-void launch_planned_w(int waves, int cus, hipStream_t s, const Params& P, const BardeenImage& B,
+void launch_planned_w(int waves, bool park, int cus, hipStream_t s, const Params& P, const BardeenImage& B,
                       const MixedCfg& M, const Work& w, int refill, int band_priority, int split,
                       const RowsOut<double>& o) {
     if (waves == 1)
@@ -456,6 +463,8 @@ void launch_planned_w(int waves, int cus, hipStream_t s, const Params& P, const 
         launch_planned_t<BLOCK, MIXED, 3>(cus, s, P, B, M, w, refill, band_priority, split, o);
     else if (waves == 4)
         launch_planned_t<BLOCK, MIXED, 4>(cus, s, P, B, M, w, refill, band_priority, split, o);
+    else if (park)
+        launch_planned_t<BLOCK, MIXED, 2, true>(cus, s, P, B, M, w, refill, band_priority, split, o);
     else
         launch_planned_t<BLOCK, MIXED, 2>(cus, s, P, B, M, w, refill, band_priority, split, o);
 }
@@ -488,20 +497,24 @@ kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B,
     // by its longest FP64 photons, which run fastest alone on their SIMD (one
     // wavefront per SIMD: 4.3 against 5.1 ms at 512^2); above, throughput
     // decides and two wavefronts per SIMD are faster (8.9 against 10.4 ms at
-    // 1024^2).  The FP64 form is faster at two in both cases.
-    const int waves = o->waves_per_simd != 0 ? o->waves_per_simd : (mixed && split && pixels <= (int64_t(1) << 19) ? 1 : 2);
+    // 1024^2).  The FP64 form is faster at two in both cases.  At two, the
+    // error sums are parked in LDS above 2^19 pixels (1024^2, a=0.99:
+    // FP64 14.4 against 15.0 ms, FP32 form 7.5 against 8.2 ms) and not below
+    // (512^2 FP64: 4.9 against 5.25 ms).
+    const bool large = pixels > (int64_t(1) << 19);
+    const int waves = o->waves_per_simd != 0 ? o->waves_per_simd : (mixed && split && !large ? 1 : 2);
     switch (o->block_size) {
         case 64:
-            mixed ? launch_planned_w<64, true>(waves, cus, s, P, B, M, w, refill, prio, split, bo)
-                  : launch_planned_w<64, false>(waves, cus, s, P, B, M, w, refill, prio, split, bo);
+            mixed ? launch_planned_w<64, true>(waves, large, cus, s, P, B, M, w, refill, prio, split, bo)
+                  : launch_planned_w<64, false>(waves, large, cus, s, P, B, M, w, refill, prio, split, bo);
             break;
         case 128:
-            mixed ? launch_planned_w<128, true>(waves, cus, s, P, B, M, w, refill, prio, split, bo)
-                  : launch_planned_w<128, false>(waves, cus, s, P, B, M, w, refill, prio, split, bo);
+            mixed ? launch_planned_w<128, true>(waves, large, cus, s, P, B, M, w, refill, prio, split, bo)
+                  : launch_planned_w<128, false>(waves, large, cus, s, P, B, M, w, refill, prio, split, bo);
             break;
         default:
-            mixed ? launch_planned_w<256, true>(waves, cus, s, P, B, M, w, refill, prio, split, bo)
-                  : launch_planned_w<256, false>(waves, cus, s, P, B, M, w, refill, prio, split, bo);
+            mixed ? launch_planned_w<256, true>(waves, large, cus, s, P, B, M, w, refill, prio, split, bo)
+                  : launch_planned_w<256, false>(waves, large, cus, s, P, B, M, w, refill, prio, split, bo);
     }
     KR_HIP(hipGetLastError());
     for (int round = 0; round < REPAIR_ROUNDS; ++round) {

@@ -135,7 +135,7 @@ extern "C" int krh_trace_image_stored(const kr_params* p, const kr_image* img, c
         kr::bardeen_state(B, kr::pixel_centre(B.fov, j, n), -kr::pixel_centre(B.fov, i, n), y0);
         kr::RayState<double> R;
         kr::ray_init<double, true>(P, y0, R);
-        while (R.state == kr::RUNNING) kr::ray_attempt_with<double, true, false>(P, R, kr::StoredStepper<double, true>{K});
+        while (R.state == kr::RUNNING) kr::ray_attempt_with<double, true, false>(P, R, kr::StoredStepper<double, true, true>{K});
         kr::RayResult<double> r;
         kr::ray_result<double, false>(R, r);
         store<double, true, false>(r, o, idx);
@@ -166,7 +166,7 @@ extern "C" int krh_trace_image_mixed(const kr_params* p, const kr_image* img, do
         kr::ray_init<double, true>(P, y0, R);
         int64_t f64 = 0;
         double slopes[4 * kr::NA];
-        const kr::MixedStepper step{G, M, &f64, kr::SlopeStore<double>{slopes, 1}};
+        const kr::MixedStepper<> step{G, M, &f64, kr::SlopeStore<double>{slopes, 1}};
         while (R.state == kr::RUNNING) kr::ray_attempt_with<double, true, false>(P, R, step);
         kr::RayResult<double> r;
         kr::ray_result<double, false>(R, r);

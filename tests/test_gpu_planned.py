@@ -27,16 +27,21 @@ def _images(hip, spin, incl, res, rtol, opts):
     return plain, plain_ms, planned, stats
 
 
-def test_fp64_planned_schedule_is_bitwise_the_plain_trace(hip, golden2) -> None:
+@pytest.mark.parametrize("case", [(0.9, 60.0, 512), (0.99, 85.0, 1024)])
+def test_fp64_planned_schedule_is_bitwise_the_plain_trace(hip, golden2, case) -> None:
+    """512^2 runs the kernel that keeps the DP5 error sums in registers, 1024^2 (over 2^19 pixels) the one
+    that parks them in LDS: both are the plain trace bit for bit."""
     from kerrray_amd.planned import PlannedOptions
 
-    plain, plain_ms, planned, stats = _images(hip, 0.9, 60.0, 512, 1e-6, PlannedOptions(fp32_bulk=False))
-    print(f"[fp64 planned 512^2] {stats.kernel_ms:.3f} ms (plain {plain_ms:.3f} ms), band {stats.planned_band}, "
-          f"repaired {stats.repaired}", flush=True)
+    spin, incl, res = case
+    plain, plain_ms, planned, stats = _images(hip, spin, incl, res, 1e-6, PlannedOptions(fp32_bulk=False))
+    print(f"[fp64 planned a={spin} {res}^2] {stats.kernel_ms:.3f} ms (plain {plain_ms:.3f} ms), "
+          f"band {stats.planned_band}, repaired {stats.repaired}", flush=True)
     for name in FIELDS:
         assert np.array_equal(getattr(planned.result, name), getattr(plain.result, name)), name
     assert stats.repaired == 0 and stats.fp64_fraction == 1.0
-    assert np.array_equal(planned.result.state, golden2["numpy_state"].astype(np.int64))
+    if res == 512:
+        assert np.array_equal(planned.result.state, golden2["numpy_state"].astype(np.int64))
 
 
 @pytest.mark.parametrize("case", [(0.9, 60.0, 512, 1e-6), (0.99, 85.0, 1024, 1e-6), (0.5, 17.0, 256, 1e-7)])
@@ -60,8 +65,9 @@ def test_fp32_bulk_with_fp64_repair_keeps_the_fp64_mask(hip, case) -> None:
 @pytest.mark.parametrize("case", [(0.9, 60.0, 512, 1e-6), (0.99, 85.0, 1024, 1e-6)])
 def test_split_roles_give_the_same_rows(hip, case) -> None:
     """Wavefronts of one precision each (split_roles, with the automatic occupancy: one wavefront per SIMD
-    at 512^2, two at 1024^2) trace every photon exactly as mixed wavefronts at two per SIMD do: a photon's
-    arithmetic depends neither on its neighbours nor on the register allocation."""
+    at 512^2, two at 1024^2 with the error sums parked in LDS) trace every photon exactly as mixed
+    wavefronts at one and at two per SIMD do: a photon's arithmetic depends neither on its neighbours nor
+    on where its values are kept."""
     from kerrray_amd.camera import BardeenCamera
     from kerrray_amd.geometry import Spacetime
     from kerrray_amd.planned import PlannedOptions, compute_shadow_planned
@@ -74,11 +80,14 @@ def test_split_roles_give_the_same_rows(hip, case) -> None:
                                        options=PlannedOptions(fp32_bulk=True, split_roles=False, waves_per_simd=2))
     split, s1 = compute_shadow_planned(st, cam, integ, term, backend=hip,
                                        options=PlannedOptions(fp32_bulk=True, split_roles=True))
-    print(f"[split roles a={spin} {res}^2] {s1.kernel_ms:.3f} ms (mixed wavefronts {s0.kernel_ms:.3f} ms), "
-          f"repaired {s1.repaired}", flush=True)
-    for name in FIELDS:
-        assert np.array_equal(getattr(split.result, name), getattr(mixed.result, name)), name
-    assert s1.fp64_attempts == s0.fp64_attempts and s1.repaired == s0.repaired
+    one, s2 = compute_shadow_planned(st, cam, integ, term, backend=hip,
+                                     options=PlannedOptions(fp32_bulk=True, split_roles=False, waves_per_simd=1))
+    print(f"[split roles a={spin} {res}^2] {s1.kernel_ms:.3f} ms (mixed wavefronts {s0.kernel_ms:.3f} ms at two "
+          f"per SIMD, {s2.kernel_ms:.3f} ms at one), repaired {s1.repaired}", flush=True)
+    for other, so in ((mixed, s0), (one, s2)):
+        for name in FIELDS:
+            assert np.array_equal(getattr(split.result, name), getattr(other.result, name)), name
+        assert s1.fp64_attempts == so.fp64_attempts and s1.repaired == so.repaired
 
 
 def test_precision_form_mask_equals_the_oracle_fp64_mask_1024(hip, oracle) -> None:
