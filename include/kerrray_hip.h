@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define KR_ABI_VERSION 4
+#define KR_ABI_VERSION 5
 
 typedef enum kr_status {
     KR_OK = 0,
@@ -376,9 +376,19 @@ kr_status kr_planned_read_stats(const void* d_work, size_t work_bytes, kr_planne
 typedef struct kr_tape_s* kr_tape;
 
 /* Traces the n points (alpha, beta) of image `img` in FP64 (adaptive DP5,
- * no events) into d_out, exactly as kr_trace_points_dev, and records for
- * every escaped photon its accepted step sizes and a checkpoint of its state
- * every `segment` accepted steps.  d_theta_inf / d_phi_inf (n each) receive
+ * no events) into d_out, exactly as kr_trace_points_dev, and records every
+ * escaped photon's accepted steps:
+ *   segment 0: the Jacobian tape.  Every accepted step's stage states are
+ *     recorded and turned, one step per lane, into the step's Jacobian,
+ *     through which every photon's 2 x 4 Jacobian d(theta_inf, phi_inf)/
+ *     d(alpha, beta, spin, theta_o) is accumulated (632 B of device memory
+ *     per accepted step while building, 64 B per photon kept); a
+ *     vector-Jacobian product is then J^T g per photon.  Fails with
+ *     KR_ERR_DEVICE if the device lacks the memory.
+ *   segment 1..4096: the step sizes and a checkpoint of the state every
+ *     `segment` accepted steps (56 B per step); each vector-Jacobian product
+ *     recomputes the steps from the checkpoints and sweeps them backwards.
+ * d_theta_inf / d_phi_inf (n each) receive
  * the escape direction at r = infinity: (theta, phi) carried from the final
  * state to u = 1/r = 0 along the geodesic (NaN for photons that did not
  * escape).  The map differentiated later is the sequence of accepted steps
@@ -388,17 +398,19 @@ typedef struct kr_tape_s* kr_tape;
 kr_status kr_tape_create(kr_handle h, const kr_params* p, const kr_image* img, const double* d_alpha,
                          const double* d_beta, int64_t n, int32_t segment, const kr_outputs* d_out,
                          double* d_theta_inf, double* d_phi_inf, kr_tape* tape);
-/* Photons on the tape (the escaped ones), accepted steps stored, and the
- * recording kernel's time; -1 for a NULL tape. */
+/* Photons on the tape (the escaped ones), accepted steps recorded, the
+ * recording kernel's time and the Jacobian accumulation kernels' time (0 for
+ * segment >= 1); -1 for a NULL tape. */
 int64_t kr_tape_recorded(kr_tape t);
 int64_t kr_tape_steps(kr_tape t);
 double kr_tape_record_ms(kr_tape t);
+double kr_tape_jacobian_ms(kr_tape t);
 /* Vector-Jacobian product: for every point i, d_grad[4 i + j] =
  * g_theta[i] d theta_inf / d x_j + g_phi[i] d phi_inf / d x_j with
  * x = (alpha, beta, spin, theta_o); 0 for photons that did not escape.
  * Device arrays, enqueued on `stream` (NULL is the default stream, as for
- * the other _dev calls).  Calls on one tape must not overlap (they share its
- * segment buffer). */
+ * the other _dev calls).  Calls on one tape must not overlap (they share
+ * its work buffers). */
 kr_status kr_tape_vjp(kr_tape t, const double* d_g_theta, const double* d_g_phi, double* d_grad, void* stream);
 /* Flags of the tape's runs (synchronises the device): bit 8 = a
  * re-integration differed from the trace, bit 9 = a recomputed final state

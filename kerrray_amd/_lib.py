@@ -18,7 +18,7 @@ KR_OK, KR_ERR_INVALID, KR_ERR_DEVICE, KR_ERR_RADICAND, KR_ERR_UNSUPPORTED, KR_ER
 KR_FLOAT64, KR_FLOAT32 = 0, 1
 KR_FLAG_RADICAND, KR_FLAG_NOT_NULL, KR_FLAG_SKY, KR_FLAG_DISK, KR_FLAG_EDGE = 1, 2, 4, 8, 16
 KR_CAMERA_BARDEEN, KR_CAMERA_ZAMO = 0, 1
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 # Every symbol the header declares (checked by the tests).
 EXPORTED = (
@@ -53,6 +53,7 @@ EXPORTED = (
     "kr_tape_recorded",
     "kr_tape_steps",
     "kr_tape_record_ms",
+    "kr_tape_jacobian_ms",
     "kr_tape_vjp",
     "kr_tape_flags",
     "kr_tape_destroy",
@@ -237,6 +238,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.kr_tape_steps.argtypes = [vp]
     L.kr_tape_record_ms.restype = ctypes.c_double
     L.kr_tape_record_ms.argtypes = [vp]
+    L.kr_tape_jacobian_ms.restype = ctypes.c_double
+    L.kr_tape_jacobian_ms.argtypes = [vp]
     L.kr_tape_vjp.restype = ctypes.c_int
     L.kr_tape_vjp.argtypes = [vp, vp, vp, vp, vp]
     L.kr_tape_flags.restype = ctypes.c_int

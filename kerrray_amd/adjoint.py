@@ -9,8 +9,14 @@ The reference has no derivatives of its ray map. Here the map is:
 - then the geodesic carried from the final state to r = infinity.
 
 Its outputs are the escape direction (theta_inf, phi_inf) of every escaping
-photon. csrc/kr_adjoint.hip records the steps and checkpoints on the device
-(``kr_tape_create``) and runs the reverse sweep (``kr_tape_vjp``).
+photon. csrc/kr_adjoint.hip records the steps on the device
+(``kr_tape_create``) and runs the vector-Jacobian products (``kr_tape_vjp``):
+- ``segment=0`` (default): every accepted step's stage states are recorded and
+  turned into the step's Jacobian, one step per lane; every photon's 2 x 4
+  Jacobian is accumulated through them once, and a product is J^T g
+  (632 B of device memory per step while building, 64 B per photon kept);
+- ``segment>=1``: step sizes and a checkpoint every ``segment`` steps
+  (56 B per step); each product recomputes and sweeps the steps backwards.
 
 * :class:`EscapeDirection` is a ``torch.autograd.Function``: (alpha, beta,
   spin, theta_o) -> (theta_inf, phi_inf, escaped), whose backward is one
@@ -50,7 +56,7 @@ __all__ = [
     "escape_directions", "fit_spin_inclination", "lensing_jacobian",
 ]
 
-DEFAULT_SEGMENT = 32
+DEFAULT_SEGMENT = 0  # Jacobian tape; >= 1: checkpoints every `segment` steps
 TAPE_FLAG_REPLAY = 1 << 8  # a re-integration differed from the trace
 TAPE_FLAG_RECOMPUTE = 1 << 9  # a recomputed final state differed from the traced one
 
@@ -137,6 +143,7 @@ class Tape:
         self.recorded = int(_lib.lib().kr_tape_recorded(tape))
         self.steps = int(_lib.lib().kr_tape_steps(tape))
         self.record_ms = float(_lib.lib().kr_tape_record_ms(tape))
+        self.jacobian_ms = float(_lib.lib().kr_tape_jacobian_ms(tape))
 
     def vjp(self, g_theta: torch.Tensor, g_phi: torch.Tensor) -> torch.Tensor:
         """(n, 4): g_theta d theta_inf/dx + g_phi d phi_inf/dx, x = (alpha, beta, spin, theta_o)."""

@@ -20,6 +20,17 @@
 // spin a variable) and the closed-form momentum derivatives of the
 // Hamiltonian right-hand side (numba_kernel.py:158-197).
 //
+// Step Jacobians instead of a reverse sweep: t and phi enter no right-hand
+// side, so a step's Jacobian with respect to (y, p_t, p_phi, a) is the
+// identity plus an increment D with seven nonzero columns (r, theta, p_r,
+// p_theta, p_t, p_phi, a).  D follows from the stage states alone, by
+// forward-mode tangents through the stages:
+//   Ydot_i = e + h sum_{j<i} a_ij kdot_j,  kdot_i = J_f(Y_i) Ydot_i (+ df/dp),
+//   D = h sum_i b_i kdot_i,
+// independently for every step, after which the cotangent recurrence is
+//   ybar_k = ybar_{k+1} + D_k^T ybar_{k+1}  (active columns; t and phi keep
+//   theirs),  (p_t, p_phi, a)bar += D_k^T ybar_{k+1}  (parameter columns).
+//
 // HIP-free: the kernels include it, and so does the host build of the
 // kernel source used by the tests.
 #pragma once
@@ -28,6 +39,13 @@
 
 namespace kr {
 namespace ad {
+
+// The derivative arithmetic (dual numbers, vector-Jacobian products, step
+// Jacobians) is not bound to the reference's roundings, only the primal steps
+// are: it may fuse a * b + c into one FMA.  The primal functions this file
+// calls (rhs, dp5_step) and the escape tail on plain doubles lie outside
+// these regions and keep the build's separate roundings.
+#pragma clang fp contract(fast)
 
 // Forward-mode dual number with N tangent directions.
 template <int N>
@@ -154,18 +172,38 @@ KR_HD Dual<N> dsqrt(const Dual<N>& x) {
     return z;
 }
 
+#pragma clang fp contract(off)
+
 KR_HD double dsin(double x) { return sin(x); }
 KR_HD double dcos(double x) { return cos(x); }
 KR_HD double dsqrt(double x) { return sqrt(x); }
 
+#pragma clang fp contract(fast)
+
+template <int N>
+KR_HD void dsincos(const Dual<N>& x, Dual<N>& s, Dual<N>& c) {
+    const double sv = sin(x.v), cv = cos(x.v);
+    s.v = sv;
+    c.v = cv;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        s.d[i] = cv * x.d[i];
+        c.d[i] = -sv * x.d[i];
+    }
+}
+
 // The inverse Kerr metric and its r and theta derivatives (the 15 terms of
-// numba_kernel.py:111-154, in the same order) with the spin a a variable.
-template <class S>
-KR_HD void metric_terms(const S& r, const S& th, const S& a, double m, S (&g)[15]) {
-    const S s = dsin(th);
-    const S c = dcos(th);
+// numba_kernel.py:111-154, in the same order) as dual numbers in (r, theta,
+// a).  The same quantities as the reference's expressions, arranged for
+// derivatives: three reciprocals (Sigma, Delta, sin^2) serve every quotient,
+// and sin 2 theta = 2 sin cos.
+template <int N>
+KR_HD void metric_terms(const Dual<N>& r, const Dual<N>& th, const Dual<N>& a, double m, Dual<N> (&g)[15]) {
+    using S = Dual<N>;
+    S s, c;
+    dsincos(th, s, c);
     const S s2 = s * s;
-    const S sin2th = dsin(2.0 * th);
+    const S sin2th = 2.0 * (s * c);
     const S r2 = r * r;
     const S a2 = a * a;
     const S sig = r2 + a2 * (c * c);
@@ -174,11 +212,17 @@ KR_HD void metric_terms(const S& r, const S& th, const S& a, double m, S (&g)[15
     const S big_a = r2a2 * r2a2 - a2 * dlt * s2;
     const S sd = sig * dlt;
     const S ma2 = (2.0 * m) * a;
-    g[0] = -big_a / sd;
-    g[1] = -(ma2 * r) / sd;
-    g[2] = dlt / sig;
-    g[3] = 1.0 / sig;
-    g[4] = (dlt - a2 * s2) / (sd * s2);
+    const S i_sig = 1.0 / sig;
+    const S i_dlt = 1.0 / dlt;
+    const S i_s2 = 1.0 / s2;
+    const S i_sd = i_sig * i_dlt;
+    const S i_sd2 = i_sd * i_sd;
+    const S i_sig2 = i_sig * i_sig;
+    g[0] = -(big_a * i_sd);
+    g[1] = -((ma2 * r) * i_sd);
+    g[2] = dlt * i_sig;
+    g[3] = i_sig;
+    g[4] = (dlt - a2 * s2) * (i_sd * i_s2);
     const S sig_r = 2.0 * r;
     const S sig_th = -(a2 * sin2th);
     const S dlt_r = 2.0 * (r - m);
@@ -186,18 +230,16 @@ KR_HD void metric_terms(const S& r, const S& th, const S& a, double m, S (&g)[15
     const S a_th = -(a2 * dlt * sin2th);
     const S sd_r = sig_r * dlt + sig * dlt_r;
     const S sd_th = sig_th * dlt;
-    const S sd2 = sd * sd;
-    const S sig2 = sig * sig;
-    g[5] = -(a_r * sd - big_a * sd_r) / sd2;
-    g[6] = -(ma2 * (sd - r * sd_r)) / sd2;
-    g[7] = (dlt_r * sig - dlt * sig_r) / sig2;
-    g[8] = -sig_r / sig2;
-    g[9] = -sig_r / (sig2 * s2) + a2 * sd_r / sd2;
-    g[10] = -(a_th * sd - big_a * sd_th) / sd2;
-    g[11] = ma2 * r * sd_th / sd2;
-    g[12] = -(dlt * sig_th) / sig2;
-    g[13] = -sig_th / sig2;
-    g[14] = -(sig_th * s2 + sig * sin2th) / (sig2 * (s2 * s2)) + a2 * sd_th / sd2;
+    g[5] = -((a_r * sd - big_a * sd_r) * i_sd2);
+    g[6] = -((ma2 * (sd - r * sd_r)) * i_sd2);
+    g[7] = (dlt_r * sig - dlt * sig_r) * i_sig2;
+    g[8] = -(sig_r * i_sig2);
+    g[9] = -(sig_r * (i_sig2 * i_s2)) + (a2 * sd_r) * i_sd2;
+    g[10] = -((a_th * sd - big_a * sd_th) * i_sd2);
+    g[11] = (ma2 * r * sd_th) * i_sd2;
+    g[12] = -((dlt * sig_th) * i_sig2);
+    g[13] = -(sig_th * i_sig2);
+    g[14] = -((sig_th * s2 + sig * sin2th) * (i_sig2 * (i_s2 * i_s2))) + (a2 * sd_th) * i_sd2;
 }
 
 // v^T df, for the right-hand side f(y; p_t, p_phi, a) of the six integrated
@@ -296,6 +338,172 @@ KR_HD void step_adjoint(const Params& P, double m, double a, const double (&y)[N
     }
 }
 
+// ---------------------------------------------------------------------------
+// Step Jacobians.  Columns of D: the active state components r, theta, p_r,
+// p_theta (integrated-state indices 1, 2, 4, 5), then p_t, p_phi, a.  D[i][c]
+// is d(y_new[i] - y[i]) / d x_c.
+constexpr int JC = 7;
+constexpr int JW = NA * JC;  // entries of one step's D, row-major
+constexpr int ACTIVE[4] = {1, 2, 4, 5};
+
+// The right-hand side's partial derivatives at one state: every row depends
+// on r, theta and a; the momenta enter only the rows listed.
+struct RhsJac {
+    double r[NA], th[NA], a[NA];
+    double pr1, pr4, pr5;           // d f_1, f_4, f_5 / d p_r
+    double pth2, pth4, pth5;        // d f_2, f_4, f_5 / d p_theta
+    double pt0, pt3, pt4, pt5;      // d f_0, f_3, f_4, f_5 / d p_t
+    double pph0, pph3, pph4, pph5;  // d f_0, f_3, f_4, f_5 / d p_phi
+};
+
+// J_f at the state Y (the transposes of rhs_vjp's products, entry by entry).
+KR_HD void rhs_jacobian(double m, double a, const double (&Y)[NA], double pt, double pph, RhsJac& J) {
+    Dual<3> g[15];
+    metric_terms(seed<3>(Y[1], 0), seed<3>(Y[2], 1), seed<3>(a, 2), m, g);
+    if (pph == 0.0) {  // polar-axis guard of the primal right-hand side
+        g[4] = constant<3>(0.0);
+        g[9] = constant<3>(0.0);
+        g[14] = constant<3>(0.0);
+    }
+    const double pr = Y[4], pth = Y[5];
+    const double pt2 = pt * pt, ptpph = pt * pph, pr2 = pr * pr, pth2 = pth * pth, pph2 = pph * pph;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double* o = k == 0 ? J.r : (k == 1 ? J.th : J.a);
+        o[0] = g[0].d[k] * pt + g[1].d[k] * pph;
+        o[1] = g[2].d[k] * pr;
+        o[2] = g[3].d[k] * pth;
+        o[3] = g[1].d[k] * pt + g[4].d[k] * pph;
+        o[4] = -0.5 * (g[5].d[k] * pt2 + 2.0 * g[6].d[k] * ptpph + g[7].d[k] * pr2 + g[8].d[k] * pth2 +
+                       g[9].d[k] * pph2);
+        o[5] = -0.5 * (g[10].d[k] * pt2 + 2.0 * g[11].d[k] * ptpph + g[12].d[k] * pr2 + g[13].d[k] * pth2 +
+                       g[14].d[k] * pph2);
+    }
+    J.pr1 = g[2].v;
+    J.pr4 = -(g[7].v * pr);
+    J.pr5 = -(g[12].v * pr);
+    J.pth2 = g[3].v;
+    J.pth4 = -(g[8].v * pth);
+    J.pth5 = -(g[13].v * pth);
+    J.pt0 = g[0].v;
+    J.pt3 = g[1].v;
+    J.pt4 = -(g[5].v * pt + g[6].v * pph);
+    J.pt5 = -(g[10].v * pt + g[11].v * pph);
+    J.pph0 = g[1].v;
+    J.pph3 = g[4].v;
+    J.pph4 = -(g[6].v * pt + g[9].v * pph);
+    J.pph5 = -(g[11].v * pt + g[14].v * pph);
+}
+
+// Columns [C0, C0 + NC) of one step's increment D from its six stage states:
+// stage(s, Y) loads stage s's state into Y; out(i, c, v) receives D[i][c].
+template <int C0, int NC, class StageIn, class Out>
+KR_HD void step_increment(double m, double a, double pt, double pph, double h, const StageIn& stage, const Out& out) {
+    using namespace dp;
+    const double A[6][5] = {{0, 0, 0, 0, 0},         {a10, 0, 0, 0, 0},     {a20, a21, 0, 0, 0},
+                            {a30, a31, a32, 0, 0},   {a40, a41, a42, a43, 0}, {a50, a51, a52, a53, a54}};
+    const double b[6] = {a60, 0.0, a62, a63, a64, a65};
+    double kact[5][4][NC];  // active rows of the stage tangents 0..4
+    double acc[NA][NC];     // sum_i b_i kdot_i
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[i][c] = 0.0;
+    double dep = 0.0;  // a result of the previous stage
+#pragma unroll
+    for (int st = 0; st < 6; ++st) {
+        double Y[NA];
+        stage(st, Y);
+        // One stage's metric derivatives at a time: the stages are
+        // independent until their tangents combine, and interleaved they
+        // would hold several stages' registers at once.
+        KR_AFTER(dep, Y[0], Y[1], Y[2], Y[3], Y[4], Y[5]);
+        RhsJac J;
+        rhs_jacobian(m, a, Y, pt, pph, J);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int col = C0 + c;
+            // Tangent of the stage state (active rows) along column col.
+            double d[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                double sum = 0.0;
+#pragma unroll
+                for (int j = 0; j < st; ++j) sum += A[st][j] * kact[j][r][c];
+                const double base = (col == r) ? 1.0 : 0.0;
+                d[r] = st == 0 ? base : base + h * sum;
+            }
+            double k[NA];
+#pragma unroll
+            for (int i = 0; i < NA; ++i) k[i] = J.r[i] * d[0] + J.th[i] * d[1];
+            k[1] += J.pr1 * d[2];
+            k[4] += J.pr4 * d[2] + J.pth4 * d[3];
+            k[5] += J.pr5 * d[2] + J.pth5 * d[3];
+            k[2] += J.pth2 * d[3];
+            if (col == 4) {
+                k[0] += J.pt0;
+                k[3] += J.pt3;
+                k[4] += J.pt4;
+                k[5] += J.pt5;
+            } else if (col == 5) {
+                k[0] += J.pph0;
+                k[3] += J.pph3;
+                k[4] += J.pph4;
+                k[5] += J.pph5;
+            } else if (col == 6) {
+#pragma unroll
+                for (int i = 0; i < NA; ++i) k[i] += J.a[i];
+            }
+            if (st < 5) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) kact[st][r][c] = k[ACTIVE[r]];
+            }
+            if (st != 1) {
+#pragma unroll
+                for (int i = 0; i < NA; ++i) acc[i][c] += b[st] * k[i];
+            }
+            if (c == 0) dep = k[0] + k[5];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) out(i, C0 + c, h * acc[i][c]);
+}
+
+// The exact FP64 Dormand-Prince attempt (the primal's own dp5_step, so the
+// trajectory is the traced one bit for bit) handing its stage states to sink.
+template <class Sink>
+struct StageStepper {
+    Sink sink;
+    KR_HD void operator()(const Params& P, const RayState<double>& R, double h_eff, double (&y_new)[NA],
+                          double (&err)[NA], double (&k6)[NA], double& s_new, double& c_new) const {
+        dp5_step(P, R.y, R.p_t, R.p_ph, h_eff, R.f, y_new, err, k6, s_new, c_new, sink);
+    }
+};
+
+// One step of the cotangent recurrence with the step's increment D (entry
+// e = i * JC + c from d(e)).
+template <class In>
+KR_HD void increment_vjp(const In& d, double (&ybar)[NA], double& ptbar, double& pphbar, double& abar) {
+    double v[NA];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) v[i] = ybar[i];
+    double s[JC];
+#pragma unroll
+    for (int c = 0; c < JC; ++c) {
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < NA; ++i) acc += d(i * JC + c) * v[i];
+        s[c] = acc;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ybar[ACTIVE[r]] += s[r];
+    ptbar += s[4];
+    pphbar += s[5];
+    abar += s[6];
+}
+
 // Bardeen image-plane state (benchmarks/rayset.py:158-165 with E = 1,
 // inward) as a function of (alpha, beta, a, theta_o).  p_theta is written
 // as -beta: with Theta(theta_o) = beta^2 and sign_theta = -1 for beta >= 0,
@@ -323,6 +531,8 @@ KR_HD void bardeen_state_of(const Dual<N>& alpha, const Dual<N>& beta, const Dua
     y[7] = xi;
 }
 
+
+#pragma clang fp contract(off)
 
 // ---------------------------------------------------------------------------
 // Escape direction at infinity.  An escaping photon ends at some r_N >=
@@ -395,33 +605,51 @@ KR_HD void escape_tail(const S& r, const S& th0, const S& ph0, const S& pth0, co
     ph_inf = ph;
 }
 
+// Jacobian of the escape direction at infinity (row 0: theta_inf, row 1:
+// phi_inf) with respect to the final state (the NA integrated components:
+// t and p_r do not enter, phi enters additively) and p_t, p_phi, a
+// (columns NA .. NA + 2).
+KR_HD void escape_tail_jacobian(double m, double a, const double (&y)[NA], double pt, double pph,
+                                double (&J)[2][NA + 3]) {
+    using D = Dual<5>;
+    const D r = seed<5>(y[1], 0), th = seed<5>(y[2], 1), pth = seed<5>(y[5], 2), dpt = seed<5>(pt, 3),
+            dpph = seed<5>(pph, 4), da = constant<5>(a);
+    D out[2];
+    escape_tail(r, th, constant<5>(y[3]), pth, dpt, dpph, da, m, out[0], out[1]);
+    // d/da through the tail, by a separate one-direction pass.
+    using E = Dual<1>;
+    E out_a[2];
+    escape_tail(constant<1>(y[1]), constant<1>(y[2]), constant<1>(y[3]), constant<1>(y[5]), constant<1>(pt),
+                constant<1>(pph), seed<1>(a, 0), m, out_a[0], out_a[1]);
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+        J[o][0] = 0.0;
+        J[o][1] = out[o].d[0];
+        J[o][2] = out[o].d[1];
+        J[o][3] = o == 1 ? 1.0 : 0.0;
+        J[o][4] = 0.0;
+        J[o][5] = out[o].d[2];
+        J[o][6] = out[o].d[3];
+        J[o][7] = out[o].d[4];
+        J[o][8] = out_a[o].d[0];
+    }
+}
+
 // Cotangent of the final state for the cotangent (g_theta, g_phi) of the
 // escape direction at infinity: fills ybar (r, theta, phi, p_theta; p_r and
 // t do not enter) and the p_t, p_phi and a cotangents.
 KR_HD void escape_tail_vjp(double m, double a, const double (&y)[NA], double pt, double pph, double g_theta,
                            double g_phi, double (&ybar)[NA], double& ptbar, double& pphbar, double& abar) {
-    using D = Dual<5>;
-    const D r = seed<5>(y[1], 0), th = seed<5>(y[2], 1), pth = seed<5>(y[5], 2), dpt = seed<5>(pt, 3),
-            dpph = seed<5>(pph, 4), da = constant<5>(a);
-    D th_inf, ph_inf;
-    escape_tail(r, th, constant<5>(y[3]), pth, dpt, dpph, da, m, th_inf, ph_inf);
-    double t[5];
+    double J[2][NA + 3];
+    escape_tail_jacobian(m, a, y, pt, pph, J);
+    double v[NA + 3];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) t[k] = g_theta * th_inf.d[k] + g_phi * ph_inf.d[k];
-    ybar[0] = 0.0;
-    ybar[1] = t[0];
-    ybar[2] = t[1];
-    ybar[3] = g_phi;  // phi enters additively
-    ybar[4] = 0.0;
-    ybar[5] = t[2];
-    ptbar = t[3];
-    pphbar = t[4];
-    // d/da through the tail, by a separate one-direction pass.
-    using E = Dual<1>;
-    E th1, ph1;
-    escape_tail(constant<1>(y[1]), constant<1>(y[2]), constant<1>(y[3]), constant<1>(y[5]), constant<1>(pt),
-                constant<1>(pph), seed<1>(a, 0), m, th1, ph1);
-    abar = g_theta * th1.d[0] + g_phi * ph1.d[0];
+    for (int k = 0; k < NA + 3; ++k) v[k] = g_theta * J[0][k] + g_phi * J[1][k];
+#pragma unroll
+    for (int c = 0; c < NA; ++c) ybar[c] = v[c];
+    ptbar = v[NA];
+    pphbar = v[NA + 1];
+    abar = v[NA + 2];
 }
 
 }  // namespace ad

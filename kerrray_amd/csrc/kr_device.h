@@ -45,6 +45,18 @@ using std::sqrt;
 #define KR_HD __host__ __device__ __forceinline__
 #endif
 
+// KR_AFTER(dep, x0..x5): in device code, makes the six values x0..x5 depend
+// on dep (an empty asm statement that claims to read dep and rewrite them;
+// no instruction is emitted), so no work on x can start before dep is
+// computed.  Serialises independent computations that the compiler would
+// otherwise interleave, and the registers they hold.  No effect on the host.
+#if defined(KR_HOST_BUILD) || !defined(__HIP_DEVICE_COMPILE__)
+#define KR_AFTER(dep, x0, x1, x2, x3, x4, x5) ((void)0)
+#else
+#define KR_AFTER(dep, x0, x1, x2, x3, x4, x5) \
+    asm volatile("" : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5) : "v"(dep))
+#endif
+
 namespace kr {
 
 // Flag bits a kernel ORs into the caller's flag word (the KR_FLAG_* values of
@@ -234,31 +246,45 @@ KR_HD bool all_finite(const T (&a)[NA], const T (&b)[NA]) {
     return ok;
 }
 
+// Receives the stage states of a Dormand-Prince attempt (none by default).
+struct NoStages {
+    template <typename T>
+    KR_HD void operator()(int, const T (&)[NA]) const {}
+};
+
 // numba_kernel.py:201-238.  k0 = f(y) on entry (FSAL); on exit y_new, err and
-// k6 = f(y_new), plus sin/cos of theta at y_new.
-template <typename T>
+// k6 = f(y_new), plus sin/cos of theta at y_new.  `stages(s, ys)` sees the
+// state at which stage s's slope is evaluated (s = 0..5; stage 0's is y).
+template <typename T, class Stages = NoStages>
 KR_HD void dp5_step(const Params& P, const T (&y)[NA], T p_t, T p_ph, T h, const T (&k0)[NA],
-                                         T (&y_new)[NA], T (&err)[NA], T (&k6)[NA], double& s_new, double& c_new) {
+                                         T (&y_new)[NA], T (&err)[NA], T (&k6)[NA], double& s_new, double& c_new,
+                                         const Stages& stages = Stages{}) {
     using namespace dp;
     T k1[NA], k2[NA], k3[NA], k4[NA], k5[NA], ys[NA];
     double s, c;
     const T pt = p_t + T(0), pph = p_ph + T(0);
+    stages(0, y);
 #pragma unroll
     for (int i = 0; i < NA; ++i) ys[i] = y[i] + h * (T(a10) * k0[i]);
+    stages(1, ys);
     rhs(P, ys, pt, pph, k1, s, c);
 #pragma unroll
     for (int i = 0; i < NA; ++i) ys[i] = y[i] + h * (T(a20) * k0[i] + T(a21) * k1[i]);
+    stages(2, ys);
     rhs(P, ys, pt, pph, k2, s, c);
 #pragma unroll
     for (int i = 0; i < NA; ++i) ys[i] = y[i] + h * (T(a30) * k0[i] + T(a31) * k1[i] + T(a32) * k2[i]);
+    stages(3, ys);
     rhs(P, ys, pt, pph, k3, s, c);
 #pragma unroll
     for (int i = 0; i < NA; ++i)
         ys[i] = y[i] + h * (T(a40) * k0[i] + T(a41) * k1[i] + T(a42) * k2[i] + T(a43) * k3[i]);
+    stages(4, ys);
     rhs(P, ys, pt, pph, k4, s, c);
 #pragma unroll
     for (int i = 0; i < NA; ++i)
         ys[i] = y[i] + h * (T(a50) * k0[i] + T(a51) * k1[i] + T(a52) * k2[i] + T(a53) * k3[i] + T(a54) * k4[i]);
+    stages(5, ys);
     rhs(P, ys, pt, pph, k5, s, c);
     T esum[NA];
 #pragma unroll

@@ -183,10 +183,20 @@ extern "C" int krh_trace_image_mixed(const kr_params* p, const kr_image* img, do
 
 // The discrete adjoint on the CPU, with the kernels' algorithm: trace,
 // record accepted steps and checkpoints every `segment` steps, then per
-// photon sweep the segments backwards, recomputing each from its checkpoint.
+// photon sweep the segments backwards, recomputing each from its checkpoint;
+// segment 0: record every accepted step's stage states and chain the step
+// Jacobians built from them.
 // theta/phi = escape direction at infinity (NaN unless escaped);
 // grad[4 i + j] = g_theta[i] d theta_inf/dx_j + g_phi[i] d phi_inf/dx_j,
 // x = (alpha, beta, spin, theta_o); 0 for photons that did not escape.
+// Copies the stage states of one attempt into a 6 x NA buffer.
+struct HostStages {
+    double* buf;
+    void operator()(int s, const double (&ys)[kr::NA]) const {
+        for (int c = 0; c < kr::NA; ++c) buf[s * kr::NA + c] = ys[c];
+    }
+};
+
 extern "C" int krh_escape_vjp(const kr_params* p, const kr_image* img, const double* alpha, const double* beta,
                               int64_t n, const double* g_theta, const double* g_phi, int segment, int64_t* state,
                               double* theta, double* phi, double* grad, int nt) {
@@ -212,7 +222,7 @@ extern "C" int krh_escape_vjp(const kr_params* p, const kr_image* img, const dou
             kr::ray_attempt<double, true, false>(P, R);
             if (R.n_steps != k) {
                 hs.push_back(h_eff);
-                if (R.n_steps % segment == 0 && R.state == kr::RUNNING)
+                if (segment > 0 && R.n_steps % segment == 0 && R.state == kr::RUNNING)
                     for (int c = 0; c < kr::NA; ++c) ck.push_back(R.y[c]);
             }
         }
@@ -226,9 +236,35 @@ extern "C" int krh_escape_vjp(const kr_params* p, const kr_image* img, const dou
         const double pt = y0[4], pph = y0[7];
         kr::ad::escape_tail(R.y[1], R.y[2], R.y[3], R.y[5], pt, pph, a, mass, theta[i], phi[i]);
         const int64_t N = static_cast<int64_t>(hs.size());
-        const int64_t nseg = (N + segment - 1) / segment;
         double ybar[kr::NA], ptbar, pphbar, abar;
         kr::ad::escape_tail_vjp(mass, a, R.y, pt, pph, g_theta[i], g_phi[i], ybar, ptbar, pphbar, abar);
+        if (segment == 0) {
+            // Step Jacobians from the recorded stage states, then the
+            // transposed chain.
+            std::vector<double> stg;
+            kr::RayState<double> Q;
+            kr::ray_init<double, true>(P, y0, Q);
+            double cur[6 * kr::NA];
+            const kr::ad::StageStepper<HostStages> rec{HostStages{cur}};
+            while (Q.state == kr::RUNNING) {
+                const int64_t k = Q.n_steps;
+                kr::ray_attempt_with<double, true, false>(P, Q, rec);
+                if (Q.n_steps != k) stg.insert(stg.end(), cur, cur + 6 * kr::NA);
+            }
+            for (int c = 0; c < kr::NA; ++c)
+                if (!(Q.y[c] == R.y[c])) bad |= 2;
+            for (int64_t k = N - 1; k >= 0; --k) {
+                double D[kr::ad::JW];
+                const double* sk = &stg[static_cast<size_t>(k) * 6 * kr::NA];
+                auto stage = [&](int s, double (&Y)[kr::NA]) {
+                    for (int c = 0; c < kr::NA; ++c) Y[c] = sk[s * kr::NA + c];
+                };
+                auto out = [&](int r, int c, double v) { D[r * kr::ad::JC + c] = v; };
+                kr::ad::step_increment<0, kr::ad::JC>(mass, a, pt, pph, hs[k], stage, out);
+                kr::ad::increment_vjp([&](int e) { return D[e]; }, ybar, ptbar, pphbar, abar);
+            }
+        }
+        const int64_t nseg = segment == 0 ? 0 : (N + segment - 1) / segment;
         std::vector<double> seg(static_cast<size_t>(segment) * kr::NA);
         for (int64_t s = nseg - 1; s >= 0; --s) {
             double y[kr::NA];
@@ -297,6 +333,24 @@ extern "C" void krh_step_adjoint(const kr_params* p, double m, double a, const d
     out[6] = ptb;
     out[7] = pphb;
     out[8] = ab;
+}
+
+// One step's increment D (NA x 7, row-major; columns r, theta, p_r, p_theta,
+// p_t, p_phi, a) from the state Y, through the step's own stage states.
+extern "C" void krh_step_increment(const kr_params* p, double m, double a, const double* Y, double pt, double pph,
+                                   double h, double* out) {
+    const kr::Params P = params_of(p);
+    double y[kr::NA], k0[kr::NA], y_new[kr::NA], err[kr::NA], k6[kr::NA], s, c;
+    for (int i = 0; i < kr::NA; ++i) y[i] = Y[i];
+    kr::rhs(P, y, pt, pph, k0, s, c);
+    double stg[6 * kr::NA];
+    kr::dp5_step(P, y, pt, pph, h, k0, y_new, err, k6, s, c, HostStages{stg});
+    auto stage = [&](int st, double (&Ys)[kr::NA]) {
+        for (int i = 0; i < kr::NA; ++i) Ys[i] = stg[st * kr::NA + i];
+    };
+    auto put = [&](int r, int col, double v) { out[r * kr::ad::JC + col] = v; };
+    kr::ad::step_increment<0, 4>(m, a, pt, pph, h, stage, put);
+    kr::ad::step_increment<4, 3>(m, a, pt, pph, h, stage, put);
 }
 
 // div_by (kr_device.h) against the IEEE quotient on n operand pairs:

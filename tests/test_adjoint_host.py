@@ -51,6 +51,9 @@ def hostref():
     L.krh_step_adjoint.restype = None
     L.krh_step_adjoint.argtypes = [vp, ctypes.c_double, ctypes.c_double, vp, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_double, vp, vp]
+    L.krh_step_increment.restype = None
+    L.krh_step_increment.argtypes = [vp, ctypes.c_double, ctypes.c_double, vp, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, vp]
     return L
 
 
@@ -163,12 +166,46 @@ def test_step_adjoint_satisfies_the_dot_product_identity(hostref) -> None:
             assert abs(lhs - rhs) <= 1e-6 * (abs(lhs) + np.linalg.norm(out) * np.linalg.norm(d) * 1e-3), (lhs, rhs)
 
 
+def test_step_increment_is_the_transposed_step_adjoint(hostref) -> None:
+    """The step Jacobian (identity plus the increment D over the columns r, theta, p_r, p_theta, p_t, p_phi, a;
+    the t and phi columns are the identity's) contracted with a cotangent equals the reverse sweep of the step."""
+    from kerrray_amd.backends import kernel_params
+    from kerrray_amd.geometry import Spacetime
+
+    integ, term = options(rtol=1e-8, atol=1e-10)
+    rng = np.random.default_rng(5)
+    active = [1, 2, 4, 5]
+    for spin in (0.0, 0.3, 0.9):
+        for trial in range(12):
+            r = rng.uniform(3, 40)
+            Y = np.array([rng.uniform(0, 50), r, rng.uniform(0.3, 2.8), rng.uniform(-3, 3), rng.uniform(-1, 1),
+                          rng.uniform(-4, 4)])
+            pt, pph, h = -1.0, (0.0 if trial == 0 else rng.uniform(-4, 4)), 0.03 * r
+            p = kernel_params(Spacetime(1.0, spin), integ, term)
+            D = np.empty(42)
+            hostref.krh_step_increment(ctypes.addressof(p), 1.0, spin, _p(Y), pt, pph, h, _p(D))
+            D = D.reshape(6, 7)
+            ybar_new = rng.normal(size=6)
+            out = np.empty(9)
+            hostref.krh_step_adjoint(ctypes.addressof(p), 1.0, spin, _p(Y), pt, pph, h, _p(ybar_new.copy()),
+                                     _p(out))
+            via_d = np.zeros(9)
+            via_d[:6] = ybar_new
+            s = ybar_new @ D
+            via_d[active] += s[:4]
+            via_d[6:] = s[4:]
+            scale = np.linalg.norm(ybar_new) * (1.0 + np.abs(D).max())
+            assert np.max(np.abs(via_d - out)) <= 1e-12 * scale, (spin, trial, via_d - out)
+
+
+@pytest.mark.parametrize("segment", [16, 0])
 @pytest.mark.parametrize("spin", [0.5, 0.9])
-def test_adjoint_equals_forward_variational_integration(hostref, oracle, spin) -> None:
+def test_adjoint_equals_forward_variational_integration(hostref, oracle, spin, segment) -> None:
+    """segment 16: checkpointed reverse sweep; segment 0: chained step Jacobians from recorded stage states."""
     integ, term = options(rtol=1e-8, atol=1e-10)
     rng = np.random.default_rng(2)
     alpha, beta = points(rng, 48, spin, 60.0, near_edge=16)
-    state, th, ph, J = host_jacobian(hostref, spin, 60.0, alpha, beta, integ, term)
+    state, th, ph, J = host_jacobian(hostref, spin, 60.0, alpha, beta, integ, term, segment=segment)
     V = oracle.escape_variational(type("S", (), {"mass": 1.0, "spin": spin})(), math.radians(60.0), 1000.0,
                                   alpha, beta, integ, term)
     assert np.array_equal(state, V.state)
