@@ -11,10 +11,17 @@
 // busy.  Per-photon arithmetic is that of the plain kernel (kr_device.h), so
 // every image's results equal the plain per-image kernel's bit for bit.
 //
-// The job table (integration constants and image scalars of every image)
-// is copied into LDS at kernel start; a lane reads its own job's constants
-// from there.  A batch of one image uses the uniform-constant variant
-// (constants in kernel arguments, scalar registers).
+// Each image's integration constants (and the first photon of each image, for
+// the lookup) are copied into LDS at kernel start; a lane reads its own
+// job's constants from there at every attempt, and the image scalars from the
+// job table in global memory when it starts a photon.  A batch of one image
+// uses the uniform-constant variant (constants in kernel arguments, scalar
+// registers).
+//
+// At two wavefronts per SIMD, when the device's LDS allows, the stage slopes
+// k1..k4 of each lane's Dormand-Prince attempt live in LDS too (and the
+// error sums are parked there during the last right-hand side), as in the
+// planned kernel: 192 B per lane, which takes the attempt out of scratch.
 //
 // Photons are handed out in a processing order of the images that need not
 // be the job order: with kr_tune.order = 1 the images whose photons take the
@@ -48,11 +55,11 @@ constexpr int MAX_LDS_JOBS = 128;
 template <typename T>
 using BatchOut = RowsOut<T>;
 
-__device__ __forceinline__ int find_job(const JobDev* jobs, int n_jobs, int64_t idx) {
+__device__ __forceinline__ int find_job(const int64_t* firsts, int n_jobs, int64_t idx) {
     int lo = 0, hi = n_jobs - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].first <= idx)
+        if (firsts[mid] <= idx)
             lo = mid;
         else
             hi = mid - 1;
@@ -69,17 +76,32 @@ struct BatchArgs {
     int32_t* flags;
 };
 
-template <typename T, bool UNIFORM, int BLOCK, int WAVES>
+// Dynamic LDS of batch_kernel, in 8-byte words: the stage slopes (4 NA BLOCK
+// doubles, LDS_SLOPES only), then the first photon of every job (n_jobs),
+// then every job's Params.
+constexpr size_t slope_bytes(int block) { return static_cast<size_t>(4 * NA) * block * sizeof(double); }
+
+size_t batch_lds_bytes(bool uniform, bool lds_slopes, int block, int n_jobs) {
+    return (lds_slopes ? slope_bytes(block) : 0) +
+           (uniform ? 0 : static_cast<size_t>(n_jobs) * (sizeof(int64_t) + sizeof(Params)));
+}
+
+template <typename T, bool UNIFORM, int BLOCK, int WAVES, bool LDS_SLOPES>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void batch_kernel(
     BatchArgs a, JobDev uniform_job, BatchOut<T> out) {
-    extern __shared__ JobDev lds_jobs[];
+    static_assert(sizeof(Params) % sizeof(uint64_t) == 0, "Params is copied in 8-byte words");
+    extern __shared__ double lds[];
+    int64_t* firsts = reinterpret_cast<int64_t*>(lds + (LDS_SLOPES ? 4 * NA * BLOCK : 0));
+    Params* params = reinterpret_cast<Params*>(firsts + (UNIFORM ? 0 : a.n_jobs));
     if (!UNIFORM) {
-        const int words = static_cast<int>(a.n_jobs * sizeof(JobDev) / sizeof(uint64_t));
-        const uint64_t* src = reinterpret_cast<const uint64_t*>(a.jobs);
-        uint64_t* dst = reinterpret_cast<uint64_t*>(lds_jobs);
-        for (int w = threadIdx.x; w < words; w += BLOCK) dst[w] = src[w];
+        constexpr int PW = static_cast<int>(sizeof(Params) / sizeof(uint64_t));
+        for (int k = threadIdx.x; k < a.n_jobs; k += BLOCK) firsts[k] = a.jobs[k].first;
+        uint64_t* dst = reinterpret_cast<uint64_t*>(params);
+        for (int w = threadIdx.x; w < a.n_jobs * PW; w += BLOCK)
+            dst[w] = reinterpret_cast<const uint64_t*>(&a.jobs[w / PW].P)[w % PW];
         __syncthreads();
     }
+    const SlopeStore<T> K{reinterpret_cast<T*>(lds) + threadIdx.x, BLOCK};
     const int lane = threadIdx.x & 63;
     RayState<T> R;
     R.state = RUNNING;
@@ -106,8 +128,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
                     const int64_t mine = static_cast<int64_t>(base) + __popcll(below);
                     if (mine < a.total) {
                         idx = mine;
-                        job = UNIFORM ? 0 : find_job(lds_jobs, a.n_jobs, idx);
-                        const JobDev& J = UNIFORM ? uniform_job : lds_jobs[job];
+                        job = UNIFORM ? 0 : find_job(firsts, a.n_jobs, idx);
+                        const JobDev& J = UNIFORM ? uniform_job : a.jobs[job];
                         const int64_t local = idx - J.first;
                         const int64_t i = local / J.B.n, j = local - i * J.B.n;
                         idx = J.out_first + local;  // from here on: the output row
@@ -115,7 +137,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
                         const int32_t bad = image_state(J.B, pixel_centre(J.B.fov, j, J.B.n),
                                                         -pixel_centre(J.B.fov, i, J.B.n), y0);
                         if (bad && a.flags) atomicOr(a.flags, bad);
-                        ray_init<T, true>(J.P, y0, R);
+                        ray_init<T, true>(UNIFORM ? uniform_job.P : params[job], y0, R);
                         if (R.state == RUNNING)
                             busy = true;
                         else
@@ -127,8 +149,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
         }
         if (!any_busy) break;  // exhausted and nothing in flight
         if (busy) {
-            const Params& P = UNIFORM ? uniform_job.P : lds_jobs[job].P;
-            ray_attempt<T, true, false>(P, R);
+            const Params& P = UNIFORM ? uniform_job.P : params[job];
+            if (LDS_SLOPES)
+                ray_attempt_with<T, true, false>(P, R, StoredStepper<T, true, true>{K});
+            else
+                ray_attempt<T, true, false>(P, R);
             if (R.state != RUNNING) {
                 store_row(out, idx, R);
                 busy = false;
@@ -147,11 +172,17 @@ __global__ void batch_reset_kernel(unsigned long long* counter, int32_t* flags) 
 
 using BatchFn = void (*)(BatchArgs, JobDev, BatchOut<double>);
 
+template <bool U, int BLOCK>
+BatchFn pick_waves(int waves, bool lds_slopes) {
+    if (waves >= 3) return batch_kernel<double, U, BLOCK, 3, false>;
+    return lds_slopes ? batch_kernel<double, U, BLOCK, 2, true> : batch_kernel<double, U, BLOCK, 2, false>;
+}
+
 template <bool U>
-BatchFn pick(int block, int waves) {
-    if (block == 64) return waves >= 3 ? batch_kernel<double, U, 64, 3> : batch_kernel<double, U, 64, 2>;
-    if (block == 128) return waves >= 3 ? batch_kernel<double, U, 128, 3> : batch_kernel<double, U, 128, 2>;
-    return waves >= 3 ? batch_kernel<double, U, 256, 3> : batch_kernel<double, U, 256, 2>;
+BatchFn pick(int block, int waves, bool lds_slopes) {
+    if (block == 64) return pick_waves<U, 64>(waves, lds_slopes);
+    if (block == 128) return pick_waves<U, 128>(waves, lds_slopes);
+    return pick_waves<U, 256>(waves, lds_slopes);
 }
 
 }  // namespace kr
@@ -169,6 +200,8 @@ struct kr_batch_s {
     kr_tune tune{};
     kr_outputs out{};
     int cus = 0;
+    bool lds_slopes = false;  // stage slopes in LDS (two wavefronts per SIMD, where LDS allows)
+    size_t lds_bytes = 0;     // dynamic LDS per block
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -232,9 +265,9 @@ hipError_t enqueue(kr_batch_s* b, hipStream_t s) {
                    b->d_flags};
     BatchOut<double> o{static_cast<double*>(b->out.Y), b->out.lam, b->out.n_steps, b->out.n_rejected, b->out.state,
                        b->out.max_null_error, b->out.max_energy_drift, b->out.max_lz_drift, b->out.max_carter_drift};
-    const size_t lds = uniform ? 0 : b->jobs.size() * sizeof(JobDev);
-    BatchFn fn = uniform ? pick<true>(block, b->tune.waves_per_simd) : pick<false>(block, b->tune.waves_per_simd);
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(block), lds, s, args, b->jobs[0], o);
+    BatchFn fn = uniform ? pick<true>(block, b->tune.waves_per_simd, b->lds_slopes)
+                         : pick<false>(block, b->tune.waves_per_simd, b->lds_slopes);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(block), b->lds_bytes, s, args, b->jobs[0], o);
     return hipGetLastError();
 }
 
@@ -270,8 +303,30 @@ extern "C" kr_status kr_batch_create(kr_handle h, const kr_job* jobs, int32_t n_
         b->d_flags = reinterpret_cast<int32_t*>(b->d_counter + 32);
         e = hipMemcpy(b->d_jobs, b->jobs.data(), table, hipMemcpyHostToDevice);
     }
+    // Stage slopes in LDS at two wavefronts per SIMD when a block's share
+    // fits the device's per-block and per-CU LDS.
+    int lds_block = 0, lds_cu = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device);
+    if (e == hipSuccess)
+        e = hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, h->device);
+    if (e == hipSuccess) {
+        const bool uniform = b->jobs.size() == 1;
+        const int block = tune->block_size;
+        const size_t blocks_per_cu = static_cast<size_t>((4 * tune->waves_per_simd * 64 + block - 1) / block);
+        const size_t with = batch_lds_bytes(uniform, true, block, n_jobs);
+        b->lds_slopes = tune->waves_per_simd == 2 && with <= static_cast<size_t>(lds_block) &&
+                        with * blocks_per_cu <= static_cast<size_t>(lds_cu);
+        if (b->lds_slopes && with > 65536) {
+            BatchFn fn = uniform ? pick<true>(block, 2, true) : pick<false>(block, 2, true);
+            b->lds_slopes = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                static_cast<int>(with)) == hipSuccess;
+            (void)hipGetLastError();
+        }
+        b->lds_bytes = batch_lds_bytes(uniform, b->lds_slopes, block, n_jobs);
+    }
     if (e != hipSuccess) {
-        if (b->dmem) hipFree(b->dmem);
+        if (b->dmem) (void)hipFree(b->dmem);
         delete b;
         return host::fail(KR_ERR_DEVICE, std::string("kr_batch_create: ") + hipGetErrorString(e));
     }
@@ -305,7 +360,7 @@ extern "C" kr_status kr_batch_graph_launch(kr_batch b, void* stream) {
             if (e2 != hipSuccess) e = e2;
         }
         if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
-        hipStreamDestroy(cap);
+        (void)hipStreamDestroy(cap);
         if (e != hipSuccess) return host::fail(KR_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
     }
     KR_HIP(hipGraphLaunch(b->exec, s));
@@ -323,11 +378,11 @@ extern "C" kr_status kr_batch_flags(kr_batch b, int32_t* flags) {
 extern "C" kr_status kr_batch_destroy(kr_batch b) {
     host::clear_error();
     if (!b) return KR_OK;
-    hipSetDevice(b->h->device);
-    hipDeviceSynchronize();
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
-    if (b->dmem) hipFree(b->dmem);
+    (void)hipSetDevice(b->h->device);
+    (void)hipDeviceSynchronize();
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->dmem) (void)hipFree(b->dmem);
     delete b;
     return KR_OK;
 }
