@@ -4,7 +4,9 @@ Polygon radii and centroid, the sub-pixel edge of a captured mask and the
 comparison with an analytic curve, as in src/kerrray/raytracing/boundary.py:74-307:
 
 * ``polygon_radii``: exact ray/segment intersection, smallest t > 0 with
-  0 <= s <= 1 (widened by 1e-9 so a ray through a vertex is not lost);
+  0 <= s <= 1 (widened by 1e-9 so a ray through a vertex is not lost),
+  searching only the segment a ray points into when the polygon is
+  star-shaped about the centre;
 * ``polygon_centroid``: shoelace area centroid;
 * ``extract_boundary_from_mask``: bilinear 0.5 level set of the mask (zero
   outside the image) marched along polar rays in steps of 1/8 pixel, on the
@@ -33,8 +35,40 @@ STEP_FRACTION = 0.125
 EDGE_TOLERANCE = 1e-9
 
 
+def _ray_segment_t(qx: Float, qy: Float, ex: Float, ey: Float, dx: Float, dy: Float) -> Float:
+    """Ray parameter t of the intersection of rays (dx, dy) from the origin with segments q + s e (inf where the
+    intersection has t <= 0 or s outside [0, 1] widened by EDGE_TOLERANCE); broadcasts."""
+    det = dx * (-ey) - dy * (-ex)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t = (qx * (-ey) - qy * (-ex)) / det
+        s = (dx * qy - dy * qx) / det
+    valid = np.isfinite(t) & (t > 0.0) & (s >= -EDGE_TOLERANCE) & (s <= 1.0 + EDGE_TOLERANCE)
+    return np.where(valid, t, np.inf)
+
+
+def _star_candidates(qx: Float, qy: Float, phi: Float) -> NDArray[np.int64] | None:
+    """For a polygon whose vertices turn monotonically once around the origin (star-shaped about it), the
+    segment each ray direction phi points into and its two neighbours, (len(phi), 3); None otherwise."""
+    n = qx.size
+    va = np.arctan2(qy, qx)
+    steps = np.diff(np.unwrap(np.append(va, va[0])))
+    turn = steps.sum()
+    if np.all(steps > 0.0) and abs(turn - 2.0 * np.pi) < 1e-9:
+        sign = 1.0
+    elif np.all(steps < 0.0) and abs(turn + 2.0 * np.pi) < 1e-9:
+        sign = -1.0
+    else:
+        return None
+    u = sign * np.concatenate([[0.0], np.cumsum(steps)])  # increasing from 0 to 2 pi
+    x = np.mod(sign * (phi - va[0]), 2.0 * np.pi)
+    j = np.clip(np.searchsorted(u, x, side="right") - 1, 0, n - 1)
+    return np.stack([(j - 1) % n, j, (j + 1) % n], axis=1)
+
+
 def polygon_radii(alpha: ArrayLike, beta: ArrayLike, centre: tuple[float, float], angles: ArrayLike) -> Float:
-    """Distance from ``centre`` to the closed polygon along each angle."""
+    """Distance from ``centre`` to the closed polygon along each angle (the smallest valid ray parameter over
+    all segments).  For a polygon star-shaped about the centre, only the segment a ray points into and its
+    neighbours can hold it, and only those are evaluated (same arithmetic, same result)."""
     px = np.asarray(alpha, dtype=np.float64).ravel()
     py = np.asarray(beta, dtype=np.float64).ravel()
     if px.size < 3 or px.shape != py.shape:
@@ -43,12 +77,11 @@ def polygon_radii(alpha: ArrayLike, beta: ArrayLike, centre: tuple[float, float]
     qx, qy = px - float(centre[0]), py - float(centre[1])
     ex, ey = np.roll(qx, -1) - qx, np.roll(qy, -1) - qy
     dx, dy = np.cos(phi)[:, None], np.sin(phi)[:, None]
-    det = dx * (-ey[None, :]) - dy * (-ex[None, :])
-    with np.errstate(divide="ignore", invalid="ignore"):
-        t = (qx[None, :] * (-ey[None, :]) - qy[None, :] * (-ex[None, :])) / det
-        s = (dx * qy[None, :] - dy * qx[None, :]) / det
-    valid = np.isfinite(t) & (t > 0.0) & (s >= -EDGE_TOLERANCE) & (s <= 1.0 + EDGE_TOLERANCE)
-    radii = np.where(valid, t, np.inf).min(axis=1)
+    cand = _star_candidates(qx, qy, phi)
+    if cand is not None:
+        radii = _ray_segment_t(qx[cand], qy[cand], ex[cand], ey[cand], dx, dy).min(axis=1)
+    else:
+        radii = _ray_segment_t(qx[None, :], qy[None, :], ex[None, :], ey[None, :], dx, dy).min(axis=1)
     if not np.all(np.isfinite(radii)):
         raise ValueError("a polar ray from the centre does not intersect the curve")
     return radii

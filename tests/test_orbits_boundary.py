@@ -66,6 +66,41 @@ def test_polygon_radii_of_a_circle() -> None:
     assert abs(cx - 2.0) < 1e-12 and abs(cy + 1.0) < 1e-12
 
 
+def _brute_radii(x, y, centre, angles):
+    """Smallest valid ray parameter over every segment: the definition polygon_radii evaluates."""
+    qx, qy = x - centre[0], y - centre[1]
+    ex, ey = np.roll(qx, -1) - qx, np.roll(qy, -1) - qy
+    dx, dy = np.cos(angles)[:, None], np.sin(angles)[:, None]
+    det = dx * (-ey[None, :]) - dy * (-ex[None, :])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t = (qx[None, :] * (-ey[None, :]) - qy[None, :] * (-ex[None, :])) / det
+        s = (dx * qy[None, :] - dy * qx[None, :]) / det
+    ok = np.isfinite(t) & (t > 0) & (s >= -1e-9) & (s <= 1 + 1e-9)
+    return np.where(ok, t, np.inf).min(axis=1)
+
+
+@pytest.mark.parametrize("spin,incl", [(0.0, 60.0), (0.5, 17.0), (0.99, 85.0), (0.999, 89.0)])
+def test_polygon_radii_star_search_equals_every_segment(spin, incl) -> None:
+    """The search over the segment a ray points into (the Bardeen curve is star-shaped about its centroid)
+    returns the brute-force minimum bit for bit, rays through vertices and both orientations included."""
+    x, y = shadow_curve(Spacetime(1.0, spin), incl, 3000)
+    c = polygon_centroid(x, y)
+    rng = np.random.default_rng(3)
+    ang = np.concatenate([2 * np.pi * np.arange(720) / 720, rng.uniform(-7, 7, 300),
+                          np.arctan2(y - c[1], x - c[0])[::29]])
+    assert np.array_equal(polygon_radii(x, y, c, ang), _brute_radii(x, y, c, ang))
+    assert np.array_equal(polygon_radii(x[::-1], y[::-1], c, ang), _brute_radii(x[::-1], y[::-1], c, ang))
+
+
+def test_polygon_radii_of_a_non_star_polygon_takes_the_nearest_crossing() -> None:
+    """A polygon that folds back (not star-shaped about the centre): every segment is searched."""
+    th = np.linspace(0, 2 * np.pi, 400, endpoint=False)
+    r = 2.0 + 1.5 * np.sin(5 * th)
+    x, y = r * np.cos(th + 0.6 * np.sin(3 * th)), r * np.sin(th + 0.6 * np.sin(3 * th))
+    ang = np.linspace(0, 2 * np.pi, 97)
+    assert np.array_equal(polygon_radii(x, y, (0.0, 0.0), ang), _brute_radii(x, y, (0.0, 0.0), ang))
+
+
 def test_oracle_mask_boundary_of_a_synthetic_disk(oracle) -> None:
     cam = BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=10.0, resolution=128)
     alpha, beta = cam.pixel_coordinates()
