@@ -501,6 +501,8 @@ def main() -> int:
     ap.add_argument("--form", choices=["auto", "fp64_plain", "fp64_planned", "fp32_fp64_repair"], default="auto")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 256 MiB device-to-device once (the reference kernel for the HBM counters)")
+    ap.add_argument("--batch-only", metavar="BLOCK,WAVES,REFILL",
+                    help="only run the 64-image ragged batch once at this shape (no autotune; for profilers)")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -529,6 +531,17 @@ def main() -> int:
     torch.cuda.init()
     log(f"device {torch.cuda.get_device_name(0)}")
     hip = HipBackend(0)
+    if args.batch_only:
+        from kerrray_amd import _lib
+        from kerrray_amd.batch import DeviceBatch, ragged_workload
+
+        shape = [int(v) for v in args.batch_only.split(",")]
+        batch = DeviceBatch(hip, ragged_workload(0, 64), _lib.KrTune(*shape, 1))
+        ms = batch.timed(repeats=1, graph=False)
+        batch.check_flags()
+        print(json.dumps({"batch_only": shape, "photons": batch.n, "kernel_ms": ms}), flush=True)
+        batch.close()
+        return 0
     st = Spacetime(1.0, SPIN)
     integ = IntegratorOptions(method="rk45", rtol=RTOL, atol=ATOL)
     term = TerminationOptions()
