@@ -159,14 +159,28 @@ KR_HD double div_by(double a, double b, double y) {
     return copysign(q2, q0);
 }
 
+// sin(x) and cos(x).  On the device one sincos call: the device library
+// derives both from one argument reduction and one polynomial pair, so the
+// results are bit for bit those of separate sin and cos calls, at about half
+// the instructions (separate calls are not merged by the compiler).  The host
+// build keeps the C library's separate calls.
+KR_HD void sin_cos(double x, double& s, double& c) {
+#if defined(KR_HOST_BUILD) || !defined(__HIP_DEVICE_COMPILE__)
+    s = sin(x);
+    c = cos(x);
+#else
+    sincos(x, &s, &c);
+#endif
+}
+
 // numba_kernel.py:111-154 (= geometry/metric.py:166-195,265-313). Also
 // returns sin(th), cos(th) for reuse by the classifier and diagnostics.
 KR_HD void inverse_metric_terms(const Params& P, double r, double th, double (&out)[15],
                                                      double& s_out, double& c_out) {
     const double m = P.geom[G_MASS];
     const double a2 = P.geom[G_A2];
-    const double s = sin(th);
-    const double c = cos(th);
+    double s, c;
+    sin_cos(th, s, c);
     const double s2 = s * s;
     const double sin2th = sin(2.0 * th);
     const double r2 = r * r;
