@@ -145,7 +145,8 @@ KR_HD Dual<N> operator/(double c, const Dual<N>& y) {
 template <int N>
 KR_HD Dual<N> dsin(const Dual<N>& x) {
     Dual<N> z;
-    const double s = sin(x.v), c = cos(x.v);
+    double s, c;
+    sin_cos(x.v, s, c);
     z.v = s;
 #pragma unroll
     for (int i = 0; i < N; ++i) z.d[i] = c * x.d[i];
@@ -155,7 +156,8 @@ KR_HD Dual<N> dsin(const Dual<N>& x) {
 template <int N>
 KR_HD Dual<N> dcos(const Dual<N>& x) {
     Dual<N> z;
-    const double s = sin(x.v), c = cos(x.v);
+    double s, c;
+    sin_cos(x.v, s, c);
     z.v = c;
 #pragma unroll
     for (int i = 0; i < N; ++i) z.d[i] = -s * x.d[i];
@@ -177,12 +179,14 @@ KR_HD Dual<N> dsqrt(const Dual<N>& x) {
 KR_HD double dsin(double x) { return sin(x); }
 KR_HD double dcos(double x) { return cos(x); }
 KR_HD double dsqrt(double x) { return sqrt(x); }
+KR_HD void dsincos(double x, double& s, double& c) { sin_cos(x, s, c); }
 
 #pragma clang fp contract(fast)
 
 template <int N>
 KR_HD void dsincos(const Dual<N>& x, Dual<N>& s, Dual<N>& c) {
-    const double sv = sin(x.v), cv = cos(x.v);
+    double sv, cv;
+    sin_cos(x.v, sv, cv);
     s.v = sv;
     c.v = cv;
 #pragma unroll
@@ -512,7 +516,8 @@ template <int N>
 KR_HD void bardeen_state_of(const Dual<N>& alpha, const Dual<N>& beta, const Dual<N>& a, const Dual<N>& th_o,
                             double m, double r_o, Dual<N> (&y)[8]) {
     using S = Dual<N>;
-    const S sin_o = dsin(th_o), cos_o = dcos(th_o);
+    S sin_o, cos_o;
+    dsincos(th_o, sin_o, cos_o);
     const S xi = -(alpha * sin_o);
     const S cot = cos_o / sin_o;
     const S a2 = a * a;
@@ -556,7 +561,8 @@ constexpr int TAIL_STEPS = 16;
 template <class S>
 KR_HD void tail_rhs(const S& u, const S& th, const S& pth, const S& pt, const S& pph, const S& a, double m,
                     S& dth, S& dph, S& dpth) {
-    const S s = dsin(th), c = dcos(th);
+    S s, c;
+    dsincos(th, s, c);
     const S s2 = s * s, sc = s * c;
     const S u2 = u * u, a2 = a * a;
     const S a2u2 = a2 * u2;

@@ -225,8 +225,6 @@ __global__ __launch_bounds__(64) void tail_jacobian_kernel(double mass, double s
         for (int k = 0; k < NA + 3; ++k) cot[(o * (NA + 3) + k) * T.n_rec + t] = J[o][k];
 }
 
-constexpr int CHAIN_DEPTH = 3;  // step increments in flight per lane
-
 // d(theta_inf, phi_inf)/d(alpha, beta, spin, theta_o) of every recorded
 // photon: both rows of the tail Jacobian carried back through the step
 // increments (D_k^T, last step first) and then through the initial state.
@@ -248,33 +246,31 @@ __global__ __launch_bounds__(64) void photon_jacobian_kernel(BardeenImage B, dou
         pphbar[o] = c[(NA + 1) * T.n_rec];
         abar[o] = c[(NA + 2) * T.n_rec];
     }
-    // A ring of CHAIN_DEPTH increments in registers: each is reloaded with
-    // the step CHAIN_DEPTH further back as soon as it has been applied, so
-    // every load has CHAIN_DEPTH - 1 steps of arithmetic to arrive in.
+    // Double-buffered and branch-free: step k - 1's increment is loading
+    // while step k's is applied.  Every load is issued (at a clamped, valid
+    // step) and the update of a step past the end is masked, so the loop has
+    // no branch and its wait can leave the newer 42 loads in flight (the
+    // counter of outstanding loads holds at most 63).
     const double* d = T.D + (T.block_unit[blockIdx.x] * ad::JW) * 64 + lane;
     constexpr int64_t STEP = ad::JW * 64;
-    double ring[CHAIN_DEPTH][ad::JW];
     const int64_t n = T.steps[t];
+    double ring[2][ad::JW];
+    const int64_t k0 = n > 0 ? n - 1 : 0;
 #pragma unroll
-    for (int j = 0; j < CHAIN_DEPTH; ++j) {
-        if (n - 1 - j >= 0) {
+    for (int e = 0; e < ad::JW; ++e) ring[0][e] = d[k0 * STEP + e * 64];
+    for (int64_t k = n - 1; k >= 0; k -= 2) {
 #pragma unroll
-            for (int e = 0; e < ad::JW; ++e) ring[j][e] = d[(n - 1 - j) * STEP + e * 64];
-        }
-    }
-    for (int64_t k = n - 1; k >= 0; k -= CHAIN_DEPTH) {
-#pragma unroll
-        for (int j = 0; j < CHAIN_DEPTH; ++j) {
+        for (int j = 0; j < 2; ++j) {
             const int64_t kk = k - j;
-            if (kk >= 0) {
+            const int64_t next = kk - 1 > 0 ? kk - 1 : 0;
 #pragma unroll
-                for (int o = 0; o < 2; ++o)
-                    ad::increment_vjp([&](int e) { return ring[j][e]; }, ybar[o], ptbar[o], pphbar[o], abar[o]);
-                if (kk - CHAIN_DEPTH >= 0) {
+            for (int e = 0; e < ad::JW; ++e) ring[1 - j][e] = d[next * STEP + e * 64];
+            __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of this step's arithmetic
+            const bool live = kk >= 0;
 #pragma unroll
-                    for (int e = 0; e < ad::JW; ++e) ring[j][e] = d[(kk - CHAIN_DEPTH) * STEP + e * 64];
-                }
-            }
+            for (int o = 0; o < 2; ++o)
+                ad::increment_vjp([&](int e) { return live ? ring[j][e] : 0.0; }, ybar[o], ptbar[o], pphbar[o],
+                                  abar[o]);
         }
     }
 #pragma unroll
