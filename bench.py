@@ -68,7 +68,7 @@ def device_outputs(torch, n: int, dtype):
 
 
 def golden_mask() -> np.ndarray | None:
-    path = os.path.join(ROOT, "tests", "golden", "config2_a09_i60_512.npz")
+    path = os.path.join(ROOT, "tests", "golden", "bardeen512_a09_i60.npz")
     if not os.path.exists(path):
         return None
     with np.load(path, allow_pickle=False) as z:

@@ -35,12 +35,12 @@ def _load(name: str) -> dict:
 
 @pytest.fixture(scope="session")
 def golden1() -> dict:
-    return _load("config1_a0_i60_64.npz")
+    return _load("bardeen64_a0_i60.npz")
 
 
 @pytest.fixture(scope="session")
 def golden2() -> dict:
-    return _load("config2_a09_i60_512.npz")
+    return _load("bardeen512_a09_i60.npz")
 
 
 @pytest.fixture(scope="session")

@@ -111,7 +111,7 @@ def sha(a: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-def config1(workers: int) -> None:
+def schwarzschild64(workers: int) -> None:
     """Schwarzschild a=0, i=60 deg, r_o=1000, 64x64 on [-10,10]^2, rtol 1e-8 / atol 1e-10: every pixel."""
     spin, incl, r_o, fov, n, rtol, atol = 0.0, 60.0, 1000.0, 10.0, 64, 1e-8, 1e-10
     alpha, beta = pixel_grid(fov, n)
@@ -121,9 +121,9 @@ def config1(workers: int) -> None:
     t1 = time.time()
     npy = run_parallel(_numpy_chunk, spin, Y0, rtol, atol, workers, 512)
     t2 = time.time()
-    print(f"config1: numba-interpreted {t1 - t0:.1f}s, numpy {t2 - t1:.1f}s", flush=True)
+    print(f"schwarzschild64: numba-interpreted {t1 - t0:.1f}s, numpy {t2 - t1:.1f}s", flush=True)
     np.savez_compressed(
-        os.path.join(HERE, "config1_a0_i60_64.npz"),
+        os.path.join(HERE, "bardeen64_a0_i60.npz"),
         spin=spin, inclination_deg=incl, r_o=r_o, fov=fov, resolution=n, rtol=rtol, atol=atol,
         Y0=Y0,
         **{f"numba_{k}": v for k, v in nb.items()},
@@ -151,7 +151,7 @@ def band_indices(spin: float, incl: float, fov: float, n: int, width_px: float) 
     return np.flatnonzero(best <= width_px * d)
 
 
-def config2(workers: int) -> None:
+def kerr512(workers: int) -> None:
     """Kerr a=0.9, i=60 deg, r_o=1000, 512x512 on [-10,10]^2, rtol 1e-6 / atol 1e-8."""
     spin, incl, r_o, fov, n, rtol, atol = 0.9, 60.0, 1000.0, 10.0, 512, 1e-6, 1e-8
     alpha, beta = pixel_grid(fov, n)
@@ -165,11 +165,11 @@ def config2(workers: int) -> None:
     pick = np.union1d(band, sample)
     nb = run_parallel(_numba_interp_chunk, spin, Y0[pick], rtol, atol, workers, 32)
     t3 = time.time()
-    print(f"config2: IC {t1 - t0:.1f}s, numpy {t2 - t1:.1f}s, numba-interpreted {len(pick)} rays {t3 - t2:.1f}s",
+    print(f"kerr512: IC {t1 - t0:.1f}s, numpy {t2 - t1:.1f}s, numba-interpreted {len(pick)} rays {t3 - t2:.1f}s",
           flush=True)
     state = npy["state"].astype(np.int8)
     np.savez_compressed(
-        os.path.join(HERE, "config2_a09_i60_512.npz"),
+        os.path.join(HERE, "bardeen512_a09_i60.npz"),
         spin=spin, inclination_deg=incl, r_o=r_o, fov=fov, resolution=n, rtol=rtol, atol=atol,
         Y0_sha256=sha(Y0), Y0_pick=Y0[pick], pick=pick, band=band,
         numpy_state=state, numpy_n_steps=npy["n_steps"].astype(np.int32),
@@ -356,7 +356,7 @@ def disk(workers: int) -> None:
 def edges(workers: int) -> None:
     """extract_boundary_from_mask (boundary.py:116-170) on reference masks.
 
-    The config-2 mask (512 x 512, fov 10) and the two 64 x 64 ZAMO masks of
+    The 512 x 512 Kerr mask (a = 0.9, i = 60 deg, fov 10) and the two 64 x 64 ZAMO masks of
     docs/shadow.md section 5 (fov 12) about their captured centroids, 360
     directions; a 128 x 128 synthetic disk about a given centre, 90 directions.
     """
@@ -364,7 +364,7 @@ def edges(workers: int) -> None:
         sys.path.insert(0, REF_SRC)
     from kerrray.raytracing.boundary import extract_boundary_from_mask, mask_centroid
 
-    with np.load(os.path.join(HERE, "config2_a09_i60_512.npz")) as z:
+    with np.load(os.path.join(HERE, "bardeen512_a09_i60.npz")) as z:
         mask2 = z["numpy_state"].reshape(512, 512) == 2
     with np.load(os.path.join(HERE, "zamo_cameras.npz")) as z:
         zamo = [z[f"c{k}_numpy_state"].reshape(64, 64) == 2 for k in (0, 1)]
@@ -430,10 +430,10 @@ def impacts(workers: int) -> None:
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--workers", type=int, default=8)
-    ap.add_argument("--only", choices=["config1", "config2", "bisection", "zamo", "disk", "edges", "impacts"],
+    ap.add_argument("--only", choices=["schwarzschild64", "kerr512", "bisection", "zamo", "disk", "edges", "impacts"],
                     default=None)
     args = ap.parse_args()
-    for name, fn in (("config1", config1), ("config2", config2), ("bisection", bisection), ("zamo", zamo),
+    for name, fn in (("schwarzschild64", schwarzschild64), ("kerr512", kerr512), ("bisection", bisection), ("zamo", zamo),
                      ("disk", disk), ("edges", edges), ("impacts", impacts)):
         if args.only in (None, name):
             fn(args.workers)

@@ -13,7 +13,7 @@ is the reference's own backend-agreement pattern (tests/test_numba_backend.py:92
   Delta ~ 1e-6, and t, phi diverge logarithmically at the horizon. There
   the smooth product Delta(r) p_r (= +-sqrt(R(r))) is held to 10 x rtol and
   t, phi to 1e-3 relative; the reference's own NumPy and Numba backends
-  differ by ~760 x rtol in p_r on the same inputs (tests/golden, config 1).
+  differ by ~760 x rtol in p_r on the same inputs (tests/golden, the 64 x 64 Schwarzschild image).
 
 The reference numbers come from tests/golden (the Numba kernel source and
 the NumPy integrator run on the same Bardeen initial states) and from the CPU
@@ -63,7 +63,7 @@ def report(tag: str, mine, ref_Y, ref_steps, ref_rej, rtol: float, ref_state=Non
     return same_seq, float(bitwise.mean())
 
 
-def test_config1_schwarzschild_64(hip, golden1) -> None:
+def test_schwarzschild_64_image(hip, golden1) -> None:
     from kerrray_amd.camera import BardeenCamera
     from kerrray_amd.geometry import Spacetime
     from kerrray_amd.shadow import compute_shadow
@@ -75,13 +75,13 @@ def test_config1_schwarzschild_64(hip, golden1) -> None:
     assert np.array_equal(img.captured, b2 < 27.0), "closed-form Schwarzschild mask"
     assert np.array_equal(img.result.state, golden1["numba_state"])
     assert img.n_failed == 0 and img.n_other == 0
-    same, _ = report("config1", img.result, golden1["numba_Y"], golden1["numba_n_steps"],
+    same, _ = report("schwarzschild 64^2", img.result, golden1["numba_Y"], golden1["numba_n_steps"],
                      golden1["numba_n_rejected"], 1e-8, golden1["numba_state"], spin=0.0)
     assert same.mean() >= 0.99
     assert np.all(img.result.max_energy_drift == 0.0) and np.all(img.result.max_lz_drift == 0.0)
 
 
-def test_config2_kerr_512_mask_is_bit_identical(hip, golden2) -> None:
+def test_kerr_512_mask_is_bit_identical(hip, golden2) -> None:
     from kerrray_amd.camera import BardeenCamera
     from kerrray_amd.geometry import Spacetime
     from kerrray_amd.shadow import compute_shadow
@@ -91,17 +91,17 @@ def test_config2_kerr_512_mask_is_bit_identical(hip, golden2) -> None:
     img = compute_shadow(Spacetime(1.0, 0.9), cam, integ, term, backend=hip)
     ref_state = golden2["numpy_state"].astype(np.int64)
     diff = np.flatnonzero(img.result.state != ref_state)
-    print(f"[config2] kernel {hip.last_kernel_ms:.3f} ms, {len(diff)} pixels differ from the reference mask", flush=True)
+    print(f"[kerr 512^2] kernel {hip.last_kernel_ms:.3f} ms, {len(diff)} pixels differ from the reference mask", flush=True)
     assert np.array_equal(img.result.state, ref_state)
     assert np.array_equal(img.captured[::-1, :], img.captured)
     steps_same = np.mean(img.result.n_steps == golden2["numpy_n_steps"])
-    print(f"[config2] n_steps identical to the reference on {steps_same:.5f} of pixels", flush=True)
+    print(f"[kerr 512^2] n_steps identical to the reference on {steps_same:.5f} of pixels", flush=True)
     assert steps_same >= 0.99
     pick = golden2["pick"]
     sub = type("R", (), {})()
     sub.Y, sub.n_steps, sub.n_rejected = img.result.Y[pick], img.result.n_steps[pick], img.result.n_rejected[pick]
-    same, _ = report("config2 band+sample", sub, golden2["numba_Y"], golden2["numba_n_steps"],
-                     golden2["numba_n_rejected"], 1e-6)  # north star: every class within 10 x rtol
+    same, _ = report("kerr 512^2 band+sample", sub, golden2["numba_Y"], golden2["numba_n_steps"],
+                     golden2["numba_n_rejected"], 1e-6)  # every class within 10 x rtol
     assert same.mean() >= 0.99
     lam_rel = np.abs(img.result.lam[pick] - golden2["numba_lam"]) / np.maximum(1.0, golden2["numba_lam"])
     assert np.all(lam_rel[same] <= 1e-5)

@@ -66,7 +66,7 @@ def same(a, b, fields=FIELDS) -> None:
         assert np.array_equal(np.asarray(x), np.asarray(y)), name
 
 
-def test_config1_image_is_bitwise_the_reference(hostref, golden1) -> None:
+def test_schwarzschild_64_image_is_bitwise_the_reference(hostref, golden1) -> None:
     integ, term = options(rtol=1e-8, atol=1e-10)
     cam = BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=10.0, resolution=64)
     r = run_image(hostref, Spacetime(1.0, 0.0), cam, integ, term)
@@ -74,7 +74,7 @@ def test_config1_image_is_bitwise_the_reference(hostref, golden1) -> None:
         assert np.array_equal(getattr(r, name), golden1[f"numba_{name}"]), name
 
 
-def test_config2_image_is_bitwise_the_reference(hostref, golden2) -> None:
+def test_kerr_512_image_is_bitwise_the_reference(hostref, golden2) -> None:
     integ, term = options(rtol=1e-6, atol=1e-8)
     cam = BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=10.0, resolution=512)
     r = run_image(hostref, Spacetime(1.0, 0.9), cam, integ, term)

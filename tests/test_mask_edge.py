@@ -4,7 +4,7 @@ The oracle's restatement (NumPy, ``oracle.mask_edge``) and the kernel source
 built for the CPU (``mask_edge_radius``, one direction at a time) both equal
 the reference's ``extract_boundary_from_mask`` bit for bit on four masks
 (tests/golden/mask_edges.npz, written by tests/golden/make_golden.py): the
-512 x 512 config-2 mask, the two 64 x 64 ZAMO masks of docs/shadow.md
+512 x 512 Kerr mask (a = 0.9, i = 60 deg), the two 64 x 64 ZAMO masks of docs/shadow.md
 section 5 and a synthetic disk. The marching uses only IEEE arithmetic and
 floor on values the host gives it (cos and sin of the directions), so the
 device must match bit for bit too (tests/test_gpu_edge.py).

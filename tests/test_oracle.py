@@ -33,7 +33,7 @@ def integ(**kw):
     return ns(**{**INTEG, **kw})
 
 
-def test_config1_oracle_is_bitwise_the_reference_kernel(oracle, golden1) -> None:
+def test_schwarzschild_64_oracle_is_bitwise_the_reference_kernel(oracle, golden1) -> None:
     Y0 = oracle.bardeen_image(1.0, 0.0, 1000.0, 60.0, 10.0, 64)
     assert np.array_equal(Y0, golden1["Y0"]), "Bardeen initial states differ from rayset._off_equatorial_state"
     r = oracle.integrate_batch(ns(mass=1.0, spin=0.0), Y0, integ(rtol=1e-8, atol=1e-10), TERM)
@@ -47,7 +47,7 @@ def test_config1_oracle_is_bitwise_the_reference_kernel(oracle, golden1) -> None
     assert np.all(r.max_energy_drift == 0.0) and np.all(r.max_lz_drift == 0.0)
 
 
-def test_config1_mask_is_the_schwarzschild_closed_form(oracle) -> None:
+def test_schwarzschild_64_mask_is_the_closed_form(oracle) -> None:
     """a = 0: captured <=> xi^2 + eta = alpha^2 + beta^2 < 27 M^2; nearest centres are 0.019 M away."""
     Y0 = oracle.bardeen_image(1.0, 0.0, 1000.0, 60.0, 10.0, 64)
     r = oracle.integrate_batch(ns(mass=1.0, spin=0.0), Y0, integ(rtol=1e-8, atol=1e-10), TERM)
@@ -59,7 +59,7 @@ def test_config1_mask_is_the_schwarzschild_closed_form(oracle) -> None:
     assert np.array_equal(r.state == 2, b2 < 27.0)
 
 
-def test_config2_oracle_matches_reference_golden(oracle, golden2) -> None:
+def test_kerr_512_oracle_matches_reference_golden(oracle, golden2) -> None:
     """a = 0.9, i = 60 deg, 512^2, rtol 1e-6: bitwise on the edge band + sample, mask on every pixel."""
     import hashlib
 
