@@ -31,10 +31,10 @@ from kerrray_amd.camera import BardeenCamera, Camera, image_struct
 from kerrray_amd.geodesics import IntegratorOptions, TerminationOptions
 from kerrray_amd.geometry import Spacetime
 from kerrray_amd.orbits import shadow_curve
-from kerrray_amd.photons import TerminationState
+from kerrray_amd.photons import BatchResult, TerminationState
 from kerrray_amd.shadow import ShadowImage, state_counts
 
-__all__ = ["EdgeTable", "PlannedOptions", "compute_shadow_planned", "edge_table"]
+__all__ = ["EdgeTable", "PlannedOptions", "compute_shadow_planned", "edge_table", "trace_planned"]
 
 
 @dataclass(frozen=True)
@@ -125,18 +125,11 @@ class PlannedStats:
         return self.fp64_attempts / max(1, self.total_attempts)
 
 
-def compute_shadow_planned(
-    st: Spacetime,
-    cam: Camera | BardeenCamera,
-    integ: IntegratorOptions,
-    term: TerminationOptions,
-    *,
-    backend: str | HipBackend = "hip",
-    options: PlannedOptions = PlannedOptions(),
-) -> tuple[ShadowImage, PlannedStats]:
-    engine = get_backend(backend) if isinstance(backend, str) else backend
-    if not isinstance(engine, HipBackend):
-        raise TypeError("planned traces run on the hip backend")
+def trace_planned(engine: HipBackend, st: Spacetime, cam: Camera | BardeenCamera, integ: IntegratorOptions,
+                  term: TerminationOptions, options: PlannedOptions = PlannedOptions(),
+                  ) -> tuple[BatchResult, _lib.KrPlannedStats, float]:
+    """Every pixel of ``cam`` through ``kr_trace_image_planned``: the rows, the plan's counts and the call's
+    device time (ms, also left in ``engine.last_kernel_ms``)."""
     t0 = time.perf_counter()
     table = edge_table(st, cam.inclination_deg)
     if isinstance(cam, Camera):
@@ -158,11 +151,27 @@ def compute_shadow_planned(
                 ctypes.byref(stats), ctypes.byref(ms)),
             "kr_trace_image_planned",
         )
-    result = out.result(time.perf_counter() - t0)
+    engine.last_kernel_ms = ms.value
+    return out.result(time.perf_counter() - t0), stats, ms.value
+
+
+def compute_shadow_planned(
+    st: Spacetime,
+    cam: Camera | BardeenCamera,
+    integ: IntegratorOptions,
+    term: TerminationOptions,
+    *,
+    backend: str | HipBackend = "hip",
+    options: PlannedOptions = PlannedOptions(),
+) -> tuple[ShadowImage, PlannedStats]:
+    engine = get_backend(backend) if isinstance(backend, str) else backend
+    if not isinstance(engine, HipBackend):
+        raise TypeError("planned traces run on the hip backend")
+    result, stats, ms = trace_planned(engine, st, cam, integ, term, options)
     alpha, beta = cam.pixel_coordinates()
     state = result.state.reshape(cam.shape)
     image = ShadowImage(alpha=alpha, beta=beta, state=state, captured=state == int(TerminationState.CAPTURED),
                         counts=state_counts(state), result=result, camera=cam, spacetime=st)
     ps = PlannedStats(stats.planned_band, stats.planned_inside, stats.planned_outside, stats.repaired,
-                      stats.fp64_attempts, int(np.sum(result.n_steps + result.n_rejected)), ms.value)
+                      stats.fp64_attempts, int(np.sum(result.n_steps + result.n_rejected)), ms)
     return image, ps

@@ -185,18 +185,31 @@ def compute_shadow(
     backend: str | Backend = "hip",
     progress: ProgressCallback | None = None,
     chunk_size: int = DEFAULT_CHUNK_SIZE,
+    schedule: str = "auto",
 ) -> ShadowImage:
     """Trace every pixel of ``cam`` and classify it (shadow.py:179-211).
 
-    On the ``hip`` backend one launch generates and traces every pixel's
-    photon. Another :class:`Backend` gets the initial states as an (N, 8)
-    array and traces them in chunks through :func:`trace_rays`.
+    On the ``hip`` backend the device generates and traces every pixel's
+    photon. With ``schedule="auto"`` an FP64 Dormand-Prince image runs through
+    the planned persistent kernel (the edge band's long photons first; every
+    row bit for bit the plain kernel's, about 1.4x faster at 512^2), and
+    anything else (RK4, float32) through the plain kernel, one thread per
+    pixel; ``schedule="plain"`` always takes the plain kernel. Another
+    :class:`Backend` gets the initial states as an (N, 8) array and traces
+    them in chunks through :func:`trace_rays`.
     """
     if not isinstance(cam, (Camera, BardeenCamera)):
         raise TypeError("compute_shadow needs a Camera or a BardeenCamera")
+    if schedule not in ("auto", "plain"):
+        raise ValueError(f"unknown schedule {schedule!r}: use 'auto' or 'plain'")
     engine = get_backend(backend) if isinstance(backend, str) else backend
     if isinstance(engine, HipBackend):
-        result = trace_image(engine, st, cam, integ, term)
+        if schedule == "auto" and integ.method == "rk45" and integ.np_dtype == np.float64:
+            from kerrray_amd.planned import PlannedOptions, trace_planned
+
+            result = trace_planned(engine, st, cam, integ, term, PlannedOptions(fp32_bulk=False))[0]
+        else:
+            result = trace_image(engine, st, cam, integ, term)
         if progress is not None:
             progress(1.0)
     else:

@@ -21,7 +21,7 @@ def _images(hip, spin, incl, res, rtol, opts):
     st = Spacetime(1.0, spin)
     integ, term = options(rtol=rtol, atol=1e-2 * rtol)
     cam = BardeenCamera(radius=1000.0, inclination_deg=incl, fov=10.0, resolution=res)
-    plain = compute_shadow(st, cam, integ, term, backend=hip)
+    plain = compute_shadow(st, cam, integ, term, backend=hip, schedule="plain")
     plain_ms = hip.last_kernel_ms
     planned, stats = compute_shadow_planned(st, cam, integ, term, backend=hip, options=opts)
     return plain, plain_ms, planned, stats
@@ -107,3 +107,27 @@ def test_precision_form_mask_equals_the_oracle_fp64_mask_1024(hip, oracle) -> No
     print(f"[precision 1024^2 a=0.99 i=85] mask diff vs oracle FP64 {diff}, band {stats.planned_band}, "
           f"repaired {stats.repaired}, fp64 attempts {stats.fp64_fraction:.3f}", flush=True)
     assert diff == 0
+
+
+@pytest.mark.parametrize("case", [("bardeen", 0.9, 60.0, 256), ("zamo", -0.5, 30.0, 128), ("zamo", 0.0, 60.0, 64),
+                                  ("bardeen", 0.998, 89.0, 96)])
+def test_compute_shadow_auto_schedule_is_bitwise_the_plain_kernel(hip, case) -> None:
+    """compute_shadow's default schedule (the planned FP64 kernel for Dormand-Prince in float64) returns the plain
+    kernel's rows bit for bit, for both cameras, both spin signs and near-extremal spin."""
+    from kerrray_amd.camera import BardeenCamera, Camera
+    from kerrray_amd.geometry import Spacetime
+    from kerrray_amd.shadow import compute_shadow
+
+    kind, spin, incl, res = case
+    st = Spacetime(1.0, spin)
+    integ, term = options(rtol=1e-7, atol=1e-9)
+    cam = (BardeenCamera(radius=1000.0, inclination_deg=incl, fov=10.0, resolution=res) if kind == "bardeen"
+           else Camera(radius=1000.0, inclination_deg=incl, fov=10.0, resolution=res))
+    auto = compute_shadow(st, cam, integ, term, backend=hip)
+    auto_ms = hip.last_kernel_ms
+    plain = compute_shadow(st, cam, integ, term, backend=hip, schedule="plain")
+    print(f"[compute_shadow {kind} a={spin} i={incl} {res}^2] auto {auto_ms:.3f} ms, plain {hip.last_kernel_ms:.3f} ms",
+          flush=True)
+    for name in FIELDS:
+        assert np.array_equal(getattr(auto.result, name), getattr(plain.result, name)), name
+    assert np.array_equal(auto.captured, plain.captured)

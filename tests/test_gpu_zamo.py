@@ -82,7 +82,7 @@ def test_planned_zamo_image_has_the_fp64_mask(hip, fp32_bulk) -> None:
     st = Spacetime(1.0, 0.9)
     cam = Camera(radius=1000.0, inclination_deg=60.0, fov=10.0, resolution=512)
     integ, term = options(rtol=1e-6, atol=1e-8)
-    plain = compute_shadow(st, cam, integ, term, backend=hip)
+    plain = compute_shadow(st, cam, integ, term, backend=hip, schedule="plain")
     planned, stats = compute_shadow_planned(st, cam, integ, term, backend=hip,
                                             options=PlannedOptions(fp32_bulk=fp32_bulk))
     diff = int(np.count_nonzero(planned.captured != plain.captured))
