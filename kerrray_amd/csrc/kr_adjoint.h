@@ -480,9 +480,10 @@ KR_HD void step_increment(double m, double a, double pt, double pph, double h, c
 template <class Sink>
 struct StageStepper {
     Sink sink;
-    KR_HD void operator()(const Params& P, const RayState<double>& R, double h_eff, double (&y_new)[NA],
+    KR_HD bool operator()(const Params& P, const RayState<double>& R, double h_eff, double (&y_new)[NA],
                           double (&err)[NA], double (&k6)[NA], double& s_new, double& c_new) const {
         dp5_step(P, R.y, R.p_t, R.p_ph, h_eff, R.f, y_new, err, k6, s_new, c_new, sink);
+        return true;
     }
 };
 

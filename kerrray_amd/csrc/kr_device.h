@@ -427,6 +427,47 @@ KR_HD T step_factor(T safety, T fmin, T fmax, T expo, T en, bool rejected_before
     return factor;
 }
 
+// x^(-1/5) for 1e-5 <= x <= 1e5 in FP64 to within a few ulp (not the library
+// pow's rounding): an FP32 estimate (hardware log2 and exp2 on the device)
+// refined by two Newton corrections y <- y + (y / 5)(1 - x y^5), each of
+// which squares the relative error (1e-7, 1e-14, then rounding).
+KR_HD double pow_m_fifth(double x) {
+    const float xf = static_cast<float>(x);
+#if defined(KR_HOST_BUILD) || !defined(__HIP_DEVICE_COMPILE__)
+    double y = static_cast<double>(exp2f(-0.2f * log2f(xf)));
+#else
+    double y = static_cast<double>(__builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf(xf)));
+#endif
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double y2 = y * y;
+        const double t = x * (y2 * y2 * y);
+        y = fma(y * 0.2, 1.0 - t, y);
+    }
+    return y;
+}
+
+// The step factor of an FP32-bulk attempt of the precision form: the
+// controller of numba_kernel.py:272-281 in FP64 (same constants and clamps)
+// with en^(-1/5) from pow_m_fifth instead of the library pow, which is
+// a fifth of such an attempt's instructions.  Outside [1e-5, 1e5] the clamps
+// decide whatever the power: 0.9 en^(-1/5) exceeds 5 below and is under 0.2
+// above.
+template <typename T>
+KR_HD T bulk_step_factor(T en, bool rejected_before) {
+    T factor;
+    if (en < T(1e-5))
+        factor = static_cast<T>(fixed::factor_max);
+    else if (en > T(1e5))
+        factor = static_cast<T>(fixed::factor_min);
+    else
+        factor = pymin(static_cast<T>(fixed::factor_max),
+                       pymax(static_cast<T>(fixed::factor_min),
+                             static_cast<T>(fixed::safety) * static_cast<T>(pow_m_fifth(static_cast<double>(en)))));
+    if (rejected_before) factor = pymin(factor, T(1));
+    return factor;
+}
+
 // numba_kernel.py:293-311.  sin_th is sin(y[th]) in the working type.
 template <typename T>
 KR_HD int classify(const Params& P, const T (&v)[NA], T p_ph, const T (&f)[NA], double lam,
@@ -543,12 +584,13 @@ KR_HD void ray_init(const Params& P, const double (&y64)[8], RayState<T>& R) {
 // working type.
 template <typename T, bool ADAPTIVE>
 struct ExactStepper {
-    KR_HD void operator()(const Params& P, const RayState<T>& R, T h_eff, T (&y_new)[NA], T (&err)[NA],
+    KR_HD bool operator()(const Params& P, const RayState<T>& R, T h_eff, T (&y_new)[NA], T (&err)[NA],
                           T (&k6)[NA], double& s_new, double& c_new) const {
         if (ADAPTIVE)
             dp5_step(P, R.y, R.p_t, R.p_ph, h_eff, R.f, y_new, err, k6, s_new, c_new);
         else
             rk4_step(P, R.y, R.p_t, R.p_ph, h_eff, R.f, y_new, k6, s_new, c_new);
+        return true;
     }
 };
 
@@ -557,12 +599,13 @@ struct ExactStepper {
 template <typename T, bool ADAPTIVE, bool PARK = false>
 struct StoredStepper {
     SlopeStore<T> K;
-    KR_HD void operator()(const Params& P, const RayState<T>& R, T h_eff, T (&y_new)[NA], T (&err)[NA],
+    KR_HD bool operator()(const Params& P, const RayState<T>& R, T h_eff, T (&y_new)[NA], T (&err)[NA],
                           T (&k6)[NA], double& s_new, double& c_new) const {
         if (ADAPTIVE)
             dp5_step_stored<T, PARK>(P, R.y, R.p_t, R.p_ph, h_eff, R.f, K, y_new, err, k6, s_new, c_new);
         else
             rk4_step(P, R.y, R.p_t, R.p_ph, h_eff, R.f, y_new, k6, s_new, c_new);
+        return true;
     }
 };
 
@@ -586,7 +629,9 @@ KR_HD void ray_attempt_with(const Params& P, RayState<T>& R, const Stepper& step
     if (last) h_eff = static_cast<T>(remaining);
     T y_new[NA], k6[NA], err[NA];
     double s_new, c_new;
-    step(P, R, h_eff, y_new, err, k6, s_new, c_new);
+    // The stepper reports whether the attempt was the exact one (the
+    // reference's arithmetic throughout, controller included).
+    const bool exact = step(P, R, h_eff, y_new, err, k6, s_new, c_new);
     bool finite = all_finite(y_new, k6);
     bool accept = finite;
     T h_new = R.h;
@@ -597,9 +642,10 @@ KR_HD void ray_attempt_with(const Params& P, RayState<T>& R, const Stepper& step
             accept = false;
         } else {
             accept = en <= T(1);
-            const T factor = step_factor(static_cast<T>(fixed::safety), static_cast<T>(fixed::factor_min),
-                                         static_cast<T>(fixed::factor_max), static_cast<T>(fixed::exponent), en,
-                                         R.rejected_before);
+            const T factor = exact ? step_factor(static_cast<T>(fixed::safety), static_cast<T>(fixed::factor_min),
+                                                 static_cast<T>(fixed::factor_max),
+                                                 static_cast<T>(fixed::exponent), en, R.rejected_before)
+                                   : bulk_step_factor(en, R.rejected_before);
             if (!(accept && last)) {
                 h_new = h_eff * factor;
                 if (static_cast<double>(h_new) > h_cap) h_new = static_cast<T>(h_cap);

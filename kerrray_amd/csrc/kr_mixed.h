@@ -201,14 +201,15 @@ struct MixedStepper {
     MixedCfg M;
     int64_t* fp64_attempts;  // per-lane counters (registers), may be null
     SlopeStore<double> K;
-    KR_HD void operator()(const Params& P, const RayState<double>& R, double h_eff, double (&y_new)[NA],
+    KR_HD bool operator()(const Params& P, const RayState<double>& R, double h_eff, double (&y_new)[NA],
                           double (&err)[NA], double (&k6)[NA], double& s_new, double& c_new) const {
         if (needs_fp64(M, R)) {
             if (fp64_attempts) *fp64_attempts += 1;
             dp5_step_stored<double, PARK>(P, R.y, R.p_t, R.p_ph, h_eff, R.f, K, y_new, err, k6, s_new, c_new);
-        } else {
-            dp5_step_mixed(G, R.y, R.p_t + 0.0, R.p_ph + 0.0, h_eff, R.f, y_new, err, k6, s_new, c_new);
+            return true;
         }
+        dp5_step_mixed(G, R.y, R.p_t + 0.0, R.p_ph + 0.0, h_eff, R.f, y_new, err, k6, s_new, c_new);
+        return false;
     }
 };
 

@@ -353,6 +353,19 @@ extern "C" void krh_step_increment(const kr_params* p, double m, double a, const
     kr::ad::step_increment<4, 3>(m, a, pt, pph, h, stage, put);
 }
 
+// The FP32-bulk controller's pieces on n inputs: out[3 i] = pow_m_fifth(x_i)
+// (x_i in [1e-5, 1e5]), out[3 i + 1] = the exact step factor of en = x_i,
+// out[3 i + 2] = the bulk step factor (neither rejected before).
+extern "C" void krh_bulk_controller(const double* x, int64_t n, double* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        const double en = x[i];
+        out[3 * i] = (en >= 1e-5 && en <= 1e5) ? kr::pow_m_fifth(en) : 0.0;
+        out[3 * i + 1] = kr::step_factor(kr::fixed::safety, kr::fixed::factor_min, kr::fixed::factor_max,
+                                         kr::fixed::exponent, en, false);
+        out[3 * i + 2] = kr::bulk_step_factor(en, false);
+    }
+}
+
 // div_by (kr_device.h) against the IEEE quotient on n operand pairs:
 // returns the number of results whose bits differ.
 extern "C" int64_t krh_div_by_mismatches(const double* a, const double* b, int64_t n) {

@@ -166,3 +166,28 @@ def test_shared_reciprocal_quotients_are_the_ieee_quotients(hostref) -> None:
     for a, b in cases:
         a, b = np.ascontiguousarray(a, dtype=np.float64), np.ascontiguousarray(b, dtype=np.float64)
         assert hostref.krh_div_by_mismatches(a.ctypes.data, b.ctypes.data, a.size) == 0
+
+
+def test_bulk_controller_power_and_clamps(hostref) -> None:
+    """The FP32-bulk controller of the precision form: en^(-1/5) within 2 ulp of the extended-precision value on
+    [1e-5, 1e5] (the host seeds it from glibc's float log2/exp2, the device from the hardware ones; two Newton
+    corrections make either seed converge), and the step factor equal to the exact controller's wherever the
+    clamps decide (en outside [1e-5, 1e5], en = 0, en below ~1.9e-4 or above ~4.1) and within 2 ulp elsewhere."""
+    hostref.krh_bulk_controller.restype = None
+    hostref.krh_bulk_controller.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+    rng = np.random.default_rng(12)
+    x = np.concatenate([np.logspace(-5, 5, 200_001), 10.0 ** rng.uniform(-5, 5, 200_000),
+                        np.array([1e-5, 1e5, 1.0, 2.0, 0.5, np.nextafter(1.0, 0), np.nextafter(1e-5, 1)]),
+                        np.array([0.0, 1e-300, 1e-9, np.nextafter(1e-5, 0), np.nextafter(1e5, 2e5), 1e9, 1e300])])
+    x = np.ascontiguousarray(x)
+    out = np.empty((x.size, 3))
+    hostref.krh_bulk_controller(x.ctypes.data, x.size, out.ctypes.data)
+    inside = (x >= 1e-5) & (x <= 1e5)
+    exact_pow = np.asarray(x[inside], dtype=np.longdouble) ** np.longdouble(-0.2)
+    rel = np.abs((out[inside, 0] - exact_pow) / exact_pow).astype(np.float64)
+    assert rel.max() <= 2 * 2.0**-52, rel.max()
+    exact_f, bulk_f = out[:, 1], out[:, 2]
+    clamped = (exact_f == 5.0) | (exact_f == 0.2)
+    assert np.array_equal(bulk_f[~inside], exact_f[~inside])
+    assert np.array_equal(bulk_f[clamped], exact_f[clamped])
+    assert np.max(np.abs(bulk_f - exact_f) / exact_f) <= 2 * 2.0**-52
