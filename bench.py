@@ -91,6 +91,11 @@ def cpu_baseline(n_threads: int) -> dict:
         t0 = time.perf_counter()
         O.integrate_batch(st, Y0, integ, term, n_threads=n_threads, dynamic=True)
         best = min(best, time.perf_counter() - t0)
+    static = math.inf
+    for _ in range(2):
+        t0 = time.perf_counter()
+        O.integrate_batch(st, Y0, integ, term, n_threads=n_threads, dynamic=False)
+        static = min(static, time.perf_counter() - t0)
     one = Y0[::64]
     t1 = math.inf
     for _ in range(2):
@@ -103,8 +108,25 @@ def cpu_baseline(n_threads: int) -> dict:
         "cores": n_threads,
         "kind": "port",
         "sample": f"the full 512x512 workload (262144 photons), best of 3, OpenMP dynamic over photons on {n_threads} threads",
+        "static_schedule": {"value": len(Y0) / static,
+                            "sample": f"the same workload, best of 2, OpenMP static over photons on {n_threads} threads"},
         "single_core": {"value": len(one) / t1, "sample": "every 64th pixel (4096 photons), 1 thread, best of 2"},
+        "host": host_cpu(),
     }
+
+
+def host_cpu() -> dict:
+    """The host's CPU model (from /proc/cpuinfo), its logical CPU count and this process's CPU affinity."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for row in f:
+                if row.startswith("model name"):
+                    model = row.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
 
 
 def persistent_single(hip, ref_state) -> dict:

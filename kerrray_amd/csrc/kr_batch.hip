@@ -351,6 +351,11 @@ extern "C" kr_status kr_batch_graph_launch(kr_batch b, void* stream) {
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (!b->exec) {
         // Capture on a private stream so the caller's stream is never in capture mode.
+        // A graph left by an earlier capture whose instantiation failed is released first.
+        if (b->graph) {
+            (void)hipGraphDestroy(b->graph);
+            b->graph = nullptr;
+        }
         hipStream_t cap;
         KR_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
         hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
