@@ -201,7 +201,7 @@ class PlannedRunner:
         return {"band_px": o.band_px, "horizon_margin": o.horizon_margin, "axis_sin": o.axis_sin,
                 "turn_rmax": o.turn_rmax, "turn_drdl": o.turn_drdl, "refill_interval": o.refill_interval,
                 "block_size": o.block_size, "longest_first": o.longest_first, "split_roles": o.split_roles,
-                "waves_per_simd": o.waves_per_simd or "auto"}
+                "waves_per_simd": o.waves_per_simd or "auto", "grid_fraction": o.grid_fraction or "auto"}
 
 
 class PlainRunner:

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define KR_ABI_VERSION 5
+#define KR_ABI_VERSION 6
 
 typedef enum kr_status {
     KR_OK = 0,
@@ -323,7 +323,13 @@ kr_status kr_batch_destroy(kr_batch b);
  *    then run alone on their SIMD), two otherwise.
  *  split_roles (fp32_bulk only): the first ceil(band / 64) wavefronts trace
  *    only the FP64 band and strip, the others only FP32 photons, so no
- *    wavefront runs both attempts. */
+ *    wavefront runs both attempts.
+ *  grid_fraction: share of the resident-wavefront slots (waves_per_simd per
+ *    SIMD) the persistent kernel is launched on, in (0, 1].  0: 7/8 for the
+ *    split FP32 form at one wavefront per SIMD up to 2^18 pixels (its FP32
+ *    photons then finish well before the longest FP64 photon, and fewer
+ *    active wavefronts let the clock those photons run at rise), 1
+ *    otherwise. */
 typedef struct kr_planned_opts {
     double horizon_margin;
     double axis_sin;
@@ -338,6 +344,7 @@ typedef struct kr_planned_opts {
     int32_t waves_per_simd; /* 0 automatic, or 1-4: occupancy the trace kernel's registers are allocated for */
     int32_t split_roles;    /* 1: wavefronts trace FP64 or FP32 photons, never both (fp32_bulk only) */
     double axis_lz; /* pixels with |alpha sin theta_o| < axis_lz are integrated in FP64 (near-polar photons) */
+    double grid_fraction; /* 0 automatic, or (0, 1]: share of the wavefront slots launched */
 } kr_planned_opts;
 
 typedef struct kr_planned_stats {

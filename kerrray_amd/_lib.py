@@ -18,7 +18,7 @@ KR_OK, KR_ERR_INVALID, KR_ERR_DEVICE, KR_ERR_RADICAND, KR_ERR_UNSUPPORTED, KR_ER
 KR_FLOAT64, KR_FLOAT32 = 0, 1
 KR_FLAG_RADICAND, KR_FLAG_NOT_NULL, KR_FLAG_SKY, KR_FLAG_DISK, KR_FLAG_EDGE = 1, 2, 4, 8, 16
 KR_CAMERA_BARDEEN, KR_CAMERA_ZAMO = 0, 1
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 # Every symbol the header declares (checked by the tests).
 EXPORTED = (
@@ -151,6 +151,7 @@ class KrPlannedOpts(ctypes.Structure):
         ("waves_per_simd", ctypes.c_int32),
         ("split_roles", ctypes.c_int32),
         ("axis_lz", ctypes.c_double),
+        ("grid_fraction", ctypes.c_double),
     ]
 
 

@@ -442,31 +442,36 @@ kr_status check_opts(const kr_planned_opts* o) {
     if (o->split_roles != 0 && o->split_roles != 1) return host::fail(KR_ERR_INVALID, "split_roles must be 0 or 1");
     if (!(o->horizon_margin >= 0.0) || !(o->axis_sin >= 0.0) || !(o->turn_drdl >= 0.0) || !(o->turn_rmax >= 0.0))
         return host::fail(KR_ERR_INVALID, "switch thresholds must be >= 0");
+    if (!(o->grid_fraction >= 0.0 && o->grid_fraction <= 1.0))
+        return host::fail(KR_ERR_INVALID, "grid_fraction must be 0 (automatic) or in (0, 1]");
     return KR_OK;
 }
 
 template <int BLOCK, bool MIXED, int WAVES, bool PARK = false>
-void launch_planned_t(int cus, hipStream_t s, const Params& P, const BardeenImage& B, const MixedCfg& M,
-                      const Work& w, int refill, int band_priority, int split, const RowsOut<double>& o) {
-    const int grid = cus * (4 * WAVES * 64 / BLOCK);  // one resident wave per SIMD slot
+void launch_planned_t(int cus, double frac, hipStream_t s, const Params& P, const BardeenImage& B,
+                      const MixedCfg& M, const Work& w, int refill, int band_priority, int split,
+                      const RowsOut<double>& o) {
+    // One resident wavefront per SIMD slot, or the share frac of them.
+    const int slots = cus * (4 * WAVES * 64 / BLOCK);
+    const int grid = std::max(1, static_cast<int>(slots * frac));
     hipLaunchKernelGGL((planned_trace_kernel<BLOCK, MIXED, WAVES, PARK>), dim3(grid), dim3(BLOCK), 0, s, P, B, M, w.work,
                        w.counters, refill, band_priority, split, w.used_fp64, o, w.flags);
 }
 
 template <int BLOCK, bool MIXED>
-void launch_planned_w(int waves, bool park, int cus, hipStream_t s, const Params& P, const BardeenImage& B,
+void launch_planned_w(int waves, bool park, int cus, double frac, hipStream_t s, const Params& P, const BardeenImage& B,
                       const MixedCfg& M, const Work& w, int refill, int band_priority, int split,
                       const RowsOut<double>& o) {
     if (waves == 1)
-        launch_planned_t<BLOCK, MIXED, 1>(cus, s, P, B, M, w, refill, band_priority, split, o);
+        launch_planned_t<BLOCK, MIXED, 1>(cus, frac, s, P, B, M, w, refill, band_priority, split, o);
     else if (waves == 3)
-        launch_planned_t<BLOCK, MIXED, 3>(cus, s, P, B, M, w, refill, band_priority, split, o);
+        launch_planned_t<BLOCK, MIXED, 3>(cus, frac, s, P, B, M, w, refill, band_priority, split, o);
     else if (waves == 4)
-        launch_planned_t<BLOCK, MIXED, 4>(cus, s, P, B, M, w, refill, band_priority, split, o);
+        launch_planned_t<BLOCK, MIXED, 4>(cus, frac, s, P, B, M, w, refill, band_priority, split, o);
     else if (park)
-        launch_planned_t<BLOCK, MIXED, 2, true>(cus, s, P, B, M, w, refill, band_priority, split, o);
+        launch_planned_t<BLOCK, MIXED, 2, true>(cus, frac, s, P, B, M, w, refill, band_priority, split, o);
     else
-        launch_planned_t<BLOCK, MIXED, 2>(cus, s, P, B, M, w, refill, band_priority, split, o);
+        launch_planned_t<BLOCK, MIXED, 2>(cus, frac, s, P, B, M, w, refill, band_priority, split, o);
 }
 
 kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B, const kr_planned_opts* o,
@@ -503,18 +508,26 @@ kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B,
     // (512^2 FP64: 4.9 against 5.25 ms).
     const bool large = pixels > (int64_t(1) << 19);
     const int waves = o->waves_per_simd != 0 ? o->waves_per_simd : (mixed && split && !large ? 1 : 2);
+    // Automatic grid: at one wavefront per SIMD up to 2^18 pixels the split
+    // FP32 form's bulk finishes well before its longest FP64 photon, and
+    // launching 7/8 of the slots lowers the power drawn meanwhile, so the
+    // clock that photon runs at rises (512^2: 3.50 against 3.68 ms; 3/4:
+    // 3.47 ms; at 5/8 the bulk becomes the bound, 3.78 ms).  Otherwise all
+    // slots (1024^2 at two per SIMD: 7/8 of them take 6.70 against 6.20 ms).
+    const bool small = pixels <= (int64_t(1) << 18);
+    const double frac = o->grid_fraction > 0.0 ? o->grid_fraction : (mixed && split && waves == 1 && small ? 0.875 : 1.0);
     switch (o->block_size) {
         case 64:
-            mixed ? launch_planned_w<64, true>(waves, large, cus, s, P, B, M, w, refill, prio, split, bo)
-                  : launch_planned_w<64, false>(waves, large, cus, s, P, B, M, w, refill, prio, split, bo);
+            mixed ? launch_planned_w<64, true>(waves, large, cus, frac, s, P, B, M, w, refill, prio, split, bo)
+                  : launch_planned_w<64, false>(waves, large, cus, frac, s, P, B, M, w, refill, prio, split, bo);
             break;
         case 128:
-            mixed ? launch_planned_w<128, true>(waves, large, cus, s, P, B, M, w, refill, prio, split, bo)
-                  : launch_planned_w<128, false>(waves, large, cus, s, P, B, M, w, refill, prio, split, bo);
+            mixed ? launch_planned_w<128, true>(waves, large, cus, frac, s, P, B, M, w, refill, prio, split, bo)
+                  : launch_planned_w<128, false>(waves, large, cus, frac, s, P, B, M, w, refill, prio, split, bo);
             break;
         default:
-            mixed ? launch_planned_w<256, true>(waves, large, cus, s, P, B, M, w, refill, prio, split, bo)
-                  : launch_planned_w<256, false>(waves, large, cus, s, P, B, M, w, refill, prio, split, bo);
+            mixed ? launch_planned_w<256, true>(waves, large, cus, frac, s, P, B, M, w, refill, prio, split, bo)
+                  : launch_planned_w<256, false>(waves, large, cus, frac, s, P, B, M, w, refill, prio, split, bo);
     }
     KR_HIP(hipGetLastError());
     for (int round = 0; round < REPAIR_ROUNDS; ++round) {

@@ -65,6 +65,12 @@ class PlannedOptions:
     # FP32 bulk only: wavefronts trace either the FP64 band and strip or FP32
     # photons, never both in one wavefront.
     split_roles: bool = True
+    # Share of the resident-wavefront slots the trace kernel is launched on,
+    # in (0, 1]; 0: automatic (7/8 for the split FP32 form at one wavefront
+    # per SIMD up to 2^18 pixels, whose FP32 photons finish well before the
+    # longest FP64 photon, so that fewer active wavefronts leave the clock
+    # higher for it; all slots otherwise).
+    grid_fraction: float = 0.0
 
     def kr(self) -> _lib.KrPlannedOpts:
         o = _lib.KrPlannedOpts()
@@ -81,6 +87,7 @@ class PlannedOptions:
         o.waves_per_simd = self.waves_per_simd
         o.axis_lz = self.axis_lz
         o.split_roles = 1 if self.split_roles else 0
+        o.grid_fraction = self.grid_fraction
         return o
 
 

@@ -90,6 +90,35 @@ def test_split_roles_give_the_same_rows(hip, case) -> None:
         assert s1.fp64_attempts == so.fp64_attempts and s1.repaired == so.repaired
 
 
+def test_grid_fraction_gives_the_same_rows(hip) -> None:
+    """The persistent trace kernel launched on 7/8 (automatic at 512^2), all and 3/10 of the wavefront slots
+    traces every photon alike (work is drawn from a queue, so no photon depends on the grid); fractions
+    outside (0, 1] are rejected."""
+    from kerrray_amd.camera import BardeenCamera
+    from kerrray_amd.geometry import Spacetime
+    from kerrray_amd.planned import PlannedOptions, compute_shadow_planned
+
+    st = Spacetime(1.0, 0.9)
+    integ, term = options(rtol=1e-6, atol=1e-8)
+    cam = BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=10.0, resolution=512)
+    runs = {}
+    for frac in (0.0, 1.0, 0.3):
+        img, stats = compute_shadow_planned(st, cam, integ, term, backend=hip,
+                                            options=PlannedOptions(fp32_bulk=True, grid_fraction=frac))
+        runs[frac] = (img, stats)
+        print(f"[grid fraction {frac or 'auto'}] {stats.kernel_ms:.3f} ms, repaired {stats.repaired}", flush=True)
+    base, s0 = runs[0.0]
+    for frac in (1.0, 0.3):
+        other, so = runs[frac]
+        for name in FIELDS:
+            assert np.array_equal(getattr(base.result, name), getattr(other.result, name)), (frac, name)
+        assert s0.fp64_attempts == so.fp64_attempts and s0.repaired == so.repaired
+    for bad in (-0.1, 1.5, float("nan")):
+        with pytest.raises(ValueError, match="grid_fraction"):
+            compute_shadow_planned(st, BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=10.0, resolution=64),
+                                   integ, term, backend=hip, options=PlannedOptions(grid_fraction=bad))
+
+
 def test_precision_form_mask_equals_the_oracle_fp64_mask_1024(hip, oracle) -> None:
     """1024^2, a=0.99, i=85 deg, rtol 1e-6: the FP32+FP64-repair mask, the GPU FP64 mask and the CPU oracle's."""
     from kerrray_amd.camera import BardeenCamera
