@@ -525,6 +525,8 @@ def main() -> int:
                     help="only copy 256 MiB device-to-device once (the reference kernel for the HBM counters)")
     ap.add_argument("--batch-only", metavar="BLOCK,WAVES,REFILL",
                     help="only run the 64-image ragged batch once at this shape (no autotune; for profilers)")
+    ap.add_argument("--precision-only", choices=["fp32", "fp64"],
+                    help="only trace the 1024^2, a=0.99, i=85 deg image in this planned form, three times (for profilers)")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -563,6 +565,14 @@ def main() -> int:
         batch.check_flags()
         print(json.dumps({"batch_only": shape, "photons": batch.n, "kernel_ms": ms}), flush=True)
         batch.close()
+        return 0
+    if args.precision_only:
+        runner = PlannedRunner(hip, args.precision_only == "fp32", Spacetime(1.0, 0.99),
+                               IntegratorOptions(method="rk45", rtol=1e-6, atol=1e-8), TerminationOptions(),
+                               res=1024, spin=0.99, incl=85.0)
+        runner()
+        ms = timed(runner, 2)
+        print(json.dumps({"precision_only": args.precision_only, "photons": runner.n, "kernel_ms": ms}), flush=True)
         return 0
     st = Spacetime(1.0, SPIN)
     integ = IntegratorOptions(method="rk45", rtol=RTOL, atol=ATOL)
