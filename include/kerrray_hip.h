@@ -326,10 +326,10 @@ kr_status kr_batch_destroy(kr_batch b);
  *    wavefront runs both attempts.
  *  grid_fraction: share of the resident-wavefront slots (waves_per_simd per
  *    SIMD) the persistent kernel is launched on, in (0, 1].  0: 7/8 for the
- *    split FP32 form at one wavefront per SIMD up to 2^18 pixels (its FP32
- *    photons then finish well before the longest FP64 photon, and fewer
- *    active wavefronts let the clock those photons run at rise), 1
- *    otherwise. */
+ *    split FP32 form at one wavefront per SIMD up to 2^18 pixels and rtol
+ *    >= 1e-6 (its FP32 photons then finish well before the longest FP64
+ *    photon, and fewer active wavefronts let the clock those photons run at
+ *    rise), 1 otherwise. */
 typedef struct kr_planned_opts {
     double horizon_margin;
     double axis_sin;

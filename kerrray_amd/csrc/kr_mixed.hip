@@ -508,14 +508,19 @@ kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B,
     // (512^2 FP64: 4.9 against 5.25 ms).
     const bool large = pixels > (int64_t(1) << 19);
     const int waves = o->waves_per_simd != 0 ? o->waves_per_simd : (mixed && split && !large ? 1 : 2);
-    // Automatic grid: at one wavefront per SIMD up to 2^18 pixels the split
-    // FP32 form's bulk finishes well before its longest FP64 photon, and
-    // launching 7/8 of the slots lowers the power drawn meanwhile, so the
-    // clock that photon runs at rises (512^2: 3.50 against 3.68 ms; 3/4:
-    // 3.47 ms; at 5/8 the bulk becomes the bound, 3.78 ms).  Otherwise all
-    // slots (1024^2 at two per SIMD: 7/8 of them take 6.70 against 6.20 ms).
+    // Automatic grid: at one wavefront per SIMD, up to 2^18 pixels and rtol
+    // >= 1e-6, the split FP32 form's bulk finishes well before its longest
+    // FP64 photon, and launching 7/8 of the slots lowers the power drawn
+    // meanwhile, so the clock that photon runs at rises (512^2, a=0.9:
+    // 3.50 against 3.68 ms, 3/4: 3.47 ms, 5/8: the bulk becomes the bound,
+    // 3.78 ms; a=0.99, i=85: 4.24 against 4.43 ms; 362^2: equal).  At tighter
+    // tolerances the FP32 bulk takes many more attempts and bounds the time
+    // (512^2, a=0, rtol 1e-8: 7.84 against 7.29 ms; 256^2, rtol 1e-7: 4.53
+    // against 4.41 ms), and above 2^18 pixels throughput decides: all slots.
     const bool small = pixels <= (int64_t(1) << 18);
-    const double frac = o->grid_fraction > 0.0 ? o->grid_fraction : (mixed && split && waves == 1 && small ? 0.875 : 1.0);
+    const bool loose = P.consts[C_RTOL] >= 1e-6;
+    const double frac =
+        o->grid_fraction > 0.0 ? o->grid_fraction : (mixed && split && waves == 1 && small && loose ? 0.875 : 1.0);
     switch (o->block_size) {
         case 64:
             mixed ? launch_planned_w<64, true>(waves, large, cus, frac, s, P, B, M, w, refill, prio, split, bo)

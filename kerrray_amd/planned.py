@@ -67,9 +67,9 @@ class PlannedOptions:
     split_roles: bool = True
     # Share of the resident-wavefront slots the trace kernel is launched on,
     # in (0, 1]; 0: automatic (7/8 for the split FP32 form at one wavefront
-    # per SIMD up to 2^18 pixels, whose FP32 photons finish well before the
-    # longest FP64 photon, so that fewer active wavefronts leave the clock
-    # higher for it; all slots otherwise).
+    # per SIMD up to 2^18 pixels and rtol >= 1e-6, whose FP32 photons finish
+    # well before the longest FP64 photon, so that fewer active wavefronts
+    # leave the clock higher for it; all slots otherwise).
     grid_fraction: float = 0.0
 
     def kr(self) -> _lib.KrPlannedOpts:
