@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define KR_ABI_VERSION 6
+#define KR_ABI_VERSION 7
 
 typedef enum kr_status {
     KR_OK = 0,
@@ -37,7 +37,8 @@ typedef enum kr_status {
     KR_ERR_DEVICE = 2,      /* HIP runtime failure / no device: BackendUnavailable */
     KR_ERR_RADICAND = 3,    /* negative Carter radicand in an initial condition: ValueError */
     KR_ERR_UNSUPPORTED = 4, /* option the kernels do not implement: ValueError */
-    KR_ERR_NOT_NULL = 5     /* a ZAMO camera's photon is not null: RuntimeError (camera.py:290-291,366-371) */
+    KR_ERR_NOT_NULL = 5,    /* a ZAMO camera's photon is not null: RuntimeError (camera.py:290-291,366-371) */
+    KR_ERR_REPAIR = 6       /* the precision form's FP64 mask repair hit its round limit: RuntimeError */
 } kr_status;
 
 typedef struct kr_context* kr_handle;
@@ -155,6 +156,10 @@ kr_status kr_integrate_batch_dev(kr_handle h, const kr_params* p, const double* 
 /* Mask edge: a direction whose starting point is outside the captured region
  * (ValueError in raytracing/boundary.py:163-164). */
 #define KR_FLAG_EDGE 16
+/* Planned traces: the FP64 verify/repair of the mask still changed a
+ * pixel's class in its last allowed round, so pixels near that change may
+ * keep their FP32 class (KR_ERR_REPAIR on the host entry points). */
+#define KR_FLAG_REPAIR 32
 
 /* The initial states of every pixel of `img`, row-major (n*n, 8) float64
  * into d_Y0: initial_states(cam, st, reverse) (camera.py:298-381) for a ZAMO
@@ -304,7 +309,7 @@ kr_status kr_batch_destroy(kr_batch b);
  *    edge from the start (with those of the polar strip, see axis_lz), and
  *    after the trace every pixel within band_px of a captured/not-captured
  *    transition of the mask, or whose photon ended neither captured nor
- *    escaped, that was not.
+ *    escaped, that was not (repeated to a fixed point, KR_REPAIR_ROUNDS).
  *  fp32_bulk: 0 = every photon in FP64 (the longest-first schedule only);
  *    1 = other photons take FP32 right-hand sides with the FP64 update,
  *    error norm and step control, switching an attempt to FP64 when
@@ -323,7 +328,8 @@ kr_status kr_batch_destroy(kr_batch b);
  *    then run alone on their SIMD), two otherwise.
  *  split_roles (fp32_bulk only): the first ceil(band / 64) wavefronts trace
  *    only the FP64 band and strip, the others only FP32 photons, so no
- *    wavefront runs both attempts.
+ *    wavefront runs both attempts.  Needs longest_first (which puts the band
+ *    first in the work list); ignored without it.
  *  grid_fraction: share of the resident-wavefront slots (waves_per_simd per
  *    SIMD) the persistent kernel is launched on, in (0, 1].  0: 7/8 for the
  *    split FP32 form at one wavefront per SIMD up to 2^18 pixels and rtol
@@ -351,9 +357,20 @@ typedef struct kr_planned_stats {
     int64_t planned_band;    /* pixels scheduled first and integrated in FP64 (band and polar strip) */
     int64_t planned_inside;  /* other pixels inside the analytic edge */
     int64_t planned_outside; /* other pixels outside it */
-    int64_t repaired;        /* pixels the mask check sent back to FP64 */
+    int64_t repaired;        /* pixels the mask check sent back to FP64, over all repair rounds */
     int64_t fp64_attempts;   /* integration attempts taken in FP64 */
+    int64_t repair_rounds;   /* repair rounds that had pixels to re-trace (0 .. KR_REPAIR_ROUNDS) */
+    int64_t unrepaired;      /* pixels queued for a round past the limit (KR_FLAG_REPAIR when > 0) */
+    int32_t flags;           /* KR_FLAG_* bits of the trace (initial states, repair limit) */
+    int32_t reserved;
 } kr_planned_stats;
+
+/* Verify/repair of a planned trace: after the trace, every pixel not traced
+ * in FP64 that failed or lies within band_px of a captured/not-captured
+ * transition is re-traced in FP64 (round 1); each later round re-traces the
+ * FP32 pixels within band_px of a pixel whose class the previous round
+ * changed, until a round changes no class, for at most this many rounds. */
+#define KR_REPAIR_ROUNDS 4
 
 /* Device workspace of kr_trace_image_planned_dev for an image of `pixels`
  * pixels and an edge table of n_angles radii. */
@@ -370,9 +387,11 @@ kr_status kr_trace_image_planned_dev(kr_handle h, const kr_params* p, const kr_i
                                      double centre_alpha, double centre_beta, const double* d_radii,
                                      int32_t n_angles, const kr_outputs* d_out, void* d_work, size_t work_bytes,
                                      void* stream);
-/* Counters of the last planned trace that used the workspace d_work, read
- * after the work enqueued on `stream` (synchronises that stream). */
+/* Counters and KR_FLAG_* bits of the last planned trace that used the
+ * workspace d_work, read after the work enqueued on `stream` (synchronises
+ * that stream). */
 kr_status kr_planned_read_stats(const void* d_work, size_t work_bytes, kr_planned_stats* stats, void* stream);
+
 
 /* ---------------------------------------------------------------------
  * Differentiation form: escape-direction gradients by the discrete adjoint

@@ -14,11 +14,12 @@ import threading
 
 from kerrray_amd.build import LIB_PATH
 
-KR_OK, KR_ERR_INVALID, KR_ERR_DEVICE, KR_ERR_RADICAND, KR_ERR_UNSUPPORTED, KR_ERR_NOT_NULL = 0, 1, 2, 3, 4, 5
+KR_OK, KR_ERR_INVALID, KR_ERR_DEVICE, KR_ERR_RADICAND, KR_ERR_UNSUPPORTED, KR_ERR_NOT_NULL, KR_ERR_REPAIR = range(7)
 KR_FLOAT64, KR_FLOAT32 = 0, 1
-KR_FLAG_RADICAND, KR_FLAG_NOT_NULL, KR_FLAG_SKY, KR_FLAG_DISK, KR_FLAG_EDGE = 1, 2, 4, 8, 16
+KR_FLAG_RADICAND, KR_FLAG_NOT_NULL, KR_FLAG_SKY, KR_FLAG_DISK, KR_FLAG_EDGE, KR_FLAG_REPAIR = 1, 2, 4, 8, 16, 32
+KR_REPAIR_ROUNDS = 4
 KR_CAMERA_BARDEEN, KR_CAMERA_ZAMO = 0, 1
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 # Every symbol the header declares (checked by the tests).
 EXPORTED = (
@@ -162,6 +163,10 @@ class KrPlannedStats(ctypes.Structure):
         ("planned_outside", ctypes.c_int64),
         ("repaired", ctypes.c_int64),
         ("fp64_attempts", ctypes.c_int64),
+        ("repair_rounds", ctypes.c_int64),
+        ("unrepaired", ctypes.c_int64),
+        ("flags", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -325,6 +330,9 @@ def check(status: int, what: str) -> None:
 
 def raise_for_flags(flags: int) -> None:
     """Raise what a device flag word (``KR_FLAG_*`` bits) stands for, as :func:`check` would."""
+    if flags & KR_FLAG_REPAIR:
+        raise KrError(KR_ERR_REPAIR, "the FP64 verify/repair of the mask did not reach a fixed point within "
+                                     f"{KR_REPAIR_ROUNDS} rounds")
     if flags & KR_FLAG_RADICAND:
         raise ValueError("a Carter radicand is negative beyond round-off at the observer")
     if flags & KR_FLAG_SKY:

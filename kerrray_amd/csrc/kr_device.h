@@ -61,7 +61,7 @@ namespace kr {
 
 // Flag bits a kernel ORs into the caller's flag word (the KR_FLAG_* values of
 // the C ABI), and the camera kinds of an image (KR_CAMERA_*).
-enum : int32_t { FLAG_RADICAND = 1, FLAG_NOT_NULL = 2, FLAG_SKY = 4, FLAG_DISK = 8, FLAG_EDGE = 16 };
+enum : int32_t { FLAG_RADICAND = 1, FLAG_NOT_NULL = 2, FLAG_SKY = 4, FLAG_DISK = 8, FLAG_EDGE = 16, FLAG_REPAIR = 32 };
 enum : int64_t { CAMERA_BARDEEN = 0, CAMERA_ZAMO = 1 };
 
 // Termination codes (src/kerrray/photons/classification.py:60-69).

@@ -130,12 +130,27 @@ inline kr_status validate(kr_context* h, const kr_params* p, int64_t n, kr::Para
     return KR_OK;
 }
 
+// Every BatchResult array of an output block is set (and the disk event's
+// two when the event is on).
+inline kr_status check_outputs(const kr_outputs* o, bool disk) {
+    if (!o) return fail(KR_ERR_INVALID, "outputs struct is NULL");
+    if (!o->Y || !o->lam || !o->n_steps || !o->n_rejected || !o->state || !o->max_null_error ||
+        !o->max_energy_drift || !o->max_lz_drift || !o->max_carter_drift)
+        return fail(KR_ERR_INVALID, "an output array is NULL");
+    if (disk && (!o->event_Y || !o->event_hit))
+        return fail(KR_ERR_INVALID, "disk event enabled but event_Y/event_hit is NULL");
+    return KR_OK;
+}
+
 // The error a device flag word stands for (KR_FLAG_* bits; 0 is KR_OK).
 inline kr_status flag_status(int32_t flags) {
     static_assert(KR_FLAG_RADICAND == kr::FLAG_RADICAND && KR_FLAG_NOT_NULL == kr::FLAG_NOT_NULL &&
                       KR_FLAG_SKY == kr::FLAG_SKY && KR_FLAG_DISK == kr::FLAG_DISK && KR_FLAG_EDGE == kr::FLAG_EDGE &&
-                      KR_CAMERA_ZAMO == kr::CAMERA_ZAMO,
+                      KR_FLAG_REPAIR == kr::FLAG_REPAIR && KR_CAMERA_ZAMO == kr::CAMERA_ZAMO,
                   "flag and camera values of the ABI and the kernels");
+    if (flags & KR_FLAG_REPAIR)
+        return fail(KR_ERR_REPAIR,
+                    "the FP64 verify/repair of the mask did not reach a fixed point within its round limit");
     if (flags & KR_FLAG_RADICAND)
         return fail(KR_ERR_RADICAND, "a Carter radicand is negative beyond round-off at the observer");
     if (flags & KR_FLAG_SKY)

@@ -19,7 +19,10 @@
 //     ended neither captured nor escaped, that was not traced in FP64 goes
 //     on a repair list.
 //  4. repair: the listed pixels are traced again with the exact FP64 kernel
-//     and their outputs overwritten.
+//     and their outputs overwritten; the FP32 pixels near any of them whose
+//     class changed go on the next round's list, for up to REPAIR_ROUNDS
+//     rounds (kr_repair.h).  A class change in the last round raises
+//     FLAG_REPAIR.
 // With the FP32 bulk off this is the FP64 trace with a better schedule: the
 // results equal kr_trace_image's bit for bit.
 #include <hip/hip_runtime.h>
@@ -33,6 +36,7 @@
 #include "kr_device.h"
 #include "kr_host.h"
 #include "kr_mixed.h"
+#include "kr_repair.h"
 
 namespace kr {
 
@@ -64,14 +68,13 @@ struct PlanCounters {
     unsigned long long inside;         // pixels inside the analytic edge, outside bin 0
     unsigned long long work;           // work-list counter of the persistent kernel
     unsigned long long work_fp32;      // second counter: the FP32 part of the list when roles are split
-    unsigned long long repair;         // repair-list length
     unsigned long long fp64_attempts;
-    unsigned long long repaired;       // pixels repaired over all rounds
+    // Length of the repair list of each round; listed[REPAIR_ROUNDS] counts
+    // the pixels the last round would have queued for one more.
+    unsigned long long listed[REPAIR_ROUNDS + 1];
 };
-
-// A repaired pixel may change class and so move the boundary; a second
-// verify/repair round covers its new neighbours.
-constexpr int REPAIR_ROUNDS = 2;
+constexpr int PLAN_COUNTER_WORDS = static_cast<int>(sizeof(PlanCounters) / 8);
+static_assert(PLAN_COUNTER_WORDS <= 64, "planned_reset_kernel clears the counters with one wavefront");
 
 struct PixelPlan {
     int bin;
@@ -323,60 +326,55 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
     }
 }
 
-// Pixels that must be (and were not) traced in FP64.
-__global__ void verify_kernel(const int64_t* state, const uint8_t* used_fp64, int64_t n, int band_px,
-                              int64_t* repair, PlanCounters* c) {
+// The verify pass: round 0's repair list (kr_repair.h).  Each lane marks
+// its own pixel queued; no lane reads another's mark.
+__global__ void verify_kernel(const int64_t* state, uint8_t* marks, int64_t n, int band_px, int32_t* list,
+                              PlanCounters* c) {
     const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= n * n) return;
-    if (used_fp64[idx]) return;
-    const int64_t s = state[idx];
-    const bool cap = s == CAPTURED;
-    bool need = s != CAPTURED && s != ESCAPED;
-    const int64_t i = idx / n, j = idx - (idx / n) * n;
-    for (int di = -band_px; di <= band_px && !need; ++di) {
-        const int64_t ii = i + di;
-        if (ii < 0 || ii >= n) continue;
-        for (int dj = -band_px; dj <= band_px; ++dj) {
-            const int64_t jj = j + dj;
-            if (jj < 0 || jj >= n) continue;
-            if ((state[ii * n + jj] == CAPTURED) != cap) {
-                need = true;
-                break;
-            }
-        }
-    }
-    if (need) {
-        repair[atomicAdd(&c->repair, 1ull)] = idx;
-        atomicAdd(&c->repaired, 1ull);
+    if (verify_pixel(state, marks, n, band_px, idx)) {
+        marks[idx] = MARK_QUEUED;
+        list[atomicAdd(&c->listed[0], 1ull)] = static_cast<int32_t>(idx);
     }
 }
 
-__global__ void repair_reset_kernel(PlanCounters* c) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) c->repair = 0ull;
-}
-
+// Repair round `round`: re-traces its list in exact FP64 and queues, for the
+// next round, the unmarked pixels within band_px of every pixel whose class
+// changed.  Pushes past the last round only count, and raise FLAG_REPAIR.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void repair_kernel(
-    Params P, BardeenImage B, const int64_t* repair, const PlanCounters* c, uint8_t* used_fp64, RowsOut<double> out) {
-    const int64_t count = static_cast<int64_t>(c->repair);
+    Params P, BardeenImage B, int round, int band_px, const int32_t* list, int32_t* next, PlanCounters* c,
+    uint8_t* marks, RowsOut<double> out, int32_t* flags) {
+    const int64_t count = static_cast<int64_t>(c->listed[round]);
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
     for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < count; p += stride) {
-        const int64_t idx = repair[p];
+        const int64_t idx = list[p];
         const int64_t i = idx / B.n, j = idx - i * B.n;
+        const bool was_captured = out.state[idx] == CAPTURED;
         double y0[8];
         image_state(B, pixel_centre(B.fov, j, B.n), -pixel_centre(B.fov, i, B.n), y0);
         RayState<double> R;
         ray_init<double, true>(P, y0, R);
         while (R.state == RUNNING) ray_attempt<double, true, false>(P, R);
         store_row(out, idx, R);
-        used_fp64[idx] = 1;
+        mark_or(marks, idx, MARK_FP64);
+        if ((R.state == CAPTURED) == was_captured) continue;
+        for_each_neighbour(B.n, band_px, idx, [&](int64_t pos) {
+            if (mark_or(marks, pos, MARK_QUEUED) != 0) return;  // FP64 already, or listed before
+            const unsigned long long slot = atomicAdd(&c->listed[round + 1], 1ull);
+            if (round + 1 < REPAIR_ROUNDS)
+                next[slot] = static_cast<int32_t>(pos);
+            else
+                atomicOr(flags, FLAG_REPAIR);
+        });
     }
 }
 
 __global__ void planned_reset_kernel(PlanCounters* c, int32_t* flags) {
-    if (blockIdx.x == 0 && threadIdx.x < sizeof(PlanCounters) / 8)
+    if (blockIdx.x == 0 && threadIdx.x < PLAN_COUNTER_WORDS)
         reinterpret_cast<unsigned long long*>(c)[threadIdx.x] = 0ull;
     if (blockIdx.x == 0 && threadIdx.x == 0 && flags) *flags = 0;
 }
+
 
 }  // namespace kr
 
@@ -389,31 +387,36 @@ struct Work {
     int32_t* flags;
     double* radii;
     int64_t* work;
-    int64_t* repair;
-    uint8_t* used_fp64;
+    int32_t* lists[2];  // repair lists of even and odd rounds
+    uint8_t* used_fp64; // per-pixel marks (kr_repair.h); the plan's class bytes before the trace
 };
 
-// pixels < 0: unknown here, the outside count is then left at -1.
-void fill_stats(const PlanCounters& pc, int64_t pixels, kr_planned_stats* stats) {
+void fill_stats(const PlanCounters& pc, int32_t flags, kr_planned_stats* stats) {
     stats->planned_band = static_cast<int64_t>(pc.seg[0]);
     stats->planned_inside = static_cast<int64_t>(pc.inside);
     int64_t planned = 0;
     for (int b = 0; b < NBINS; ++b) planned += static_cast<int64_t>(pc.seg[b]);
     stats->planned_outside = planned - stats->planned_band - stats->planned_inside;
-    (void)pixels;
-    stats->repaired = static_cast<int64_t>(pc.repaired);
+    stats->repaired = 0;
+    stats->repair_rounds = 0;
+    for (int r = 0; r < REPAIR_ROUNDS; ++r) {
+        stats->repaired += static_cast<int64_t>(pc.listed[r]);
+        if (pc.listed[r]) stats->repair_rounds = r + 1;
+    }
+    stats->unrepaired = static_cast<int64_t>(pc.listed[REPAIR_ROUNDS]);
     stats->fp64_attempts = static_cast<int64_t>(pc.fp64_attempts);
+    stats->flags = flags;
+    stats->reserved = 0;
 }
 
-size_t workspace_bytes(int64_t pixels, int32_t n_angles) {
-    host::Carver c;
-    c.take(sizeof(PlanCounters));
-    c.take(64);
-    c.take(static_cast<int64_t>(n_angles) * 8);
-    c.take(pixels * 8);
-    c.take(pixels * 8);
-    c.take(pixels);
-    return c.total;
+// Workspace layout, 256-byte aligned parts: counters, flag word, radii, work
+// list (i64 per pixel), two repair lists (i32 per pixel each), marks (u8).
+constexpr size_t part(int64_t bytes) { return (static_cast<size_t>(bytes) + 255) & ~static_cast<size_t>(255); }
+constexpr size_t FLAGS_OFFSET = part(sizeof(PlanCounters));
+
+constexpr size_t workspace_bytes(int64_t pixels, int32_t n_angles) {
+    return part(sizeof(PlanCounters)) + part(64) + part(static_cast<int64_t>(n_angles) * 8) + part(pixels * 8) +
+           2 * part(pixels * 4) + part(pixels);
 }
 
 Work carve_work(void* base, int64_t pixels, int32_t n_angles) {
@@ -424,7 +427,8 @@ Work carve_work(void* base, int64_t pixels, int32_t n_angles) {
     w.flags = reinterpret_cast<int32_t*>(b + c.take(64));
     w.radii = reinterpret_cast<double*>(b + c.take(static_cast<int64_t>(n_angles) * 8));
     w.work = reinterpret_cast<int64_t*>(b + c.take(pixels * 8));
-    w.repair = reinterpret_cast<int64_t*>(b + c.take(pixels * 8));
+    w.lists[0] = reinterpret_cast<int32_t*>(b + c.take(pixels * 4));
+    w.lists[1] = reinterpret_cast<int32_t*>(b + c.take(pixels * 4));
     w.used_fp64 = reinterpret_cast<uint8_t*>(b + c.take(pixels));
     return w;
 }
@@ -497,7 +501,8 @@ kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B,
                    d_out.max_null_error, d_out.max_energy_drift, d_out.max_lz_drift, d_out.max_carter_drift};
     const bool mixed = o->fp32_bulk != 0;
     const int refill = o->refill_interval, prio = o->band_priority ? 1 : 0;
-    const int split = o->split_roles ? 1 : 0;
+    // Split roles rely on the band coming first in the work list.
+    const int split = o->split_roles && o->longest_first ? 1 : 0;
     // Automatic occupancy: up to 2^19 pixels the split FP32 form is bounded
     // by its longest FP64 photons, which run fastest alone on their SIMD (one
     // wavefront per SIMD: 4.3 against 5.1 ms at 512^2); above, throughput
@@ -535,14 +540,13 @@ kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B,
                   : launch_planned_w<256, false>(waves, large, cus, frac, s, P, B, M, w, refill, prio, split, bo);
     }
     KR_HIP(hipGetLastError());
+    hipLaunchKernelGGL(verify_kernel, dim3(pblocks), dim3(256), 0, s, d_out.state, w.used_fp64, B.n, o->band_px,
+                       w.lists[0], w.counters);
+    KR_HIP(hipGetLastError());
     for (int round = 0; round < REPAIR_ROUNDS; ++round) {
-        hipLaunchKernelGGL(repair_reset_kernel, dim3(1), dim3(64), 0, s, w.counters);
-        KR_HIP(hipGetLastError());
-        hipLaunchKernelGGL(verify_kernel, dim3(pblocks), dim3(256), 0, s, d_out.state, w.used_fp64, B.n, o->band_px,
-                           w.repair, w.counters);
-        KR_HIP(hipGetLastError());
-        hipLaunchKernelGGL(repair_kernel, dim3(static_cast<unsigned>(cus * 2)), dim3(256), 0, s, P, B, w.repair,
-                           w.counters, w.used_fp64, bo);
+        hipLaunchKernelGGL(repair_kernel, dim3(static_cast<unsigned>(cus * 2)), dim3(256), 0, s, P, B, round,
+                           o->band_px, w.lists[round & 1], w.lists[(round + 1) & 1], w.counters, w.used_fp64, bo,
+                           w.flags);
         KR_HIP(hipGetLastError());
     }
     return KR_OK;
@@ -557,6 +561,7 @@ kr_status check_common(kr_handle h, const kr_params* p, const kr_image* img, con
         return host::fail(KR_ERR_UNSUPPORTED, "planned traces are float64 rk45 without a disk event");
     if ((st = check_opts(o)) != KR_OK) return st;
     if (n_angles < 8 || n_angles > (1 << 20)) return host::fail(KR_ERR_INVALID, "n_angles must be in [8, 2^20]");
+    if (B->n * B->n > (int64_t(1) << 31)) return host::fail(KR_ERR_INVALID, "planned traces take at most 2^31 pixels");
     return KR_OK;
 }
 
@@ -575,8 +580,8 @@ extern "C" kr_status kr_trace_image_planned_dev(kr_handle h, const kr_params* p,
     BardeenImage B;
     kr_status st = check_common(h, p, img, o, n_angles, &P, &B);
     if (st != KR_OK) return st;
-    if (!d_out || !d_out->Y || !d_out->state || !d_radii || !d_work)
-        return host::fail(KR_ERR_INVALID, "an output, radii or workspace pointer is NULL");
+    if ((st = host::check_outputs(d_out, false)) != KR_OK) return st;
+    if (!d_radii || !d_work) return host::fail(KR_ERR_INVALID, "the radii or workspace pointer is NULL");
     if (work_bytes < workspace_bytes(B.n * B.n, n_angles)) return host::fail(KR_ERR_INVALID, "workspace too small");
     KR_HIP(hipSetDevice(h->device));
     Work w = carve_work(d_work, B.n * B.n, n_angles);
@@ -594,9 +599,8 @@ extern "C" kr_status kr_trace_image_planned(kr_handle h, const kr_params* p, con
     BardeenImage B;
     kr_status st = check_common(h, p, img, o, n_angles, &P, &B);
     if (st != KR_OK) return st;
-    if (!out || !out->Y || !out->lam || !out->n_steps || !out->n_rejected || !out->state || !out->max_null_error ||
-        !out->max_energy_drift || !out->max_lz_drift || !out->max_carter_drift || !radii)
-        return host::fail(KR_ERR_INVALID, "an output or radii array is NULL");
+    if ((st = host::check_outputs(out, false)) != KR_OK) return st;
+    if (!radii) return host::fail(KR_ERR_INVALID, "the radii array is NULL");
     KR_HIP(hipSetDevice(h->device));
     const int64_t n = B.n * B.n;
     host::Carver c;
@@ -635,10 +639,8 @@ extern "C" kr_status kr_trace_image_planned(kr_handle h, const kr_params* p, con
     KR_HIP(hipMemcpyAsync(out->max_lz_drift, d.max_lz_drift, n * 8, hipMemcpyDeviceToHost, s));
     KR_HIP(hipMemcpyAsync(out->max_carter_drift, d.max_carter_drift, n * 8, hipMemcpyDeviceToHost, s));
     KR_HIP(hipStreamSynchronize(s));
+    if (stats) fill_stats(pc, flags, stats);
     if (flags) return host::flag_status(flags);
-    if (stats) {
-        fill_stats(pc, B.n * B.n, stats);
-    }
     if (kernel_ms) {
         float ms = 0.f;
         KR_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
@@ -651,11 +653,13 @@ extern "C" kr_status kr_planned_read_stats(const void* d_work, size_t work_bytes
                                            void* stream) {
     host::clear_error();
     if (!d_work || !stats) return host::fail(KR_ERR_INVALID, "workspace or stats is NULL");
-    if (work_bytes < sizeof(PlanCounters)) return host::fail(KR_ERR_INVALID, "workspace is too small");
+    if (work_bytes < FLAGS_OFFSET + 4) return host::fail(KR_ERR_INVALID, "workspace is too small");
     hipStream_t s = static_cast<hipStream_t>(stream);
     PlanCounters pc;
+    int32_t flags = 0;
     KR_HIP(hipMemcpyAsync(&pc, d_work, sizeof(pc), hipMemcpyDeviceToHost, s));
+    KR_HIP(hipMemcpyAsync(&flags, static_cast<const char*>(d_work) + FLAGS_OFFSET, 4, hipMemcpyDeviceToHost, s));
     KR_HIP(hipStreamSynchronize(s));
-    fill_stats(pc, -1, stats);
+    fill_stats(pc, flags, stats);
     return KR_OK;
 }

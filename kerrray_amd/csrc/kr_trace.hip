@@ -380,16 +380,6 @@ kr_status copy_back(kr_handle h, const BatchBuffers& b, const kr_outputs& out, i
     return KR_OK;
 }
 
-kr_status check_outputs(const kr_outputs* o, bool disk) {
-    if (!o) return host::fail(KR_ERR_INVALID, "outputs struct is NULL");
-    if (!o->Y || !o->lam || !o->n_steps || !o->n_rejected || !o->state || !o->max_null_error ||
-        !o->max_energy_drift || !o->max_lz_drift || !o->max_carter_drift)
-        return host::fail(KR_ERR_INVALID, "an output array is NULL");
-    if (disk && (!o->event_Y || !o->event_hit))
-        return host::fail(KR_ERR_INVALID, "disk event enabled but event_Y/event_hit is NULL");
-    return KR_OK;
-}
-
 }  // namespace
 
 extern "C" kr_status kr_integrate_batch_dev(kr_handle h, const kr_params* p, const double* d_Y0, int64_t n,
@@ -399,7 +389,7 @@ extern "C" kr_status kr_integrate_batch_dev(kr_handle h, const kr_params* p, con
     kr_status st = host::validate(h, p, n, &P);
     if (st != KR_OK) return st;
     const bool disk = P.ipar[I_DISK] != 0;
-    if ((st = check_outputs(d_out, disk)) != KR_OK) return st;
+    if ((st = host::check_outputs(d_out, disk)) != KR_OK) return st;
     if (n > 0 && !d_Y0) return host::fail(KR_ERR_INVALID, "Y0 is NULL");
     KR_HIP(hipSetDevice(h->device));
     ICArgs ic{};
@@ -415,7 +405,7 @@ extern "C" kr_status kr_integrate_batch(kr_handle h, const kr_params* p, const d
     kr_status st = host::validate(h, p, n, &P);
     if (st != KR_OK) return st;
     const bool disk = P.ipar[I_DISK] != 0;
-    if ((st = check_outputs(out, disk)) != KR_OK) return st;
+    if ((st = host::check_outputs(out, disk)) != KR_OK) return st;
     if (n > 0 && !Y0) return host::fail(KR_ERR_INVALID, "Y0 is NULL");
     KR_HIP(hipSetDevice(h->device));
     BatchBuffers b;
@@ -444,7 +434,7 @@ extern "C" kr_status kr_trace_image_dev(kr_handle h, const kr_params* p, const k
     if (st != KR_OK) return st;
     Params P;
     if ((st = host::validate(h, p, B.n * B.n, &P)) != KR_OK) return st;
-    if ((st = check_outputs(d_out, P.ipar[I_DISK] != 0)) != KR_OK) return st;
+    if ((st = host::check_outputs(d_out, P.ipar[I_DISK] != 0)) != KR_OK) return st;
     KR_HIP(hipSetDevice(h->device));
     ICArgs ic{};
     ic.B = B;
@@ -487,7 +477,7 @@ kr_status image_host(kr_handle h, const kr_params* p, const kr_image* img, const
     Params P;
     if ((st = host::validate(h, p, n, &P)) != KR_OK) return st;
     const bool events = P.ipar[I_DISK] != 0;
-    if ((st = check_outputs(out, events)) != KR_OK) return st;
+    if ((st = host::check_outputs(out, events)) != KR_OK) return st;
     DiskShade D{};
     if (disk) {
         if ((st = disk_setup(disk, img, P, &D)) != KR_OK) return st;
@@ -553,7 +543,7 @@ extern "C" kr_status kr_render_disk_dev(kr_handle h, const kr_params* p, const k
     const int64_t n = B.n * B.n;
     Params P;
     if ((st = host::validate(h, p, n, &P)) != KR_OK) return st;
-    if ((st = check_outputs(d_out, true)) != KR_OK) return st;
+    if ((st = host::check_outputs(d_out, true)) != KR_OK) return st;
     DiskShade D{};
     if ((st = disk_setup(disk, img, P, &D)) != KR_OK) return st;
     if (!d_intensity || !d_g || !d_r_hit) return host::fail(KR_ERR_INVALID, "intensity, g or r_hit is NULL");
@@ -669,7 +659,7 @@ extern "C" kr_status kr_trace_points_dev(kr_handle h, const kr_params* p, const 
     Params P;
     if ((st = host::validate(h, p, n, &P)) != KR_OK) return st;
     if (P.ipar[I_DISK] != 0) return host::fail(KR_ERR_UNSUPPORTED, "the point entry points have no disk event");
-    if ((st = check_outputs(d_out, false)) != KR_OK) return st;
+    if ((st = host::check_outputs(d_out, false)) != KR_OK) return st;
     if (n > 0 && (!d_alpha || !d_beta)) return host::fail(KR_ERR_INVALID, "alpha/beta is NULL");
     KR_HIP(hipSetDevice(h->device));
     ICArgs ic{};

@@ -9,6 +9,9 @@
 #include "kerrray_hip.h"
 #include "kr_device.h"
 #include "kr_image.h"
+#include "kr_repair.h"
+#include <utility>
+#include <vector>
 #include <cmath>
 #include <cstring>
 
@@ -404,4 +407,42 @@ extern "C" int krh_mask_edge(const uint8_t* mask, int64_t rows, int64_t cols, do
     for (int64_t k = 0; k < n_angles; ++k)
         if (!kr::mask_edge_radius(mask, rows, cols, a0, b0, d, ca, cb, cosk[k], sink[k], n_samples, radii[k])) ++bad;
     return bad;
+}
+
+// The verify/repair rounds of a planned trace (kr_mixed.hip: verify_kernel,
+// repair_kernel) run serially with the same kr_repair.h functions, the FP64
+// trace of a pixel replaced by a given class: state holds the traced classes
+// (updated in place), state64 the FP64 class of every pixel, marks the mark
+// bytes (MARK_FP64 for pixels traced in FP64).  listed[r] (max_rounds + 1
+// entries) receives round r's list length, listed[max_rounds] the pushes
+// past the last round.  Returns FLAG_REPAIR when the limit was hit, else 0.
+extern "C" int krh_repair_rounds(int64_t n, int band_px, int64_t* state, const int64_t* state64, uint8_t* marks,
+                                 int max_rounds, int64_t* listed) {
+    std::vector<int64_t> cur, next;
+    for (int64_t idx = 0; idx < n * n; ++idx)
+        if (kr::verify_pixel(state, marks, n, band_px, idx)) cur.push_back(idx);
+    for (int64_t idx : cur) marks[idx] = kr::MARK_QUEUED;
+    int flags = 0;
+    for (int round = 0; round < max_rounds; ++round) {
+        listed[round] = static_cast<int64_t>(cur.size());
+        int64_t pushed = 0;
+        next.clear();
+        for (int64_t idx : cur) {
+            const bool was_captured = state[idx] == kr::CAPTURED;
+            state[idx] = state64[idx];
+            kr::mark_or(marks, idx, kr::MARK_FP64);
+            if ((state[idx] == kr::CAPTURED) == was_captured) continue;
+            kr::for_each_neighbour(n, band_px, idx, [&](int64_t pos) {
+                if (kr::mark_or(marks, pos, kr::MARK_QUEUED) != 0) return;
+                ++pushed;
+                if (round + 1 < max_rounds)
+                    next.push_back(pos);
+                else
+                    flags |= kr::FLAG_REPAIR;
+            });
+        }
+        listed[round + 1] = pushed;
+        std::swap(cur, next);
+    }
+    return flags;
 }

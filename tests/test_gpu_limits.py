@@ -85,3 +85,30 @@ def test_render_disk_needs_matching_event_constants(hip) -> None:
                                        ctypes.byref(kd), ctypes.byref(out.outputs), *(a.ctypes.data for a in arrays),
                                        None)
     assert status == _lib.KR_ERR_INVALID and "r_in" in _lib.last_error()
+
+
+@pytest.mark.parametrize("missing", ["lam", "max_carter_drift", "n_rejected"])
+def test_planned_device_trace_needs_every_output_array(hip, missing) -> None:
+    """The planned kernels store every BatchResult array of a row, so each must be given."""
+    import torch
+
+    from bench import device_outputs
+    from kerrray_amd.backends import kernel_params
+    from kerrray_amd.planned import PlannedOptions, edge_table
+    from kerrray_amd.shadow import kr_image
+
+    st = Spacetime(1.0, 0.9)
+    integ, term = options(rtol=1e-6, atol=1e-8)
+    n = 16 * 16
+    bufs, outs = device_outputs(torch, n, torch.float64)
+    setattr(outs, missing, None)
+    table = edge_table(st, 60.0)
+    radii = torch.from_numpy(table.radii).cuda()
+    L = _lib.lib()
+    nbytes = L.kr_planned_workspace_bytes(n, table.n_angles)
+    work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    p, img, ko = kernel_params(st, integ, term), kr_image(st, 1000.0, 60.0, 10.0, 16), PlannedOptions().kr()
+    status = L.kr_trace_image_planned_dev(hip.handle.ptr, ctypes.byref(p), ctypes.byref(img), ctypes.byref(ko),
+                                          table.centre[0], table.centre[1], radii.data_ptr(), table.n_angles,
+                                          ctypes.byref(outs), work.data_ptr(), nbytes, None)
+    assert status == _lib.KR_ERR_INVALID and "output" in _lib.last_error()
