@@ -392,6 +392,26 @@ kr_status kr_trace_image_planned_dev(kr_handle h, const kr_params* p, const kr_i
  * that stream). */
 kr_status kr_planned_read_stats(const void* d_work, size_t work_bytes, kr_planned_stats* stats, void* stream);
 
+/* kr_trace_image_planned_dev without a caller workspace and without any
+ * device allocation: the plan, work list, repair lists and counters live in
+ * a static device array of the library (17 MiB on every device the library
+ * runs on, held until the process ends), whose address kr_create finds once
+ * per device.  No counterpart in the reference; it is for callers
+ * that cannot allocate device memory (code under graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Images of at most
+ * KR_NOALLOC_MAX_PIXELS pixels and edge tables of at most
+ * KR_NOALLOC_MAX_ANGLES radii; larger ones are rejected with
+ * KR_ERR_INVALID.  Every call on a device shares that device's array, so
+ * calls that overlap on one device (other streams, other handles, other
+ * threads) race; calls in one stream are ordered and safe.  The trace's
+ * KR_FLAG_* bits are ORed into *d_flags (caller-zeroed device word; may be
+ * NULL).  Results equal kr_trace_image_planned_dev's bit for bit. */
+#define KR_NOALLOC_MAX_PIXELS (1 << 20)
+#define KR_NOALLOC_MAX_ANGLES 8192
+kr_status kr_trace_image_planned_noalloc(kr_handle h, const kr_params* p, const kr_image* img,
+                                         const kr_planned_opts* o, double centre_alpha, double centre_beta,
+                                         const double* d_radii, int32_t n_angles, const kr_outputs* d_out,
+                                         int32_t* d_flags, void* stream);
 
 /* ---------------------------------------------------------------------
  * Differentiation form: escape-direction gradients by the discrete adjoint

@@ -18,6 +18,7 @@ KR_OK, KR_ERR_INVALID, KR_ERR_DEVICE, KR_ERR_RADICAND, KR_ERR_UNSUPPORTED, KR_ER
 KR_FLOAT64, KR_FLOAT32 = 0, 1
 KR_FLAG_RADICAND, KR_FLAG_NOT_NULL, KR_FLAG_SKY, KR_FLAG_DISK, KR_FLAG_EDGE, KR_FLAG_REPAIR = 1, 2, 4, 8, 16, 32
 KR_REPAIR_ROUNDS = 4
+KR_NOALLOC_MAX_PIXELS, KR_NOALLOC_MAX_ANGLES = 1 << 20, 8192
 KR_CAMERA_BARDEEN, KR_CAMERA_ZAMO = 0, 1
 ABI_VERSION = 7
 
@@ -50,6 +51,7 @@ EXPORTED = (
     "kr_trace_image_planned",
     "kr_trace_image_planned_dev",
     "kr_planned_read_stats",
+    "kr_trace_image_planned_noalloc",
     "kr_tape_create",
     "kr_tape_recorded",
     "kr_tape_steps",
@@ -235,6 +237,9 @@ def _declare(L: ctypes.CDLL) -> None:
                                              vp]
     L.kr_planned_read_stats.restype = ctypes.c_int
     L.kr_planned_read_stats.argtypes = [vp, ctypes.c_size_t, P(KrPlannedStats), vp]
+    L.kr_trace_image_planned_noalloc.restype = ctypes.c_int
+    L.kr_trace_image_planned_noalloc.argtypes = [vp, P(KrParams), P(KrImage), P(KrPlannedOpts), ctypes.c_double,
+                                                 ctypes.c_double, vp, ctypes.c_int32, P(KrOutputs), vp, vp]
     L.kr_tape_create.restype = ctypes.c_int
     L.kr_tape_create.argtypes = [vp, P(KrParams), P(KrImage), vp, vp, ctypes.c_int64, ctypes.c_int32, P(KrOutputs),
                                  vp, vp, P(vp)]

@@ -19,6 +19,7 @@ struct kr_context {
     hipEvent_t ev1 = nullptr;
     void* buf = nullptr;
     size_t buf_bytes = 0;
+    void* scratch = nullptr;  // device address of the planned trace's static scratch on this device
 };
 
 #define KR_HIP(expr)                                                                                   \
@@ -169,6 +170,10 @@ inline kr_status bardeen_image(const kr_image* img, bool need_grid, kr::BardeenI
     const char* err = kr::bardeen_image_scalars(img, need_grid, B);
     return err ? fail(KR_ERR_INVALID, err) : KR_OK;
 }
+
+// Finds the device address of the planned trace's static scratch array on
+// the current device (once per device; kr_mixed.hip) for kr_create.
+kr_status planned_scratch_setup(int device, void** address);
 
 }  // namespace host
 }  // namespace kr

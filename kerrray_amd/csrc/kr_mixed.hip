@@ -29,6 +29,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -375,6 +376,10 @@ __global__ void planned_reset_kernel(PlanCounters* c, int32_t* flags) {
     if (blockIdx.x == 0 && threadIdx.x == 0 && flags) *flags = 0;
 }
 
+// ORs the trace's flag word into the caller's.
+__global__ void flags_out_kernel(const int32_t* flags, int32_t* out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && *flags) atomicOr(out, *flags);
+}
 
 }  // namespace kr
 
@@ -565,7 +570,38 @@ kr_status check_common(kr_handle h, const kr_params* p, const kr_image* img, con
     return KR_OK;
 }
 
+// The workspace of kr_trace_image_planned_noalloc: a static device array,
+// one per device, sized for the largest image and edge table it accepts.
+// Host code never uses the array's name as an address (on the host it is
+// only a stand-in); kr_create looks up each device's own copy once, with
+// hipGetSymbolAddress, and the entry point passes that address on.
+constexpr size_t NOALLOC_SCRATCH_BYTES = workspace_bytes(KR_NOALLOC_MAX_PIXELS, KR_NOALLOC_MAX_ANGLES);
+static_assert(NOALLOC_SCRATCH_BYTES < (size_t(18) << 20), "the static scratch is documented as 17 MiB");
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_address[MAX_DEVICES] = {};
+
 }  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+kr_status kr::host::planned_scratch_setup(int device, void** address) {
+    *address = nullptr;
+    if (device < 0 || device >= MAX_DEVICES) return fail(KR_ERR_INVALID, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_address[device]) {
+        int current = -1;
+        KR_HIP(hipGetDevice(&current));
+        if (current != device) return fail(KR_ERR_INVALID, "planned_scratch_setup: the device must be current");
+        void* ptr = nullptr;
+        KR_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(KR_ERR_DEVICE, "hipGetSymbolAddress returned no address for the planned scratch");
+        g_scratch_address[device] = ptr;
+    }
+    *address = g_scratch_address[device];
+    return KR_OK;
+}
 
 extern "C" size_t kr_planned_workspace_bytes(int64_t pixels, int32_t n_angles) {
     return workspace_bytes(pixels, n_angles);
@@ -588,6 +624,34 @@ extern "C" kr_status kr_trace_image_planned_dev(kr_handle h, const kr_params* p,
     hipStream_t s = static_cast<hipStream_t>(stream);
     KR_HIP(hipMemcpyAsync(w.radii, d_radii, static_cast<size_t>(n_angles) * 8, hipMemcpyDeviceToDevice, s));
     return enqueue_planned(h, P, B, o, centre_alpha, centre_beta, n_angles, w, *d_out, s);
+}
+
+extern "C" kr_status kr_trace_image_planned_noalloc(kr_handle h, const kr_params* p, const kr_image* img,
+                                                    const kr_planned_opts* o, double centre_alpha,
+                                                    double centre_beta, const double* d_radii, int32_t n_angles,
+                                                    const kr_outputs* d_out, int32_t* d_flags, void* stream) {
+    host::clear_error();
+    Params P;
+    BardeenImage B;
+    kr_status st = check_common(h, p, img, o, n_angles, &P, &B);
+    if (st != KR_OK) return st;
+    if ((st = host::check_outputs(d_out, false)) != KR_OK) return st;
+    if (!d_radii) return host::fail(KR_ERR_INVALID, "the radii pointer is NULL");
+    const int64_t pixels = B.n * B.n;
+    if (pixels > KR_NOALLOC_MAX_PIXELS || n_angles > KR_NOALLOC_MAX_ANGLES)
+        return host::fail(KR_ERR_INVALID, "kr_trace_image_planned_noalloc takes at most KR_NOALLOC_MAX_PIXELS (2^20) "
+                                          "pixels and KR_NOALLOC_MAX_ANGLES (8192) radii");
+    if (!h->scratch) return host::fail(KR_ERR_DEVICE, "the handle has no planned scratch address");
+    KR_HIP(hipSetDevice(h->device));
+    Work w = carve_work(h->scratch, pixels, n_angles);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    KR_HIP(hipMemcpyAsync(w.radii, d_radii, static_cast<size_t>(n_angles) * 8, hipMemcpyDeviceToDevice, s));
+    if ((st = enqueue_planned(h, P, B, o, centre_alpha, centre_beta, n_angles, w, *d_out, s)) != KR_OK) return st;
+    if (d_flags) {
+        hipLaunchKernelGGL(flags_out_kernel, dim3(1), dim3(64), 0, s, w.flags, d_flags);
+        KR_HIP(hipGetLastError());
+    }
+    return KR_OK;
 }
 
 extern "C" kr_status kr_trace_image_planned(kr_handle h, const kr_params* p, const kr_image* img,

@@ -301,6 +301,13 @@ extern "C" kr_status kr_create(int device, kr_handle* out) {
     KR_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     KR_HIP(hipEventCreate(&h->ev0));
     KR_HIP(hipEventCreate(&h->ev1));
+    if (kr_status st = host::planned_scratch_setup(device, &h->scratch); st != KR_OK) {
+        hipEventDestroy(h->ev1);
+        hipEventDestroy(h->ev0);
+        hipStreamDestroy(h->stream);
+        delete h;
+        return st;
+    }
     *out = h;
     return KR_OK;
 }
