@@ -256,6 +256,54 @@ kr_status kr_bisect_edge(kr_handle h, const kr_params* p, const kr_image* img, d
                          int64_t* nonmonotone, double* kernel_ms);
 
 /* ---------------------------------------------------------------------
+ * Critical impact parameter of equatorial photons: the bracketed search of
+ * bisect_critical_impacts (src/kerrray/validation/kerr_bisection.py:177-300)
+ * on the device.  Each search keeps a bracket [lo, hi], CAPTURED at lo and
+ * ESCAPED at hi, for photons launched inward in the equatorial plane from
+ * r0 with E = 1, Q = 0 and L_z = lz_sign[k] * b (photon_from_constants,
+ * src/kerrray/geodesics/initial_conditions.py:83-112, with sign_r = -1 and
+ * sign_theta = +1).  The endpoints are classified first; then every round
+ * traces n_probe evenly spaced interior impact parameters of every open
+ * search in one launch and narrows the brackets in a second (lo = the
+ * largest captured probe, hi = the smallest escaped one), until each
+ * bracket is no wider than tol or has taken max_iterations rounds.
+ * ------------------------------------------------------------------- */
+typedef struct kr_impact_opts {
+    double r0;             /* launch radius (outside the outer horizon) */
+    double theta;          /* launch polar angle (pi / 2 for the reference's searches) */
+    double cos2_theta;     /* math.cos(theta) ** 2, as the reference computes it */
+    double sin2_theta;     /* math.sin(theta) ** 2 */
+    double tol;            /* bracket width at which a search stops (> 0) */
+    double capture_radius; /* r_+ + horizon_epsilon: the relabelling radius below */
+    int64_t max_iterations;
+    int32_t n_probe;       /* interior probes per search and round, 1 .. 1024 */
+    int32_t horizon_crossing_as_capture; /* 1: OUT_OF_DOMAIN with finite r <= capture_radius counts as CAPTURED */
+} kr_impact_opts;
+
+/* What a search run found.  error: 0 none; 1 an endpoint was not CAPTURED
+ * at lo / ESCAPED at hi; 2 a probe ended neither CAPTURED nor ESCAPED; 3 a
+ * probe escaped below a captured one (the reference raises RuntimeError for
+ * each).  On an error, the brackets are those before the failing round, and
+ * probe_b / probe_state / was_open hold that round (round 0: the endpoint
+ * check, two probes per search). */
+typedef struct kr_impact_result {
+    int64_t rays;      /* photons traced, endpoints included */
+    int64_t max_steps; /* largest accepted-step count of any of them */
+    int64_t crossings; /* photons relabelled CAPTURED */
+    int32_t error;
+    int32_t error_round;
+} kr_impact_result;
+
+/* n_search searches (1 .. 1024) of one spacetime (p's geometry).  lo, hi
+ * (in: the starting bracket; out: the final one) and iterations (out) have
+ * n_search entries; probe_b and probe_state have n_search * max(2, n_probe),
+ * was_open n_search (written on an error only).  Host arrays; kernel_ms may
+ * be NULL. */
+kr_status kr_impact_search(kr_handle h, const kr_params* p, const kr_impact_opts* o, int32_t n_search,
+                           const double* lz_sign, double* lo, double* hi, int64_t* iterations, double* probe_b,
+                           int64_t* probe_state, uint8_t* was_open, kr_impact_result* res, double* kernel_ms);
+
+/* ---------------------------------------------------------------------
  * Ragged batches of images in one persistent launch.  No counterpart in the
  * reference, which traces one image at a time (compute_shadow ->
  * trace_rays, src/kerrray/raytracing/rays.py:72-117); each image here is

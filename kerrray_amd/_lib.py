@@ -41,6 +41,7 @@ EXPORTED = (
     "kr_render_disk",
     "kr_render_disk_dev",
     "kr_bisect_edge",
+    "kr_impact_search",
     "kr_batch_create",
     "kr_batch_photons",
     "kr_batch_run",
@@ -172,6 +173,30 @@ class KrPlannedStats(ctypes.Structure):
     ]
 
 
+class KrImpactOpts(ctypes.Structure):
+    _fields_ = [
+        ("r0", ctypes.c_double),
+        ("theta", ctypes.c_double),
+        ("cos2_theta", ctypes.c_double),
+        ("sin2_theta", ctypes.c_double),
+        ("tol", ctypes.c_double),
+        ("capture_radius", ctypes.c_double),
+        ("max_iterations", ctypes.c_int64),
+        ("n_probe", ctypes.c_int32),
+        ("horizon_crossing_as_capture", ctypes.c_int32),
+    ]
+
+
+class KrImpactResult(ctypes.Structure):
+    _fields_ = [
+        ("rays", ctypes.c_int64),
+        ("max_steps", ctypes.c_int64),
+        ("crossings", ctypes.c_int64),
+        ("error", ctypes.c_int32),
+        ("error_round", ctypes.c_int32),
+    ]
+
+
 class KrTune(ctypes.Structure):
     _fields_ = [
         ("block_size", ctypes.c_int32),
@@ -213,6 +238,9 @@ def _declare(L: ctypes.CDLL) -> None:
         vp, P(KrParams), P(KrImage), ctypes.c_double, ctypes.c_double, vp, vp, ctypes.c_int64, vp, vp,
         ctypes.c_int32, ctypes.c_int32, c_int64_p, c_int64_p, c_double_p,
     ]
+    L.kr_impact_search.restype = ctypes.c_int
+    L.kr_impact_search.argtypes = [vp, P(KrParams), P(KrImpactOpts), ctypes.c_int32, vp, vp, vp, vp, vp, vp, vp,
+                                   P(KrImpactResult), c_double_p]
     L.kr_batch_create.restype = ctypes.c_int
     L.kr_batch_create.argtypes = [vp, P(KrJob), ctypes.c_int32, P(KrTune), P(KrOutputs), P(vp)]
     L.kr_batch_photons.restype = ctypes.c_int64

@@ -32,7 +32,7 @@ from kerrray_amd.geodesics import (
     EventOptions, IntegratorOptions, TerminationOptions, max_attempts,
 )
 from kerrray_amd.geometry import Spacetime, outer_horizon
-from kerrray_amd.photons import AXIS_SIN_TOLERANCE, THETA_DOMAIN_TOLERANCE, BatchResult
+from kerrray_amd.photons import AXIS_SIN_TOLERANCE, THETA_DOMAIN_TOLERANCE, BatchResult, ResultRows
 
 __all__ = [
     "BACKEND_NAMES", "Backend", "BackendUnavailable", "HipBackend", "available_backends", "get_backend",
@@ -118,42 +118,25 @@ def _validate_batch(Y0: ArrayLike) -> NDArray[np.float64]:
     return np.ascontiguousarray(arr)
 
 
-class HostBatch:
-    """Freshly allocated host outputs of one batch and the ``kr_outputs`` view of them."""
+def output_struct(rows: ResultRows) -> _lib.KrOutputs:
+    """The ``kr_outputs`` block pointing at the (C-contiguous) arrays of ``rows``."""
+    o = _lib.KrOutputs()
+    for name, _ in _lib.KrOutputs._fields_:  # the struct's fields are the store's array names
+        arr = getattr(rows, name)
+        if arr is not None:
+            if not arr.flags.c_contiguous:
+                raise ValueError(f"output array {name} must be C-contiguous")
+            setattr(o, name, arr.ctypes.data)
+    return o
+
+
+class HostBatch(ResultRows):
+    """Freshly allocated host outputs of one batch and their ``kr_outputs`` view (``outputs``)."""
 
     def __init__(self, n: int, dtype: np.dtype, disk: bool) -> None:
-        self.Y = np.zeros((n, STATE_SIZE), dtype=dtype)
-        self.lam = np.zeros(n)
-        self.n_steps = np.zeros(n, np.int64)
-        self.n_rejected = np.zeros(n, np.int64)
-        self.state = np.zeros(n, np.int64)
-        self.maxima = [np.zeros(n) for _ in range(4)]
-        self.event_Y = np.zeros((n, STATE_SIZE), dtype=dtype) if disk else None
-        self.event_hit = np.zeros(n, np.uint8) if disk else None
-        o = _lib.KrOutputs()
-        o.Y = self.Y.ctypes.data
-        o.lam = self.lam.ctypes.data
-        o.n_steps = self.n_steps.ctypes.data
-        o.n_rejected = self.n_rejected.ctypes.data
-        o.state = self.state.ctypes.data
-        o.max_null_error = self.maxima[0].ctypes.data
-        o.max_energy_drift = self.maxima[1].ctypes.data
-        o.max_lz_drift = self.maxima[2].ctypes.data
-        o.max_carter_drift = self.maxima[3].ctypes.data
-        o.event_Y = self.event_Y.ctypes.data if disk else None
-        o.event_hit = self.event_hit.ctypes.data if disk else None
-        self.outputs = o
+        super().__init__(n, dtype, disk)
         self.disk = disk
-
-    def result(self, runtime_s: float) -> BatchResult:
-        return BatchResult(
-            Y=self.Y, state=self.state, n_steps=self.n_steps, lam=self.lam,
-            max_null_error=self.maxima[0], max_energy_drift=self.maxima[1],
-            max_lz_drift=self.maxima[2], max_carter_drift=self.maxima[3], runtime_s=runtime_s,
-            event_Y=self.event_Y if self.disk else None,
-            event_hit=self.event_hit.astype(bool) if self.disk else None,
-            n_rejected=self.n_rejected,
-        )
+        self.outputs = output_struct(self)
 
 
 class HipBackend:
@@ -194,21 +177,32 @@ class HipBackend:
             raise ValueError("method 'dop853' is available in integrate() only; the hip backend supports 'rk4' and 'rk45'")
         t0 = time.perf_counter()
         y0 = _validate_batch(Y0)
+        rows = ResultRows(y0.shape[0], integ.np_dtype, events is not None and events.disk_plane)
+        self.integrate_into(st, y0, integ, term, events, rows)
+        return rows.result(time.perf_counter() - t0)
+
+    def integrate_into(self, st: Spacetime, Y0: ArrayLike, integ: IntegratorOptions, term: TerminationOptions,
+                       events: EventOptions | None, rows: ResultRows) -> None:
+        """:meth:`integrate_batch` writing into the given row store (as many rows as ``Y0``)."""
+        if integ.method == "dop853":
+            raise ValueError("method 'dop853' is available in integrate() only; the hip backend supports 'rk4' and 'rk45'")
+        y0 = _validate_batch(Y0)
         n = y0.shape[0]
         disk = events is not None and events.disk_plane
+        if rows.n != n or rows.Y.dtype != integ.np_dtype or (disk and rows.event_Y is None):
+            raise ValueError("the row store does not match the batch (rows, working dtype or event arrays)")
         params = kernel_params(st, integ, term, events)
-        out = HostBatch(n, integ.np_dtype, disk)
+        outputs = output_struct(rows)
         ms = ctypes.c_double(0.0)
         with self.handle._lock:
             _lib.check(
                 _lib.lib().kr_integrate_batch(
-                    self.handle.ptr, ctypes.byref(params), y0.ctypes.data, n, ctypes.byref(out.outputs),
+                    self.handle.ptr, ctypes.byref(params), y0.ctypes.data, n, ctypes.byref(outputs),
                     ctypes.byref(ms),
                 ),
                 "kr_integrate_batch",
             )
         self.last_kernel_ms = ms.value
-        return out.result(time.perf_counter() - t0)
 
     def warm_up(self, st: Spacetime, integ: IntegratorOptions, term: TerminationOptions) -> float:
         """One short ray (``lambda_max = step_size``); returns the seconds it took (numba_backend.py:239-253)."""

@@ -867,6 +867,41 @@ KR_HD bool bardeen_state(const BardeenImage& B, double alpha, double beta, doubl
     return ok;
 }
 
+// Launch constants of a critical-impact search: photons from (r0, theta)
+// with E = 1, Q = 0 (kerr_bisection.py:117-118 -> photon_from_constants).
+struct ImpactLaunch {
+    double m, a;        // mass, spin * mass
+    double r0, theta;
+    double c2, s2;      // math.cos(theta) ** 2, math.sin(theta) ** 2
+};
+
+// photon_from_constants(st, r0, theta, E = 1, L_z, Q = 0, sign_r = -1,
+// sign_theta = +1) with carter_potentials (initial_conditions.py:50-112),
+// operation for operation (Python's x ** 2 on a float is x * x exactly).
+// Returns false when a radicand is negative beyond round-off.
+KR_HD bool impact_state(const ImpactLaunch& L, double lz, double (&y)[8]) {
+    const double e = 1.0, q = 0.0, r = L.r0, a = L.a;
+    const double dlt = r * r - 2.0 * L.m * r + a * a;
+    const double term1 = e * (r * r + a * a) - a * lz;
+    const double d = lz - a * e;
+    const double term2 = dlt * (q + d * d);
+    const double big_r = term1 * term1 - term2;
+    const double scale_r = term1 * term1 + fabs(term2);
+    const double lz_term = lz == 0.0 ? 0.0 : lz * lz / L.s2;
+    const double big_theta = q - L.c2 * (lz_term - a * a * e * e);
+    const double scale_theta = fabs(q) + L.c2 * (lz_term + a * a * e * e);
+    const bool ok = !(big_r < -1e-12 * pymax(scale_r, 1.0)) && !(big_theta < -1e-12 * pymax(scale_theta, 1.0));
+    y[0] = 0.0;
+    y[1] = r;
+    y[2] = L.theta;
+    y[3] = 0.0;
+    y[4] = -e;
+    y[5] = -1.0 * sqrt(pymax(big_r, 0.0)) / dlt;
+    y[6] = 1.0 * sqrt(pymax(big_theta, 0.0));
+    y[7] = lz;
+    return ok;
+}
+
 // Camera.pixel_coordinates (camera.py:206-222): fov (2 j + 1 - n) / n.
 KR_HD double pixel_centre(double fov, int64_t j, int64_t n) {
     const double nd = static_cast<double>(n);

@@ -6,11 +6,13 @@
 // per prange iteration) behind numba_backend.py:131-256.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "kerrray_hip.h"
 #include "kr_device.h"
@@ -151,6 +153,117 @@ __global__ void bisect_update_kernel(double* lo, double* hi, const uint8_t* esca
         hi[k] = probe_radius(l, h, first, n_probe);
         if (first > 0) lo[k] = probe_radius(l, h, first - 1, n_probe);
     }
+}
+
+// Critical-impact search (kr_impact_search).  Searches live on the device;
+// a round is one probe launch (lane = (search, probe)) and one update launch
+// (one workgroup, thread = search).  After an error every later launch
+// returns at once, leaving the failing round's probes and open flags intact.
+struct ImpactSearchDev {
+    double lo, hi;
+    double lz_sign;
+    int64_t iterations;
+    int32_t open;
+    int32_t pad;
+};
+
+struct ImpactCounters {
+    unsigned long long rays, max_steps, crossings;
+    int32_t error, error_round;
+    int32_t round;  // rounds enqueued so far (0: the endpoint check)
+    int32_t pad;
+};
+
+struct ImpactCfg {
+    ImpactLaunch L;
+    double tol, r_capture;
+    int64_t max_iterations;
+    int32_t n_search, n_probe, relabel;
+};
+
+template <typename T, bool ADAPTIVE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void impact_probe_kernel(
+    Params P, ImpactCfg C, ImpactSearchDev* s, ImpactCounters* c, double* probe_b, int64_t* probe_state,
+    int endpoints, int32_t* flags) {
+    if (c->error) return;
+    const int per = endpoints ? 2 : C.n_probe;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<int64_t>(C.n_search) * per) return;
+    const int k = static_cast<int>(t / per), j = static_cast<int>(t - static_cast<int64_t>(k) * per);
+    const ImpactSearchDev S = s[k];
+    if (!endpoints && !S.open) return;
+    // kerr_bisection.py:233: lo + width * (j + 1) / (n_probe + 1)
+    const double b = endpoints ? (j == 0 ? S.lo : S.hi)
+                               : S.lo + (S.hi - S.lo) * (static_cast<double>(j + 1) / static_cast<double>(per + 1));
+    double y0[8];
+    if (!impact_state(C.L, S.lz_sign * b, y0)) atomicOr(flags, FLAG_RADICAND);
+    RayResult<T> r;
+    integrate_ray<T, ADAPTIVE, false>(P, y0, r);
+    int64_t code = r.state;
+    const double r_end = static_cast<double>(r.y[1]);
+    if (C.relabel && code == OUT_OF_DOMAIN && isfinite(r_end) && r_end <= C.r_capture) {  // kerr_bisection.py:121-126
+        code = CAPTURED;
+        atomicAdd(&c->crossings, 1ull);
+    }
+    probe_b[t] = b;
+    probe_state[t] = code;
+    atomicAdd(&c->rays, 1ull);
+    atomicMax(&c->max_steps, static_cast<unsigned long long>(r.n_steps));
+}
+
+// One workgroup; thread k owns search k.  Checks the round, then narrows the
+// brackets (kerr_bisection.py:143-156,240-256) only if no search failed.
+__global__ __launch_bounds__(1024) void impact_update_kernel(ImpactCfg C, ImpactSearchDev* s, ImpactCounters* c,
+                                                             const double* probe_b, const int64_t* probe_state,
+                                                             int endpoints) {
+    __shared__ int bad_state, bad_order;
+    if (c->error) return;
+    const int k = threadIdx.x;
+    if (k == 0) bad_state = bad_order = 0;
+    __syncthreads();
+    const int per = endpoints ? 2 : C.n_probe;
+    ImpactSearchDev S = k < C.n_search ? s[k] : ImpactSearchDev{};
+    const bool active = k < C.n_search && (endpoints || S.open);
+    double cap_max = 0.0, esc_min = 0.0;
+    bool any_cap = false, any_esc = false;
+    if (active) {
+        for (int j = 0; j < per; ++j) {
+            const int64_t code = probe_state[static_cast<int64_t>(k) * per + j];
+            const double b = probe_b[static_cast<int64_t>(k) * per + j];
+            if (endpoints) {
+                if (code != (j == 0 ? CAPTURED : ESCAPED)) atomicOr(&bad_state, 1);
+                continue;
+            }
+            if (code == CAPTURED) {
+                cap_max = any_cap ? pymax(cap_max, b) : b;
+                any_cap = true;
+            } else if (code == ESCAPED) {
+                esc_min = any_esc ? pymin(esc_min, b) : b;
+                any_esc = true;
+            } else {
+                atomicOr(&bad_state, 1);
+            }
+        }
+        if (any_cap && any_esc && cap_max > esc_min) atomicOr(&bad_order, 1);
+    }
+    __syncthreads();
+    if (bad_state || bad_order) {
+        if (k == 0) {
+            c->error = endpoints ? 1 : (bad_state ? 2 : 3);
+            c->error_round = c->round;
+        }
+        return;
+    }
+    if (active && !endpoints) {
+        if (any_cap) S.lo = pymax(S.lo, cap_max);
+        if (any_esc) S.hi = pymin(S.hi, esc_min);
+        S.iterations += 1;
+    }
+    if (k < C.n_search) {
+        S.open = (S.hi - S.lo > C.tol) && (S.iterations < C.max_iterations);
+        s[k] = S;
+    }
+    if (k == 0) c->round += 1;
 }
 
 // Initial states of every pixel of an image, row-major (n*n, 8): one lane per
@@ -757,6 +870,149 @@ extern "C" kr_status kr_bisect_edge(kr_handle h, const kr_params* p, const kr_im
     if (failures) *failures = static_cast<int64_t>(cnt[0]);
     if (nonmonotone) *nonmonotone = static_cast<int64_t>(cnt[1]);
     if (flags) return host::flag_status(flags);
+    if (kernel_ms) {
+        float ms = 0.f;
+        KR_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        *kernel_ms = ms;
+    }
+    return KR_OK;
+}
+
+namespace {
+
+void launch_impact_probe(const Params& P, int dtype, const ImpactCfg& C, ImpactSearchDev* s, ImpactCounters* c,
+                         double* pb, int64_t* ps, int endpoints, int32_t* flags, hipStream_t st) {
+    const int64_t lanes = static_cast<int64_t>(C.n_search) * (endpoints ? 2 : C.n_probe);
+    const dim3 grid(static_cast<unsigned>((lanes + 255) / 256)), block(256);
+    const bool adaptive = P.ipar[I_ADAPTIVE] != 0;
+    if (dtype == KR_FLOAT64) {
+        if (adaptive)
+            hipLaunchKernelGGL((impact_probe_kernel<double, true>), grid, block, 0, st, P, C, s, c, pb, ps, endpoints,
+                               flags);
+        else
+            hipLaunchKernelGGL((impact_probe_kernel<double, false>), grid, block, 0, st, P, C, s, c, pb, ps, endpoints,
+                               flags);
+    } else {
+        if (adaptive)
+            hipLaunchKernelGGL((impact_probe_kernel<float, true>), grid, block, 0, st, P, C, s, c, pb, ps, endpoints,
+                               flags);
+        else
+            hipLaunchKernelGGL((impact_probe_kernel<float, false>), grid, block, 0, st, P, C, s, c, pb, ps, endpoints,
+                               flags);
+    }
+}
+
+}  // namespace
+
+extern "C" kr_status kr_impact_search(kr_handle h, const kr_params* p, const kr_impact_opts* o, int32_t n_search,
+                                      const double* lz_sign, double* lo, double* hi, int64_t* iterations,
+                                      double* probe_b, int64_t* probe_state, uint8_t* was_open, kr_impact_result* res,
+                                      double* kernel_ms) {
+    host::clear_error();
+    if (!o || !res) return host::fail(KR_ERR_INVALID, "options or result is NULL");
+    if (n_search < 1 || n_search > 1024) return host::fail(KR_ERR_INVALID, "n_search must be in [1, 1024]");
+    if (o->n_probe < 1 || o->n_probe > 1024) return host::fail(KR_ERR_INVALID, "n_probe must be in [1, 1024]");
+    if (!(o->tol > 0.0) || o->max_iterations < 1)
+        return host::fail(KR_ERR_INVALID, "need tol > 0 and max_iterations >= 1");
+    if (o->horizon_crossing_as_capture != 0 && o->horizon_crossing_as_capture != 1)
+        return host::fail(KR_ERR_INVALID, "horizon_crossing_as_capture must be 0 or 1");
+    if (!lz_sign || !lo || !hi || !iterations || !probe_b || !probe_state || !was_open)
+        return host::fail(KR_ERR_INVALID, "an array is NULL");
+    const int per = o->n_probe > 2 ? o->n_probe : 2;
+    const int64_t lanes = static_cast<int64_t>(n_search) * per;
+    Params P;
+    kr_status st = host::validate(h, p, lanes, &P);
+    if (st != KR_OK) return st;
+    if (P.ipar[I_DISK] != 0) return host::fail(KR_ERR_UNSUPPORTED, "the impact search has no disk event");
+    const double m = P.geom[G_MASS], a = P.geom[G_A];
+    if (!(o->r0 > m + std::sqrt(m * m - P.geom[G_A2])) || !std::isfinite(o->r0))
+        return host::fail(KR_ERR_INVALID, "r0 must lie outside the outer horizon");
+    for (int k = 0; k < n_search; ++k) {
+        if (lz_sign[k] != 1.0 && lz_sign[k] != -1.0) return host::fail(KR_ERR_INVALID, "lz_sign must be +1 or -1");
+        if (!(lo[k] < hi[k])) return host::fail(KR_ERR_INVALID, "each bracket needs lo < hi");
+    }
+    ImpactCfg C{};
+    C.L = ImpactLaunch{m, a, o->r0, o->theta, o->cos2_theta, o->sin2_theta};
+    C.tol = o->tol;
+    C.r_capture = o->capture_radius;
+    C.max_iterations = o->max_iterations;
+    C.n_search = n_search;
+    C.n_probe = o->n_probe;
+    C.relabel = o->horizon_crossing_as_capture;
+    KR_HIP(hipSetDevice(h->device));
+    host::Carver cv;
+    const size_t os = cv.take(n_search * static_cast<int64_t>(sizeof(ImpactSearchDev))),
+                 oc = cv.take(sizeof(ImpactCounters)), ob = cv.take(lanes * 8), ost = cv.take(lanes * 8),
+                 of = cv.take(4);
+    if ((st = host::reserve(h, cv.total)) != KR_OK) return st;
+    char* base = static_cast<char*>(h->buf);
+    ImpactSearchDev* d_s = reinterpret_cast<ImpactSearchDev*>(base + os);
+    ImpactCounters* d_c = reinterpret_cast<ImpactCounters*>(base + oc);
+    double* d_b = reinterpret_cast<double*>(base + ob);
+    int64_t* d_state = reinterpret_cast<int64_t*>(base + ost);
+    int32_t* d_flags = reinterpret_cast<int32_t*>(base + of);
+    std::vector<ImpactSearchDev> S(n_search);
+    double widest = 0.0;
+    for (int k = 0; k < n_search; ++k) {
+        S[k] = ImpactSearchDev{lo[k], hi[k], lz_sign[k], 0, 0, 0};
+        widest = std::max(widest, hi[k] - lo[k]);
+    }
+    ImpactCounters zero{};
+    hipStream_t s = h->stream;
+    KR_HIP(hipMemcpyAsync(d_s, S.data(), S.size() * sizeof(ImpactSearchDev), hipMemcpyHostToDevice, s));
+    KR_HIP(hipMemcpyAsync(d_c, &zero, sizeof(zero), hipMemcpyHostToDevice, s));
+    KR_HIP(fill_i32(d_flags, 1, s));
+    KR_HIP(hipEventRecord(h->ev0, s));
+    // Rounds: the bracket shrinks (n_probe + 1)-fold per round, so the host
+    // enqueues that many (plus a margin) without waiting, then checks.
+    const double shrink = std::log(static_cast<double>(o->n_probe) + 1.0);
+    const double want = widest > o->tol ? std::ceil(std::log(widest / o->tol) / shrink) + 2.0 : 1.0;
+    int64_t batch = static_cast<int64_t>(std::min(want, 4096.0));
+    int64_t enqueued = 0;
+    ImpactCounters cnt{};
+    for (int pass = 0;; ++pass) {
+        if (pass == 0) {
+            launch_impact_probe(P, p->dtype, C, d_s, d_c, d_b, d_state, 1, d_flags, s);
+            KR_HIP(hipGetLastError());
+            hipLaunchKernelGGL(impact_update_kernel, dim3(1), dim3(1024), 0, s, C, d_s, d_c, d_b, d_state, 1);
+            KR_HIP(hipGetLastError());
+        }
+        const int64_t todo = std::min<int64_t>(batch, o->max_iterations - enqueued);
+        for (int64_t r = 0; r < todo; ++r) {
+            launch_impact_probe(P, p->dtype, C, d_s, d_c, d_b, d_state, 0, d_flags, s);
+            KR_HIP(hipGetLastError());
+            hipLaunchKernelGGL(impact_update_kernel, dim3(1), dim3(1024), 0, s, C, d_s, d_c, d_b, d_state, 0);
+            KR_HIP(hipGetLastError());
+        }
+        enqueued += todo;
+        KR_HIP(hipEventRecord(h->ev1, s));
+        KR_HIP(hipMemcpyAsync(S.data(), d_s, S.size() * sizeof(ImpactSearchDev), hipMemcpyDeviceToHost, s));
+        KR_HIP(hipMemcpyAsync(&cnt, d_c, sizeof(cnt), hipMemcpyDeviceToHost, s));
+        KR_HIP(hipStreamSynchronize(s));
+        bool open = false;
+        for (const ImpactSearchDev& x : S) open = open || x.open;
+        if (cnt.error || !open || enqueued >= o->max_iterations) break;
+        batch = 4;  // the estimate fell short (rounding): a few more rounds at a time
+    }
+    int32_t flags = 0;
+    KR_HIP(hipMemcpyAsync(&flags, d_flags, 4, hipMemcpyDeviceToHost, s));
+    if (cnt.error) {
+        KR_HIP(hipMemcpyAsync(probe_b, d_b, lanes * 8, hipMemcpyDeviceToHost, s));
+        KR_HIP(hipMemcpyAsync(probe_state, d_state, lanes * 8, hipMemcpyDeviceToHost, s));
+    }
+    KR_HIP(hipStreamSynchronize(s));
+    if (flags) return host::flag_status(flags);
+    for (int k = 0; k < n_search; ++k) {
+        lo[k] = S[k].lo;
+        hi[k] = S[k].hi;
+        iterations[k] = S[k].iterations;
+        was_open[k] = S[k].open ? 1 : 0;
+    }
+    res->rays = static_cast<int64_t>(cnt.rays);
+    res->max_steps = static_cast<int64_t>(cnt.max_steps);
+    res->crossings = static_cast<int64_t>(cnt.crossings);
+    res->error = cnt.error;
+    res->error_round = cnt.error_round;
     if (kernel_ms) {
         float ms = 0.f;
         KR_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));

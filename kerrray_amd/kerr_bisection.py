@@ -1,26 +1,31 @@
-"""Critical impact parameter by bracketed classification search, on any backend.
+"""Critical impact parameter of equatorial photons by bracketed classification.
 
-Mirrors ``CriticalImpactBisection``, ``probe_states`` and
-``bisect_critical_impacts`` of the reference
-(src/kerrray/validation/kerr_bisection.py:59-300). Equatorial photons launched
-inward from ``r0`` with impact parameter ``b = |L_z| / E`` are captured below
-the critical value ``b_c`` of the circular photon orbit and escape above it
-(Bardeen, Press and Teukolsky 1972). The search keeps a bracket
-``[b_lo, b_hi]``, captured at ``b_lo`` and escaped at ``b_hi``, and classifies
-``n_probe`` interior rays per iteration. Prograde and retrograde searches
-share one batch.
+Equatorial photons launched inward from ``r0`` with impact parameter
+``b = |L_z| / E`` fall in below the critical value ``b_c`` of the circular
+photon orbit and escape above it (Bardeen, Press and Teukolsky 1972). A
+search holds a bracket ``[b_lo, b_hi]``, captured at ``b_lo`` and escaped at
+``b_hi``, and narrows it by classifying ``n_probe`` evenly spaced interior
+photons per round until it is no wider than ``tol`` or has taken
+``max_iterations`` rounds. The contract (result fields, argument checks,
+error messages, the optional relabelling of horizon crossings) is that of
+``bisect_critical_impacts`` and ``probe_states`` in
+src/kerrray/validation/kerr_bisection.py:59-300.
 
-The reference integrates the probes with its NumPy integrator directly;
-here ``probe_states`` takes a ``backend`` (the ``hip`` backend by default),
-whose ``integrate_batch`` traces the batch. Initial states come from
-:func:`kerrray_amd.geodesics.photon_from_constants`, the reference's
-expressions. Any ray that ends neither ``CAPTURED`` nor ``ESCAPED``, an
-invalid starting bracket, or an ``ESCAPED`` probe below a ``CAPTURED`` one
-raises ``RuntimeError``, as in the reference.
+On the hip backend the whole search runs on the device
+(``kr_impact_search``): the brackets of all searches stay in device memory,
+one launch traces every open search's probes of a round, a second narrows
+the brackets, and the host waits only once all rounds are enqueued. With any
+other backend (the CPU oracle in the tests) the same rounds run on the
+host: the brackets are arrays over the searches, each round's probes are one
+``integrate_batch`` call, and the narrowing is vectorised. Both give the
+reference's brackets: the probe impact parameters, the updates
+``lo = max(lo, largest captured)``, ``hi = min(hi, smallest escaped)`` and the
+stopping rule are the reference's, operation for operation.
 """
 
 from __future__ import annotations
 
+import ctypes
 import math
 import time
 from dataclasses import dataclass
@@ -29,17 +34,21 @@ from typing import Any
 import numpy as np
 from numpy.typing import NDArray
 
-from kerrray_amd.backends import get_backend
+from kerrray_amd import _lib
+from kerrray_amd.backends import HipBackend, get_backend, kernel_params
 from kerrray_amd.geodesics import IntegratorOptions, TerminationOptions, photon_from_constants
 from kerrray_amd.geometry import Spacetime, outer_horizon
 from kerrray_amd.photons import TerminationState
 
 __all__ = ["CriticalImpactBisection", "bisect_critical_impacts", "probe_states"]
 
+_CAPTURED = int(TerminationState.CAPTURED)
+_ESCAPED = int(TerminationState.ESCAPED)
+
 
 @dataclass(frozen=True)
 class CriticalImpactBisection:
-    """Result of one bracketing search (kerr_bisection.py:59-88)."""
+    """Outcome of one search (the reference's result record, kerr_bisection.py:59-88)."""
 
     spin: float
     prograde: bool
@@ -54,80 +63,147 @@ class CriticalImpactBisection:
     horizon_crossings: int = 0
 
 
-def _lz_sign(st: Spacetime, prograde: bool) -> float:
-    """``+1`` or ``-1`` such that ``a L_z > 0`` for prograde (``L_z > 0`` for ``a = 0``)."""
-    return 1.0 if prograde == (st.a >= 0.0) else -1.0
+def _lz_signs(st: Spacetime, prograde: NDArray[np.bool_]) -> NDArray[np.float64]:
+    """+1 where a prograde photon (a L_z > 0; L_z > 0 for a = 0) is wanted, -1 otherwise."""
+    return np.where(np.asarray(prograde, bool) == (st.a >= 0.0), 1.0, -1.0)
 
 
 def probe_states(st: Spacetime, bs: NDArray[np.float64], prograde: NDArray[np.bool_], r0: float,
                  integ: IntegratorOptions, term: TerminationOptions, *, horizon_crossing_as_capture: bool = False,
                  backend: Any = "hip") -> tuple[NDArray[np.int64], int, int]:
-    """Classify inward equatorial rays with impact parameters ``bs`` in one batch (kerr_bisection.py:96-127).
+    """Trace inward equatorial photons with impact parameters ``bs`` on ``backend`` in one batch.
 
-    Returns the termination codes, the largest accepted-step count and the
-    number of rays relabelled ``CAPTURED`` (an ``OUT_OF_DOMAIN`` ray whose
-    finite final radius is within the capture margin, when
-    ``horizon_crossing_as_capture`` is set).
+    Returns their termination codes, the largest accepted-step count and how
+    many ``OUT_OF_DOMAIN`` photons with a finite final radius inside the
+    capture margin were counted as ``CAPTURED`` (only with
+    ``horizon_crossing_as_capture``).
     """
-    y0 = np.stack([
-        photon_from_constants(st, r0, 0.5 * math.pi, 1.0, _lz_sign(st, bool(p)) * b, 0.0, sign_r=-1, sign_theta=1)
-        for b, p in zip(bs, prograde, strict=True)
-    ])
+    lz = _lz_signs(st, prograde) * np.asarray(bs, np.float64)
+    y0 = np.array([photon_from_constants(st, r0, 0.5 * math.pi, 1.0, float(v), 0.0, sign_r=-1, sign_theta=1)
+                   for v in lz])
     engine = get_backend(backend) if isinstance(backend, str) else backend
-    result = engine.integrate_batch(st, y0, integ, term)
-    states = np.array(result.state, dtype=np.int64, copy=True)
-    n_crossed = 0
+    res = engine.integrate_batch(st, y0, integ, term)
+    codes = np.asarray(res.state, np.int64).copy()
+    relabelled = 0
     if horizon_crossing_as_capture:
-        r_end = np.asarray(result.Y[:, 1], dtype=np.float64)
-        crossed = (states == TerminationState.OUT_OF_DOMAIN) & np.isfinite(r_end) & (
-            r_end <= outer_horizon(st) + term.horizon_epsilon)
-        states[crossed] = TerminationState.CAPTURED
-        n_crossed = int(np.count_nonzero(crossed))
-    return states, int(np.max(result.n_steps)), n_crossed
+        r_end = np.asarray(res.Y[:, 1], np.float64)
+        inside = (codes == int(TerminationState.OUT_OF_DOMAIN)) & np.isfinite(r_end)
+        inside &= r_end <= outer_horizon(st) + term.horizon_epsilon
+        codes[inside] = _CAPTURED
+        relabelled = int(inside.sum())
+    return codes, int(res.n_steps.max()), relabelled
 
 
-class _Search:
-    """The bracket of one search."""
-
-    def __init__(self, prograde: bool, b_lo: float, b_hi: float) -> None:
-        self.prograde = prograde
-        self.lo = b_lo
-        self.hi = b_hi
-        self.iterations = 0
-
-    @property
-    def width(self) -> float:
-        return self.hi - self.lo
-
-    def update(self, bs: NDArray[np.float64], states: NDArray[np.int64]) -> None:
-        captured = bs[states == TerminationState.CAPTURED]
-        escaped = bs[states == TerminationState.ESCAPED]
-        if captured.size and escaped.size and captured.max() > escaped.min():
-            raise RuntimeError(f"non-monotone classification in b for prograde={self.prograde}: captured up to "
-                               f"{captured.max()!r} but escaped from {escaped.min()!r}")
-        if captured.size:
-            self.lo = max(self.lo, float(captured.max()))
-        if escaped.size:
-            self.hi = min(self.hi, float(escaped.min()))
-        self.iterations += 1
+def _state_names(codes) -> list[str]:
+    return [TerminationState(int(c)).name for c in codes]
 
 
-def _check_states(bs: NDArray[np.float64], states: NDArray[np.int64]) -> None:
-    bad = [(float(b), TerminationState(int(s)).name) for b, s in zip(bs, states, strict=True)
-           if int(s) not in (TerminationState.CAPTURED, TerminationState.ESCAPED)]
-    if bad:
-        raise RuntimeError(f"rays ended neither CAPTURED nor ESCAPED (b, state): {bad}")
+def _fail_endpoints(b_lo: float, b_hi: float, prograde: bool, lo_code: int, hi_code: int) -> RuntimeError:
+    lo_name, hi_name = _state_names([lo_code, hi_code])
+    return RuntimeError(f"bracket [{b_lo}, {b_hi}] invalid for prograde={prograde}: "
+                        f"{lo_name} at b_lo, {hi_name} at b_hi")
+
+
+def _fail_states(bs, codes) -> RuntimeError:
+    bad = [(float(b), name) for b, c, name in zip(bs, codes, _state_names(codes)) if c not in (_CAPTURED, _ESCAPED)]
+    return RuntimeError(f"rays ended neither CAPTURED nor ESCAPED (b, state): {bad}")
+
+
+def _fail_order(prograde: bool, cap_max: float, esc_min: float) -> RuntimeError:
+    return RuntimeError(f"non-monotone classification in b for prograde={prograde}: captured up to "
+                        f"{cap_max!r} but escaped from {esc_min!r}")
+
+
+def _narrow(b: NDArray[np.float64], codes: NDArray[np.int64], prograde: NDArray[np.bool_],
+            lo: NDArray[np.float64], hi: NDArray[np.float64]) -> None:
+    """One round's bracket update for the searches whose probes are the rows of ``b`` (in place)."""
+    if np.any((codes != _CAPTURED) & (codes != _ESCAPED)):
+        raise _fail_states(b.ravel(), codes.ravel())
+    cap, esc = codes == _CAPTURED, codes == _ESCAPED
+    cap_max = np.where(cap, b, -np.inf).max(axis=1)
+    esc_min = np.where(esc, b, np.inf).min(axis=1)
+    for k in np.flatnonzero(cap.any(axis=1) & esc.any(axis=1) & (cap_max > esc_min)):
+        raise _fail_order(bool(prograde[k]), float(cap_max[k]), float(esc_min[k]))
+    lo[:] = np.where(cap.any(axis=1) & (cap_max > lo), cap_max, lo)
+    hi[:] = np.where(esc.any(axis=1) & (esc_min < hi), esc_min, hi)
+
+
+def _host_search(st, integ, term, r0, b_lo, b_hi, tol, max_iterations, n_probe, prograde, relabel, engine):
+    n = prograde.size
+    lo, hi = np.full(n, b_lo), np.full(n, b_hi)
+    rounds = np.zeros(n, np.int64)
+    kw = dict(horizon_crossing_as_capture=relabel, backend=engine)
+    codes, max_steps, crossings = probe_states(st, np.tile([b_lo, b_hi], n), np.repeat(prograde, 2), r0, integ,
+                                               term, **kw)
+    rays = 2 * n
+    for k in range(n):
+        if (codes[2 * k], codes[2 * k + 1]) != (_CAPTURED, _ESCAPED):
+            raise _fail_endpoints(b_lo, b_hi, bool(prograde[k]), codes[2 * k], codes[2 * k + 1])
+    fractions = np.arange(1, n_probe + 1) / (n_probe + 1)
+    while True:
+        live = np.flatnonzero((hi - lo > tol) & (rounds < max_iterations))
+        if live.size == 0:
+            break
+        b = lo[live, None] + (hi[live] - lo[live])[:, None] * fractions[None, :]
+        codes, steps, crossed = probe_states(st, b.ravel(), np.repeat(prograde[live], n_probe), r0, integ, term,
+                                             **kw)
+        rays += b.size
+        crossings += crossed
+        max_steps = max(max_steps, steps)
+        l, h = lo[live], hi[live]
+        _narrow(b, codes.reshape(b.shape), prograde[live], l, h)
+        lo[live], hi[live] = l, h
+        rounds[live] += 1
+    return lo, hi, rounds, rays, max_steps, crossings
+
+
+def _device_search(st, integ, term, r0, b_lo, b_hi, tol, max_iterations, n_probe, prograde, relabel,
+                   engine: HipBackend):
+    n = prograde.size
+    theta = 0.5 * math.pi
+    o = _lib.KrImpactOpts()
+    o.r0, o.theta = float(r0), theta
+    o.cos2_theta, o.sin2_theta = math.cos(theta) ** 2, math.sin(theta) ** 2
+    o.tol, o.capture_radius = float(tol), outer_horizon(st) + term.horizon_epsilon
+    o.max_iterations, o.n_probe, o.horizon_crossing_as_capture = int(max_iterations), int(n_probe), int(relabel)
+    signs = np.ascontiguousarray(_lz_signs(st, prograde))
+    lo, hi = np.full(n, float(b_lo)), np.full(n, float(b_hi))
+    rounds = np.zeros(n, np.int64)
+    per = max(2, int(n_probe))
+    probe_b, probe_code = np.zeros(n * per), np.zeros(n * per, np.int64)
+    was_open = np.zeros(n, np.uint8)
+    res = _lib.KrImpactResult()
+    params = kernel_params(st, integ, term)
+    ms = ctypes.c_double(0.0)
+    with engine.handle._lock:
+        _lib.check(_lib.lib().kr_impact_search(
+            engine.handle.ptr, ctypes.byref(params), ctypes.byref(o), n, signs.ctypes.data, lo.ctypes.data,
+            hi.ctypes.data, rounds.ctypes.data, probe_b.ctypes.data, probe_code.ctypes.data, was_open.ctypes.data,
+            ctypes.byref(res), ctypes.byref(ms)), "kr_impact_search")
+    engine.last_kernel_ms = ms.value
+    if res.error == 1:
+        for k in range(n):
+            if (probe_code[2 * k], probe_code[2 * k + 1]) != (_CAPTURED, _ESCAPED):
+                raise _fail_endpoints(b_lo, b_hi, bool(prograde[k]), probe_code[2 * k], probe_code[2 * k + 1])
+    if res.error in (2, 3):  # re-run the failing round's checks on its probes to raise the reference's message
+        live = np.flatnonzero(was_open)
+        b = probe_b.reshape(n, per)[live, :n_probe]
+        codes = probe_code.reshape(n, per)[live, :n_probe]
+        _narrow(b, codes, prograde[live], lo[live].copy(), hi[live].copy())
+        raise RuntimeError(f"the device search reported error {res.error} that the host check did not reproduce")
+    return lo, hi, rounds, int(res.rays), int(res.max_steps), int(res.crossings)
 
 
 def bisect_critical_impacts(st: Spacetime, integ: IntegratorOptions, term: TerminationOptions, *, r0: float,
                             b_lo: float, b_hi: float, tol: float, max_iterations: int, n_probe: int = 8,
                             directions: tuple[bool, ...] = (True, False), horizon_crossing_as_capture: bool = False,
                             backend: Any = "hip") -> tuple[CriticalImpactBisection, ...]:
-    """Bracketing search for ``b_c`` in each of ``directions`` (prograde ``True``) (kerr_bisection.py:177-300).
+    """Bracketing search for ``b_c`` in each of ``directions`` (``True`` prograde, ``False`` retrograde).
 
-    Raises ``ValueError`` for an invalid bracket, tolerance or probe count,
-    and ``RuntimeError`` for a ray that ends neither ``CAPTURED`` nor
-    ``ESCAPED``, misclassified bracket endpoints, or a non-monotone outcome.
+    ``ValueError`` for an invalid bracket, tolerance or probe count;
+    ``RuntimeError`` for a photon that ends neither ``CAPTURED`` nor
+    ``ESCAPED``, misclassified bracket endpoints, or a captured probe above an
+    escaped one.
     """
     if not (0.0 <= b_lo < b_hi):
         raise ValueError(f"need 0 <= b_lo < b_hi, got {b_lo!r}, {b_hi!r}")
@@ -136,36 +212,16 @@ def bisect_critical_impacts(st: Spacetime, integ: IntegratorOptions, term: Termi
     if n_probe < 1 or max_iterations < 1:
         raise ValueError("n_probe and max_iterations must be >= 1")
     t0 = time.perf_counter()
-    searches = [_Search(p, b_lo, b_hi) for p in directions]
-    probe = dict(horizon_crossing_as_capture=horizon_crossing_as_capture, backend=backend)
-    bs = np.array([b_lo, b_hi] * len(searches))
-    pro = np.array([p for s in searches for p in (s.prograde, s.prograde)])
-    states, max_steps, n_crossed = probe_states(st, bs, pro, r0, integ, term, **probe)
-    n_rays = bs.size
-    for k, s in enumerate(searches):
-        lo_state, hi_state = TerminationState(int(states[2 * k])), TerminationState(int(states[2 * k + 1]))
-        if lo_state != TerminationState.CAPTURED or hi_state != TerminationState.ESCAPED:
-            raise RuntimeError(f"bracket [{b_lo}, {b_hi}] invalid for prograde={s.prograde}: "
-                               f"{lo_state.name} at b_lo, {hi_state.name} at b_hi")
-    fractions = np.arange(1, n_probe + 1) / (n_probe + 1)
-    while True:
-        open_searches = [s for s in searches if s.width > tol and s.iterations < max_iterations]
-        if not open_searches:
-            break
-        bs = np.concatenate([s.lo + s.width * fractions for s in open_searches])
-        pro = np.concatenate([np.full(n_probe, s.prograde) for s in open_searches])
-        states, steps, crossed = probe_states(st, bs, pro, r0, integ, term, **probe)
-        _check_states(bs, states)
-        n_rays += bs.size
-        n_crossed += crossed
-        max_steps = max(max_steps, steps)
-        for k, s in enumerate(open_searches):
-            sl = slice(k * n_probe, (k + 1) * n_probe)
-            s.update(bs[sl], states[sl])
+    prograde = np.array(directions, dtype=bool)
+    engine = get_backend(backend) if isinstance(backend, str) else backend
+    search = _device_search if isinstance(engine, HipBackend) else _host_search
+    lo, hi, rounds, rays, max_steps, crossings = search(st, integ, term, r0, float(b_lo), float(b_hi), tol,
+                                                        max_iterations, n_probe, prograde,
+                                                        horizon_crossing_as_capture, engine)
     runtime = time.perf_counter() - t0
     return tuple(
-        CriticalImpactBisection(spin=st.spin, prograde=s.prograde, b_c=0.5 * (s.lo + s.hi), b_lo=s.lo, b_hi=s.hi,
-                                width=s.width, iterations=s.iterations, n_rays=n_rays, max_steps=max_steps,
-                                runtime_s=runtime, horizon_crossings=n_crossed)
-        for s in searches
+        CriticalImpactBisection(spin=st.spin, prograde=bool(p), b_c=0.5 * (float(l) + float(h)), b_lo=float(l),
+                                b_hi=float(h), width=float(h) - float(l), iterations=int(r), n_rays=rays,
+                                max_steps=max_steps, runtime_s=runtime, horizon_crossings=crossings)
+        for p, l, h, r in zip(prograde, lo, hi, rounds)
     )

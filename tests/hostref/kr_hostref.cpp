@@ -446,3 +446,18 @@ extern "C" int krh_repair_rounds(int64_t n, int band_px, int64_t* state, const i
     }
     return flags;
 }
+
+// The device's initial state of a critical-impact probe (kr_device.h:
+// impact_state) for n angular momenta lz; Y (n, 8).  Returns the number of
+// states whose radicand check failed.
+extern "C" int krh_impact_states(double m, double a, double r0, double theta, double c2, double s2, const double* lz,
+                                 int64_t n, double* Y) {
+    const kr::ImpactLaunch L{m, a, r0, theta, c2, s2};
+    int bad = 0;
+    for (int64_t k = 0; k < n; ++k) {
+        double y[8];
+        if (!kr::impact_state(L, lz[k], y)) ++bad;
+        for (int c = 0; c < 8; ++c) Y[k * 8 + c] = y[c];
+    }
+    return bad;
+}

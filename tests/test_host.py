@@ -132,6 +132,50 @@ def test_trace_rays_chunking_and_progress() -> None:
         concatenate_results([])
 
 
+class _FakeWriter(_FakeBackend):
+    """A backend that writes into the caller's row store (the hip backend's ``integrate_into`` path)."""
+
+    def __init__(self) -> None:
+        self.calls: list[int] = []
+
+    def integrate_into(self, st, Y0, integ, term, events, rows) -> None:
+        y = np.asarray(Y0, float)
+        self.calls.append(len(y))
+        assert rows.n == len(y) and rows.Y.base is not None  # a view on the one allocation
+        rows.Y[:] = y
+        rows.state[:] = np.arange(len(y))
+
+
+def test_trace_rays_writes_chunks_in_place() -> None:
+    Y0 = np.random.default_rng(1).normal(size=(50, 8))
+    fake = _FakeWriter()
+    res = trace_rays(Spacetime(), Y0, IntegratorOptions(), TerminationOptions(), backend=fake, chunk_size=16)
+    assert fake.calls == [16, 16, 16, 2]
+    assert np.array_equal(res.Y, Y0)
+    assert np.array_equal(res.state, np.concatenate([np.arange(16)] * 3 + [np.arange(2)]))
+    assert res.event_Y is None and res.event_hit is None
+
+
+def test_concatenate_results_keeps_order_and_drops_fields_a_part_lacks() -> None:
+    def part(n, start, events, rejected=True):
+        z = np.zeros(n)
+        return BatchResult(Y=np.full((n, 8), float(start)), state=np.arange(start, start + n, dtype=np.int64),
+                           n_steps=np.ones(n, np.int64), lam=z, max_null_error=z, max_energy_drift=z,
+                           max_lz_drift=z, max_carter_drift=z, runtime_s=0.5,
+                           event_Y=np.zeros((n, 8)) if events else None,
+                           event_hit=np.ones(n, bool) if events else None,
+                           n_rejected=np.zeros(n, np.int64) if rejected else None)
+
+    a, b = part(3, 0, True), part(2, 3, True)
+    joined = concatenate_results([a, b])
+    assert np.array_equal(joined.state, np.arange(5)) and joined.runtime_s == 1.0
+    assert joined.event_hit.dtype == bool and joined.event_hit.all() and joined.event_Y.shape == (5, 8)
+    mixed = concatenate_results([a, part(2, 3, False, rejected=False)], runtime_s=7.0)
+    assert mixed.event_Y is None and mixed.event_hit is None and mixed.n_rejected is None
+    assert mixed.runtime_s == 7.0 and np.array_equal(mixed.Y[3:, 0], [3.0, 3.0])
+    assert concatenate_results([a]) is a
+
+
 def _header_symbols() -> list[str]:
     text = open(os.path.join(ROOT, "include", "kerrray_hip.h")).read()
     return sorted(set(re.findall(r"^(?:kr_status|int|int64_t|size_t|double|const char\*)\s+(kr_\w+)\(", text, flags=re.M)))

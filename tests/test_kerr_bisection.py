@@ -44,6 +44,35 @@ def test_photon_from_constants_equals_the_reference(gold) -> None:
         assert np.array_equal(y, ref), (spin, r0, lz)
 
 
+def test_device_probe_states_equal_photon_from_constants(gold) -> None:
+    """kr_device.h's impact_state (host build) against photon_from_constants, bit for bit, on the golden rows
+    and on a sweep of impact parameters through b_c for several spins."""
+    import ctypes
+    import subprocess
+
+    from conftest import ROOT
+
+    hostref = os.path.join(ROOT, "tests", "hostref")
+    subprocess.run(["make", "-s", "-C", hostref], check=True, capture_output=True)
+    L = ctypes.CDLL(os.path.join(hostref, "libkr_hostref.so"))
+    L.krh_impact_states.restype = ctypes.c_int
+    L.krh_impact_states.argtypes = [ctypes.c_double] * 6 + [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+    theta = 0.5 * math.pi
+    rows = [(float(s), float(r0), float(lz)) for s, r0, lz in gold["constants_rows"]]
+    rng = np.random.default_rng(3)
+    for spin in (0.0, 0.5, -0.9, 0.998):
+        for r0 in (50.0, 1000.0):
+            rows += [(spin, r0, float(v)) for v in rng.uniform(-9.0, 9.0, 64)]
+    for spin, r0, lz in rows:
+        st = Spacetime(1.0, spin)
+        ref = photon_from_constants(st, r0, theta, 1.0, lz, 0.0, sign_r=-1, sign_theta=1)
+        got = np.zeros(8)
+        arr = np.array([lz])
+        assert L.krh_impact_states(st.mass, st.a, r0, theta, math.cos(theta) ** 2, math.sin(theta) ** 2,
+                                   arr.ctypes.data, 1, got.ctypes.data) == 0
+        assert np.array_equal(got, ref), (spin, r0, lz)
+
+
 def run(k, backend):
     spin, r0, rtol, lo, hi, tol, n_probe, dirs = SEARCHES[k]
     integ, term = options(rtol=rtol, atol=1e-2 * rtol, escape_radius=r0)
