@@ -427,14 +427,115 @@ def impacts(workers: int) -> None:
     np.savez_compressed(os.path.join(HERE, "critical_impacts.npz"), **out)
 
 
+# Option sets whose packed kernel constants are recorded: (spin, method, dtype,
+# rtol, atol, step_size, max_steps, lambda_max, h_min, h_max (nan: None),
+# horizon_epsilon, escape_radius, disk (0/1), r_in, r_out).
+PACKING_SETS = (
+    (0.9, "rk45", "float64", 1e-6, 1e-8, 0.01, 100_000, 1e4, 1e-12, float("nan"), 1e-6, 1000.0, 0, 6.0, 20.0),
+    (0.9, "rk45", "float32", 1e-6, 1e-8, 0.01, 100_000, 1e4, 1e-12, float("nan"), 1e-3, 1000.0, 0, 6.0, 20.0),
+    (0.5, "rk4", "float64", 1e-9, 1e-11, 0.05, 5_000, 2e3, 1e-12, float("nan"), 1e-6, 500.0, 0, 6.0, 20.0),
+    (0.7, "rk45", "float64", 1e-8, 1e-10, 0.02, 20_000, 5e3, 1e-10, 0.5, 1e-6, 1000.0, 0, 6.0, 20.0),
+    (0.9, "rk45", "float64", 1e-9, 1e-11, 0.01, 100_000, 1e4, 1e-12, float("nan"), 1e-6, 1000.0, 1, 2.3209, 20.0),
+    (0.9, "rk45", "float32", 1e-5, 1e-7, 0.01, 100_000, 1e4, 1e-12, float("nan"), 1e-3, 1000.0, 1, 6.0, 25.0),
+    (-0.5, "rk45", "float64", 1e-9, 1e-11, 0.01, 100_000, 1e4, 1e-12, 2.0, 1e-6, 50.0, 0, 6.0, 20.0),
+)
+
+
+def packing(workers: int) -> None:
+    """The reference's own packing of the kernel constants (numba_backend.py:87-128) for PACKING_SETS.
+
+    Both in float64 (what the C ABI's kr_params carries) and in the working
+    dtype (what the reference's kernel receives).
+    """
+    if REF_SRC not in sys.path:
+        sys.path.insert(0, REF_SRC)
+    from kerrray.geodesics import EventOptions, IntegratorOptions, TerminationOptions
+    from kerrray.geometry import Spacetime
+    from kerrray.raytracing.backends.numba_backend import _kernel_constants, _tableaus
+
+    out: dict[str, np.ndarray] = {}
+    rows = []
+    for k, (spin, method, dtype, rtol, atol, h0, max_steps, lam_max, h_min, h_max, eps, r_esc, disk, r_in,
+            r_out) in enumerate(PACKING_SETS):
+        st = Spacetime(1.0, spin)
+        integ = IntegratorOptions(method=method, rtol=rtol, atol=atol, step_size=h0, max_steps=max_steps,
+                                  lambda_max=lam_max, h_min=h_min, h_max=None if math.isnan(h_max) else h_max,
+                                  dtype=dtype)
+        term = TerminationOptions(horizon_epsilon=eps, escape_radius=r_esc)
+        events = EventOptions(disk_plane=True, r_in=r_in, r_out=r_out) if disk else None
+        for tag, dt in (("f64", np.dtype(np.float64)), ("work", integ.np_dtype)):
+            geom, consts, params, ipar = _kernel_constants(st, integ, term, events, dt)
+            tab_a, tab_e, rk4_b, rk4_c = _tableaus(dt)
+            for name, arr in (("geom", geom), ("consts", consts), ("params", params), ("ipar", ipar),
+                              ("tab_a", tab_a), ("tab_e", tab_e), ("rk4_b", rk4_b), ("rk4_c", rk4_c)):
+                out[f"p{k}_{tag}_{name}"] = np.asarray(arr)
+        rows.append([spin, 1.0 if method == "rk45" else 0.0, 1.0 if dtype == "float32" else 0.0, rtol, atol, h0,
+                     max_steps, lam_max, h_min, h_max, eps, r_esc, disk, r_in, r_out])
+    out["sets"] = np.asarray(rows, dtype=np.float64)
+    np.savez_compressed(os.path.join(HERE, "packing.npz"), **out)
+    print(f"packing: {len(PACKING_SETS)} option sets", flush=True)
+
+
+def _float32_chunk(args):
+    Y0, interpreted = args
+    if REF_SRC not in sys.path:
+        sys.path.insert(0, REF_SRC)
+    from kerrray.geodesics import IntegratorOptions, TerminationOptions, integrate_batch
+    from kerrray.geometry import Spacetime
+    from kerrray.raytracing.backends.numba_backend import integrate_batch_numba
+
+    st = Spacetime(1.0, 0.9)
+    integ = IntegratorOptions(method="rk45", rtol=1e-6, atol=1e-8, dtype="float32")
+    term = TerminationOptions(horizon_epsilon=1e-3)
+    if interpreted:
+        res = integrate_batch_numba(st, Y0, integ, term, interpreted=True)
+    else:
+        res = integrate_batch(st, Y0, integ, term)
+    out = _fields(res)
+    out["max_energy_drift"], out["max_lz_drift"] = res.max_energy_drift, res.max_lz_drift
+    return out
+
+
+def float32_zamo(workers: int) -> None:
+    """The reference's float32 mode on the image of docs/numerical_analysis.md section 7 (EXP-011).
+
+    ZAMO camera, a = 0.9, i = 60 deg, r_o = 1000, 64 x 64 on fov 12, rk45
+    rtol 1e-6, atol 1e-8, float32, horizon_epsilon 1e-3 (the table there:
+    558 captured, 3538 escaped).  Rows of the Numba kernel run interpreted
+    (glibc math in the working dtype) and of the NumPy integrator.
+    """
+    if REF_SRC not in sys.path:
+        sys.path.insert(0, REF_SRC)
+    from kerrray.geometry import Spacetime
+    from kerrray.raytracing.camera import Camera, initial_states
+
+    cam = Camera(radius=1000.0, inclination_deg=60.0, fov=12.0, resolution=64)
+    Y0 = initial_states(cam, Spacetime(1.0, 0.9))
+    out: dict[str, np.ndarray] = {"Y0": Y0}
+    t0 = time.time()
+    chunks = [Y0[i:i + 32] for i in range(0, len(Y0), 32)]
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        parts = list(ex.map(_float32_chunk, [(c, True) for c in chunks]))
+    t1 = time.time()
+    numba = {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+    numpy_rows = _float32_chunk((Y0, False))
+    for tag, rows in (("numba", numba), ("numpy", numpy_rows)):
+        for k, v in rows.items():
+            out[f"{tag}_{k}"] = v
+    print(f"float32 zamo: interpreted Numba {t1 - t0:.1f}s, NumPy {time.time() - t1:.1f}s, "
+          f"captured {int((numba['state'] == 2).sum())}", flush=True)
+    np.savez_compressed(os.path.join(HERE, "float32_zamo64.npz"), **out)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--workers", type=int, default=8)
-    ap.add_argument("--only", choices=["schwarzschild64", "kerr512", "bisection", "zamo", "disk", "edges", "impacts"],
-                    default=None)
+    ap.add_argument("--only", choices=["schwarzschild64", "kerr512", "bisection", "zamo", "disk", "edges", "impacts",
+                                       "packing", "float32_zamo"], default=None)
     args = ap.parse_args()
     for name, fn in (("schwarzschild64", schwarzschild64), ("kerr512", kerr512), ("bisection", bisection), ("zamo", zamo),
-                     ("disk", disk), ("edges", edges), ("impacts", impacts)):
+                     ("disk", disk), ("edges", edges), ("impacts", impacts), ("packing", packing),
+                     ("float32_zamo", float32_zamo)):
         if args.only in (None, name):
             fn(args.workers)
 
