@@ -89,8 +89,10 @@ def cpu_baseline(n_threads: int) -> dict:
     best = math.inf
     for _ in range(3):
         t0 = time.perf_counter()
-        O.integrate_batch(st, Y0, integ, term, n_threads=n_threads, dynamic=True)
+        res = O.integrate_batch(st, Y0, integ, term, n_threads=n_threads, dynamic=True)
         best = min(best, time.perf_counter() - t0)
+    ref = golden_mask()
+    mask_ok = None if ref is None else bool(np.array_equal(np.asarray(res.state, np.int64), ref))
     static = math.inf
     for _ in range(2):
         t0 = time.perf_counter()
@@ -107,12 +109,93 @@ def cpu_baseline(n_threads: int) -> dict:
         "unit": "photons/s",
         "cores": n_threads,
         "kind": "port",
+        "label": "Numba-equivalent restatement (oracle/kr_oracle.cpp, the reference's numba_kernel.py in C++, "
+                 "FP64, -O2 -fno-fast-math -ffp-contract=off)",
+        "mask_identical_to_reference": mask_ok,
         "sample": f"the full 512x512 workload (262144 photons), best of 3, OpenMP dynamic over photons on {n_threads} threads",
         "static_schedule": {"value": len(Y0) / static,
                             "sample": f"the same workload, best of 2, OpenMP static over photons on {n_threads} threads"},
         "single_core": {"value": len(one) / t1, "sample": "every 64th pixel (4096 photons), 1 thread, best of 2"},
         "host": host_cpu(),
     }
+
+
+EXP009_RAYS, EXP009_CPU_SAMPLE = 1_000_000, 100_000
+
+
+def exp009_points(n: int) -> np.ndarray:
+    """The seeded image-plane points of the reference's CPU throughput benchmark (benchmarks/cpu.py:157-170):
+    uniform in [-12, 12]^2 M, drawn as (n, 2) pairs from a PCG64 generator with seed 0."""
+    return np.random.default_rng(0).uniform(-12.0, 12.0, size=(n, 2))
+
+
+def exp009_options():
+    from kerrray_amd.geodesics import IntegratorOptions, TerminationOptions
+
+    return IntegratorOptions(method="rk45", rtol=1e-9, atol=1e-11), TerminationOptions()
+
+
+def exp009_report(hip, log, n_threads: int | None) -> dict:
+    """The reference's one published throughput workload (EXP-009, docs/numerical_analysis.md:266-301): 1e6
+    seeded rays, ZAMO camera at r_o = 1000 M, i = 60 deg, a = 0.9, rk45 rtol 1e-9 / atol 1e-11.  On hip the
+    states are built on the device (kr_point_states_dev) and traced through kr_integrate_batch_dev, the
+    path the reference's benchmark takes through Backend.integrate_batch; the CPU leg runs the oracle on the
+    first rays of the same sequence."""
+    import torch
+
+    from kerrray_amd import _lib
+    from kerrray_amd.backends import kernel_params
+    from kerrray_amd.camera import Camera, image_plane_states_device
+    from kerrray_amd.geometry import Spacetime
+
+    st = Spacetime(1.0, 0.9)
+    cam = Camera(radius=R_O, inclination_deg=60.0, fov=12.0, resolution=2)
+    integ, term = exp009_options()
+    pts = exp009_points(EXP009_RAYS)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    y0 = image_plane_states_device(cam, st, pts[:, 0], pts[:, 1], backend=hip)
+    e1.record(stream)
+    bufs, outs = device_outputs(torch, EXP009_RAYS, torch.float64)
+    p = kernel_params(st, integ, term)
+
+    def trace():
+        _lib.check(_lib.lib().kr_integrate_batch_dev(hip.handle.ptr, ctypes.byref(p), y0.data_ptr(), EXP009_RAYS,
+                                                     ctypes.byref(outs), ctypes.c_void_p(stream.cuda_stream)),
+                   "kr_integrate_batch_dev")
+
+    trace()
+    ms = timed(trace, 2)
+    torch.cuda.synchronize()
+    states_ms = e0.elapsed_time(e1)
+    state = bufs["state"].cpu().numpy()
+    counts = np.bincount(state, minlength=7)
+    out = {"workload": "EXP-009: 1e6 rays, seeded uniform alpha,beta in [-12,12]^2 M (PCG64 seed 0), ZAMO camera "
+                       "r_o=1000 M, i=60deg, a=0.9, rk45 rtol 1e-9 atol 1e-11, FP64",
+           "reference_published": {"rays_per_s": 7949, "source": "docs/numerical_analysis.md:292-301 (Numba, 8 "
+                                   "logical CPUs of a laptop, N = 1e6)"},
+           "hip": {"rays": EXP009_RAYS, "trace_kernel_ms": ms, "states_kernel_ms": states_ms,
+                   "rays_per_s": EXP009_RAYS / (ms * 1e-3), "captured": int(counts[2]), "escaped": int(counts[1]),
+                   "other": int(EXP009_RAYS - counts[1] - counts[2]),
+                   "mean_steps": float(bufs["n_steps"].double().mean().item())}}
+    log(f"EXP-009 1e6 rays at rtol 1e-9: {ms:.1f} ms ({out['hip']['rays_per_s']:.3g} rays/s)")
+    del bufs, y0
+    torch.cuda.empty_cache()
+    if n_threads:
+        from oracle import oracle as O
+
+        sample = pts[:EXP009_CPU_SAMPLE]
+        Y0, bad = O.zamo_states(1.0, 0.9, R_O, cam.inclination_rad, 0.0, sample[:, 0], sample[:, 1])
+        t0 = time.perf_counter()
+        res = O.integrate_batch(st, Y0, integ, term, n_threads=n_threads, dynamic=True)
+        sec = time.perf_counter() - t0
+        same = bool(np.array_equal(np.asarray(res.state, np.int64), state[:EXP009_CPU_SAMPLE]))
+        out["cpu_baseline"] = {"rays_per_s": len(Y0) / sec, "cores": n_threads, "kind": "port",
+                               "sample": f"the first {EXP009_CPU_SAMPLE} rays of the same sequence, one run",
+                               "classes_identical_to_hip": same}
+        log(f"EXP-009 CPU oracle: {len(Y0) / sec:.3g} rays/s on {n_threads} threads")
+    return out
 
 
 def host_cpu() -> dict:
@@ -192,9 +275,10 @@ class PlannedRunner:
         st = _lib.KrPlannedStats()
         _lib.check(_lib.lib().kr_planned_read_stats(self.work.data_ptr(), self.nbytes, ctypes.byref(st),
                                                     ctypes.c_void_p(self.stream.cuda_stream)), "kr_planned_read_stats")
+        _lib.raise_for_flags(st.flags)
         total = int((self.bufs["n_steps"] + self.bufs["n_rejected"]).sum())
         return {"edge_band_pixels": st.planned_band, "repaired_pixels": st.repaired,
-                "fp64_attempt_fraction": st.fp64_attempts / max(1, total)}
+                "repair_rounds": st.repair_rounds, "fp64_attempt_fraction": st.fp64_attempts / max(1, total)}
 
     def options(self) -> dict:
         o = self.opts
@@ -226,9 +310,42 @@ class PlainRunner:
                    "kr_trace_image_dev")
 
 
+class RowsRunner:
+    """The reference's own path: given initial states (resident in HBM) through kr_integrate_batch_dev, which
+    runs FP64 Dormand-Prince batches on the longest-first persistent kernel."""
+
+    def __init__(self, hip, st, integ, term):
+        import torch
+
+        from kerrray_amd import _lib
+        from kerrray_amd.backends import kernel_params
+        from kerrray_amd.shadow import kr_image
+
+        self.hip, self.n = hip, RES * RES
+        self.bufs, self.outs = device_outputs(torch, self.n, torch.float64)
+        self.y0 = torch.empty((self.n, 8), dtype=torch.float64, device="cuda")
+        self.stream = torch.cuda.current_stream()
+        img = kr_image(st, R_O, INCL, FOV, RES)
+        flags = torch.zeros(1, dtype=torch.int32, device="cuda")
+        _lib.check(_lib.lib().kr_image_states_dev(hip.handle.ptr, ctypes.byref(img), 1, self.y0.data_ptr(),
+                                                  flags.data_ptr(), ctypes.c_void_p(self.stream.cuda_stream)),
+                   "kr_image_states_dev")
+        _lib.raise_for_flags(int(flags.item()))
+        self.params = kernel_params(st, integ, term)
+
+    def __call__(self) -> None:
+        from kerrray_amd import _lib
+
+        _lib.check(_lib.lib().kr_integrate_batch_dev(self.hip.handle.ptr, ctypes.byref(self.params),
+                                                     self.y0.data_ptr(), self.n, ctypes.byref(self.outs),
+                                                     ctypes.c_void_p(self.stream.cuda_stream)),
+                   "kr_integrate_batch_dev")
+
+
 def critical_path(hip, st, integ, term, n_steps, n_rej) -> dict:
-    """The photon with the most attempts in the timed image, traced alone (kr_trace_points_dev, one lane, exact
-    FP64): no launch of the image can finish before it."""
+    """The photon with the most attempts in the timed image, traced alone in exact FP64 by the plain kernel
+    (kr_trace_points_dev, one lane): an approximate floor for a launch of the image, since the timed kernels
+    run that photon with their own register layout and clock."""
     import torch
 
     from kerrray_amd import _lib
@@ -255,7 +372,7 @@ def critical_path(hip, st, integ, term, n_steps, n_rej) -> dict:
     ms = timed(one, 3)
     alone = int(bufs["n_steps"].item() + bufs["n_rejected"].item())
     return {"pixel": [i, j], "attempts": int(attempts[k]), "attempts_alone_fp64": alone, "alone_ms": ms,
-            "us_per_attempt_alone": 1e3 * ms / max(1, alone)}
+            "us_per_attempt_alone": 1e3 * ms / max(1, alone), "kernel": "kr_trace_points_dev (plain FP64, one lane)"}
 
 
 def timed(run, repeats: int) -> float:
@@ -316,6 +433,17 @@ def precision_report(hip, log) -> dict:
     e_th, e_ph = (x.cpu().numpy() for x in directions_at_infinity(hip, st, integ, term, exact.bufs["Y"], exact.bufs["state"]))
     esc = (m_state == 1) & (e_state == 1)
     dth, dph = np.abs(m_th - e_th)[esc], np.abs(m_ph - e_ph)[esc]
+    # Final rows at r >= r_esc: the two forms' last steps land at different radii, so theta and phi there
+    # differ by the path between them as well; photons with the same step count are listed on their own.
+    mY, eY = mixed.bufs["Y"].cpu().numpy(), exact.bufs["Y"].cpu().numpy()
+    same = esc & (mixed.bufs["n_steps"].cpu().numpy() == exact.bufs["n_steps"].cpu().numpy())
+    final_diff = {}
+    for name, col in (("r", 1), ("theta", 2), ("phi", 3), ("p_r", 5), ("p_theta", 6)):
+        d = np.abs(mY[:, col] - eY[:, col]) / np.maximum(1.0, np.abs(eY[:, col]))
+        final_diff[f"{name}_rel_median_escaped"] = float(np.median(d[esc]))
+        final_diff[f"{name}_rel_max_escaped"] = float(np.max(d[esc]))
+        final_diff[f"{name}_rel_max_same_step_count"] = float(np.max(d[same])) if same.any() else None
+    final_diff["escaped_with_same_step_count"] = int(same.sum())
     stats = mixed.stats()
     del mixed, exact
     torch.cuda.empty_cache()
@@ -329,7 +457,9 @@ def precision_report(hip, log) -> dict:
             "fp32_fp64_repair_ms": ms_mixed, "fp64_planned_ms": ms_exact,
             "photons_per_s_fp32_fp64_repair": res * res / (ms_mixed * 1e-3),
             "photons_per_s_fp64": res * res / (ms_exact * 1e-3),
-            "mask_diff_vs_fp64": mask_diff, "states_identical_to_fp64": bool(np.array_equal(m_state, e_state)),
+            "mask_diff_vs_fp64": mask_diff,
+            "termination_codes_identical_to_fp64": bool(np.array_equal(m_state, e_state)),
+            "final_state_difference_vs_fp64": final_diff,
             "escape_direction_error_vs_fp64": {
                 "theta_inf_median": float(np.median(dth)), "theta_inf_p99": float(np.quantile(dth, 0.99)),
                 "theta_inf_max": float(np.max(dth)), "phi_inf_median": float(np.median(dph)),
@@ -581,6 +711,7 @@ def main() -> int:
     n = RES * RES
 
     runners = {"fp64_plain": PlainRunner(hip, st, integ, term),
+               "fp64_rows": RowsRunner(hip, st, integ, term),
                "fp64_planned": PlannedRunner(hip, False, st, integ, term),
                "fp32_fp64_repair": PlannedRunner(hip, True, st, integ, term)}
     forms = {}
@@ -612,6 +743,31 @@ def main() -> int:
     kernel_ms = ev0.elapsed_time(ev1) / args.steps
     ms_per_step = 1e3 * wall / args.steps
     log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step wall, {kernel_ms:.3f} ms/launch (events)")
+    if isinstance(run, PlannedRunner):
+        run.stats()  # raises on any flag of the timed form (bad initial state, repair limit)
+    # Both forms the metric names, each over the same K steps.
+    timed_forms = {chosen: {"ms_per_step": ms_per_step, "kernel_ms": kernel_ms,
+                            "photons_per_s": n / (ms_per_step * 1e-3)}}
+    for other in ("fp64_planned", "fp32_fp64_repair"):
+        if other == chosen:
+            continue
+        orun = runners[other]
+        for _ in range(args.warmup):
+            orun()
+        torch.cuda.synchronize()
+        t0o = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            orun()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wo = 1e3 * (time.perf_counter() - t0o) / args.steps
+        timed_forms[other] = {"ms_per_step": wo, "kernel_ms": ev0.elapsed_time(ev1) / args.steps,
+                              "photons_per_s": n / (wo * 1e-3),
+                              "mask_identical_to_reference": None if ref is None else
+                              bool(np.array_equal(orun.bufs["state"].cpu().numpy(), ref))}
+        orun.stats()
+        log(f"{other} over the same {args.steps} steps: {wo:.3f} ms/step")
     crit = critical_path(hip, st, integ, term, run.bufs["n_steps"].cpu().numpy(), run.bufs["n_rejected"].cpu().numpy())
     crit["frac_of_ms_per_launch"] = crit["alone_ms"] / kernel_ms
     log(f"longest photon alone: {crit['attempts_alone_fp64']} attempts, {crit['alone_ms']:.3f} ms "
@@ -626,6 +782,8 @@ def main() -> int:
     counts = np.bincount(state, minlength=7)
     described = {
         "fp64_plain": "FP64, plain one-photon-per-lane kernel",
+        "fp64_rows": "FP64, given initial states through kr_integrate_batch_dev (longest-first persistent kernel "
+                     "ordered by constants of motion); results bitwise those of the plain kernel",
         "fp64_planned": "FP64, persistent kernel over a longest-first work list (edge band, inside, outside); "
                         "results bitwise those of the plain kernel",
         "fp32_fp64_repair": "FP32 right-hand sides with FP64 state/update/error norm/step control; the edge band "
@@ -666,8 +824,8 @@ def main() -> int:
             "frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
             "traffic": None,
             "algorithmic_flops_per_launch": fl,
-            "launch": "kr_trace_image_dev" if chosen == "fp64_plain" else
-                      "kr_trace_image_planned_dev (plan + persistent trace + verify + repair kernels)",
+            "launch": {"fp64_plain": "kr_trace_image_dev", "fp64_rows": "kr_integrate_batch_dev (plan + persistent trace)"}
+                      .get(chosen, "kr_trace_image_planned_dev (plan + persistent trace + verify + repair kernels)"),
         },
     }
     t = traffic()
@@ -675,6 +833,7 @@ def main() -> int:
         line["roofline"]["traffic"] = t[chosen]["bytes_per_launch"]
         line["roofline"]["traffic_source"] = t[chosen]["source"]
     line["forms"] = forms
+    line["timed_forms"] = timed_forms
     log("persistent kernel, single image (uniform constants)")
     forms["fp64_persistent"] = persistent_single(hip, ref)
     if not args.no_batch:
@@ -686,9 +845,11 @@ def main() -> int:
         line["precision_1024_a099_i85"] = precision_report(hip, log)
         line["adjoint_256_a09_i60"] = adjoint_report(hip, log)
         line["camera_zamo_512_and_disk"] = camera_report(hip, log)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    threads = min(threads, len(os.sched_getaffinity(0)))
+    if not args.no_extras:
+        line["exp009_context"] = exp009_report(hip, log, None if args.no_cpu_baseline else threads)
     if not args.no_cpu_baseline:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-        threads = min(threads, len(os.sched_getaffinity(0)))
         log(f"cpu baseline on {threads} threads")
         line["cpu_baseline"] = cpu_baseline(threads)
     print(json.dumps(line), flush=True)

@@ -169,6 +169,17 @@ kr_status kr_integrate_batch_dev(kr_handle h, const kr_params* p, const double* 
 kr_status kr_image_states_dev(kr_handle h, const kr_image* img, int32_t reverse, double* d_Y0, int32_t* d_flags,
                               void* stream);
 
+/* The initial states of n image points (d_alpha[k], d_beta[k]) of the
+ * camera in `img` (resolution and fov are ignored), (n, 8) float64 into
+ * d_Y0: image_plane_states(cam, st, alpha, beta)
+ * (src/kerrray/benchmarks/cpu.py:140-175) for a ZAMO camera, with the
+ * construction of initial_states (so the states of pixel centres are those
+ * of kr_image_states_dev bit for bit; the reference's two constructions
+ * agree to 1e-12), or the Bardeen photons for a Bardeen camera (reverse must
+ * be 1).  KR_FLAG_* bits go to *d_flags. */
+kr_status kr_point_states_dev(kr_handle h, const kr_image* img, int32_t reverse, const double* d_alpha,
+                              const double* d_beta, int64_t n, double* d_Y0, int32_t* d_flags, void* stream);
+
 /* ---------------------------------------------------------------------
  * Accretion-disk images: render_disk (src/kerrray/raytracing/renderer.py:
  * 163-216).  The trace stops photons at the disk plane (DISK_HIT, the event

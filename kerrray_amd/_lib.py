@@ -35,6 +35,7 @@ EXPORTED = (
     "kr_trace_image_dev",
     "kr_trace_points_dev",
     "kr_image_states_dev",
+    "kr_point_states_dev",
     "kr_zamo_frame",
     "kr_disk_shade_dev",
     "kr_mask_edge_dev",
@@ -287,6 +288,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.kr_tape_destroy.argtypes = [vp]
     L.kr_image_states_dev.restype = ctypes.c_int
     L.kr_image_states_dev.argtypes = [vp, P(KrImage), ctypes.c_int32, vp, vp, vp]
+    L.kr_point_states_dev.restype = ctypes.c_int
+    L.kr_point_states_dev.argtypes = [vp, P(KrImage), ctypes.c_int32, vp, vp, ctypes.c_int64, vp, vp, vp]
     L.kr_disk_shade_dev.restype = ctypes.c_int
     L.kr_disk_shade_dev.argtypes = [vp, P(KrDisk), ctypes.c_double, ctypes.c_double, ctypes.c_int32, vp, vp,
                                     ctypes.c_int64, vp, vp, vp, vp, vp]

@@ -48,7 +48,8 @@ from kerrray_amd.geometry import Spacetime, outer_horizon
 
 __all__ = [
     "INCLINATION_EPSILON_DEG", "NULL_CHECK_TOLERANCE", "RADICAND_TOLERANCE", "BardeenCamera", "Camera",
-    "bardeen_initial_states", "bardeen_points", "image_struct", "initial_states", "zamo_tetrad",
+    "bardeen_initial_states", "bardeen_points", "image_plane_states", "image_plane_states_device", "image_struct",
+    "initial_states", "zamo_tetrad",
 ]
 
 INCLINATION_EPSILON_DEG = 1e-3
@@ -308,6 +309,54 @@ def initial_states(cam: Camera, st: Spacetime, *, reverse: bool = True, backend:
     out = y.cpu().numpy()
     _lib.raise_for_flags(int(flags.item()))
     return out
+
+
+def image_plane_states_device(cam: Camera, st: Spacetime, alpha, beta, *, reverse: bool = True,
+                              backend: Any = "hip"):
+    """Null states (N, 8) of arbitrary image points as a device tensor (``kr_point_states_dev``).
+
+    ``alpha``, ``beta`` are float64 device tensors or arrays of N points; the
+    construction is that of :func:`initial_states`, so pixel centres give its
+    states bit for bit. Flags raise as in :func:`initial_states`.
+    """
+    import torch
+
+    from kerrray_amd import _lib
+    from kerrray_amd.backends import get_backend
+
+    if not isinstance(cam, Camera):
+        raise TypeError("image_plane_states needs a ZAMO Camera")
+    if cam.radius <= outer_horizon(st):
+        raise ValueError(f"camera radius {cam.radius:.6g} must exceed the outer horizon r_plus = {outer_horizon(st):.6g}")
+    engine = get_backend(backend) if isinstance(backend, str) else backend
+    dev = torch.device("cuda", engine.handle.device)
+    a = torch.as_tensor(alpha, dtype=torch.float64, device=dev).reshape(-1).contiguous()
+    b = torch.as_tensor(beta, dtype=torch.float64, device=dev).reshape(-1).contiguous()
+    if a.shape != b.shape:
+        raise ValueError("alpha and beta must have the same shape")
+    y = torch.empty((a.numel(), 8), dtype=torch.float64, device=dev)
+    flags = torch.zeros(1, dtype=torch.int32, device=dev)
+    img = image_struct(st, cam)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    with engine.handle._lock:
+        _lib.check(_lib.lib().kr_point_states_dev(engine.handle.ptr, ctypes.byref(img), 1 if reverse else 0,
+                                                  ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()),
+                                                  a.numel(), ctypes.c_void_p(y.data_ptr()),
+                                                  ctypes.c_void_p(flags.data_ptr()), ctypes.c_void_p(stream)),
+                   "kr_point_states_dev")
+    _lib.raise_for_flags(int(flags.item()))
+    return y
+
+
+def image_plane_states(cam: Camera, st: Spacetime, alpha: ArrayLike, beta: ArrayLike, *,
+                       backend: Any = "hip") -> NDArray[np.float64]:
+    """Past-directed null states (N, 8) for arbitrary image points of ``cam`` (benchmarks/cpu.py:140-175).
+
+    Built on the device with :func:`initial_states`' construction; the
+    reference's own construction agrees with it to 1e-12.
+    """
+    return image_plane_states_device(cam, st, np.asarray(alpha, np.float64), np.asarray(beta, np.float64),
+                                     backend=backend).cpu().numpy()
 
 
 def zamo_tetrad(st: Spacetime, r: ArrayLike, theta: ArrayLike) -> NDArray[np.float64]:

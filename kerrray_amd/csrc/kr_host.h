@@ -20,6 +20,8 @@ struct kr_context {
     void* buf = nullptr;
     size_t buf_bytes = 0;
     void* scratch = nullptr;  // device address of the planned trace's static scratch on this device
+    void* ws = nullptr;       // workspace of the _dev row traces (grow-only; freed after a device sync)
+    size_t ws_bytes = 0;
 };
 
 #define KR_HIP(expr)                                                                                   \
@@ -65,6 +67,22 @@ inline kr_status reserve(kr_context* h, size_t bytes) {
     const size_t rounded = (bytes + (1u << 20) - 1) & ~static_cast<size_t>((1u << 20) - 1);
     KR_HIP(hipMalloc(&h->buf, rounded));
     h->buf_bytes = rounded;
+    return KR_OK;
+}
+
+// Grow-only workspace of the _dev entry points, whose kernels run on the
+// caller's streams: it is replaced only after the whole device is idle.
+inline kr_status reserve_ws(kr_context* h, size_t bytes) {
+    if (bytes <= h->ws_bytes) return KR_OK;
+    if (h->ws) {
+        KR_HIP(hipDeviceSynchronize());
+        KR_HIP(hipFree(h->ws));
+        h->ws = nullptr;
+        h->ws_bytes = 0;
+    }
+    const size_t rounded = (bytes + (1u << 20) - 1) & ~static_cast<size_t>((1u << 20) - 1);
+    KR_HIP(hipMalloc(&h->ws, rounded));
+    h->ws_bytes = rounded;
     return KR_OK;
 }
 
@@ -170,6 +188,12 @@ inline kr_status bardeen_image(const kr_image* img, bool need_grid, kr::BardeenI
     const char* err = kr::bardeen_image_scalars(img, need_grid, B);
     return err ? fail(KR_ERR_INVALID, err) : KR_OK;
 }
+
+// The FP64 rk45 trace of given rows on the longest-first persistent kernel
+// (kr_mixed.hip), enqueued on s with a workspace of rows_workspace_bytes(n).
+size_t rows_workspace_bytes(int64_t n);
+kr_status enqueue_rows_planned(kr_context* h, const kr::Params& P, const double* d_Y0, int64_t n,
+                               const kr_outputs& d_out, void* d_work, hipStream_t s);
 
 // Finds the device address of the planned trace's static scratch array on
 // the current device (once per device; kr_mixed.hip) for kr_create.

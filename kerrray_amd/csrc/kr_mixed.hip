@@ -139,6 +139,71 @@ __global__ __launch_bounds__(PLAN_BLOCK) void plan_count_kernel(PlanDev pl, Bard
     }
 }
 
+// Longest-first classes of given initial states, from their constants of
+// motion.  A photon with constants (E, L_z, Q) circles near the spherical
+// photon orbits when its radial potential per E^2,
+//   R(r) = (r^2 + a^2 - a xi)^2 - Delta(r) (eta + (xi - a)^2),
+// xi = L_z / E, eta = Q / E^2, nearly has a double root outside the horizon:
+// its minimum over r, scaled by (r^2 + a^2)^2, is then near 0 (for a = 0 the
+// scaled minimum is 1 - b^2 / 27 at r = 3 M).  Bin 0: |minimum| <= band, the
+// near-critical photons; inside: minimum > 0 (no radial turning point, the
+// photon falls in); outside: the rest.  The ratios are invariant under the
+// reversal of the momentum, so backward-traced camera rows class alike.
+// Only the order of the work list depends on this; no photon's arithmetic.
+struct RowPlanDev {
+    double m, a;
+    double r_lo, r_hi;  // radii sampled for the minimum: r_+ .. 5 M
+    double band;
+};
+
+__device__ __forceinline__ PixelPlan plan_row(const RowPlanDev& pl, const double* y) {
+    const double e = -y[4], lz = y[7], th = y[2], pth = y[6];
+    double s, c;
+    sincos(th, &s, &c);
+    const double a = pl.a, a2 = a * a;
+    const double lz2s2 = lz != 0.0 ? lz * lz / (s * s) : 0.0;
+    const double q = pth * pth + c * c * (lz2s2 - a2 * e * e);
+    if (!(e != 0.0) || !isfinite(q)) return PixelPlan{1, false};
+    const double xi = lz / e, eta = q / (e * e);
+    const double k = eta + (xi - a) * (xi - a), w = a * xi;
+    constexpr int NR = 32;
+    double lowest = INFINITY;
+    for (int i = 0; i < NR; ++i) {
+        const double r = pl.r_lo + (pl.r_hi - pl.r_lo) * (static_cast<double>(i) / (NR - 1));
+        const double r2a2 = r * r + a2;
+        const double u = r2a2 - w;
+        const double R = u * u - (r * r - 2.0 * pl.m * r + a2) * k;
+        lowest = fmin(lowest, R / (r2a2 * r2a2));
+    }
+    const double d = fabs(lowest);
+    int bin = 0;
+    if (!(d <= pl.band)) {
+        bin = 1;
+        for (double edge = 2.0 * pl.band; bin < NBINS - 1 && d > edge; edge *= 2.0) ++bin;
+    }
+    return PixelPlan{bin, lowest > 0.0};
+}
+
+// plan_count_kernel for given rows.
+__global__ __launch_bounds__(PLAN_BLOCK) void plan_rows_count_kernel(RowPlanDev pl, const double* Y0, int64_t n,
+                                                                     PlanCounters* c, uint8_t* cls) {
+    __shared__ unsigned int counts[NBINS + 1];
+    if (threadIdx.x < NBINS + 1) counts[threadIdx.x] = 0u;
+    __syncthreads();
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * PLAN_BLOCK + threadIdx.x;
+    if (idx < n) {
+        const PixelPlan q = plan_row(pl, Y0 + idx * 8);
+        cls[idx] = static_cast<uint8_t>(q.bin | (q.inside ? 0x10 : 0));
+        atomicAdd(&counts[q.bin], 1u);
+        if (q.bin > 0 && q.inside) atomicAdd(&counts[NBINS], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < NBINS + 1 && counts[threadIdx.x] != 0u) {
+        unsigned long long* target = threadIdx.x < NBINS ? &c->seg[threadIdx.x] : &c->inside;
+        atomicAdd(target, static_cast<unsigned long long>(counts[threadIdx.x]));
+    }
+}
+
 // Writes the work list from the class bytes: each workgroup reserves one
 // contiguous run per order class.
 __global__ __launch_bounds__(PLAN_BLOCK) void plan_scatter_kernel(int64_t total, PlanCounters* c, const uint8_t* cls,
@@ -187,10 +252,15 @@ __global__ __launch_bounds__(PLAN_BLOCK) void plan_scatter_kernel(int64_t total,
 // photons are done.  A wavefront then never holds lanes of both precisions,
 // which would run both attempts one after the other and slow the longest
 // photons, the FP64 ones, down.
-template <int BLOCK, bool MIXED, int WAVES, bool PARK>
+//
+// ROWS: the photons are given initial states Y0 (n, 8), not image pixels
+// (FP64 only; B is then unused).
+template <int BLOCK, bool MIXED, int WAVES, bool PARK, bool ROWS = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void planned_trace_kernel(
     Params P_arg, BardeenImage B_arg, MixedCfg M, const int64_t* work, PlanCounters* c, int refill_interval,
-    int band_priority, int split, uint8_t* used_fp64, RowsOut<double> out, int32_t* flags) {
+    int band_priority, int split, uint8_t* used_fp64, RowsOut<double> out, int32_t* flags, const double* Y0,
+    int64_t total) {
+    static_assert(!(ROWS && MIXED), "given rows are traced in FP64");
     // At one wavefront per SIMD there are registers to spare but scalar
     // registers are short: the launch constants, uniform values the compiler
     // would hold in scalar registers (and spill to vector lanes and reload,
@@ -224,7 +294,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
                                             : SlopeStore<double>{private_slopes, 1};
     const int lane = threadIdx.x & 63;
     // The part of the work list this wavefront draws from, [lo, lo + count).
-    int64_t lo = 0, count = B.n * B.n;
+    int64_t lo = 0, count = total;
     unsigned long long* cursor = &c->work;
     const int64_t seg0 = MIXED && split ? static_cast<int64_t>(c->seg[0]) : 0;
     bool fp64_role = false;
@@ -255,7 +325,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
             fp64_role = false;
             exhausted = false;
             lo = seg0;
-            count = B.n * B.n - seg0;
+            count = total - seg0;
             cursor = &c->work_fp32;
         }
         if (!exhausted && (since >= refill_interval || !any_busy)) {
@@ -277,11 +347,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
                         idx = w & INDEX_MASK;
                         exact = !MIXED || (w & FP64_FLAG) != 0;
                         bin = static_cast<int>((w >> BIN_SHIFT) & 15);
-                        const int64_t i = idx / B.n, j = idx - i * B.n;
                         double y0[8];
-                        const int32_t bad =
-                            image_state(B, pixel_centre(B.fov, j, B.n), -pixel_centre(B.fov, i, B.n), y0);
-                        if (bad && flags) atomicOr(flags, bad);
+                        if (ROWS) {
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) y0[k] = Y0[idx * 8 + k];
+                        } else {
+                            const int64_t i = idx / B.n, j = idx - i * B.n;
+                            const int32_t bad =
+                                image_state(B, pixel_centre(B.fov, j, B.n), -pixel_centre(B.fov, i, B.n), y0);
+                            if (bad && flags) atomicOr(flags, bad);
+                        }
                         ray_init<double, true>(P, y0, R);
                         fp64_att = 0;
                         if (R.state == RUNNING) {
@@ -464,7 +539,7 @@ void launch_planned_t(int cus, double frac, hipStream_t s, const Params& P, cons
     const int slots = cus * (4 * WAVES * 64 / BLOCK);
     const int grid = std::max(1, static_cast<int>(slots * frac));
     hipLaunchKernelGGL((planned_trace_kernel<BLOCK, MIXED, WAVES, PARK>), dim3(grid), dim3(BLOCK), 0, s, P, B, M, w.work,
-                       w.counters, refill, band_priority, split, w.used_fp64, o, w.flags);
+                       w.counters, refill, band_priority, split, w.used_fp64, o, w.flags, nullptr, B.n * B.n);
 }
 
 template <int BLOCK, bool MIXED>
@@ -557,6 +632,10 @@ kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B,
     return KR_OK;
 }
 
+// Near-critical band of the row planner, in the scaled radial potential:
+// |1 - b^2 / 27| <= 0.05 is |b - 3 sqrt 3| <~ 0.13 M for a = 0.
+constexpr double ROW_BAND = 0.05;
+
 kr_status check_common(kr_handle h, const kr_params* p, const kr_image* img, const kr_planned_opts* o,
                        int32_t n_angles, Params* P, BardeenImage* B) {
     kr_status st = host::bardeen_image(img, true, B);
@@ -585,6 +664,45 @@ void* g_scratch_address[MAX_DEVICES] = {};
 }  // namespace
 
 static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+size_t kr::host::rows_workspace_bytes(int64_t n) { return workspace_bytes(n, 0); }
+
+// The FP64 rk45 trace of given rows (kr_integrate_batch[_dev]): the row plan,
+// then the persistent trace of the longest-first work list, exactly as the
+// planned image trace runs it; every row equals the plain kernel's.
+kr_status kr::host::enqueue_rows_planned(kr_context* h, const Params& P, const double* d_Y0, int64_t n,
+                                         const kr_outputs& d_out, void* d_work, hipStream_t s) {
+    if (n <= 0) return KR_OK;
+    int cus = 0;
+    KR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+    Work w = carve_work(d_work, n, 0);
+    const double m = P.geom[G_MASS];
+    const double r_plus = m + std::sqrt(m * m - P.geom[G_A2]);
+    RowPlanDev pl{m, P.geom[G_A], r_plus, 5.0 * m, ROW_BAND};
+    const unsigned plan_blocks = static_cast<unsigned>((n + PLAN_BLOCK - 1) / PLAN_BLOCK);
+    hipLaunchKernelGGL(planned_reset_kernel, dim3(1), dim3(64), 0, s, w.counters, w.flags);
+    KR_HIP(hipGetLastError());
+    hipLaunchKernelGGL(plan_rows_count_kernel, dim3(plan_blocks), dim3(PLAN_BLOCK), 0, s, pl, d_Y0, n, w.counters,
+                       w.used_fp64);
+    KR_HIP(hipGetLastError());
+    hipLaunchKernelGGL(plan_scatter_kernel, dim3(plan_blocks), dim3(PLAN_BLOCK), 0, s, n, w.counters, w.used_fp64,
+                       w.work, 0, 1);
+    KR_HIP(hipGetLastError());
+    RowsOut<double> o{static_cast<double*>(d_out.Y), d_out.lam, d_out.n_steps, d_out.n_rejected, d_out.state,
+                      d_out.max_null_error, d_out.max_energy_drift, d_out.max_lz_drift, d_out.max_carter_drift};
+    constexpr int BLOCK = 128, WAVES = 2;
+    const int grid = cus * (4 * WAVES * 64 / BLOCK);
+    const BardeenImage none{};
+    const MixedCfg M{};
+    if (n > (int64_t(1) << 19))
+        hipLaunchKernelGGL((planned_trace_kernel<BLOCK, false, WAVES, true, true>), dim3(grid), dim3(BLOCK), 0, s, P,
+                           none, M, w.work, w.counters, 8, 0, 0, w.used_fp64, o, w.flags, d_Y0, n);
+    else
+        hipLaunchKernelGGL((planned_trace_kernel<BLOCK, false, WAVES, false, true>), dim3(grid), dim3(BLOCK), 0, s, P,
+                           none, M, w.work, w.counters, 8, 0, 0, w.used_fp64, o, w.flags, d_Y0, n);
+    KR_HIP(hipGetLastError());
+    return KR_OK;
+}
 
 kr_status kr::host::planned_scratch_setup(int device, void** address) {
     *address = nullptr;

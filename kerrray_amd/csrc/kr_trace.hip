@@ -279,6 +279,18 @@ __global__ __launch_bounds__(256) void image_states_kernel(BardeenImage B, doubl
     for (int c = 0; c < 8; ++c) Y0[idx * 8 + c] = y[c];
 }
 
+// Initial states of given image points, one lane per point.
+__global__ __launch_bounds__(256) void point_states_kernel(BardeenImage B, const double* alpha, const double* beta,
+                                                           int64_t n, double* Y0, int32_t* flags) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    double y[8];
+    const int32_t bad = image_state(B, alpha[idx], beta[idx], y);
+    if (bad && flags) atomicOr(flags, bad);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) Y0[idx * 8 + c] = y[c];
+}
+
 // Shading of disk crossings, one lane per row: g, r_hit and I_obs of every
 // DISK_HIT row; 0 intensity and NaN elsewhere (renderer.py:190-205).
 template <typename T>
@@ -431,6 +443,10 @@ extern "C" kr_status kr_destroy(kr_handle h) {
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
     if (h->buf) hipFree(h->buf);
+    if (h->ws) {
+        hipDeviceSynchronize();
+        hipFree(h->ws);
+    }
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
     if (h->stream) hipStreamDestroy(h->stream);
@@ -502,6 +518,20 @@ kr_status copy_back(kr_handle h, const BatchBuffers& b, const kr_outputs& out, i
 
 }  // namespace
 
+namespace {
+
+// FP64 Dormand-Prince batches of at least this many rows run on the
+// longest-first persistent kernel (kr_mixed.hip); smaller ones, and every
+// other option, on the plain one-photon-per-lane kernel.  Both give the same
+// rows bit for bit.
+constexpr int64_t ROWS_PLANNED_MIN = int64_t(1) << 15;
+
+bool rows_planned(const Params& P, int dtype, int64_t n) {
+    return dtype == KR_FLOAT64 && P.ipar[I_ADAPTIVE] == 1 && P.ipar[I_DISK] == 0 && n >= ROWS_PLANNED_MIN;
+}
+
+}  // namespace
+
 extern "C" kr_status kr_integrate_batch_dev(kr_handle h, const kr_params* p, const double* d_Y0, int64_t n,
                                             const kr_outputs* d_out, void* stream) {
     host::clear_error();
@@ -512,9 +542,14 @@ extern "C" kr_status kr_integrate_batch_dev(kr_handle h, const kr_params* p, con
     if ((st = host::check_outputs(d_out, disk)) != KR_OK) return st;
     if (n > 0 && !d_Y0) return host::fail(KR_ERR_INVALID, "Y0 is NULL");
     KR_HIP(hipSetDevice(h->device));
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (rows_planned(P, p->dtype, n)) {
+        if ((st = host::reserve_ws(h, host::rows_workspace_bytes(n))) != KR_OK) return st;
+        return host::enqueue_rows_planned(h, P, d_Y0, n, *d_out, h->ws, s);
+    }
     ICArgs ic{};
     ic.Y0 = d_Y0;
-    KR_HIP(launch_trace<IC_Y0>(P, p->dtype, ic, *d_out, n, static_cast<hipStream_t>(stream)));
+    KR_HIP(launch_trace<IC_Y0>(P, p->dtype, ic, *d_out, n, s));
     return KR_OK;
 }
 
@@ -529,13 +564,18 @@ extern "C" kr_status kr_integrate_batch(kr_handle h, const kr_params* p, const d
     if (n > 0 && !Y0) return host::fail(KR_ERR_INVALID, "Y0 is NULL");
     KR_HIP(hipSetDevice(h->device));
     BatchBuffers b;
-    if ((st = carve(h, n, p->dtype, true, disk, b)) != KR_OK) return st;
+    const bool planned = rows_planned(P, p->dtype, n);
+    if ((st = carve(h, n, p->dtype, true, disk, b, planned ? host::rows_workspace_bytes(n) : 0)) != KR_OK) return st;
     hipStream_t s = h->stream;
     if (n > 0) KR_HIP(hipMemcpyAsync(b.Y0, Y0, n * 64, hipMemcpyHostToDevice, s));
     ICArgs ic{};
     ic.Y0 = b.Y0;
     KR_HIP(hipEventRecord(h->ev0, s));
-    KR_HIP(launch_trace<IC_Y0>(P, p->dtype, ic, b.d, n, s));
+    if (planned) {
+        if ((st = host::enqueue_rows_planned(h, P, b.Y0, n, b.d, b.extra, s)) != KR_OK) return st;
+    } else {
+        KR_HIP(launch_trace<IC_Y0>(P, p->dtype, ic, b.d, n, s));
+    }
     KR_HIP(hipEventRecord(h->ev1, s));
     if ((st = copy_back(h, b, *out, n, p->dtype, disk)) != KR_OK) return st;
     if (kernel_ms) {
@@ -736,6 +776,28 @@ extern "C" kr_status kr_image_states_dev(kr_handle h, const kr_image* img, int32
     KR_HIP(hipSetDevice(h->device));
     hipLaunchKernelGGL(image_states_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
                        static_cast<hipStream_t>(stream), B, d_Y0, d_flags);
+    KR_HIP(hipGetLastError());
+    return KR_OK;
+}
+
+extern "C" kr_status kr_point_states_dev(kr_handle h, const kr_image* img, int32_t reverse, const double* d_alpha,
+                                         const double* d_beta, int64_t n, double* d_Y0, int32_t* d_flags,
+                                         void* stream) {
+    host::clear_error();
+    if (!h) return host::fail(KR_ERR_INVALID, "handle is NULL");
+    BardeenImage B;
+    kr_status st = host::bardeen_image(img, false, &B);
+    if (st != KR_OK) return st;
+    if (reverse != 0 && reverse != 1) return host::fail(KR_ERR_INVALID, "reverse must be 0 or 1");
+    if (B.camera == CAMERA_BARDEEN && reverse != 1)
+        return host::fail(KR_ERR_INVALID, "Bardeen image photons are the inward rays: reverse must be 1");
+    if (B.camera == CAMERA_ZAMO) B.Z.sign = reverse ? -1.0 : 1.0;
+    if (n < 0 || n > (int64_t(1) << 31)) return host::fail(KR_ERR_INVALID, "n must be in [0, 2^31]");
+    if (n == 0) return KR_OK;
+    if (!d_alpha || !d_beta || !d_Y0) return host::fail(KR_ERR_INVALID, "alpha, beta or Y0 is NULL");
+    KR_HIP(hipSetDevice(h->device));
+    hipLaunchKernelGGL(point_states_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), B, d_alpha, d_beta, n, d_Y0, d_flags);
     KR_HIP(hipGetLastError());
     return KR_OK;
 }
