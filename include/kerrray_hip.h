@@ -383,8 +383,8 @@ kr_status kr_batch_destroy(kr_batch b);
  *    last.
  *  waves_per_simd: resident wavefronts per SIMD the persistent kernel is
  *    compiled for; more hide more FP64 latency, fewer registers each.  0:
- *    one for the split FP32 form up to 2^19 pixels (its longest FP64 photons
- *    then run alone on their SIMD), two otherwise.
+ *    one up to 2^19 pixels for the FP64 form and the split FP32 form (their
+ *    longest FP64 photons then run alone on their SIMD), two otherwise.
  *  split_roles (fp32_bulk only): the first ceil(band / 64) wavefronts trace
  *    only the FP64 band and strip, the others only FP32 photons, so no
  *    wavefront runs both attempts.  Needs longest_first (which puts the band

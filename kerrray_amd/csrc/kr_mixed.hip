@@ -583,16 +583,19 @@ kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B,
     const int refill = o->refill_interval, prio = o->band_priority ? 1 : 0;
     // Split roles rely on the band coming first in the work list.
     const int split = o->split_roles && o->longest_first ? 1 : 0;
-    // Automatic occupancy: up to 2^19 pixels the split FP32 form is bounded
-    // by its longest FP64 photons, which run fastest alone on their SIMD (one
-    // wavefront per SIMD: 4.3 against 5.1 ms at 512^2); above, throughput
-    // decides and two wavefronts per SIMD are faster (8.9 against 10.4 ms at
-    // 1024^2).  The FP64 form is faster at two in both cases.  At two, the
-    // error sums are parked in LDS above 2^19 pixels (1024^2, a=0.99:
-    // FP64 14.4 against 15.0 ms, FP32 form 7.5 against 8.2 ms) and not below
-    // (512^2 FP64: 4.9 against 5.25 ms).
+    // Automatic occupancy: up to 2^19 pixels both forms are bounded by their
+    // longest FP64 photons, which run fastest alone on their SIMD (one
+    // wavefront per SIMD, 512^2, a=0.9: split FP32 form 3.29 against 4.1 ms,
+    // FP64 form 4.4-4.6 against 4.85-4.9 ms; the FP32 form without split
+    // roles mixes precisions in a wavefront and stays at two); above,
+    // throughput decides and two wavefronts per SIMD are faster (1024^2,
+    // a=0.99: FP64 12.6 against 15.3 ms, FP32 form 6.2 against 8.7 ms).  At
+    // two, the error sums are parked in LDS above 2^19 pixels (1024^2,
+    // a=0.99: FP64 14.4 against 15.0 ms, FP32 form 7.5 against 8.2 ms, with
+    // the library pow) and not below (512^2 FP64: 4.9 against 5.25 ms).
     const bool large = pixels > (int64_t(1) << 19);
-    const int waves = o->waves_per_simd != 0 ? o->waves_per_simd : (mixed && split && !large ? 1 : 2);
+    const int waves =
+        o->waves_per_simd != 0 ? o->waves_per_simd : ((mixed ? split != 0 : true) && !large ? 1 : 2);
     // Automatic grid: at one wavefront per SIMD, up to 2^18 pixels and rtol
     // >= 1e-6, the split FP32 form's bulk finishes well before its longest
     // FP64 photon, and launching 7/8 of the slots lowers the power drawn
@@ -690,16 +693,19 @@ kr_status kr::host::enqueue_rows_planned(kr_context* h, const Params& P, const d
     KR_HIP(hipGetLastError());
     RowsOut<double> o{static_cast<double*>(d_out.Y), d_out.lam, d_out.n_steps, d_out.n_rejected, d_out.state,
                       d_out.max_null_error, d_out.max_energy_drift, d_out.max_lz_drift, d_out.max_carter_drift};
-    constexpr int BLOCK = 128, WAVES = 2;
-    const int grid = cus * (4 * WAVES * 64 / BLOCK);
+    // The planned image trace's automatic occupancy: one wavefront per SIMD up
+    // to 2^19 rows, two with the error sums parked in LDS above.
+    constexpr int BLOCK = 128;
     const BardeenImage none{};
     const MixedCfg M{};
     if (n > (int64_t(1) << 19))
-        hipLaunchKernelGGL((planned_trace_kernel<BLOCK, false, WAVES, true, true>), dim3(grid), dim3(BLOCK), 0, s, P,
-                           none, M, w.work, w.counters, 8, 0, 0, w.used_fp64, o, w.flags, d_Y0, n);
+        hipLaunchKernelGGL((planned_trace_kernel<BLOCK, false, 2, true, true>), dim3(cus * (4 * 2 * 64 / BLOCK)),
+                           dim3(BLOCK), 0, s, P, none, M, w.work, w.counters, 8, 0, 0, w.used_fp64, o, w.flags, d_Y0,
+                           n);
     else
-        hipLaunchKernelGGL((planned_trace_kernel<BLOCK, false, WAVES, false, true>), dim3(grid), dim3(BLOCK), 0, s, P,
-                           none, M, w.work, w.counters, 8, 0, 0, w.used_fp64, o, w.flags, d_Y0, n);
+        hipLaunchKernelGGL((planned_trace_kernel<BLOCK, false, 1, false, true>), dim3(cus * (4 * 1 * 64 / BLOCK)),
+                           dim3(BLOCK), 0, s, P, none, M, w.work, w.counters, 8, 0, 0, w.used_fp64, o, w.flags, d_Y0,
+                           n);
     KR_HIP(hipGetLastError());
     return KR_OK;
 }

@@ -461,3 +461,48 @@ extern "C" int krh_impact_states(double m, double a, double r0, double theta, do
     }
     return bad;
 }
+
+// The step controller's power, as the device computes it (kr_device.h), on n
+// arguments: pow_m02_cr (double) and powf_m02_cr (float).
+extern "C" void krh_pow_m02(const double* x, int64_t n, double* out) {
+    for (int64_t k = 0; k < n; ++k) out[k] = kr::pow_m02_cr(x[k]);
+}
+extern "C" void krh_powf_m02(const float* x, int64_t n, float* out) {
+    for (int64_t k = 0; k < n; ++k) out[k] = kr::powf_m02_cr(x[k]);
+}
+
+// Agreement with the C library: over n arguments log-uniform in [1e-5, 1e5]
+// (a splitmix64 sequence from seed), the number whose pow_m02_cr differs from
+// pow(x, -0.2); the first `keep` of them go to bad_x.
+extern "C" int64_t krh_pow_m02_disagreements(int64_t n, uint64_t seed, double* bad_x, int64_t keep) {
+    int64_t bad = 0;
+    uint64_t s = seed;
+    for (int64_t k = 0; k < n; ++k) {
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const double u = static_cast<double>(z >> 11) * 0x1p-53;
+        const double x = std::exp(std::log(1e-5) + u * (std::log(1e5) - std::log(1e-5)));
+        if (kr::pow_m02_cr(x) != std::pow(x, -0.2)) {
+            if (bad < keep) bad_x[bad] = x;
+            ++bad;
+        }
+    }
+    return bad;
+}
+
+// The same for powf over every float in [lo, hi] (exhaustive).
+extern "C" int64_t krh_powf_m02_disagreements(float lo, float hi, float* bad_x, int64_t keep, int64_t* tested) {
+    int64_t bad = 0, count = 0;
+    for (float x = lo; x <= hi; x = std::nextafter(x, 2.0f * hi)) {
+        ++count;
+        if (kr::powf_m02_cr(x) != std::pow(x, -0.2f)) {
+            if (bad < keep) bad_x[bad] = x;
+            ++bad;
+        }
+    }
+    *tested = count;
+    return bad;
+}
