@@ -427,6 +427,28 @@ def impacts(workers: int) -> None:
     np.savez_compressed(os.path.join(HERE, "critical_impacts.npz"), **out)
 
 
+def zamo_rows(workers: int) -> None:
+    """Final rows of the reference's NumPy integrator on the two documented 64 x 64 ZAMO images (rtol 1e-9).
+
+    The images of zamo(): a = 0.9 and a = 0 at i = 60 deg, fov 12, r_o =
+    1000, rk45 rtol 1e-9 / atol 1e-11.  Set beside the oracle's rows (the
+    Numba kernel's, bit for bit), they measure how far the reference's own
+    two CPU paths drift apart on these images.
+    """
+    with np.load(os.path.join(HERE, "zamo_cameras.npz"), allow_pickle=False) as z:
+        cams = z["cameras"]
+        y0 = {k: z[f"c{k}_Y0"] for k in (0, 1)}
+    out = {}
+    t0 = time.time()
+    for k in (0, 1):
+        res = run_parallel(_numpy_chunk, float(cams[k][0]), y0[k], 1e-9, 1e-11, workers, 256)
+        out[f"c{k}_numpy_Y"] = res["Y"]
+        out[f"c{k}_numpy_n_steps"] = res["n_steps"].astype(np.int32)
+        out[f"c{k}_numpy_n_rejected"] = res["n_rejected"].astype(np.int32)
+    print(f"zamo rows: {time.time() - t0:.1f}s", flush=True)
+    np.savez_compressed(os.path.join(HERE, "zamo64_numpy_rows.npz"), **out)
+
+
 # Option sets whose packed kernel constants are recorded: (spin, method, dtype,
 # rtol, atol, step_size, max_steps, lambda_max, h_min, h_max (nan: None),
 # horizon_epsilon, escape_radius, disk (0/1), r_in, r_out).
@@ -531,11 +553,11 @@ def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--only", choices=["schwarzschild64", "kerr512", "bisection", "zamo", "disk", "edges", "impacts",
-                                       "packing", "float32_zamo"], default=None)
+                                       "packing", "float32_zamo", "zamo_rows"], default=None)
     args = ap.parse_args()
     for name, fn in (("schwarzschild64", schwarzschild64), ("kerr512", kerr512), ("bisection", bisection), ("zamo", zamo),
                      ("disk", disk), ("edges", edges), ("impacts", impacts), ("packing", packing),
-                     ("float32_zamo", float32_zamo)):
+                     ("float32_zamo", float32_zamo), ("zamo_rows", zamo_rows)):
         if args.only in (None, name):
             fn(args.workers)
 

@@ -57,9 +57,19 @@ def test_measured_run(hip, gold, tmp_path) -> None:
     print(f"  same step count as the Numba kernel {same.mean():.4f}; crossing rows max rel "
           f"{err[same].max():.2e}; g vs reference max rel {_rel(g, gold['g'][hit]).max():.2e}", flush=True)
     # Same step sequence: rounding-level differences that grow along the
-    # path; the bound is that of rtol 1e-7, the median that of 1e-9.
+    # path.  The reference's own NumPy integrator (y_hit) differs from its
+    # Numba kernel on these crossings by up to 1.2e-9 (p99 1.3e-10, median
+    # 1.7e-12); the device may differ from the kernel by twice that, and by
+    # no more than 10 x rtol.
     row = err.max(axis=1)
-    assert same.mean() > 0.98 and row[same].max() < 1e-6 and np.median(row[same]) < 1e-9
+    spread = (np.abs(gold["y_hit"] - gold["numba_event_Y_hit"])
+              / np.maximum(1.0, np.abs(gold["numba_event_Y_hit"]))).max(axis=1)
+    print(f"  reference NumPy vs Numba crossings: max {spread.max():.2e} p99 {np.quantile(spread, 0.99):.2e} "
+          f"median {np.median(spread):.2e}; device vs Numba: max {row[same].max():.2e} "
+          f"p99 {np.quantile(row[same], 0.99):.2e} median {np.median(row[same]):.2e}", flush=True)
+    assert same.mean() > 0.98 and row[same].max() <= min(10.0 * 1e-9, 2.0 * spread.max())
+    assert np.quantile(row[same], 0.99) <= 2.0 * np.quantile(spread, 0.99)
+    assert np.median(row[same]) <= max(2.0 * np.median(spread), 1e-15)
     assert _rel(g, gold["g"][hit]).max() < 1e-7 and np.median(_rel(g, gold["g"][hit])) < 1e-10
     assert _rel(I, gold["intensity"][hit]).max() < 1e-6
     assert np.all(np.isnan(img.g.ravel()[~hit])) and np.all(img.intensity.ravel()[~hit] == 0.0)
