@@ -613,7 +613,18 @@ def ragged_batch(hip, log) -> dict:
     batch.run(graph=True)
     res = batch.results()
     ms = batch.timed(repeats=5, graph=True)
+    fp64_state = batch.bufs["state"].clone()
     batch.close()
+    # The precision form on the same batch (FP32 bulk where rtol >= 1e-6, FP64 band, verify/repair).
+    from kerrray_amd.planned import PlannedOptions
+
+    mixed = DeviceBatch(hip, jobs, tune, precision=PlannedOptions())
+    mixed.run(graph=True)
+    ms_mixed = mixed.timed(repeats=5, graph=True)
+    pst = mixed.stats()
+    same_masks = bool(((mixed.bufs["state"] == 2) == (fp64_state == 2)).all().item())
+    mixed.close()
+    log(f"precision form on the batch: {ms_mixed:.1f} ms against {ms:.1f} ms FP64, masks identical {same_masks}")
     in_job_order = DeviceBatch(hip, jobs, _lib.KrTune(tune.block_size, tune.waves_per_simd, tune.refill_interval, 0))
     in_job_order.run(graph=True)
     ms_job_order = in_job_order.timed(repeats=3, graph=True)
@@ -639,7 +650,11 @@ def ragged_batch(hip, log) -> dict:
             "kernel_ms_images_in_job_order": ms_job_order,
             "roofline_frac": ach / FP64_VECTOR_PEAK_TFLOPS, "masks_identical_to_plain_kernel": identical,
             "tune": {"block_size": tune.block_size, "waves_per_simd": tune.waves_per_simd,
-                     "refill_interval": tune.refill_interval, "candidates": len(table), "seconds": tune_s}}
+                     "refill_interval": tune.refill_interval, "candidates": len(table), "seconds": tune_s},
+            "precision_form": {"kernel_ms": ms_mixed, "photons_per_s": total / (ms_mixed * 1e-3),
+                               "masks_identical_to_fp64_batch": same_masks, "fp32_min_rtol": 1e-6,
+                               "fp64_band_photons": pst.planned_band, "repaired": pst.repaired,
+                               "repair_rounds": pst.repair_rounds, "fp64_attempts": pst.fp64_attempts}}
 
 
 def main() -> int:

@@ -451,6 +451,33 @@ kr_status kr_trace_image_planned_dev(kr_handle h, const kr_params* p, const kr_i
  * that stream). */
 kr_status kr_planned_read_stats(const void* d_work, size_t work_bytes, kr_planned_stats* stats, void* stream);
 
+/* The precision form on a batch (FP32 bulk, FP64 where needed, as
+ * kr_trace_image_planned with fp32_bulk = 1): each image's photons within
+ * band_px + 1 pixels of its analytic edge, or with |alpha sin theta_o| <
+ * axis_lz, take the exact FP64 attempt, the others FP32 right-hand sides
+ * with the FP64 update, error norm and step control; after the persistent
+ * launch every image is verified and repaired in FP64 to a fixed point
+ * (KR_REPAIR_ROUNDS), all in the same kr_batch_run / graph launch.  Image k
+ * (in job order) has its edge as n_angles radii radii[k * n_angles + j]
+ * about (centre_alpha[k], centre_beta[k]) along the angles 2 pi j /
+ * n_angles.  Host arrays, copied at creation. */
+typedef struct kr_batch_precision {
+    int32_t band_px;
+    int32_t n_angles;
+    double axis_lz;
+    double fp32_min_rtol; /* images with a smaller rtol are traced in FP64 throughout: FP32 stages hold their
+                             error estimate near FP32 round-off, which tighter tolerances cannot accept */
+    const double* centre_alpha;
+    const double* centre_beta;
+    const double* radii;
+} kr_batch_precision;
+kr_status kr_batch_create_precision(kr_handle h, const kr_job* jobs, int32_t n_jobs, const kr_tune* tune,
+                                    const kr_batch_precision* prec, const kr_outputs* d_out, kr_batch* out);
+/* Counters of the last run of a precision batch (synchronises the device):
+ * FP64 band photons, repaired pixels and rounds, FP64 attempts, flags
+ * (planned_inside and planned_outside are -1). */
+kr_status kr_batch_stats(kr_batch b, kr_planned_stats* stats);
+
 /* kr_trace_image_planned_dev without a caller workspace and without any
  * device allocation: the plan, work list, repair lists and counters live in
  * a static device array of the library (17 MiB on every device the library

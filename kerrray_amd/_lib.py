@@ -49,6 +49,8 @@ EXPORTED = (
     "kr_batch_graph_launch",
     "kr_batch_flags",
     "kr_batch_destroy",
+    "kr_batch_create_precision",
+    "kr_batch_stats",
     "kr_planned_workspace_bytes",
     "kr_trace_image_planned",
     "kr_trace_image_planned_dev",
@@ -198,6 +200,18 @@ class KrImpactResult(ctypes.Structure):
     ]
 
 
+class KrBatchPrecision(ctypes.Structure):
+    _fields_ = [
+        ("band_px", ctypes.c_int32),
+        ("n_angles", ctypes.c_int32),
+        ("axis_lz", ctypes.c_double),
+        ("fp32_min_rtol", ctypes.c_double),
+        ("centre_alpha", ctypes.c_void_p),
+        ("centre_beta", ctypes.c_void_p),
+        ("radii", ctypes.c_void_p),
+    ]
+
+
 class KrTune(ctypes.Structure):
     _fields_ = [
         ("block_size", ctypes.c_int32),
@@ -254,6 +268,11 @@ def _declare(L: ctypes.CDLL) -> None:
     L.kr_batch_flags.argtypes = [vp, P(ctypes.c_int32)]
     L.kr_batch_destroy.restype = ctypes.c_int
     L.kr_batch_destroy.argtypes = [vp]
+    L.kr_batch_create_precision.restype = ctypes.c_int
+    L.kr_batch_create_precision.argtypes = [vp, P(KrJob), ctypes.c_int32, P(KrTune), P(KrBatchPrecision), P(KrOutputs),
+                                            P(vp)]
+    L.kr_batch_stats.restype = ctypes.c_int
+    L.kr_batch_stats.argtypes = [vp, P(KrPlannedStats)]
     L.kr_planned_workspace_bytes.restype = ctypes.c_size_t
     L.kr_planned_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.c_int32]
     L.kr_trace_image_planned.restype = ctypes.c_int

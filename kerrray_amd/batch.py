@@ -40,6 +40,10 @@ from kerrray_amd.photons import BatchResult
 __all__ = ["BLOCK_SIZES", "DeviceBatch", "ImageJob", "REFILL_INTERVALS", "WAVES_PER_SIMD", "autotune", "ragged_workload"]
 
 BLOCK_SIZES = (64, 128, 256)
+FP32_MIN_RTOL = 1e-6
+"""Images of a precision batch with a smaller rtol are traced in FP64: below it the FP32 stages' round-off
+holds the error estimate above what the tolerance accepts (single images: 512^2 at rtol 1e-6 gains 1.3x,
+256^2 at 1e-7 gains nothing, 1e-8 loses)."""
 WAVES_PER_SIMD = (2, 3)
 REFILL_INTERVALS = (1, 2, 4, 8, 16, 32)
 
@@ -99,7 +103,11 @@ def _kr_job(job: ImageJob) -> _lib.KrJob:
 class DeviceBatch:
     """A batch bound to device outputs (torch tensors on ``cuda:<device>``)."""
 
-    def __init__(self, backend: HipBackend, jobs: list[ImageJob], tune: _lib.KrTune | None = None) -> None:
+    def __init__(self, backend: HipBackend, jobs: list[ImageJob], tune: _lib.KrTune | None = None, *,
+                 precision=None, fp32_min_rtol: float = FP32_MIN_RTOL) -> None:
+        """``precision``: a :class:`kerrray_amd.planned.PlannedOptions` (its ``band_px`` and ``axis_lz``) to run
+        the precision form (FP32 bulk, FP64 band, verify/repair to the FP64 mask) on every image whose rtol is
+        at least ``fp32_min_rtol`` (the others are traced in FP64); ``None``: exact FP64 throughout."""
         import torch
 
         self.torch = torch
@@ -130,8 +138,16 @@ class DeviceBatch:
         arr = (_lib.KrJob * len(self.jobs))(*[_kr_job(j) for j in self.jobs])
         self.tune = tune or _lib.KrTune(256, 2, 4, 1)
         h = ctypes.c_void_p()
-        _lib.check(_lib.lib().kr_batch_create(backend.handle.ptr, arr, len(self.jobs), ctypes.byref(self.tune),
-                                              ctypes.byref(o), ctypes.byref(h)), "kr_batch_create")
+        self.precision = precision
+        if precision is None:
+            _lib.check(_lib.lib().kr_batch_create(backend.handle.ptr, arr, len(self.jobs), ctypes.byref(self.tune),
+                                                  ctypes.byref(o), ctypes.byref(h)), "kr_batch_create")
+        else:
+            prec, self._keep = _precision_struct(self.jobs, precision, fp32_min_rtol)
+            _lib.check(_lib.lib().kr_batch_create_precision(backend.handle.ptr, arr, len(self.jobs),
+                                                            ctypes.byref(self.tune), ctypes.byref(prec),
+                                                            ctypes.byref(o), ctypes.byref(h)),
+                       "kr_batch_create_precision")
         self._b = h
 
     def run(self, graph: bool = True, stream=None) -> None:
@@ -159,6 +175,13 @@ class DeviceBatch:
         _lib.check(_lib.lib().kr_batch_flags(self._b, ctypes.byref(flags)), "kr_batch_flags")
         _lib.raise_for_flags(flags.value)
 
+    def stats(self) -> _lib.KrPlannedStats:
+        """Counters of the last run of a precision batch (band photons, repairs, FP64 attempts, flags)."""
+        st = _lib.KrPlannedStats()
+        _lib.check(_lib.lib().kr_batch_stats(self._b, ctypes.byref(st)), "kr_batch_stats")
+        _lib.raise_for_flags(st.flags)
+        return st
+
     def results(self) -> list[BatchResult]:
         """Per-image results (host copies), in job order."""
         self.torch.cuda.synchronize(self.backend.device)
@@ -184,6 +207,29 @@ class DeviceBatch:
             self.close()
         except Exception:
             pass
+
+
+def _precision_struct(jobs: list[ImageJob], options, fp32_min_rtol: float) -> tuple[_lib.KrBatchPrecision, tuple]:
+    """Each image's analytic edge (Bardeen's curve about its centroid; shrunk by the finite-distance factor
+    for a ZAMO camera) for ``kr_batch_create_precision``."""
+    from kerrray_amd.boundary import finite_distance_scale
+    from kerrray_amd.planned import edge_table
+
+    tables = []
+    for job in jobs:
+        st = Spacetime(job.mass, job.spin)
+        t = edge_table(st, job.inclination_deg)
+        k = finite_distance_scale(st, job.r_o) if job.camera == "zamo" else 1.0
+        tables.append((t.centre[0] / k, t.centre[1] / k, t.radii / k))
+    n_angles = tables[0][2].size
+    ca = np.ascontiguousarray([t[0] for t in tables], dtype=np.float64)
+    cb = np.ascontiguousarray([t[1] for t in tables], dtype=np.float64)
+    radii = np.ascontiguousarray(np.concatenate([t[2] for t in tables]), dtype=np.float64)
+    p = _lib.KrBatchPrecision()
+    p.band_px, p.n_angles, p.axis_lz = int(options.band_px), int(n_angles), float(options.axis_lz)
+    p.fp32_min_rtol = float(fp32_min_rtol)
+    p.centre_alpha, p.centre_beta, p.radii = ca.ctypes.data, cb.ctypes.data, radii.ctypes.data
+    return p, (ca, cb, radii)
 
 
 def _cache_path() -> str:

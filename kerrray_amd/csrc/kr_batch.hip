@@ -39,6 +39,8 @@
 #include "kerrray_hip.h"
 #include "kr_device.h"
 #include "kr_host.h"
+#include "kr_mixed.h"
+#include "kr_repair.h"
 
 namespace kr {
 
@@ -48,6 +50,15 @@ struct JobDev {
     int64_t first;      // first photon of this image in the processing order
     int64_t count;      // n * n
     int64_t out_first;  // first output row of this image (images in job order)
+    PlanDev plan;       // precision form: the image's analytic edge, band and polar strip
+    double r_plus;      // precision form: outer horizon (the FP32 stages carry r - r_+)
+};
+
+// Counters of the precision form of a batch.
+struct BatchCounters {
+    unsigned long long band;           // photons of the FP64 band and polar strip
+    unsigned long long fp64_attempts;  // attempts taken in exact FP64
+    unsigned long long listed[REPAIR_ROUNDS + 1];
 };
 
 constexpr int MAX_LDS_JOBS = 128;
@@ -74,6 +85,8 @@ struct BatchArgs {
     int64_t total;  // photons in the batch
     unsigned long long* counter;
     int32_t* flags;
+    uint8_t* marks;          // precision form: MARK_FP64 per output row
+    BatchCounters* counters; // precision form
 };
 
 // Dynamic LDS of batch_kernel, in 8-byte words: the stage slopes (4 NA BLOCK
@@ -86,9 +99,14 @@ size_t batch_lds_bytes(bool uniform, bool lds_slopes, int block, int n_jobs) {
            (uniform ? 0 : static_cast<size_t>(n_jobs) * (sizeof(int64_t) + sizeof(Params)));
 }
 
-template <typename T, bool UNIFORM, int BLOCK, int WAVES, bool LDS_SLOPES>
+// MIXED: the precision form.  A photon of its image's edge band or polar
+// strip (bin 0 of the image plan, kr_mixed.h) takes the exact FP64 attempt;
+// any other takes FP32 right-hand sides with the FP64 update, error norm and
+// step control.  Rows record which (marks), for the verify/repair pass.
+template <typename T, bool UNIFORM, int BLOCK, int WAVES, bool LDS_SLOPES, bool MIXED = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void batch_kernel(
     BatchArgs a, JobDev uniform_job, BatchOut<T> out) {
+    static_assert(!MIXED || sizeof(T) == sizeof(double), "the precision form integrates in FP64 and FP32");
     static_assert(sizeof(Params) % sizeof(uint64_t) == 0, "Params is copied in 8-byte words");
     extern __shared__ double lds[];
     int64_t* firsts = reinterpret_cast<int64_t*>(lds + (LDS_SLOPES ? 4 * NA * BLOCK : 0));
@@ -101,12 +119,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
             dst[w] = reinterpret_cast<const uint64_t*>(&a.jobs[w / PW].P)[w % PW];
         __syncthreads();
     }
-    const SlopeStore<T> K{reinterpret_cast<T*>(lds) + threadIdx.x, BLOCK};
+    // Stage slopes of an exact attempt: in LDS at two wavefronts per SIMD,
+    // else private (only the precision form's occasional exact attempts of a
+    // bulk photon use them there; the exact photons keep theirs in registers).
+    T private_slopes[LDS_SLOPES ? 1 : 4 * NA];
+    const SlopeStore<T> K = LDS_SLOPES ? SlopeStore<T>{reinterpret_cast<T*>(lds) + threadIdx.x, BLOCK}
+                                       : SlopeStore<T>{private_slopes, 1};
     const int lane = threadIdx.x & 63;
     RayState<T> R;
     R.state = RUNNING;
     bool busy = false;
     bool exhausted = false;  // wave-uniform: the counter has run past the end
+    bool exact = true;       // MIXED: this photon takes the exact attempt
+    Geom32 G{};
+    MixedCfg M{};
+    int64_t fp64_att = 0;
     int64_t idx = -1;
     int job = 0;
     int since = a.refill_interval;
@@ -138,10 +165,19 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
                                                         -pixel_centre(J.B.fov, i, J.B.n), y0);
                         if (bad && a.flags) atomicOr(a.flags, bad);
                         ray_init<T, true>(UNIFORM ? uniform_job.P : params[job], y0, R);
-                        if (R.state == RUNNING)
+                        if (MIXED) {
+                            exact = plan_pixel(J.plan, J.B, local).bin == 0;
+                            if (exact) atomicAdd(&a.counters->band, 1ull);
+                            G = geom32(J.P);
+                            M = MixedCfg{J.r_plus, 0.0, 0.0, 0.0};
+                            fp64_att = 0;
+                        }
+                        if (R.state == RUNNING) {
                             busy = true;
-                        else
+                        } else {
                             store_row(out, idx, R);
+                            if (MIXED) a.marks[idx] = MARK_FP64;
+                        }
                     }
                 }
             }
@@ -150,12 +186,25 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
         if (!any_busy) break;  // exhausted and nothing in flight
         if (busy) {
             const Params& P = UNIFORM ? uniform_job.P : params[job];
-            if (LDS_SLOPES)
+            if (MIXED && !exact) {
+                if constexpr (MIXED) {
+                    if (LDS_SLOPES)
+                        ray_attempt_with<double, true, false>(P, R, MixedStepper<true>{G, M, &fp64_att, K});
+                    else
+                        ray_attempt_with<double, true, false>(P, R, MixedStepper<false>{G, M, &fp64_att, K});
+                }
+            } else if (LDS_SLOPES) {
                 ray_attempt_with<T, true, false>(P, R, StoredStepper<T, true, true>{K});
-            else
+            } else {
                 ray_attempt<T, true, false>(P, R);
+            }
             if (R.state != RUNNING) {
                 store_row(out, idx, R);
+                if (MIXED) {
+                    a.marks[idx] = exact ? MARK_FP64 : 0;
+                    const int64_t n64 = exact ? R.n_steps + R.n_rej : fp64_att;
+                    if (n64) atomicAdd(&a.counters->fp64_attempts, static_cast<unsigned long long>(n64));
+                }
                 busy = false;
             }
         }
@@ -163,26 +212,90 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
     }
 }
 
-__global__ void batch_reset_kernel(unsigned long long* counter, int32_t* flags) {
+__global__ void batch_reset_kernel(unsigned long long* counter, int32_t* flags, BatchCounters* counters) {
+    constexpr int words = static_cast<int>(sizeof(BatchCounters) / 8);
+    if (blockIdx.x == 0 && threadIdx.x < words && counters)
+        reinterpret_cast<unsigned long long*>(counters)[threadIdx.x] = 0ull;
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         *counter = 0ull;
         if (flags) *flags = 0;
     }
 }
 
+// The job owning output row `row`: jobs are in processing order, rows in job
+// order; out_firsts[k] is the first row of the k-th job in job order and
+// order_job[k] its index in the job table.
+__device__ __forceinline__ const JobDev& row_job(const JobDev* jobs, const int64_t* out_firsts, const int* order_job,
+                                                 int n_jobs, int64_t row) {
+    return jobs[order_job[find_job(out_firsts, n_jobs, row)]];
+}
+
+// Verify pass of the precision form over every row of the batch: each pixel
+// against its own image's neighbours (kr_repair.h).
+__global__ void batch_verify_kernel(const JobDev* jobs, const int64_t* out_firsts, const int* order_job, int n_jobs,
+                                    int64_t total, const int64_t* state, uint8_t* marks, int band_px, int32_t* list,
+                                    BatchCounters* c) {
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (row >= total) return;
+    const JobDev& J = row_job(jobs, out_firsts, order_job, n_jobs, row);
+    const int64_t local = row - J.out_first;
+    if (verify_pixel(state + J.out_first, marks + J.out_first, J.B.n, band_px, local)) {
+        marks[row] = MARK_QUEUED;
+        list[atomicAdd(&c->listed[0], 1ull)] = static_cast<int32_t>(row);
+    }
+}
+
+// Repair round `round` of the precision form (repair_kernel of kr_mixed.hip
+// over the rows of a batch).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void batch_repair_kernel(
+    const JobDev* jobs, const int64_t* out_firsts, const int* order_job, int n_jobs, int round, int band_px,
+    const int32_t* list, int32_t* next, BatchCounters* c, uint8_t* marks, BatchOut<double> out, int32_t* flags) {
+    const int64_t count = static_cast<int64_t>(c->listed[round]);
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < count; p += stride) {
+        const int64_t row = list[p];
+        const JobDev& J = row_job(jobs, out_firsts, order_job, n_jobs, row);
+        const int64_t local = row - J.out_first;
+        const int64_t i = local / J.B.n, j = local - i * J.B.n;
+        const bool was_captured = out.state[row] == CAPTURED;
+        double y0[8];
+        image_state(J.B, pixel_centre(J.B.fov, j, J.B.n), -pixel_centre(J.B.fov, i, J.B.n), y0);
+        RayState<double> R;
+        ray_init<double, true>(J.P, y0, R);
+        while (R.state == RUNNING) ray_attempt<double, true, false>(J.P, R);
+        store_row(out, row, R);
+        mark_or(marks, row, MARK_FP64);
+        if ((R.state == CAPTURED) == was_captured) continue;
+        for_each_neighbour(J.B.n, band_px, local, [&](int64_t pos) {
+            const int64_t g = J.out_first + pos;
+            if (mark_or(marks, g, MARK_QUEUED) != 0) return;
+            const unsigned long long slot = atomicAdd(&c->listed[round + 1], 1ull);
+            if (round + 1 < REPAIR_ROUNDS)
+                next[slot] = static_cast<int32_t>(g);
+            else
+                atomicOr(flags, FLAG_REPAIR);
+        });
+    }
+}
+
 using BatchFn = void (*)(BatchArgs, JobDev, BatchOut<double>);
 
-template <bool U, int BLOCK>
+template <bool U, int BLOCK, bool MIXED>
 BatchFn pick_waves(int waves, bool lds_slopes) {
-    if (waves >= 3) return batch_kernel<double, U, BLOCK, 3, false>;
-    return lds_slopes ? batch_kernel<double, U, BLOCK, 2, true> : batch_kernel<double, U, BLOCK, 2, false>;
+    if (waves >= 3) return batch_kernel<double, U, BLOCK, 3, false, MIXED>;
+    return lds_slopes ? batch_kernel<double, U, BLOCK, 2, true, MIXED> : batch_kernel<double, U, BLOCK, 2, false, MIXED>;
 }
 
 template <bool U>
-BatchFn pick(int block, int waves, bool lds_slopes) {
-    if (block == 64) return pick_waves<U, 64>(waves, lds_slopes);
-    if (block == 128) return pick_waves<U, 128>(waves, lds_slopes);
-    return pick_waves<U, 256>(waves, lds_slopes);
+BatchFn pick(int block, int waves, bool lds_slopes, bool mixed = false) {
+    if (mixed) {
+        if (block == 64) return pick_waves<U, 64, true>(waves, lds_slopes);
+        if (block == 128) return pick_waves<U, 128, true>(waves, lds_slopes);
+        return pick_waves<U, 256, true>(waves, lds_slopes);
+    }
+    if (block == 64) return pick_waves<U, 64, false>(waves, lds_slopes);
+    if (block == 128) return pick_waves<U, 128, false>(waves, lds_slopes);
+    return pick_waves<U, 256, false>(waves, lds_slopes);
 }
 
 }  // namespace kr
@@ -204,6 +317,15 @@ struct kr_batch_s {
     size_t lds_bytes = 0;     // dynamic LDS per block
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // Precision form (kr_batch_create_precision).
+    bool precision = false;
+    int band_px = 0;
+    void* pmem = nullptr;  // radii, marks, repair lists, counters, row tables
+    uint8_t* d_marks = nullptr;
+    int32_t* d_lists[2] = {nullptr, nullptr};
+    BatchCounters* d_counters = nullptr;
+    int64_t* d_out_firsts = nullptr;
+    int* d_order_job = nullptr;
 };
 
 namespace {
@@ -252,7 +374,7 @@ kr_status build_jobs(kr_context* h, const kr_job* jobs, int32_t n_jobs, int orde
 }
 
 hipError_t enqueue(kr_batch_s* b, hipStream_t s) {
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, b->d_counter, b->d_flags);
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, b->d_counter, b->d_flags, b->d_counters);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int cus = b->cus;
@@ -262,13 +384,26 @@ hipError_t enqueue(kr_batch_s* b, hipStream_t s) {
     const unsigned grid = static_cast<unsigned>(cus * blocks_per_cu);
     const bool uniform = b->jobs.size() == 1;
     BatchArgs args{b->d_jobs, static_cast<int>(b->jobs.size()), b->tune.refill_interval, b->total, b->d_counter,
-                   b->d_flags};
+                   b->d_flags, b->d_marks, b->d_counters};
     BatchOut<double> o{static_cast<double*>(b->out.Y), b->out.lam, b->out.n_steps, b->out.n_rejected, b->out.state,
                        b->out.max_null_error, b->out.max_energy_drift, b->out.max_lz_drift, b->out.max_carter_drift};
-    BatchFn fn = uniform ? pick<true>(block, b->tune.waves_per_simd, b->lds_slopes)
-                         : pick<false>(block, b->tune.waves_per_simd, b->lds_slopes);
+    BatchFn fn = uniform ? pick<true>(block, b->tune.waves_per_simd, b->lds_slopes, b->precision)
+                         : pick<false>(block, b->tune.waves_per_simd, b->lds_slopes, b->precision);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(block), b->lds_bytes, s, args, b->jobs[0], o);
-    return hipGetLastError();
+    e = hipGetLastError();
+    if (e != hipSuccess || !b->precision) return e;
+    const int n_jobs = static_cast<int>(b->jobs.size());
+    hipLaunchKernelGGL(batch_verify_kernel, dim3(static_cast<unsigned>((b->total + 255) / 256)), dim3(256), 0, s,
+                       b->d_jobs, b->d_out_firsts, b->d_order_job, n_jobs, b->total, b->out.state, b->d_marks,
+                       b->band_px, b->d_lists[0], b->d_counters);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    for (int round = 0; round < REPAIR_ROUNDS; ++round) {
+        hipLaunchKernelGGL(batch_repair_kernel, dim3(static_cast<unsigned>(cus * 2)), dim3(256), 0, s, b->d_jobs,
+                           b->d_out_firsts, b->d_order_job, n_jobs, round, b->band_px, b->d_lists[round & 1],
+                           b->d_lists[(round + 1) & 1], b->d_counters, b->d_marks, o, b->d_flags);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace
@@ -388,6 +523,116 @@ extern "C" kr_status kr_batch_destroy(kr_batch b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->dmem) (void)hipFree(b->dmem);
+    if (b->pmem) (void)hipFree(b->pmem);
     delete b;
+    return KR_OK;
+}
+
+extern "C" kr_status kr_batch_create_precision(kr_handle h, const kr_job* jobs, int32_t n_jobs, const kr_tune* tune,
+                                               const kr_batch_precision* prec, const kr_outputs* d_out,
+                                               kr_batch* out) {
+    host::clear_error();
+    if (!out) return host::fail(KR_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!prec) return host::fail(KR_ERR_INVALID, "precision options are NULL");
+    if (prec->band_px < 0 || prec->band_px > 64) return host::fail(KR_ERR_INVALID, "band_px must be in [0, 64]");
+    if (prec->n_angles < 8 || prec->n_angles > 8192) return host::fail(KR_ERR_INVALID, "n_angles must be in [8, 8192]");
+    if (!(prec->axis_lz >= 0.0)) return host::fail(KR_ERR_INVALID, "axis_lz must be >= 0");
+    if (!(prec->fp32_min_rtol >= 0.0)) return host::fail(KR_ERR_INVALID, "fp32_min_rtol must be >= 0");
+    if (!prec->centre_alpha || !prec->centre_beta || !prec->radii)
+        return host::fail(KR_ERR_INVALID, "centre or radii array is NULL");
+    kr_batch b = nullptr;
+    kr_status st = kr_batch_create(h, jobs, n_jobs, tune, d_out, &b);
+    if (st != KR_OK) return st;
+    if (b->total >= (int64_t(1) << 31)) {
+        kr_batch_destroy(b);
+        return host::fail(KR_ERR_INVALID, "the precision form of a batch takes fewer than 2^31 photons");
+    }
+    // Job-order tables for the row -> job lookup, and each job's plan.
+    const int nj = n_jobs;
+    std::vector<int64_t> out_firsts(nj);
+    std::vector<int> order_job(nj);
+    for (int k = 0; k < nj; ++k) {
+        // the job table is in processing order; out_first is monotone in job order
+        int pos = 0;
+        for (int m = 0; m < nj; ++m)
+            if (b->jobs[m].out_first < b->jobs[k].out_first) ++pos;
+        out_firsts[pos] = b->jobs[k].out_first;
+        order_job[pos] = k;
+    }
+    host::Carver c;
+    const size_t o_radii = c.take(static_cast<int64_t>(nj) * prec->n_angles * 8), o_marks = c.take(b->total),
+                 o_l0 = c.take(b->total * 4), o_l1 = c.take(b->total * 4), o_cnt = c.take(sizeof(BatchCounters)),
+                 o_of = c.take(nj * 8), o_oj = c.take(nj * 4);
+    hipError_t e = hipMalloc(&b->pmem, c.total);
+    if (e == hipSuccess) {
+        char* base = static_cast<char*>(b->pmem);
+        double* d_radii = reinterpret_cast<double*>(base + o_radii);
+        b->d_marks = reinterpret_cast<uint8_t*>(base + o_marks);
+        b->d_lists[0] = reinterpret_cast<int32_t*>(base + o_l0);
+        b->d_lists[1] = reinterpret_cast<int32_t*>(base + o_l1);
+        b->d_counters = reinterpret_cast<BatchCounters*>(base + o_cnt);
+        b->d_out_firsts = reinterpret_cast<int64_t*>(base + o_of);
+        b->d_order_job = reinterpret_cast<int*>(base + o_oj);
+        // The caller's arrays are in job order; the table is in processing order.
+        for (int k = 0; k < nj; ++k) {
+            JobDev& J = b->jobs[order_job[k]];
+            const double m = J.P.geom[G_MASS];
+            J.r_plus = m + std::sqrt(m * m - J.P.geom[G_A2]);
+            // An image below fp32_min_rtol is all band: every photon exact.
+            const bool fp32 = J.P.consts[C_RTOL] >= prec->fp32_min_rtol;
+            J.plan = PlanDev{prec->centre_alpha[k], prec->centre_beta[k], d_radii + static_cast<int64_t>(k) * prec->n_angles,
+                             prec->n_angles,
+                             fp32 ? (prec->band_px + 1.0) * (2.0 * J.B.fov / static_cast<double>(J.B.n)) : INFINITY,
+                             prec->axis_lz};
+        }
+        e = hipMemcpy(d_radii, prec->radii, static_cast<size_t>(nj) * prec->n_angles * 8, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(b->d_out_firsts, out_firsts.data(), nj * 8, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(b->d_order_job, order_job.data(), nj * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(b->d_jobs, b->jobs.data(), b->jobs.size() * sizeof(JobDev), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        kr_batch_destroy(b);
+        return host::fail(KR_ERR_DEVICE, std::string("kr_batch_create_precision: ") + hipGetErrorString(e));
+    }
+    b->precision = true;
+    b->band_px = prec->band_px;
+    // The precision kernel needs the device's LDS limit raised as the FP64 one does.
+    if (b->lds_slopes && b->lds_bytes > 65536) {
+        const bool uniform = b->jobs.size() == 1;
+        BatchFn fn = uniform ? pick<true>(b->tune.block_size, 2, true, true) : pick<false>(b->tune.block_size, 2, true, true);
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                static_cast<int>(b->lds_bytes)) != hipSuccess) {
+            (void)hipGetLastError();
+            kr_batch_destroy(b);
+            return host::fail(KR_ERR_DEVICE, "kr_batch_create_precision: the LDS request was refused");
+        }
+    }
+    *out = b;
+    return KR_OK;
+}
+
+extern "C" kr_status kr_batch_stats(kr_batch b, kr_planned_stats* stats) {
+    host::clear_error();
+    if (!b || !stats) return host::fail(KR_ERR_INVALID, "batch or stats is NULL");
+    if (!b->precision) return host::fail(KR_ERR_INVALID, "the batch has no precision form");
+    KR_HIP(hipSetDevice(b->h->device));
+    BatchCounters c{};
+    int32_t flags = 0;
+    KR_HIP(hipDeviceSynchronize());
+    KR_HIP(hipMemcpy(&c, b->d_counters, sizeof(c), hipMemcpyDeviceToHost));
+    KR_HIP(hipMemcpy(&flags, b->d_flags, 4, hipMemcpyDeviceToHost));
+    std::memset(stats, 0, sizeof(*stats));
+    stats->planned_band = static_cast<int64_t>(c.band);
+    stats->planned_inside = -1;
+    stats->planned_outside = -1;
+    for (int r = 0; r < REPAIR_ROUNDS; ++r) {
+        stats->repaired += static_cast<int64_t>(c.listed[r]);
+        if (c.listed[r]) stats->repair_rounds = r + 1;
+    }
+    stats->unrepaired = static_cast<int64_t>(c.listed[REPAIR_ROUNDS]);
+    stats->fp64_attempts = static_cast<int64_t>(c.fp64_attempts);
+    stats->flags = flags;
     return KR_OK;
 }

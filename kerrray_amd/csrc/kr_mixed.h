@@ -213,4 +213,49 @@ struct MixedStepper {
     }
 };
 
+// Pixels are classed in NBINS bins by their distance d to the analytic edge,
+// which is what sets a photon's length (near the edge it orbits before it
+// leaves or falls in).  Bin 0: d <= band, or |xi| = |alpha sin theta_o| <
+// axis_lz (photons with L_z near 0 pass close to the poles); bin b >= 1:
+// band 2^(b-1) < d <= band 2^b, the last bin open-ended.  Bin 0 is traced in
+// FP64; the bin also sets a wavefront's issue priority (band_priority).
+constexpr int NBINS = 8;
+
+struct PlanDev {
+    double ca, cb;          // centre of the polar radii table
+    const double* radii;    // n_angles radii at angles 2 pi k / n_angles
+    int32_t n_angles;
+    double band;            // band half-width in units of M
+    double axis_lz;         // |xi| below which a pixel belongs to bin 0
+};
+
+struct PixelPlan {
+    int bin;
+    bool inside;
+};
+
+KR_HD PixelPlan plan_pixel(const PlanDev& pl, const BardeenImage& B, int64_t idx) {
+    const int64_t i = idx / B.n, j = idx - (idx / B.n) * B.n;
+    const double alpha = pixel_centre(B.fov, j, B.n);
+    const double da = alpha - pl.ca;
+    const double db = -pixel_centre(B.fov, i, B.n) - pl.cb;
+    const double rho = sqrt(da * da + db * db);
+    double phi = atan2(db, da);
+    if (phi < 0.0) phi += 2.0 * M_PI;
+    const double u = phi * (pl.n_angles / (2.0 * M_PI));
+    int k0 = static_cast<int>(floor(u));
+    const double w = u - k0;
+    k0 = ((k0 % pl.n_angles) + pl.n_angles) % pl.n_angles;
+    const int k1 = (k0 + 1) % pl.n_angles;
+    const double rc = (1.0 - w) * pl.radii[k0] + w * pl.radii[k1];
+    const double d = fabs(rho - rc);
+    int bin = 0;
+    if (!(d <= pl.band || fabs(alpha * B.sin_o) < pl.axis_lz)) {
+        bin = 1;
+        for (double edge = 2.0 * pl.band; bin < NBINS - 1 && d > edge; edge *= 2.0) ++bin;
+    }
+    return PixelPlan{bin, rho < rc};
+}
+
+
 }  // namespace kr

@@ -13,7 +13,8 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PAIRS = {"KrParams": "kr_params", "KrOutputs": "kr_outputs", "KrImage": "kr_image", "KrDisk": "kr_disk",
          "KrJob": "kr_job", "KrPlannedOpts": "kr_planned_opts", "KrPlannedStats": "kr_planned_stats",
-         "KrTune": "kr_tune", "KrImpactOpts": "kr_impact_opts", "KrImpactResult": "kr_impact_result"}
+         "KrTune": "kr_tune", "KrImpactOpts": "kr_impact_opts", "KrImpactResult": "kr_impact_result",
+         "KrBatchPrecision": "kr_batch_precision"}
 
 
 @pytest.fixture(scope="module")

@@ -96,3 +96,38 @@ def test_autotune_picks_a_supported_shape(hip) -> None:
     assert len(table) == 4 and tune.block_size in (64, 256) and tune.refill_interval in (1, 8)
     assert min(r["ms"] for r in table) == next(
         r["ms"] for r in table if (r["block_size"], r["refill_interval"]) == (tune.block_size, tune.refill_interval))
+
+
+def test_precision_batch_keeps_every_image_fp64_mask(hip) -> None:
+    """The precision form on the batch (FP32 bulk, FP64 band, verify/repair): each image's mask equals the
+    FP64 batch's; eager runs and graph replays give the same rows."""
+    import torch
+
+    from kerrray_amd import _lib
+    from kerrray_amd.batch import DeviceBatch
+    from kerrray_amd.planned import PlannedOptions
+
+    jobs = _small_ragged()
+    exact = DeviceBatch(hip, jobs, _lib.KrTune(256, 2, 8, 1))
+    exact.run(graph=False)
+    ref = exact.results()
+    exact.close()
+    for tune in ((256, 2, 8, 1), (64, 3, 4, 0)):
+        mixed = DeviceBatch(hip, jobs, _lib.KrTune(*tune), precision=PlannedOptions())
+        mixed.run(graph=False)
+        got = mixed.results()
+        st = mixed.stats()
+        print(f"[precision batch {tune}] band {st.planned_band}, repaired {st.repaired} in {st.repair_rounds} "
+              f"rounds, fp64 attempts {st.fp64_attempts}", flush=True)
+        for job, a, b in zip(jobs, got, ref):
+            assert np.array_equal(a.state == 2, b.state == 2), job
+            assert np.array_equal(a.state, b.state), job
+        for _ in range(2):
+            mixed.bufs["state"].fill_(-7)
+            torch.cuda.synchronize()
+            mixed.run(graph=True)
+            again = mixed.results()
+            for a, b in zip(got, again):
+                for name in FIELDS:
+                    assert np.array_equal(getattr(a, name), getattr(b, name)), name
+        mixed.close()

@@ -82,3 +82,33 @@ def test_edge720_a099_i85_fp64_rtol1e8(hip) -> None:
     assert float(edge.width.max()) <= 1e-11
     assert err["max"] < 2e-7
     assert np.all(np.isfinite(edge.radius))
+
+
+def test_ragged64_precision_form_keeps_every_fp64_mask(hip) -> None:
+    """The 64-image workload in the precision form: every image's mask equals the FP64 batch's."""
+    import torch
+
+    from kerrray_amd import _lib
+    from kerrray_amd.batch import DeviceBatch, ragged_workload
+    from kerrray_amd.planned import PlannedOptions
+
+    jobs = ragged_workload(0, 64)
+    tune = _lib.KrTune(256, 2, 8, 1)
+    exact = DeviceBatch(hip, jobs, tune)
+    ms_exact = exact.timed(repeats=2)
+    ref_state = exact.bufs["state"].clone()
+    exact.close()
+    mixed = DeviceBatch(hip, jobs, tune, precision=PlannedOptions())
+    ms_mixed = mixed.timed(repeats=2)
+    st = mixed.stats()
+    n = mixed.n
+    diff = []
+    for k in range(len(jobs)):
+        lo, hi = int(mixed.offsets[k]), int(mixed.offsets[k + 1])
+        if not torch.equal(mixed.bufs["state"][lo:hi] == 2, ref_state[lo:hi] == 2):
+            diff.append(k)
+    print(f"[ragged64 precision] {ms_mixed:.1f} ms ({n / (ms_mixed * 1e-3):.3g} photons/s) against FP64 "
+          f"{ms_exact:.1f} ms ({n / (ms_exact * 1e-3):.3g}); band {st.planned_band}, repaired {st.repaired} in "
+          f"{st.repair_rounds} rounds, fp64 attempts {st.fp64_attempts}", flush=True)
+    mixed.close()
+    assert not diff, diff
