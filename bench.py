@@ -343,9 +343,10 @@ class RowsRunner:
 
 
 def critical_path(hip, st, integ, term, n_steps, n_rej) -> dict:
-    """The photon with the most attempts in the timed image, traced alone in exact FP64 by the plain kernel
-    (kr_trace_points_dev, one lane): an approximate floor for a launch of the image, since the timed kernels
-    run that photon with their own register layout and clock."""
+    """The photon with the most attempts in the timed image, traced alone in exact FP64
+    (kr_trace_points_dev: one photon, carried by a lane pair that shares out its trigonometry, as the
+    timed form's band wavefronts carry it): an approximate floor for a launch of the image, since the
+    timed kernel runs that photon with its own register layout and clock."""
     import torch
 
     from kerrray_amd import _lib
@@ -372,7 +373,7 @@ def critical_path(hip, st, integ, term, n_steps, n_rej) -> dict:
     ms = timed(one, 3)
     alone = int(bufs["n_steps"].item() + bufs["n_rejected"].item())
     return {"pixel": [i, j], "attempts": int(attempts[k]), "attempts_alone_fp64": alone, "alone_ms": ms,
-            "us_per_attempt_alone": 1e3 * ms / max(1, alone), "kernel": "kr_trace_points_dev (plain FP64, one lane)"}
+            "us_per_attempt_alone": 1e3 * ms / max(1, alone), "kernel": "kr_trace_points_dev (exact FP64, one lane pair)"}
 
 
 def timed(run, repeats: int) -> float:

@@ -388,13 +388,16 @@ kr_status kr_batch_destroy(kr_batch b);
  *  split_roles (fp32_bulk only): the first ceil(band / 64) wavefronts trace
  *    only the FP64 band and strip, the others only FP32 photons, so no
  *    wavefront runs both attempts.  Needs longest_first (which puts the band
- *    first in the work list); ignored without it.
+ *    first in the work list); ignored without it.  1: at one wavefront per
+ *    SIMD each FP64 photon is carried by two lanes (ceil(band / 32)
+ *    wavefronts), which share out its trigonometry: the same rows, a shorter
+ *    longest photon; 2: always paired; 3: never paired.
  *  grid_fraction: share of the resident-wavefront slots (waves_per_simd per
  *    SIMD) the persistent kernel is launched on, in (0, 1].  0: 7/8 for the
  *    split FP32 form at one wavefront per SIMD up to 2^18 pixels and rtol
- *    >= 1e-6 (its FP32 photons then finish well before the longest FP64
- *    photon, and fewer active wavefronts let the clock those photons run at
- *    rise), 1 otherwise. */
+ *    >= 1e-6 (15/16 with paired FP64 photons; its FP32 photons then finish
+ *    well before the longest FP64 photon, and fewer active wavefronts let the
+ *    clock those photons run at rise), 1 otherwise. */
 typedef struct kr_planned_opts {
     double horizon_margin;
     double axis_sin;
@@ -407,7 +410,7 @@ typedef struct kr_planned_opts {
     int32_t longest_first; /* 1: work list band/strip -> inside -> outside; 0: row-major */
     int32_t band_priority; /* 1: wavefronts tracing the nearest photons issue first on their SIMD */
     int32_t waves_per_simd; /* 0 automatic, or 1-4: occupancy the trace kernel's registers are allocated for */
-    int32_t split_roles;    /* 1: wavefronts trace FP64 or FP32 photons, never both (fp32_bulk only) */
+    int32_t split_roles;    /* 1-3: wavefronts trace FP64 or FP32 photons, never both (fp32_bulk only) */
     double axis_lz; /* pixels with |alpha sin theta_o| < axis_lz are integrated in FP64 (near-polar photons) */
     double grid_fraction; /* 0 automatic, or (0, 1]: share of the wavefront slots launched */
 } kr_planned_opts;

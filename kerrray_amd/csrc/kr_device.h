@@ -173,16 +173,61 @@ KR_HD void sin_cos(double x, double& s, double& c) {
 #endif
 }
 
+// The metric's trigonometry: s = sin(th), c = cos(th) and s2 = sin(2 th),
+// one lane per photon.
+struct TrigSolo {
+    KR_HD void operator()(double th, double& s, double& c, double& s2) const {
+        sin_cos(th, s, c);
+        s2 = sin(2.0 * th);
+    }
+};
+
+#ifndef KR_HOST_BUILD
+// x of the other lane of this lane's pair (lanes 2k and 2k + 1): a DPP
+// quad_perm [1, 0, 3, 2] move of each 32-bit half.  Both lanes must be
+// active.
+__device__ __forceinline__ double pair_swap(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(b), 0xB1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(b >> 32), 0xB1, 0xF, 0xF, false);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+#else
+    return x;
+#endif
+}
+
+// The same three values with both lanes of a pair carrying one photon (the
+// same state in both, so the same control flow): the even lane evaluates
+// sincos(th), the odd one sincos(2 th) (2 th is exact), and each takes what
+// it lacks from the other.  The sine of a sincos call is bit for bit the
+// sin call's, so every value is the one TrigSolo gives, for one library call
+// per lane instead of two: the calls are long, and a wavefront whose photons
+// set the end of a launch runs alone on its SIMD, where each instruction it
+// issues is time on the critical path.
+struct TrigPair {
+    bool odd;
+    __device__ void operator()(double th, double& s, double& c, double& s2) const {
+        double sa, ca;
+        sincos(odd ? 2.0 * th : th, &sa, &ca);
+        const double so = pair_swap(sa), co = pair_swap(ca);
+        s = odd ? so : sa;
+        c = odd ? co : ca;
+        s2 = odd ? sa : so;
+    }
+};
+#endif
+
 // numba_kernel.py:111-154 (= geometry/metric.py:166-195,265-313). Also
 // returns sin(th), cos(th) for reuse by the classifier and diagnostics.
+template <class Trig = TrigSolo>
 KR_HD void inverse_metric_terms(const Params& P, double r, double th, double (&out)[15],
-                                                     double& s_out, double& c_out) {
+                                                     double& s_out, double& c_out, const Trig& trig = Trig{}) {
     const double m = P.geom[G_MASS];
     const double a2 = P.geom[G_A2];
-    double s, c;
-    sin_cos(th, s, c);
+    double s, c, sin2th;
+    trig(th, s, c, sin2th);
     const double s2 = s * s;
-    const double sin2th = sin(2.0 * th);
     const double r2 = r * r;
     const double sig = r2 + a2 * (c * c);
     const double dlt = r2 - P.geom[G_TWO_M] * r + a2;
@@ -226,11 +271,11 @@ KR_HD void inverse_metric_terms(const Params& P, double r, double th, double (&o
 
 // numba_kernel.py:158-197: f = dy/dlambda for the six integrated components
 // (f[p_t] = f[p_phi] = 0 identically).
-template <typename T>
+template <typename T, class Trig = TrigSolo>
 KR_HD void rhs(const Params& P, const T (&v)[NA], T p_t, T p_ph, T (&f)[NA], double& s,
-                                    double& c) {
+                                    double& c, const Trig& trig = Trig{}) {
     double g64[15];
-    inverse_metric_terms(P, static_cast<double>(v[1]), static_cast<double>(v[2]), g64, s, c);
+    inverse_metric_terms(P, static_cast<double>(v[1]), static_cast<double>(v[2]), g64, s, c, trig);
     T g[15];
 #pragma unroll
     for (int i = 0; i < 15; ++i) g[i] = static_cast<T>(g64[i]);
@@ -269,10 +314,10 @@ struct NoStages {
 // numba_kernel.py:201-238.  k0 = f(y) on entry (FSAL); on exit y_new, err and
 // k6 = f(y_new), plus sin/cos of theta at y_new.  `stages(s, ys)` sees the
 // state at which stage s's slope is evaluated (s = 0..5; stage 0's is y).
-template <typename T, class Stages = NoStages>
+template <typename T, class Stages = NoStages, class Trig = TrigSolo>
 KR_HD void dp5_step(const Params& P, const T (&y)[NA], T p_t, T p_ph, T h, const T (&k0)[NA],
                                          T (&y_new)[NA], T (&err)[NA], T (&k6)[NA], double& s_new, double& c_new,
-                                         const Stages& stages = Stages{}) {
+                                         const Stages& stages = Stages{}, const Trig& trig = Trig{}) {
     using namespace dp;
     T k1[NA], k2[NA], k3[NA], k4[NA], k5[NA], ys[NA];
     double s, c;
@@ -281,32 +326,32 @@ KR_HD void dp5_step(const Params& P, const T (&y)[NA], T p_t, T p_ph, T h, const
 #pragma unroll
     for (int i = 0; i < NA; ++i) ys[i] = y[i] + h * (T(a10) * k0[i]);
     stages(1, ys);
-    rhs(P, ys, pt, pph, k1, s, c);
+    rhs(P, ys, pt, pph, k1, s, c, trig);
 #pragma unroll
     for (int i = 0; i < NA; ++i) ys[i] = y[i] + h * (T(a20) * k0[i] + T(a21) * k1[i]);
     stages(2, ys);
-    rhs(P, ys, pt, pph, k2, s, c);
+    rhs(P, ys, pt, pph, k2, s, c, trig);
 #pragma unroll
     for (int i = 0; i < NA; ++i) ys[i] = y[i] + h * (T(a30) * k0[i] + T(a31) * k1[i] + T(a32) * k2[i]);
     stages(3, ys);
-    rhs(P, ys, pt, pph, k3, s, c);
+    rhs(P, ys, pt, pph, k3, s, c, trig);
 #pragma unroll
     for (int i = 0; i < NA; ++i)
         ys[i] = y[i] + h * (T(a40) * k0[i] + T(a41) * k1[i] + T(a42) * k2[i] + T(a43) * k3[i]);
     stages(4, ys);
-    rhs(P, ys, pt, pph, k4, s, c);
+    rhs(P, ys, pt, pph, k4, s, c, trig);
 #pragma unroll
     for (int i = 0; i < NA; ++i)
         ys[i] = y[i] + h * (T(a50) * k0[i] + T(a51) * k1[i] + T(a52) * k2[i] + T(a53) * k3[i] + T(a54) * k4[i]);
     stages(5, ys);
-    rhs(P, ys, pt, pph, k5, s, c);
+    rhs(P, ys, pt, pph, k5, s, c, trig);
     T esum[NA];
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
         y_new[i] = y[i] + h * (T(a60) * k0[i] + T(a62) * k2[i] + T(a63) * k3[i] + T(a64) * k4[i] + T(a65) * k5[i]);
         esum[i] = T(e0) * k0[i] + T(e2) * k2[i] + T(e3) * k3[i] + T(e4) * k4[i] + T(e5) * k5[i];
     }
-    rhs(P, y_new, pt, pph, k6, s_new, c_new);
+    rhs(P, y_new, pt, pph, k6, s_new, c_new, trig);
 #pragma unroll
     for (int i = 0; i < NA; ++i) err[i] = h * (esum[i] + T(e6) * k6[i]);
 }
@@ -329,40 +374,40 @@ struct SlopeStore {
 // store (k1's slot, no longer read) while the last right-hand side is
 // evaluated: for a store in LDS that takes six more values per lane out of
 // the register file across the deepest point of the attempt.
-template <typename T, bool PARK = false>
+template <typename T, bool PARK = false, class Trig = TrigSolo>
 KR_HD void dp5_step_stored(const Params& P, const T (&y)[NA], T p_t, T p_ph, T h, const T (&k0)[NA],
                            const SlopeStore<T>& K, T (&y_new)[NA], T (&err)[NA], T (&k6)[NA], double& s_new,
-                           double& c_new) {
+                           double& c_new, const Trig& trig = Trig{}) {
     using namespace dp;
     T k[NA], ys[NA];
     double s, c;
     const T pt = p_t + T(0), pph = p_ph + T(0);
 #pragma unroll
     for (int i = 0; i < NA; ++i) ys[i] = y[i] + h * (T(a10) * k0[i]);
-    rhs(P, ys, pt, pph, k, s, c);
+    rhs(P, ys, pt, pph, k, s, c, trig);
 #pragma unroll
     for (int i = 0; i < NA; ++i) K.at(0, i) = k[i];
 #pragma unroll
     for (int i = 0; i < NA; ++i) ys[i] = y[i] + h * (T(a20) * k0[i] + T(a21) * K.at(0, i));
-    rhs(P, ys, pt, pph, k, s, c);
+    rhs(P, ys, pt, pph, k, s, c, trig);
 #pragma unroll
     for (int i = 0; i < NA; ++i) K.at(1, i) = k[i];
 #pragma unroll
     for (int i = 0; i < NA; ++i) ys[i] = y[i] + h * (T(a30) * k0[i] + T(a31) * K.at(0, i) + T(a32) * K.at(1, i));
-    rhs(P, ys, pt, pph, k, s, c);
+    rhs(P, ys, pt, pph, k, s, c, trig);
 #pragma unroll
     for (int i = 0; i < NA; ++i) K.at(2, i) = k[i];
 #pragma unroll
     for (int i = 0; i < NA; ++i)
         ys[i] = y[i] + h * (T(a40) * k0[i] + T(a41) * K.at(0, i) + T(a42) * K.at(1, i) + T(a43) * K.at(2, i));
-    rhs(P, ys, pt, pph, k, s, c);
+    rhs(P, ys, pt, pph, k, s, c, trig);
 #pragma unroll
     for (int i = 0; i < NA; ++i) K.at(3, i) = k[i];
 #pragma unroll
     for (int i = 0; i < NA; ++i)
         ys[i] = y[i] + h * (T(a50) * k0[i] + T(a51) * K.at(0, i) + T(a52) * K.at(1, i) + T(a53) * K.at(2, i) +
                             T(a54) * K.at(3, i));
-    rhs(P, ys, pt, pph, k, s, c);  // k5
+    rhs(P, ys, pt, pph, k, s, c, trig);  // k5
     T esum[PARK ? 1 : NA];
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
@@ -371,7 +416,7 @@ KR_HD void dp5_step_stored(const Params& P, const T (&y)[NA], T p_t, T p_ph, T h
         T& e = PARK ? K.at(0, i) : esum[PARK ? 0 : i];
         e = T(e0) * k0[i] + T(e2) * K.at(1, i) + T(e3) * K.at(2, i) + T(e4) * K.at(3, i) + T(e5) * k[i];
     }
-    rhs(P, y_new, pt, pph, k6, s_new, c_new);
+    rhs(P, y_new, pt, pph, k6, s_new, c_new, trig);
 #pragma unroll
     for (int i = 0; i < NA; ++i) err[i] = h * ((PARK ? K.at(0, i) : esum[PARK ? 0 : i]) + T(e6) * k6[i]);
 }
@@ -628,8 +673,8 @@ struct RayState {
 // numba_kernel.py:344-375.  (Y0 arrives in float64 and is cast to T,
 // numba_backend.py:167; the drift baselines e0, lz0, q0 and the floors come
 // from the float64 Y0, numba_backend.py:174-178.)
-template <typename T, bool ADAPTIVE>
-KR_HD void ray_init(const Params& P, const double (&y64)[8], RayState<T>& R) {
+template <typename T, bool ADAPTIVE, class Trig = TrigSolo>
+KR_HD void ray_init(const Params& P, const double (&y64)[8], RayState<T>& R, const Trig& trig = Trig{}) {
     const double a2 = P.geom[G_A2];
 #pragma unroll
     for (int i = 0; i < NA; ++i) R.y[i] = static_cast<T>(y64[full_index(i)]);
@@ -644,7 +689,7 @@ KR_HD void ray_init(const Params& P, const double (&y64)[8], RayState<T>& R) {
     R.fl = fe * mass;
     R.fq = fe * fe * mass * mass;
     R.fe2 = fe * fe;
-    rhs(P, R.y, R.p_t, R.p_ph, R.f, R.s_cur, R.c_cur);
+    rhs(P, R.y, R.p_t, R.p_ph, R.f, R.s_cur, R.c_cur, trig);
     {
         // carter_constant(st, Y0) on the float64 Y0 (same expression as numba_kernel.py:325-336).
         const double th0 = y64[2];
@@ -675,12 +720,13 @@ KR_HD void ray_init(const Params& P, const double (&y64)[8], RayState<T>& R) {
 
 // The exact step of the reference: Dormand-Prince 5(4) or RK4 in the
 // working type.
-template <typename T, bool ADAPTIVE>
+template <typename T, bool ADAPTIVE, class Trig = TrigSolo>
 struct ExactStepper {
+    Trig trig = Trig{};
     KR_HD bool operator()(const Params& P, const RayState<T>& R, T h_eff, T (&y_new)[NA], T (&err)[NA],
                           T (&k6)[NA], double& s_new, double& c_new) const {
         if (ADAPTIVE)
-            dp5_step(P, R.y, R.p_t, R.p_ph, h_eff, R.f, y_new, err, k6, s_new, c_new);
+            dp5_step(P, R.y, R.p_t, R.p_ph, h_eff, R.f, y_new, err, k6, s_new, c_new, NoStages{}, trig);
         else
             rk4_step(P, R.y, R.p_t, R.p_ph, h_eff, R.f, y_new, k6, s_new, c_new);
         return true;
@@ -689,13 +735,14 @@ struct ExactStepper {
 
 // The exact attempt with the stage slopes in a SlopeStore (RK4 attempts
 // are short and keep theirs in registers); PARK as in dp5_step_stored.
-template <typename T, bool ADAPTIVE, bool PARK = false>
+template <typename T, bool ADAPTIVE, bool PARK = false, class Trig = TrigSolo>
 struct StoredStepper {
     SlopeStore<T> K;
+    Trig trig = Trig{};
     KR_HD bool operator()(const Params& P, const RayState<T>& R, T h_eff, T (&y_new)[NA], T (&err)[NA],
                           T (&k6)[NA], double& s_new, double& c_new) const {
         if (ADAPTIVE)
-            dp5_step_stored<T, PARK>(P, R.y, R.p_t, R.p_ph, h_eff, R.f, K, y_new, err, k6, s_new, c_new);
+            dp5_step_stored<T, PARK, Trig>(P, R.y, R.p_t, R.p_ph, h_eff, R.f, K, y_new, err, k6, s_new, c_new, trig);
         else
             rk4_step(P, R.y, R.p_t, R.p_ph, h_eff, R.f, y_new, k6, s_new, c_new);
         return true;

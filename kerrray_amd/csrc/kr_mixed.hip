@@ -201,7 +201,8 @@ __global__ __launch_bounds__(PLAN_BLOCK) void plan_scatter_kernel(int64_t total,
 // nothing.  It pays where throughput decides and costs where the longest
 // photon's latency does (one LDS round trip more per attempt).
 //
-// split (mixed form only): the first ceil(bin-0 pixels / 64) wavefronts of
+// split (mixed form only; 1 or 2): the first ceil(bin-0 pixels / 64)
+// wavefronts (ceil(bin-0 pixels / 32) with pairs, split == 2) of
 // the grid take work only from the bin-0 part of the list (the FP64
 // photons), the others only from the rest (FP32); a wavefront of the first
 // kind joins the FP32 part once the FP64 part is handed out and its own
@@ -254,9 +255,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
     unsigned long long* cursor = &c->work;
     const int64_t seg0 = MIXED && split ? static_cast<int64_t>(c->seg[0]) : 0;
     bool fp64_role = false;
+    // split == 2: the FP64 band photons are traced by two lanes each (both
+    // carry the same state; TrigPair shares out the trigonometry), 32 per
+    // wavefront.
+    const bool odd = (lane & 1) != 0;
+    bool pair = false;
     if (MIXED && split) {
         const int64_t wave = (static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x) >> 6;
-        fp64_role = wave < (seg0 + 63) / 64;
+        const int64_t per_wave = split == 2 ? 32 : 64;
+        fp64_role = wave < (seg0 + per_wave - 1) / per_wave;
+        pair = fp64_role && split == 2;
         if (fp64_role) {
             count = seg0;
         } else {
@@ -279,6 +287,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
             // The FP64 part is handed out and this wavefront's photons are
             // done: it joins the FP32 part instead of leaving its slot idle.
             fp64_role = false;
+            pair = false;
             exhausted = false;
             lo = seg0;
             count = total - seg0;
@@ -286,7 +295,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
         }
         if (!exhausted && (since >= refill_interval || !any_busy)) {
             since = 0;
-            const uint64_t idle = __ballot(!busy);
+            // In pair mode both lanes of a pair are idle together; the even
+            // ones stand for their pairs.
+            const uint64_t idle = __ballot(!busy) & (pair ? 0x5555555555555555ull : ~0ull);
             const int n_idle = __popcll(idle);
             if (n_idle > 0) {
                 const int leader = __ffsll(static_cast<unsigned long long>(idle)) - 1;
@@ -296,7 +307,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
                 if (base + static_cast<unsigned long long>(n_idle) >= static_cast<unsigned long long>(count))
                     exhausted = true;
                 if (!busy) {
-                    const uint64_t below = (lane == 0) ? 0ull : (idle & (~0ull >> (64 - lane)));
+                    const int me = pair ? (lane & ~1) : lane;
+                    const uint64_t below = (me == 0) ? 0ull : (idle & (~0ull >> (64 - me)));
                     const int64_t pos = static_cast<int64_t>(base) + __popcll(below);
                     if (pos < count) {
                         const int64_t w = work[lo + pos];
@@ -313,11 +325,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
                                 image_state(B, pixel_centre(B.fov, j, B.n), -pixel_centre(B.fov, i, B.n), y0);
                             if (bad && flags) atomicOr(flags, bad);
                         }
-                        ray_init<double, true>(P, y0, R);
+                        if (MIXED && pair)
+                            ray_init<double, true>(P, y0, R, TrigPair{odd});
+                        else
+                            ray_init<double, true>(P, y0, R);
                         fp64_att = 0;
                         if (R.state == RUNNING) {
                             busy = true;
-                        } else {
+                        } else if (!(pair && odd)) {
                             store_row(out, idx, R);
                             used_fp64[idx] = 1;
                         }
@@ -343,10 +358,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
         if (busy) {
             if (MIXED && !exact) {
                 ray_attempt_with<double, true, false>(P, R, MixedStepper<PARK>{G, M, &fp64_att, K});
+            } else if (MIXED && pair) {
+                ray_attempt_with<double, true, false>(P, R, StoredStepper<double, true, PARK, TrigPair>{K, {odd}});
             } else {
                 ray_attempt_with<double, true, false>(P, R, StoredStepper<double, true, PARK>{K});
             }
-            if (R.state != RUNNING) {
+            if (R.state != RUNNING && pair && odd) {
+                busy = false;  // the even lane of the pair writes the row
+            } else if (R.state != RUNNING) {
                 store_row(out, idx, R);
                 used_fp64[idx] = exact ? 1 : 0;
                 const int64_t n_fp64 = exact ? R.n_steps + R.n_rej : fp64_att;
@@ -479,7 +498,8 @@ kr_status check_opts(const kr_planned_opts* o) {
     if (o->waves_per_simd < 0 || o->waves_per_simd > 4)
         return host::fail(KR_ERR_INVALID, "waves_per_simd must be 0 (automatic), 1, 2, 3 or 4");
     if (!(o->axis_lz >= 0.0)) return host::fail(KR_ERR_INVALID, "axis_lz must be >= 0");
-    if (o->split_roles != 0 && o->split_roles != 1) return host::fail(KR_ERR_INVALID, "split_roles must be 0 or 1");
+    if (o->split_roles < 0 || o->split_roles > 3)
+        return host::fail(KR_ERR_INVALID, "split_roles must be 0, 1, 2 or 3");
     if (!(o->horizon_margin >= 0.0) || !(o->axis_sin >= 0.0) || !(o->turn_drdl >= 0.0) || !(o->turn_rmax >= 0.0))
         return host::fail(KR_ERR_INVALID, "switch thresholds must be >= 0");
     if (!(o->grid_fraction >= 0.0 && o->grid_fraction <= 1.0))
@@ -538,7 +558,7 @@ kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B,
     const bool mixed = o->fp32_bulk != 0;
     const int refill = o->refill_interval, prio = o->band_priority ? 1 : 0;
     // Split roles rely on the band coming first in the work list.
-    const int split = o->split_roles && o->longest_first ? 1 : 0;
+    int split = o->longest_first ? o->split_roles : 0;
     // Automatic occupancy: up to 2^19 pixels both forms are bounded by their
     // longest FP64 photons, which run fastest alone on their SIMD (one
     // wavefront per SIMD, 512^2, a=0.9: split FP32 form 3.29 against 4.1 ms,
@@ -561,10 +581,21 @@ kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B,
     // tolerances the FP32 bulk takes many more attempts and bounds the time
     // (512^2, a=0, rtol 1e-8: 7.84 against 7.29 ms; 256^2, rtol 1e-7: 4.53
     // against 4.41 ms), and above 2^18 pixels throughput decides: all slots.
+    // Pairs (split 2) where the longest FP64 photon bounds the launch, at one
+    // wavefront per SIMD (512^2, a=0.9: 3.11 against 3.51 ms, the same rows);
+    // where throughput decides the band's lanes are worth more one photon each
+    // (1024^2, a=0.99, two per SIMD: 6.59 against 6.21 ms).  3: split, never
+    // paired.
+    if (split == 1 && waves == 1) split = 2;
+    if (split == 3) split = 1;
     const bool small = pixels <= (int64_t(1) << 18);
     const bool loose = P.consts[C_RTOL] >= 1e-6;
+    // Paired band photons take twice the band's wavefronts from the FP32 part:
+    // 15/16 of the slots then (512^2, a=0.9, Bardeen image: 3.07 against 3.11
+    // ms at 7/8 and 3.15 at all; ZAMO camera: 2.94 against 3.13 and 2.79).
+    const double auto_frac = split == 2 ? 0.9375 : 0.875;
     const double frac =
-        o->grid_fraction > 0.0 ? o->grid_fraction : (mixed && split && waves == 1 && small && loose ? 0.875 : 1.0);
+        o->grid_fraction > 0.0 ? o->grid_fraction : (mixed && split && waves == 1 && small && loose ? auto_frac : 1.0);
     switch (o->block_size) {
         case 64:
             mixed ? launch_planned_w<64, true>(waves, large, cus, frac, s, P, B, M, w, refill, prio, split, bo)

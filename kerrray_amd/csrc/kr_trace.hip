@@ -106,6 +106,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     write_result<T, DISK>(out, idx, r);
 }
 
+// Given rows or points, FP64 DP5, no disk, at most PAIR_ROWS_MAX photons:
+// such a launch fills few SIMDs and lasts as long as its longest photon, so
+// each photon is carried by two lanes (lanes 2k and 2k + 1 hold the same
+// state; TrigPair shares out the trigonometry of every right-hand side) and
+// each wavefront has a SIMD of its own.  The rows are trace_kernel's bit for
+// bit; the even lane writes them.
+constexpr int64_t PAIR_ROWS_MAX = 16384;
+
+template <int IC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void trace_pair_kernel(
+    Params P, ICArgs ic, Outputs<double> out, int64_t n) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t idx = t >> 1;
+    const bool odd = (t & 1) != 0;
+    if (idx >= n) return;  // both lanes of a pair leave together
+    double y0[8];
+    if (IC == IC_Y0) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) y0[c] = ic.Y0[idx * 8 + c];
+    } else {
+        const int32_t bad = image_state(ic.B, ic.alpha[idx], ic.beta[idx], y0);
+        if (bad && ic.flags && !odd) atomicOr(ic.flags, bad);
+    }
+    const TrigPair trig{odd};
+    RayState<double> R;
+    ray_init<double, true>(P, y0, R, trig);
+    while (R.state == RUNNING) ray_attempt_with<double, true, false>(P, R, ExactStepper<double, true, TrigPair>{trig});
+    RayResult<double> r;
+    ray_result<double, false>(R, r);
+    if (!odd) write_result<double, false>(out, idx, r);
+}
+
 // One bisection round, probe kernel: lane (k, j) traces probe j of direction k.
 __device__ __forceinline__ double probe_radius(double lo, double hi, int j, int n_probe) {
     if (n_probe == 1) return 0.5 * (lo + hi);
@@ -358,6 +390,13 @@ hipError_t launch_trace_t(const Params& P, const ICArgs& ic, const kr_outputs& o
         threads = n;
     }
     if (threads == 0) return hipSuccess;
+    if constexpr (IC != IC_IMAGE && sizeof(T) == sizeof(double) && ADAPTIVE && !DISK) {
+        if (n <= PAIR_ROWS_MAX) {
+            hipLaunchKernelGGL((trace_pair_kernel<IC>), dim3(static_cast<unsigned>((2 * n + 255) / 256)), dim3(256),
+                               0, s, P, ic, typed<T>(o), n);
+            return hipGetLastError();
+        }
+    }
     const int64_t blocks = (threads + 255) / 256;
     hipLaunchKernelGGL((trace_kernel<T, ADAPTIVE, DISK, IC>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,
                        P, ic, typed<T>(o), n);

@@ -63,11 +63,15 @@ class PlannedOptions:
     # planned in FP64 from the start, with the edge band.
     axis_lz: float = 0.1
     # FP32 bulk only: wavefronts trace either the FP64 band and strip or FP32
-    # photons, never both in one wavefront.
-    split_roles: bool = True
+    # photons, never both in one wavefront (1, or True).  At one wavefront per
+    # SIMD each FP64 photon is then carried by two lanes that share out its
+    # trigonometry (the same rows, a shorter longest photon); 2: always
+    # paired, 3: never.
+    split_roles: int = 1
     # Share of the resident-wavefront slots the trace kernel is launched on,
     # in (0, 1]; 0: automatic (7/8 for the split FP32 form at one wavefront
-    # per SIMD up to 2^18 pixels and rtol >= 1e-6, whose FP32 photons finish
+    # per SIMD up to 2^18 pixels and rtol >= 1e-6, 15/16 when its FP64
+    # photons are paired, whose FP32 photons finish
     # well before the longest FP64 photon, so that fewer active wavefronts
     # leave the clock higher for it; all slots otherwise).
     grid_fraction: float = 0.0
@@ -86,7 +90,7 @@ class PlannedOptions:
         o.band_priority = 1 if self.band_priority else 0
         o.waves_per_simd = self.waves_per_simd
         o.axis_lz = self.axis_lz
-        o.split_roles = 1 if self.split_roles else 0
+        o.split_roles = int(self.split_roles)
         o.grid_fraction = self.grid_fraction
         return o
 

@@ -127,6 +127,25 @@ def test_y0_path_equals_image_path(hip) -> None:
         assert np.array_equal(getattr(img.result, name), getattr(batch, name)), name
 
 
+def test_paired_rows_equal_single_lane_rows(hip) -> None:
+    """Launches of up to 16384 rows carry each photon on a lane pair (the even lane's sincos(theta), the odd
+    lane's sincos(2 theta), exchanged by DPP); one row more takes the one-lane kernel. The rows they share
+    are equal bit for bit, and so are the point entry's."""
+    from kerrray_amd.camera import BardeenCamera, bardeen_initial_states
+    from kerrray_amd.geometry import Spacetime
+
+    st = Spacetime(1.0, 0.9)
+    integ, term = options(rtol=1e-7, atol=1e-9)
+    Y0 = bardeen_initial_states(BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=12.0, resolution=128), st)
+    paired = hip.integrate_batch(st, Y0, integ, term)
+    single = hip.integrate_batch(st, np.concatenate([Y0, Y0[:1]]), integ, term)
+    assert len(Y0) == 16384
+    for name in ("Y", "state", "n_steps", "n_rejected", "lam", "max_null_error", "max_carter_drift"):
+        assert np.array_equal(getattr(paired, name), getattr(single, name)[:16384]), name
+    print(f"[pairs] 16384 rows: {int((paired.state == 2).sum())} captured, bitwise equal to one lane each",
+          flush=True)
+
+
 def test_batch_matches_oracle_kerr(hip, oracle) -> None:
     from kerrray_amd.camera import BardeenCamera, bardeen_initial_states
     from kerrray_amd.geometry import Spacetime

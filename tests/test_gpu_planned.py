@@ -66,8 +66,10 @@ def test_fp32_bulk_with_fp64_repair_keeps_the_fp64_mask(hip, case) -> None:
 def test_split_roles_give_the_same_rows(hip, case) -> None:
     """Wavefronts of one precision each (split_roles, with the automatic occupancy: one wavefront per SIMD
     at 512^2, two at 1024^2 with the error sums parked in LDS) trace every photon exactly as mixed
-    wavefronts at one and at two per SIMD do: a photon's arithmetic depends neither on its neighbours nor
-    on where its values are kept."""
+    wavefronts at one and at two per SIMD do, and so do FP64 photons carried by lane pairs (TrigPair:
+    the even lane's sincos(theta) and the odd lane's sincos(2 theta) exchanged by DPP) and split roles
+    without pairs: a photon's arithmetic depends neither on its neighbours nor on where its values are
+    kept."""
     from kerrray_amd.camera import BardeenCamera
     from kerrray_amd.geometry import Spacetime
     from kerrray_amd.planned import PlannedOptions, compute_shadow_planned
@@ -82,9 +84,15 @@ def test_split_roles_give_the_same_rows(hip, case) -> None:
                                        options=PlannedOptions(fp32_bulk=True, split_roles=True))
     one, s2 = compute_shadow_planned(st, cam, integ, term, backend=hip,
                                      options=PlannedOptions(fp32_bulk=True, split_roles=False, waves_per_simd=1))
+    # Paired FP64 lanes (automatic at one wavefront per SIMD, forced at two) and never-paired split roles.
+    paired, s3 = compute_shadow_planned(st, cam, integ, term, backend=hip,
+                                        options=PlannedOptions(fp32_bulk=True, split_roles=2))
+    unpaired, s4 = compute_shadow_planned(st, cam, integ, term, backend=hip,
+                                          options=PlannedOptions(fp32_bulk=True, split_roles=3))
     print(f"[split roles a={spin} {res}^2] {s1.kernel_ms:.3f} ms (mixed wavefronts {s0.kernel_ms:.3f} ms at two "
-          f"per SIMD, {s2.kernel_ms:.3f} ms at one), repaired {s1.repaired}", flush=True)
-    for other, so in ((mixed, s0), (one, s2)):
+          f"per SIMD, {s2.kernel_ms:.3f} ms at one; paired {s3.kernel_ms:.3f} ms, never paired {s4.kernel_ms:.3f} ms), "
+          f"repaired {s1.repaired}", flush=True)
+    for other, so in ((mixed, s0), (one, s2), (paired, s3), (unpaired, s4)):
         for name in FIELDS:
             assert np.array_equal(getattr(split.result, name), getattr(other.result, name)), name
         assert s1.fp64_attempts == so.fp64_attempts and s1.repaired == so.repaired
