@@ -232,8 +232,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
         for (int k = 0; k < 21; ++k) asm volatile("" : "+v"(P.consts[k]));
 #pragma unroll
         for (int k = 0; k < 3; ++k) asm volatile("" : "+v"(P.params[k]));
+    }
+    if (WAVES == 1 && !MIXED) {
         // The image's initial-state scalars (read when a lane refills); the
-        // pixel count and camera kind stay scalar.
+        // pixel count and camera kind stay scalar.  (Not in the mixed kernel,
+        // which holds both attempts and has no vector registers to spare:
+        // 512^2, a=0.9, FP32 form 3.04 against 3.09 ms, FP64 form 4.44
+        // against 4.38 ms with them.)
         asm volatile("" : "+v"(B.r_o), "+v"(B.theta_o), "+v"(B.sin_o), "+v"(B.a), "+v"(B.a2c2), "+v"(B.cot2));
         asm volatile("" : "+v"(B.r2a2), "+v"(B.dlt), "+v"(B.c2), "+v"(B.s2), "+v"(B.aaEE), "+v"(B.fov));
         asm volatile("" : "+v"(B.Z.r_o), "+v"(B.Z.theta_o), "+v"(B.Z.phi_o), "+v"(B.Z.r_o2), "+v"(B.Z.e00),
@@ -584,8 +589,10 @@ kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B,
     // Pairs (split 2) where the longest FP64 photon bounds the launch, at one
     // wavefront per SIMD (512^2, a=0.9: 3.11 against 3.51 ms, the same rows);
     // where throughput decides the band's lanes are worth more one photon each
-    // (1024^2, a=0.99, two per SIMD: 6.59 against 6.21 ms).  3: split, never
-    // paired.
+    // (1024^2, a=0.99, two per SIMD: 6.59 against 6.21 ms), and so they are
+    // in the FP64 form, whose bulk bounds it (512^2: band pairs with the rest
+    // one photon per lane measured 4.82 ms at one wavefront per SIMD, 4.95 at
+    // two with band priority, against 4.50 unsplit).  3: split, never paired.
     if (split == 1 && waves == 1) split = 2;
     if (split == 3) split = 1;
     const bool small = pixels <= (int64_t(1) << 18);
