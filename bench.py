@@ -804,7 +804,8 @@ def main() -> int:
                         "results bitwise those of the plain kernel",
         "fp32_fp64_repair": "FP32 right-hand sides with FP64 state/update/error norm/step control; the edge band "
                             "and polar strip traced in exact FP64 by wavefronts of their own (one per SIMD at "
-                            "this size); FP64 verify and repair; mask identical to the FP64 mask",
+                            "this size, two lanes per photon sharing its trigonometry); FP64 verify and repair; "
+                            "mask identical to the FP64 mask",
     }
     line = {
         "metric": METRIC,
