@@ -342,6 +342,37 @@ class RowsRunner:
                    "kr_integrate_batch_dev")
 
 
+def end_to_end(hip, st, integ, term, ref_state) -> dict:
+    """The same 512^2 image through the host-buffer entry points, wall clock per call (PCIe included):
+    the FP32 form through kr_trace_image_planned (compute_shadow_planned), and the drop-in row path the
+    reference's own callers take, HipBackend.integrate_batch on host initial states (H2D of Y0, D2H of
+    the rows). Reported beside `value`, never as it."""
+    from kerrray_amd.camera import BardeenCamera, bardeen_initial_states
+    from kerrray_amd.planned import PlannedOptions, compute_shadow_planned
+
+    cam = BardeenCamera(radius=R_O, inclination_deg=INCL, fov=FOV, resolution=RES)
+    n = cam.n_rays
+
+    def wall(fn, repeats: int = 5) -> tuple[float, object]:
+        out = fn()
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            out = fn()
+        return 1e3 * (time.perf_counter() - t0) / repeats, out
+
+    ms_p, (img, _) = wall(lambda: compute_shadow_planned(st, cam, integ, term, backend=hip,
+                                                         options=PlannedOptions(fp32_bulk=True)))
+    y0 = bardeen_initial_states(cam, st)
+    ms_b, res = wall(lambda: hip.integrate_batch(st, y0, integ, term))
+    same_p = None if ref_state is None else bool(np.array_equal(img.result.state, ref_state))
+    same_b = None if ref_state is None else bool(np.array_equal(res.state, ref_state))
+    return {"workload": "the timed 512^2 image through host buffers, wall clock per call, best of 1 + 5",
+            "fp32_fp64_repair_host_ms": ms_p, "fp32_fp64_repair_host_photons_per_s": n / (ms_p * 1e-3),
+            "fp32_fp64_repair_host_mask_identical": same_p,
+            "integrate_batch_host_y0_ms": ms_b, "integrate_batch_host_y0_photons_per_s": n / (ms_b * 1e-3),
+            "integrate_batch_host_y0_mask_identical": same_b}
+
+
 def critical_path(hip, st, integ, term, n_steps, n_rej) -> dict:
     """The photon with the most attempts in the timed image, traced alone in exact FP64
     (kr_trace_points_dev: one photon, carried by a lane pair that shares out its trigonometry, as the
@@ -784,6 +815,9 @@ def main() -> int:
                               bool(np.array_equal(orun.bufs["state"].cpu().numpy(), ref))}
         orun.stats()
         log(f"{other} over the same {args.steps} steps: {wo:.3f} ms/step")
+    e2e = end_to_end(hip, st, integ, term, ref)
+    log(f"host buffers (PCIe included): FP32 form {e2e['fp32_fp64_repair_host_ms']:.2f} ms per call, "
+        f"integrate_batch on host Y0 {e2e['integrate_batch_host_y0_ms']:.2f} ms")
     crit = critical_path(hip, st, integ, term, run.bufs["n_steps"].cpu().numpy(), run.bufs["n_rejected"].cpu().numpy())
     crit["frac_of_ms_per_launch"] = crit["alone_ms"] / kernel_ms
     log(f"longest photon alone: {crit['attempts_alone_fp64']} attempts, {crit['alone_ms']:.3f} ms "
@@ -851,6 +885,7 @@ def main() -> int:
         line["roofline"]["traffic_source"] = t[chosen]["source"]
     line["forms"] = forms
     line["timed_forms"] = timed_forms
+    line["end_to_end"] = e2e
     log("persistent kernel, single image (uniform constants)")
     forms["fp64_persistent"] = persistent_single(hip, ref)
     if not args.no_batch:
