@@ -366,7 +366,8 @@ def end_to_end(hip, st, integ, term, ref_state) -> dict:
     ms_b, res = wall(lambda: hip.integrate_batch(st, y0, integ, term))
     same_p = None if ref_state is None else bool(np.array_equal(img.result.state, ref_state))
     same_b = None if ref_state is None else bool(np.array_equal(res.state, ref_state))
-    return {"workload": "the timed 512^2 image through host buffers, wall clock per call, best of 1 + 5",
+    return {"workload": "the timed 512^2 image through host buffers, wall clock per call, mean of 5 after one "
+                        "warm-up call (fresh host output arrays each call, as the reference's callers get)",
             "fp32_fp64_repair_host_ms": ms_p, "fp32_fp64_repair_host_photons_per_s": n / (ms_p * 1e-3),
             "fp32_fp64_repair_host_mask_identical": same_p,
             "integrate_batch_host_y0_ms": ms_b, "integrate_batch_host_y0_photons_per_s": n / (ms_b * 1e-3),
