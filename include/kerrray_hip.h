@@ -389,9 +389,9 @@ kr_status kr_batch_destroy(kr_batch b);
  *    only the FP64 band and strip, the others only FP32 photons, so no
  *    wavefront runs both attempts.  Needs longest_first (which puts the band
  *    first in the work list); ignored without it.  1: at one wavefront per
- *    SIMD each FP64 photon is carried by two lanes (ceil(band / 32)
- *    wavefronts), which share out its trigonometry: the same rows, a shorter
- *    longest photon; 2: always paired; 3: never paired.
+ *    SIMD and rtol >= 1e-7 each FP64 photon is carried by two lanes
+ *    (ceil(band / 32) wavefronts), which share out its trigonometry: the
+ *    same rows, a shorter longest photon; 2: always paired; 3: never paired.
  *  grid_fraction: share of the resident-wavefront slots (waves_per_simd per
  *    SIMD) the persistent kernel is launched on, in (0, 1].  0: 7/8 for the
  *    split FP32 form at one wavefront per SIMD up to 2^18 pixels and rtol

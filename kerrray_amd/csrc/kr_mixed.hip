@@ -592,8 +592,12 @@ kr_status enqueue_planned(kr_context* h, const Params& P, const BardeenImage& B,
     // (1024^2, a=0.99, two per SIMD: 6.59 against 6.21 ms), and so they are
     // in the FP64 form, whose bulk bounds it (512^2: band pairs with the rest
     // one photon per lane measured 4.82 ms at one wavefront per SIMD, 4.95 at
-    // two with band priority, against 4.50 unsplit).  3: split, never paired.
-    if (split == 1 && waves == 1) split = 2;
+    // two with band priority, against 4.50 unsplit), and at tight tolerances,
+    // where the FP32 bulk takes many more attempts (256^2, a=0, rtol 1e-8:
+    // 5.49 paired against 5.14 ms; 362^2, a=0.5, rtol 1e-7: 3.88 against
+    // 4.69; 512^2, a=0.99, i=85, rtol 1e-6: 3.77 against 4.10).  3: split,
+    // never paired.
+    if (split == 1 && waves == 1 && P.consts[C_RTOL] >= 1e-7) split = 2;
     if (split == 3) split = 1;
     const bool small = pixels <= (int64_t(1) << 18);
     const bool loose = P.consts[C_RTOL] >= 1e-6;

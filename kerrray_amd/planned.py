@@ -64,9 +64,9 @@ class PlannedOptions:
     axis_lz: float = 0.1
     # FP32 bulk only: wavefronts trace either the FP64 band and strip or FP32
     # photons, never both in one wavefront (1, or True).  At one wavefront per
-    # SIMD each FP64 photon is then carried by two lanes that share out its
-    # trigonometry (the same rows, a shorter longest photon); 2: always
-    # paired, 3: never.
+    # SIMD and rtol >= 1e-7 each FP64 photon is then carried by two lanes
+    # that share out its trigonometry (the same rows, a shorter longest
+    # photon); 2: always paired, 3: never.
     split_roles: int = 1
     # Share of the resident-wavefront slots the trace kernel is launched on,
     # in (0, 1]; 0: automatic (7/8 for the split FP32 form at one wavefront
