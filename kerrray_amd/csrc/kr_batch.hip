@@ -56,7 +56,6 @@ struct JobDev {
 
 // Counters of the precision form of a batch.
 struct BatchCounters {
-    unsigned long long band;           // photons of the FP64 band and polar strip
     unsigned long long fp64_attempts;  // attempts taken in exact FP64
     unsigned long long listed[REPAIR_ROUNDS + 1];
 };
@@ -78,14 +77,19 @@ __device__ __forceinline__ int find_job(const int64_t* firsts, int n_jobs, int64
     return lo;
 }
 
+// A launch integrates the photons at positions [begin, begin + count) of the
+// batch's work order: photon work[p] of the processing order (work == NULL:
+// photon p).
 struct BatchArgs {
     const JobDev* jobs;  // device copy of the job table
     int n_jobs;
     int refill_interval;
-    int64_t total;  // photons in the batch
+    const int32_t* work;  // precision form: exact photons first, then the FP32 ones
+    int64_t begin;
+    int64_t count;
     unsigned long long* counter;
     int32_t* flags;
-    uint8_t* marks;          // precision form: MARK_FP64 per output row
+    uint8_t* marks;          // precision form: MARK_FP64 per output row (NULL otherwise)
     BatchCounters* counters; // precision form
 };
 
@@ -99,10 +103,14 @@ size_t batch_lds_bytes(bool uniform, bool lds_slopes, int block, int n_jobs) {
            (uniform ? 0 : static_cast<size_t>(n_jobs) * (sizeof(int64_t) + sizeof(Params)));
 }
 
-// MIXED: the precision form.  A photon of its image's edge band or polar
-// strip (bin 0 of the image plan, kr_mixed.h) takes the exact FP64 attempt;
-// any other takes FP32 right-hand sides with the FP64 update, error norm and
-// step control.  Rows record which (marks), for the verify/repair pass.
+// The precision form runs a batch as two launches over one work order: the
+// photons of every image's edge band or polar strip (bin 0 of the image plan,
+// kr_mixed.h) and every photon of an image below fp32_min_rtol first, in the
+// exact FP64 kernel (MIXED = false), then all the others in the MIXED kernel:
+// FP32 right-hand sides with the FP64 update, error norm and step control.
+// So no wavefront carries both attempts, and the exact photons run in the
+// kernel that holds only the exact attempt.  Rows record which (marks), for
+// the verify/repair pass.
 template <typename T, bool UNIFORM, int BLOCK, int WAVES, bool LDS_SLOPES, bool MIXED = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void batch_kernel(
     BatchArgs a, JobDev uniform_job, BatchOut<T> out) {
@@ -130,7 +138,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
     R.state = RUNNING;
     bool busy = false;
     bool exhausted = false;  // wave-uniform: the counter has run past the end
-    bool exact = true;       // MIXED: this photon takes the exact attempt
     Geom32 G{};
     MixedCfg M{};
     int64_t fp64_att = 0;
@@ -148,13 +155,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
                 unsigned long long base = 0;
                 if (lane == leader) base = atomicAdd(a.counter, static_cast<unsigned long long>(n_idle));
                 base = __shfl(base, leader);
-                if (base + static_cast<unsigned long long>(n_idle) >= static_cast<unsigned long long>(a.total))
+                if (base + static_cast<unsigned long long>(n_idle) >= static_cast<unsigned long long>(a.count))
                     exhausted = true;
                 if (!busy) {
                     const uint64_t below = (lane == 0) ? 0ull : (idle & (~0ull >> (64 - lane)));
                     const int64_t mine = static_cast<int64_t>(base) + __popcll(below);
-                    if (mine < a.total) {
-                        idx = mine;
+                    if (mine < a.count) {
+                        const int64_t pos = a.begin + mine;
+                        idx = a.work ? static_cast<int64_t>(a.work[pos]) : pos;  // in processing order
                         job = UNIFORM ? 0 : find_job(firsts, a.n_jobs, idx);
                         const JobDev& J = UNIFORM ? uniform_job : a.jobs[job];
                         const int64_t local = idx - J.first;
@@ -166,8 +174,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
                         if (bad && a.flags) atomicOr(a.flags, bad);
                         ray_init<T, true>(UNIFORM ? uniform_job.P : params[job], y0, R);
                         if (MIXED) {
-                            exact = plan_pixel(J.plan, J.B, local).bin == 0;
-                            if (exact) atomicAdd(&a.counters->band, 1ull);
                             G = geom32(J.P);
                             M = MixedCfg{J.r_plus, 0.0, 0.0, 0.0};
                             fp64_att = 0;
@@ -176,7 +182,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
                             busy = true;
                         } else {
                             store_row(out, idx, R);
-                            if (MIXED) a.marks[idx] = MARK_FP64;
+                            if (a.marks) a.marks[idx] = MARK_FP64;
                         }
                     }
                 }
@@ -186,13 +192,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
         if (!any_busy) break;  // exhausted and nothing in flight
         if (busy) {
             const Params& P = UNIFORM ? uniform_job.P : params[job];
-            if (MIXED && !exact) {
-                if constexpr (MIXED) {
-                    if (LDS_SLOPES)
-                        ray_attempt_with<double, true, false>(P, R, MixedStepper<true>{G, M, &fp64_att, K});
-                    else
-                        ray_attempt_with<double, true, false>(P, R, MixedStepper<false>{G, M, &fp64_att, K});
-                }
+            if constexpr (MIXED) {
+                if (LDS_SLOPES)
+                    ray_attempt_with<double, true, false>(P, R, MixedStepper<true>{G, M, &fp64_att, K});
+                else
+                    ray_attempt_with<double, true, false>(P, R, MixedStepper<false>{G, M, &fp64_att, K});
             } else if (LDS_SLOPES) {
                 ray_attempt_with<T, true, false>(P, R, StoredStepper<T, true, true>{K});
             } else {
@@ -200,9 +204,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
             }
             if (R.state != RUNNING) {
                 store_row(out, idx, R);
-                if (MIXED) {
-                    a.marks[idx] = exact ? MARK_FP64 : 0;
-                    const int64_t n64 = exact ? R.n_steps + R.n_rej : fp64_att;
+                if (a.marks) {
+                    a.marks[idx] = MIXED ? 0 : MARK_FP64;
+                    const int64_t n64 = MIXED ? fp64_att : R.n_steps + R.n_rej;
                     if (n64) atomicAdd(&a.counters->fp64_attempts, static_cast<unsigned long long>(n64));
                 }
                 busy = false;
@@ -212,14 +216,33 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
     }
 }
 
-__global__ void batch_reset_kernel(unsigned long long* counter, int32_t* flags, BatchCounters* counters) {
+// Clears the launches' cursors (counter[0 .. n_counters)), the flags and the
+// precision counters.
+__global__ void batch_reset_kernel(unsigned long long* counter, int n_counters, int32_t* flags,
+                                   BatchCounters* counters) {
     constexpr int words = static_cast<int>(sizeof(BatchCounters) / 8);
     if (blockIdx.x == 0 && threadIdx.x < words && counters)
         reinterpret_cast<unsigned long long*>(counters)[threadIdx.x] = 0ull;
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        *counter = 0ull;
-        if (flags) *flags = 0;
+    if (blockIdx.x == 0 && threadIdx.x < n_counters) counter[threadIdx.x] = 0ull;
+    if (threadIdx.x == 0 && blockIdx.x == 0 && flags) *flags = 0;
+}
+
+// One byte per photon of the batch in processing order: 1 if its image's plan
+// traces it in exact FP64 (bin 0), else 0.  Run once, when a precision batch
+// is created, to build the work order.
+__global__ void batch_bins_kernel(const JobDev* jobs, int n_jobs, int64_t total, uint8_t* exact) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= total) return;
+    int lo = 0, hi = n_jobs - 1;  // the job table is in processing order: jobs[k].first ascends
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].first <= p)
+            lo = mid;
+        else
+            hi = mid - 1;
     }
+    const JobDev& J = jobs[lo];
+    exact[p] = plan_pixel(J.plan, J.B, p - J.first).bin == 0 ? 1 : 0;
 }
 
 // The job owning output row `row`: jobs are in processing order, rows in job
@@ -326,6 +349,8 @@ struct kr_batch_s {
     BatchCounters* d_counters = nullptr;
     int64_t* d_out_firsts = nullptr;
     int* d_order_job = nullptr;
+    int32_t* d_work = nullptr;  // work order: the n_exact exact photons, then the FP32 ones
+    int64_t n_exact = 0;
 };
 
 namespace {
@@ -374,7 +399,7 @@ kr_status build_jobs(kr_context* h, const kr_job* jobs, int32_t n_jobs, int orde
 }
 
 hipError_t enqueue(kr_batch_s* b, hipStream_t s) {
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, b->d_counter, b->d_flags, b->d_counters);
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, b->d_counter, 2, b->d_flags, b->d_counters);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int cus = b->cus;
@@ -383,15 +408,23 @@ hipError_t enqueue(kr_batch_s* b, hipStream_t s) {
     const int blocks_per_cu = (waves_per_cu * 64 + block - 1) / block;
     const unsigned grid = static_cast<unsigned>(cus * blocks_per_cu);
     const bool uniform = b->jobs.size() == 1;
-    BatchArgs args{b->d_jobs, static_cast<int>(b->jobs.size()), b->tune.refill_interval, b->total, b->d_counter,
-                   b->d_flags, b->d_marks, b->d_counters};
     BatchOut<double> o{static_cast<double*>(b->out.Y), b->out.lam, b->out.n_steps, b->out.n_rejected, b->out.state,
                        b->out.max_null_error, b->out.max_energy_drift, b->out.max_lz_drift, b->out.max_carter_drift};
-    BatchFn fn = uniform ? pick<true>(block, b->tune.waves_per_simd, b->lds_slopes, b->precision)
-                         : pick<false>(block, b->tune.waves_per_simd, b->lds_slopes, b->precision);
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(block), b->lds_bytes, s, args, b->jobs[0], o);
-    e = hipGetLastError();
-    if (e != hipSuccess || !b->precision) return e;
+    // The exact part (every photon, without the precision form), then the FP32 part; launches over an empty
+    // range are left out.  Each launch draws from a cursor of its own.
+    const int64_t parts[3] = {0, b->precision ? b->n_exact : b->total, b->total};
+    for (int part = 0; part < 2; ++part) {
+        if (parts[part + 1] == parts[part]) continue;
+        const bool mixed = part == 1;
+        BatchArgs args{b->d_jobs, static_cast<int>(b->jobs.size()), b->tune.refill_interval, b->d_work,
+                       parts[part], parts[part + 1] - parts[part], b->d_counter + part, b->d_flags, b->d_marks,
+                       b->d_counters};
+        BatchFn fn = uniform ? pick<true>(block, b->tune.waves_per_simd, b->lds_slopes, mixed)
+                             : pick<false>(block, b->tune.waves_per_simd, b->lds_slopes, mixed);
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(block), b->lds_bytes, s, args, b->jobs[0], o);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (!b->precision) return hipSuccess;
     const int n_jobs = static_cast<int>(b->jobs.size());
     hipLaunchKernelGGL(batch_verify_kernel, dim3(static_cast<unsigned>((b->total + 255) / 256)), dim3(256), 0, s,
                        b->d_jobs, b->d_out_firsts, b->d_order_job, n_jobs, b->total, b->out.state, b->d_marks,
@@ -563,7 +596,7 @@ extern "C" kr_status kr_batch_create_precision(kr_handle h, const kr_job* jobs, 
     host::Carver c;
     const size_t o_radii = c.take(static_cast<int64_t>(nj) * prec->n_angles * 8), o_marks = c.take(b->total),
                  o_l0 = c.take(b->total * 4), o_l1 = c.take(b->total * 4), o_cnt = c.take(sizeof(BatchCounters)),
-                 o_of = c.take(nj * 8), o_oj = c.take(nj * 4);
+                 o_of = c.take(nj * 8), o_oj = c.take(nj * 4), o_work = c.take(b->total * 4);
     hipError_t e = hipMalloc(&b->pmem, c.total);
     if (e == hipSuccess) {
         char* base = static_cast<char*>(b->pmem);
@@ -574,6 +607,7 @@ extern "C" kr_status kr_batch_create_precision(kr_handle h, const kr_job* jobs, 
         b->d_counters = reinterpret_cast<BatchCounters*>(base + o_cnt);
         b->d_out_firsts = reinterpret_cast<int64_t*>(base + o_of);
         b->d_order_job = reinterpret_cast<int*>(base + o_oj);
+        b->d_work = reinterpret_cast<int32_t*>(base + o_work);
         // The caller's arrays are in job order; the table is in processing order.
         for (int k = 0; k < nj; ++k) {
             JobDev& J = b->jobs[order_job[k]];
@@ -591,6 +625,33 @@ extern "C" kr_status kr_batch_create_precision(kr_handle h, const kr_job* jobs, 
         if (e == hipSuccess) e = hipMemcpy(b->d_order_job, order_job.data(), nj * 4, hipMemcpyHostToDevice);
         if (e == hipSuccess)
             e = hipMemcpy(b->d_jobs, b->jobs.data(), b->jobs.size() * sizeof(JobDev), hipMemcpyHostToDevice);
+        // The work order: each photon's bin from the device plan (the marks array serves as the buffer), then
+        // the exact photons and the FP32 ones, each part in processing order.
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(batch_bins_kernel, dim3(static_cast<unsigned>((b->total + 255) / 256)), dim3(256), 0,
+                               nullptr, b->d_jobs, nj, b->total, b->d_marks);
+            e = hipGetLastError();
+        }
+        std::vector<uint8_t> exact;
+        std::vector<int32_t> work;
+        if (e == hipSuccess) {
+            try {
+                exact.resize(static_cast<size_t>(b->total));
+                work.resize(static_cast<size_t>(b->total));
+            } catch (const std::bad_alloc&) {
+                e = hipErrorOutOfMemory;
+            }
+        }
+        if (e == hipSuccess) e = hipMemcpy(exact.data(), b->d_marks, static_cast<size_t>(b->total), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) {
+            int64_t n0 = 0;
+            for (int64_t p = 0; p < b->total; ++p)
+                if (exact[p]) work[n0++] = static_cast<int32_t>(p);
+            b->n_exact = n0;
+            for (int64_t p = 0; p < b->total; ++p)
+                if (!exact[p]) work[n0++] = static_cast<int32_t>(p);
+            e = hipMemcpy(b->d_work, work.data(), static_cast<size_t>(b->total) * 4, hipMemcpyHostToDevice);
+        }
     }
     if (e != hipSuccess) {
         kr_batch_destroy(b);
@@ -624,7 +685,7 @@ extern "C" kr_status kr_batch_stats(kr_batch b, kr_planned_stats* stats) {
     KR_HIP(hipMemcpy(&c, b->d_counters, sizeof(c), hipMemcpyDeviceToHost));
     KR_HIP(hipMemcpy(&flags, b->d_flags, 4, hipMemcpyDeviceToHost));
     std::memset(stats, 0, sizeof(*stats));
-    stats->planned_band = static_cast<int64_t>(c.band);
+    stats->planned_band = b->n_exact;
     stats->planned_inside = -1;
     stats->planned_outside = -1;
     for (int r = 0; r < REPAIR_ROUNDS; ++r) {
