@@ -341,8 +341,10 @@ typedef struct kr_tune {
 
 typedef struct kr_batch_s* kr_batch;
 
-/* Uploads the job table (at most 128 images) and binds the device output
- * arrays: rows are the images' pixels in job order, each image row-major. */
+/* Uploads the job table and binds the device output arrays: rows are the
+ * images' pixels in job order, each image row-major.  Up to 128 images the
+ * launch keeps every image's constants in LDS; a larger batch reads them
+ * from the job table in device memory.  At most 2^31 - 1 images. */
 kr_status kr_batch_create(kr_handle h, const kr_job* jobs, int32_t n_jobs, const kr_tune* tune,
                           const kr_outputs* d_out, kr_batch* out);
 /* Total photons (output rows) of the batch, or -1. */

@@ -11,12 +11,13 @@
 // busy.  Per-photon arithmetic is that of the plain kernel (kr_device.h), so
 // every image's results equal the plain per-image kernel's bit for bit.
 //
-// Each image's integration constants (and the first photon of each image, for
-// the lookup) are copied into LDS at kernel start; a lane reads its own
-// job's constants from there at every attempt, and the image scalars from the
-// job table in global memory when it starts a photon.  A batch of one image
-// uses the uniform-constant variant (constants in kernel arguments, scalar
-// registers).
+// Up to 128 images, each image's integration constants (and the first photon
+// of each image, for the lookup) are copied into LDS at kernel start; a lane
+// reads its own job's constants from there at every attempt, and the image
+// scalars from the job table in global memory when it starts a photon.  A
+// larger batch reads the constants from the job table too, where the L2
+// keeps them.  A batch of one image uses the uniform-constant variant
+// (constants in kernel arguments, scalar registers).
 //
 // At two wavefronts per SIMD, when the device's LDS allows, the stage slopes
 // k1..k4 of each lane's Dormand-Prince attempt live in LDS too (and the
@@ -81,7 +82,8 @@ __device__ __forceinline__ int find_job(const int64_t* firsts, int n_jobs, int64
 // batch's work order: photon work[p] of the processing order (work == NULL:
 // photon p).
 struct BatchArgs {
-    const JobDev* jobs;  // device copy of the job table
+    const JobDev* jobs;     // device copy of the job table
+    const int64_t* firsts;  // jobs[k].first, k = 0 .. n_jobs - 1
     int n_jobs;
     int refill_interval;
     const int32_t* work;  // precision form: exact photons first, then the FP32 ones
@@ -93,14 +95,22 @@ struct BatchArgs {
     BatchCounters* counters; // precision form
 };
 
+// Where a launch finds each image's first photon and integration constants:
+// a batch of one image in the kernel arguments (scalar registers); up to
+// MAX_LDS_JOBS images copied into LDS at kernel start; more images read from
+// the job table in global memory (L2) at every attempt.
+enum : int { TABLE_ARGS = 0, TABLE_LDS = 1, TABLE_GLOBAL = 2 };
+
+int table_of(int n_jobs) { return n_jobs == 1 ? TABLE_ARGS : n_jobs <= MAX_LDS_JOBS ? TABLE_LDS : TABLE_GLOBAL; }
+
 // Dynamic LDS of batch_kernel, in 8-byte words: the stage slopes (4 NA BLOCK
-// doubles, LDS_SLOPES only), then the first photon of every job (n_jobs),
-// then every job's Params.
+// doubles, LDS_SLOPES only), then with TABLE_LDS the first photon of every
+// job (n_jobs), then every job's Params.
 constexpr size_t slope_bytes(int block) { return static_cast<size_t>(4 * NA) * block * sizeof(double); }
 
-size_t batch_lds_bytes(bool uniform, bool lds_slopes, int block, int n_jobs) {
+size_t batch_lds_bytes(int table, bool lds_slopes, int block, int n_jobs) {
     return (lds_slopes ? slope_bytes(block) : 0) +
-           (uniform ? 0 : static_cast<size_t>(n_jobs) * (sizeof(int64_t) + sizeof(Params)));
+           (table == TABLE_LDS ? static_cast<size_t>(n_jobs) * (sizeof(int64_t) + sizeof(Params)) : 0);
 }
 
 // The precision form runs a batch as two launches over one work order: the
@@ -111,22 +121,24 @@ size_t batch_lds_bytes(bool uniform, bool lds_slopes, int block, int n_jobs) {
 // So no wavefront carries both attempts, and the exact photons run in the
 // kernel that holds only the exact attempt.  Rows record which (marks), for
 // the verify/repair pass.
-template <typename T, bool UNIFORM, int BLOCK, int WAVES, bool LDS_SLOPES, bool MIXED = false>
+template <typename T, int TABLE, int BLOCK, int WAVES, bool LDS_SLOPES, bool MIXED = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void batch_kernel(
     BatchArgs a, JobDev uniform_job, BatchOut<T> out) {
     static_assert(!MIXED || sizeof(T) == sizeof(double), "the precision form integrates in FP64 and FP32");
     static_assert(sizeof(Params) % sizeof(uint64_t) == 0, "Params is copied in 8-byte words");
+    constexpr bool UNIFORM = TABLE == TABLE_ARGS;
     extern __shared__ double lds[];
-    int64_t* firsts = reinterpret_cast<int64_t*>(lds + (LDS_SLOPES ? 4 * NA * BLOCK : 0));
-    Params* params = reinterpret_cast<Params*>(firsts + (UNIFORM ? 0 : a.n_jobs));
-    if (!UNIFORM) {
+    int64_t* lds_firsts = reinterpret_cast<int64_t*>(lds + (LDS_SLOPES ? 4 * NA * BLOCK : 0));
+    Params* params = reinterpret_cast<Params*>(lds_firsts + (TABLE == TABLE_LDS ? a.n_jobs : 0));
+    if (TABLE == TABLE_LDS) {
         constexpr int PW = static_cast<int>(sizeof(Params) / sizeof(uint64_t));
-        for (int k = threadIdx.x; k < a.n_jobs; k += BLOCK) firsts[k] = a.jobs[k].first;
+        for (int k = threadIdx.x; k < a.n_jobs; k += BLOCK) lds_firsts[k] = a.firsts[k];
         uint64_t* dst = reinterpret_cast<uint64_t*>(params);
         for (int w = threadIdx.x; w < a.n_jobs * PW; w += BLOCK)
             dst[w] = reinterpret_cast<const uint64_t*>(&a.jobs[w / PW].P)[w % PW];
         __syncthreads();
     }
+    const int64_t* firsts = TABLE == TABLE_LDS ? lds_firsts : a.firsts;
     // Stage slopes of an exact attempt: in LDS at two wavefronts per SIMD,
     // else private (only the precision form's occasional exact attempts of a
     // bulk photon use them there; the exact photons keep theirs in registers).
@@ -172,7 +184,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
                         const int32_t bad = image_state(J.B, pixel_centre(J.B.fov, j, J.B.n),
                                                         -pixel_centre(J.B.fov, i, J.B.n), y0);
                         if (bad && a.flags) atomicOr(a.flags, bad);
-                        ray_init<T, true>(UNIFORM ? uniform_job.P : params[job], y0, R);
+                        ray_init<T, true>(UNIFORM ? uniform_job.P : TABLE == TABLE_LDS ? params[job] : J.P, y0, R);
                         if (MIXED) {
                             G = geom32(J.P);
                             M = MixedCfg{J.r_plus, 0.0, 0.0, 0.0};
@@ -191,7 +203,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
         }
         if (!any_busy) break;  // exhausted and nothing in flight
         if (busy) {
-            const Params& P = UNIFORM ? uniform_job.P : params[job];
+            const Params& P = UNIFORM ? uniform_job.P : TABLE == TABLE_LDS ? params[job] : a.jobs[job].P;
             if constexpr (MIXED) {
                 if (LDS_SLOPES)
                     ray_attempt_with<double, true, false>(P, R, MixedStepper<true>{G, M, &fp64_att, K});
@@ -303,22 +315,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 
 using BatchFn = void (*)(BatchArgs, JobDev, BatchOut<double>);
 
-template <bool U, int BLOCK, bool MIXED>
+template <int TABLE, int BLOCK, bool MIXED>
 BatchFn pick_waves(int waves, bool lds_slopes) {
-    if (waves >= 3) return batch_kernel<double, U, BLOCK, 3, false, MIXED>;
-    return lds_slopes ? batch_kernel<double, U, BLOCK, 2, true, MIXED> : batch_kernel<double, U, BLOCK, 2, false, MIXED>;
+    if (waves >= 3) return batch_kernel<double, TABLE, BLOCK, 3, false, MIXED>;
+    return lds_slopes ? batch_kernel<double, TABLE, BLOCK, 2, true, MIXED>
+                      : batch_kernel<double, TABLE, BLOCK, 2, false, MIXED>;
 }
 
-template <bool U>
-BatchFn pick(int block, int waves, bool lds_slopes, bool mixed = false) {
+template <int TABLE>
+BatchFn pick_table(int block, int waves, bool lds_slopes, bool mixed) {
     if (mixed) {
-        if (block == 64) return pick_waves<U, 64, true>(waves, lds_slopes);
-        if (block == 128) return pick_waves<U, 128, true>(waves, lds_slopes);
-        return pick_waves<U, 256, true>(waves, lds_slopes);
+        if (block == 64) return pick_waves<TABLE, 64, true>(waves, lds_slopes);
+        if (block == 128) return pick_waves<TABLE, 128, true>(waves, lds_slopes);
+        return pick_waves<TABLE, 256, true>(waves, lds_slopes);
     }
-    if (block == 64) return pick_waves<U, 64, false>(waves, lds_slopes);
-    if (block == 128) return pick_waves<U, 128, false>(waves, lds_slopes);
-    return pick_waves<U, 256, false>(waves, lds_slopes);
+    if (block == 64) return pick_waves<TABLE, 64, false>(waves, lds_slopes);
+    if (block == 128) return pick_waves<TABLE, 128, false>(waves, lds_slopes);
+    return pick_waves<TABLE, 256, false>(waves, lds_slopes);
+}
+
+BatchFn pick(int table, int block, int waves, bool lds_slopes, bool mixed = false) {
+    if (table == TABLE_ARGS) return pick_table<TABLE_ARGS>(block, waves, lds_slopes, mixed);
+    if (table == TABLE_LDS) return pick_table<TABLE_LDS>(block, waves, lds_slopes, mixed);
+    return pick_table<TABLE_GLOBAL>(block, waves, lds_slopes, mixed);
 }
 
 }  // namespace kr
@@ -331,6 +350,7 @@ struct kr_batch_s {
     JobDev* d_jobs = nullptr;
     unsigned long long* d_counter = nullptr;
     int32_t* d_flags = nullptr;
+    int64_t* d_firsts = nullptr;  // jobs[k].first in processing order
     std::vector<JobDev> jobs;
     int64_t total = 0;
     kr_tune tune{};
@@ -369,7 +389,6 @@ kr_status check_tune(const kr_tune* t) {
 kr_status build_jobs(kr_context* h, const kr_job* jobs, int32_t n_jobs, int order, std::vector<JobDev>& out,
                      int64_t& total) {
     if (!jobs || n_jobs < 1) return host::fail(KR_ERR_INVALID, "need at least one job");
-    if (n_jobs > MAX_LDS_JOBS) return host::fail(KR_ERR_UNSUPPORTED, "at most 128 images per batch");
     out.resize(n_jobs);
     total = 0;
     for (int k = 0; k < n_jobs; ++k) {
@@ -407,7 +426,7 @@ hipError_t enqueue(kr_batch_s* b, hipStream_t s) {
     const int waves_per_cu = 4 * b->tune.waves_per_simd;
     const int blocks_per_cu = (waves_per_cu * 64 + block - 1) / block;
     const unsigned grid = static_cast<unsigned>(cus * blocks_per_cu);
-    const bool uniform = b->jobs.size() == 1;
+    const int n_jobs = static_cast<int>(b->jobs.size());
     BatchOut<double> o{static_cast<double*>(b->out.Y), b->out.lam, b->out.n_steps, b->out.n_rejected, b->out.state,
                        b->out.max_null_error, b->out.max_energy_drift, b->out.max_lz_drift, b->out.max_carter_drift};
     // The exact part (every photon, without the precision form), then the FP32 part; launches over an empty
@@ -416,16 +435,14 @@ hipError_t enqueue(kr_batch_s* b, hipStream_t s) {
     for (int part = 0; part < 2; ++part) {
         if (parts[part + 1] == parts[part]) continue;
         const bool mixed = part == 1;
-        BatchArgs args{b->d_jobs, static_cast<int>(b->jobs.size()), b->tune.refill_interval, b->d_work,
+        BatchArgs args{b->d_jobs, b->d_firsts, n_jobs, b->tune.refill_interval, b->d_work,
                        parts[part], parts[part + 1] - parts[part], b->d_counter + part, b->d_flags, b->d_marks,
                        b->d_counters};
-        BatchFn fn = uniform ? pick<true>(block, b->tune.waves_per_simd, b->lds_slopes, mixed)
-                             : pick<false>(block, b->tune.waves_per_simd, b->lds_slopes, mixed);
+        BatchFn fn = pick(table_of(n_jobs), block, b->tune.waves_per_simd, b->lds_slopes, mixed);
         hipLaunchKernelGGL(fn, dim3(grid), dim3(block), b->lds_bytes, s, args, b->jobs[0], o);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (!b->precision) return hipSuccess;
-    const int n_jobs = static_cast<int>(b->jobs.size());
     hipLaunchKernelGGL(batch_verify_kernel, dim3(static_cast<unsigned>((b->total + 255) / 256)), dim3(256), 0, s,
                        b->d_jobs, b->d_out_firsts, b->d_order_job, n_jobs, b->total, b->out.state, b->d_marks,
                        b->band_px, b->d_lists[0], b->d_counters);
@@ -464,12 +481,20 @@ extern "C" kr_status kr_batch_create(kr_handle h, const kr_job* jobs, int32_t n_
     hipError_t e = hipSetDevice(h->device);
     const size_t table = b->jobs.size() * sizeof(JobDev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, h->device);
-    if (e == hipSuccess) e = hipMalloc(&b->dmem, table + 512);
+    // Device block: the job table, then (256-byte aligned) the launches' cursors and the flags word (512 B),
+    // then the first photon of every job.
+    const size_t o_counter = (table + 255) & ~size_t(255);
+    if (e == hipSuccess) e = hipMalloc(&b->dmem, o_counter + 512 + b->jobs.size() * sizeof(int64_t));
     if (e == hipSuccess) {
         b->d_jobs = static_cast<JobDev*>(b->dmem);
-        b->d_counter = reinterpret_cast<unsigned long long*>(static_cast<char*>(b->dmem) + ((table + 255) & ~size_t(255)));
+        b->d_counter = reinterpret_cast<unsigned long long*>(static_cast<char*>(b->dmem) + o_counter);
         b->d_flags = reinterpret_cast<int32_t*>(b->d_counter + 32);
+        b->d_firsts = reinterpret_cast<int64_t*>(static_cast<char*>(b->dmem) + o_counter + 512);
+        std::vector<int64_t> firsts(b->jobs.size());
+        for (size_t k = 0; k < b->jobs.size(); ++k) firsts[k] = b->jobs[k].first;
         e = hipMemcpy(b->d_jobs, b->jobs.data(), table, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(b->d_firsts, firsts.data(), firsts.size() * sizeof(int64_t), hipMemcpyHostToDevice);
     }
     // Stage slopes in LDS at two wavefronts per SIMD when a block's share
     // fits the device's per-block and per-CU LDS.
@@ -478,20 +503,20 @@ extern "C" kr_status kr_batch_create(kr_handle h, const kr_job* jobs, int32_t n_
     if (e == hipSuccess)
         e = hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, h->device);
     if (e == hipSuccess) {
-        const bool uniform = b->jobs.size() == 1;
+        const int table = table_of(n_jobs);
         const int block = tune->block_size;
         const size_t blocks_per_cu = static_cast<size_t>((4 * tune->waves_per_simd * 64 + block - 1) / block);
-        const size_t with = batch_lds_bytes(uniform, true, block, n_jobs);
+        const size_t with = batch_lds_bytes(table, true, block, n_jobs);
         b->lds_slopes = tune->waves_per_simd == 2 && with <= static_cast<size_t>(lds_block) &&
                         with * blocks_per_cu <= static_cast<size_t>(lds_cu);
         if (b->lds_slopes && with > 65536) {
-            BatchFn fn = uniform ? pick<true>(block, 2, true) : pick<false>(block, 2, true);
+            BatchFn fn = pick(table, block, 2, true);
             b->lds_slopes = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                                 static_cast<int>(with)) == hipSuccess;
             (void)hipGetLastError();
         }
-        b->lds_bytes = batch_lds_bytes(uniform, b->lds_slopes, block, n_jobs);
+        b->lds_bytes = batch_lds_bytes(table, b->lds_slopes, block, n_jobs);
     }
     if (e != hipSuccess) {
         if (b->dmem) (void)hipFree(b->dmem);
@@ -661,8 +686,7 @@ extern "C" kr_status kr_batch_create_precision(kr_handle h, const kr_job* jobs, 
     b->band_px = prec->band_px;
     // The precision kernel needs the device's LDS limit raised as the FP64 one does.
     if (b->lds_slopes && b->lds_bytes > 65536) {
-        const bool uniform = b->jobs.size() == 1;
-        BatchFn fn = uniform ? pick<true>(b->tune.block_size, 2, true, true) : pick<false>(b->tune.block_size, 2, true, true);
+        BatchFn fn = pick(table_of(static_cast<int>(b->jobs.size())), b->tune.block_size, 2, true, true);
         if (hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
                                 static_cast<int>(b->lds_bytes)) != hipSuccess) {
             (void)hipGetLastError();

@@ -131,3 +131,55 @@ def test_precision_batch_keeps_every_image_fp64_mask(hip) -> None:
                 for name in FIELDS:
                     assert np.array_equal(getattr(a, name), getattr(b, name)), name
         mixed.close()
+
+
+def _many_small(n_images: int = 200):
+    """More images than the launch keeps in LDS (128): the job table is read from device memory."""
+    from kerrray_amd.batch import ImageJob
+
+    rng = np.random.Generator(np.random.PCG64(7))
+    return [
+        ImageJob(float(rng.uniform(0.0, 0.99)), float(rng.choice([17.0, 60.0, 85.0])), int(rng.choice([8, 12, 16, 24])),
+                 10.0 ** -int(e), 1e-2 * 10.0 ** -int(e))
+        for e in rng.integers(5, 9, n_images)
+    ]
+
+
+@pytest.mark.parametrize("tune", [(128, 2, 8, 1), (64, 3, 4, 0)])
+def test_batch_of_more_than_128_images_equals_plain_kernel_per_image(hip, tune) -> None:
+    """200 images in one launch (constants read from the job table in device memory): every image's rows equal
+    the plain kernel's."""
+    from kerrray_amd import _lib
+    from kerrray_amd.batch import DeviceBatch
+
+    jobs = _many_small()
+    batch = DeviceBatch(hip, jobs, _lib.KrTune(*tune))
+    batch.run(graph=True)
+    res = batch.results()
+    batch.close()
+    for job, got in zip(jobs, res):
+        ref = _plain(hip, job)
+        for name in FIELDS:
+            assert np.array_equal(getattr(got, name), getattr(ref, name)), (job, name)
+
+
+def test_precision_batch_of_more_than_128_images_keeps_every_fp64_mask(hip) -> None:
+    from kerrray_amd import _lib
+    from kerrray_amd.batch import DeviceBatch
+    from kerrray_amd.planned import PlannedOptions
+
+    jobs = _many_small()
+    tune = _lib.KrTune(128, 2, 8, 1)
+    exact = DeviceBatch(hip, jobs, tune)
+    exact.run(graph=False)
+    ref = exact.results()
+    exact.close()
+    mixed = DeviceBatch(hip, jobs, tune, precision=PlannedOptions())
+    mixed.run(graph=True)
+    got = mixed.results()
+    st = mixed.stats()
+    mixed.close()
+    print(f"[precision batch, {len(jobs)} images] band {st.planned_band}, repaired {st.repaired} in "
+          f"{st.repair_rounds} rounds", flush=True)
+    for job, a, b in zip(jobs, got, ref):
+        assert np.array_equal(a.state, b.state), job
