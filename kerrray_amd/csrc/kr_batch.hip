@@ -371,6 +371,8 @@ struct kr_batch_s {
     int* d_order_job = nullptr;
     int32_t* d_work = nullptr;  // work order: the n_exact exact photons, then the FP32 ones
     int64_t n_exact = 0;
+    hipStream_t side = nullptr;  // precision form: the FP32 launch runs here, beside the exact one
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace {
@@ -429,18 +431,35 @@ hipError_t enqueue(kr_batch_s* b, hipStream_t s) {
     const int n_jobs = static_cast<int>(b->jobs.size());
     BatchOut<double> o{static_cast<double*>(b->out.Y), b->out.lam, b->out.n_steps, b->out.n_rejected, b->out.state,
                        b->out.max_null_error, b->out.max_energy_drift, b->out.max_lz_drift, b->out.max_carter_drift};
-    // The exact part (every photon, without the precision form), then the FP32 part; launches over an empty
-    // range are left out.  Each launch draws from a cursor of its own.
+    // The exact part (every photon, without the precision form) on `s`, and the FP32 part forked onto the
+    // batch's side stream (joined before the verify pass), so that the FP32 wavefronts fill the slots the exact
+    // launch leaves while its longest photons finish.  Launches over an empty range are left out; each draws
+    // from a cursor of its own.  With both parts the exact launch is sized to its own photons (at most every
+    // slot), which leaves the rest of the chip to the FP32 launch from the start.
     const int64_t parts[3] = {0, b->precision ? b->n_exact : b->total, b->total};
+    const bool fork = parts[1] > parts[0] && parts[2] > parts[1];
+    if (fork) {
+        if ((e = hipEventRecord(b->ev_fork, s)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(b->side, b->ev_fork, 0)) != hipSuccess) return e;
+    }
     for (int part = 0; part < 2; ++part) {
         if (parts[part + 1] == parts[part]) continue;
         const bool mixed = part == 1;
-        BatchArgs args{b->d_jobs, b->d_firsts, n_jobs, b->tune.refill_interval, b->d_work,
-                       parts[part], parts[part + 1] - parts[part], b->d_counter + part, b->d_flags, b->d_marks,
-                       b->d_counters};
+        const int64_t count = parts[part + 1] - parts[part];
+        unsigned g = grid;
+        if (fork && !mixed) {
+            const int64_t blocks = (count + block - 1) / block;
+            if (blocks < static_cast<int64_t>(g)) g = static_cast<unsigned>(blocks);
+        }
+        BatchArgs args{b->d_jobs, b->d_firsts, n_jobs, b->tune.refill_interval, b->d_work, parts[part], count,
+                       b->d_counter + part, b->d_flags, b->d_marks, b->d_counters};
         BatchFn fn = pick(table_of(n_jobs), block, b->tune.waves_per_simd, b->lds_slopes, mixed);
-        hipLaunchKernelGGL(fn, dim3(grid), dim3(block), b->lds_bytes, s, args, b->jobs[0], o);
+        hipLaunchKernelGGL(fn, dim3(g), dim3(block), b->lds_bytes, fork && mixed ? b->side : s, args, b->jobs[0], o);
         if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (fork) {
+        if ((e = hipEventRecord(b->ev_join, b->side)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(s, b->ev_join, 0)) != hipSuccess) return e;
     }
     if (!b->precision) return hipSuccess;
     hipLaunchKernelGGL(batch_verify_kernel, dim3(static_cast<unsigned>((b->total + 255) / 256)), dim3(256), 0, s,
@@ -580,6 +599,9 @@ extern "C" kr_status kr_batch_destroy(kr_batch b) {
     (void)hipDeviceSynchronize();
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->side) (void)hipStreamDestroy(b->side);
     if (b->dmem) (void)hipFree(b->dmem);
     if (b->pmem) (void)hipFree(b->pmem);
     delete b;
@@ -678,6 +700,14 @@ extern "C" kr_status kr_batch_create_precision(kr_handle h, const kr_job* jobs, 
             e = hipMemcpy(b->d_work, work.data(), static_cast<size_t>(b->total) * 4, hipMemcpyHostToDevice);
         }
     }
+    if (e != hipSuccess) {
+        kr_batch_destroy(b);
+        return host::fail(KR_ERR_DEVICE, std::string("kr_batch_create_precision: ") + hipGetErrorString(e));
+    }
+    // The side stream (default priority) and the events that fork the FP32 launch onto it and join it back.
+    e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming);
     if (e != hipSuccess) {
         kr_batch_destroy(b);
         return host::fail(KR_ERR_DEVICE, std::string("kr_batch_create_precision: ") + hipGetErrorString(e));
