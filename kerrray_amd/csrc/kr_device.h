@@ -458,6 +458,47 @@ KR_HD T error_norm(T rtol, T atol, const T (&err)[NA], const T (&y)[NA], const T
     return wsqrt(total / T(8));
 }
 
+// 1/x and sqrt(x) for the bookkeeping of FP32-bulk attempts (error norm,
+// drift diagnostics), which need not round as the reference does: the
+// hardware reciprocal estimate (about 2^-23 relative) with one Newton step
+// (about 2^-46), and the hardware square root (about 2^-23).  Zero and
+// non-finite operands give non-finite results, as the divisions do.  The
+// host build divides and calls sqrt.
+KR_HD double rcp_bulk(double x) {
+#if defined(KR_HOST_BUILD) || !defined(__HIP_DEVICE_COMPILE__)
+    return 1.0 / x;
+#else
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(fma(-x, r, 1.0), r, r);
+#endif
+}
+
+KR_HD double sqrt_bulk(double x) {
+#if defined(KR_HOST_BUILD) || !defined(__HIP_DEVICE_COMPILE__)
+    return sqrt(x);
+#else
+    return __builtin_amdgcn_sqrt(x);
+#endif
+}
+
+// error_norm for an FP32-bulk attempt: the same quantity, with the six
+// quotients and the root from rcp_bulk and sqrt_bulk.  It moves en by about
+// 1e-7 relative, which changes an acceptance only for en within that of 1;
+// the bulk's rows are not the reference's bits in any case, and its mask is
+// checked and repaired in FP64 after the trace.
+KR_HD double error_norm_bulk(double rtol, double atol, const double (&err)[NA], const double (&y)[NA],
+                             const double (&y_new)[NA]) {
+    double q[NA];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+        const double scale = atol + rtol * pymax(fabs(y[i]), fabs(y_new[i]));
+        const double v = err[i] * rcp_bulk(scale);
+        q[i] = v * v;
+    }
+    const double total = ((q[0] + q[1]) + (q[2] + q[3])) + (q[4] + q[5]);
+    return sqrt_bulk(total * 0.125);
+}
+
 // x^(-1/5) for 1e-5 <= x <= 1e5 in FP64 to within a few ulp (not the library
 // pow's rounding): an FP32 estimate (hardware log2 and exp2 on the device)
 // refined by two Newton corrections y <- y + (y / 5)(1 - x y^5), each of
@@ -638,13 +679,14 @@ KR_HD double null_error(const T (&v)[NA], T p_t, T p_ph, const T (&f)[NA]) {
 }
 
 // numba_kernel.py:325-336 with s = sin(th), c = cos(th) of double(th).
-template <typename T>
+// BULK: the quotient from rcp_bulk (FP32-bulk attempts).
+template <bool BULK = false, typename T>
 KR_HD double carter(double a2, const T (&v)[NA], T p_t, T p_ph, double s, double c) {
     const double p_th = static_cast<double>(v[5]);
     const double lz = static_cast<double>(p_ph);
     const double e = -static_cast<double>(p_t);
     double lz2_over_s2 = 0.0;
-    if (lz != 0.0) lz2_over_s2 = (lz * lz) / (s * s);
+    if (lz != 0.0) lz2_over_s2 = BULK ? (lz * lz) * rcp_bulk(s * s) : (lz * lz) / (s * s);
     return p_th * p_th + (c * c) * (lz2_over_s2 - a2 * e * e);
 }
 
@@ -776,7 +818,13 @@ KR_HD void ray_attempt_with(const Params& P, RayState<T>& R, const Stepper& step
     bool accept = finite;
     T h_new = R.h;
     if (ADAPTIVE && finite) {
-        const T en = error_norm(static_cast<T>(P.consts[C_RTOL]), static_cast<T>(P.consts[C_ATOL]), err, R.y, y_new);
+        T en;
+        if constexpr (sizeof(T) == sizeof(double)) {
+            en = exact ? error_norm(P.consts[C_RTOL], P.consts[C_ATOL], err, R.y, y_new)
+                       : error_norm_bulk(P.consts[C_RTOL], P.consts[C_ATOL], err, R.y, y_new);
+        } else {
+            en = error_norm(static_cast<T>(P.consts[C_RTOL]), static_cast<T>(P.consts[C_ATOL]), err, R.y, y_new);
+        }
         if (!wfinite(en)) {
             finite = false;
             accept = false;
@@ -832,9 +880,15 @@ KR_HD void ray_attempt_with(const Params& P, RayState<T>& R, const Stepper& step
         R.m_e = pymax(R.m_e, fabs(e - R.e0) / pymax(fabs(R.e0), R.fe));
         R.m_lz = pymax(R.m_lz, fabs(lz - R.lz0) / pymax(fabs(R.lz0), R.fl));
     }
-    R.m_q = pymax(R.m_q, fabs(carter(P.geom[G_A2], R.y, R.p_t, R.p_ph, R.s_cur, R.c_cur) - R.q0) /
-                             pymax(fabs(R.q0), R.fq));
-    R.m_null = pymax(R.m_null, null_error(R.y, R.p_t, R.p_ph, R.f) / R.fe2);
+    if (exact) {
+        R.m_q = pymax(R.m_q, fabs(carter(P.geom[G_A2], R.y, R.p_t, R.p_ph, R.s_cur, R.c_cur) - R.q0) /
+                                 pymax(fabs(R.q0), R.fq));
+        R.m_null = pymax(R.m_null, null_error(R.y, R.p_t, R.p_ph, R.f) / R.fe2);
+    } else {
+        R.m_q = pymax(R.m_q, fabs(carter<true>(P.geom[G_A2], R.y, R.p_t, R.p_ph, R.s_cur, R.c_cur) - R.q0) *
+                                 rcp_bulk(pymax(fabs(R.q0), R.fq)));
+        R.m_null = pymax(R.m_null, null_error(R.y, R.p_t, R.p_ph, R.f) * rcp_bulk(R.fe2));
+    }
     int code = classify(P, R.y, R.p_ph, R.f, R.lam, R.n_steps, R.h, true, working_sin<T>(R.s_cur, R.y[2]));
     if (DISK) {
         // numba_kernel.py:439-454
