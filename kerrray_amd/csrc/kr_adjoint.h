@@ -15,10 +15,10 @@
 //   for i = 5 .. 0:  Ybar_i = J_f(Y_i)^T kbar_i;  ybar += Ybar_i;
 //                    kbar_j += h a_ij Ybar_i (j < i);
 //                    (p_t, p_phi, a)bar += (df/d(p_t, p_phi, a))^T kbar_i.
-// J_f^T v is formed from forward-mode dual numbers in (r, theta, a) through
-// the inverse metric (the same 15 terms as numba_kernel.py:111-154, with the
-// spin a variable) and the closed-form momentum derivatives of the
-// Hamiltonian right-hand side (numba_kernel.py:158-197).
+// J_f^T v is formed by one reverse sweep through the inverse metric (the same
+// 15 terms as numba_kernel.py:111-154, with the spin a variable; metric_vjp)
+// for the r, theta and a columns, and from the closed-form momentum
+// derivatives of the Hamiltonian right-hand side (numba_kernel.py:158-197).
 //
 // Step Jacobians instead of a reverse sweep: t and phi enter no right-hand
 // side, so a step's Jacobian with respect to (y, p_t, p_phi, a) is the
@@ -246,42 +246,218 @@ KR_HD void metric_terms(const Dual<N>& r, const Dual<N>& th, const Dual<N>& a, d
     g[14] = -((sig_th * s2 + sig * sin2th) * (i_sig2 * (i_s2 * i_s2))) + (a2 * sd_th) * i_sd2;
 }
 
+// The 15 metric terms g of metric_terms at (r, theta, a) on plain doubles,
+// and the cotangent of (r, theta, a) for the cotangent gb of g: one reverse
+// sweep through the same expression graph (each intermediate's adjoint is
+// the sum, over its uses, of the use's adjoint times the use's partial
+// derivative), instead of three tangent directions carried through every
+// operation.  The forward values are kept for the sweep: about 40 doubles,
+// against four per value with Dual<3>.
+KR_HD void metric_vjp(double r, double th, double a, double m, const double (&gb)[15], double (&g)[15],
+                      double& r_b, double& th_b, double& a_b) {
+    // Forward (metric_terms' operations, values only).
+    double s, c;
+    sin_cos(th, s, c);
+    const double s2 = s * s;
+    const double sin2th = 2.0 * (s * c);
+    const double r2 = r * r;
+    const double a2 = a * a;
+    const double cc = c * c;
+    const double sig = r2 + a2 * cc;
+    const double dlt = r2 - (2.0 * m) * r + a2;
+    const double r2a2 = r2 + a2;
+    const double a2dlt = a2 * dlt;
+    const double big_a = r2a2 * r2a2 - a2dlt * s2;
+    const double sd = sig * dlt;
+    const double ma2 = (2.0 * m) * a;
+    const double i_sig = 1.0 / sig;
+    const double i_dlt = 1.0 / dlt;
+    const double i_s2 = 1.0 / s2;
+    const double i_sd = i_sig * i_dlt;
+    const double i_sd2 = i_sd * i_sd;
+    const double i_sig2 = i_sig * i_sig;
+    const double ma2r = ma2 * r;
+    const double u4 = dlt - a2 * s2, w4 = i_sd * i_s2;
+    const double sig_r = 2.0 * r;
+    const double sig_th = -(a2 * sin2th);
+    const double dlt_r = 2.0 * (r - m);
+    const double rm = r - m;
+    const double p_ar = a2 * rm;
+    const double a_r = 4.0 * (r * r2a2) - 2.0 * (p_ar * s2);
+    const double a_th = -(a2dlt * sin2th);
+    const double sd_r = sig_r * dlt + sig * dlt_r;
+    const double sd_th = sig_th * dlt;
+    const double u5 = a_r * sd - big_a * sd_r;
+    const double u6 = sd - r * sd_r;
+    const double v6 = ma2 * u6;
+    const double u7 = dlt_r * sig - dlt * sig_r;
+    const double w9 = i_sig2 * i_s2;
+    const double z9 = a2 * sd_r;
+    const double u10 = a_th * sd - big_a * sd_th;
+    const double u11 = ma2r * sd_th;
+    const double u12 = dlt * sig_th;
+    const double u14 = sig_th * s2 + sig * sin2th;
+    const double q14 = i_s2 * i_s2;
+    const double w14 = i_sig2 * q14;
+    const double z14 = a2 * sd_th;
+    g[0] = -(big_a * i_sd);
+    g[1] = -(ma2r * i_sd);
+    g[2] = dlt * i_sig;
+    g[3] = i_sig;
+    g[4] = u4 * w4;
+    g[5] = -(u5 * i_sd2);
+    g[6] = -(v6 * i_sd2);
+    g[7] = u7 * i_sig2;
+    g[8] = -(sig_r * i_sig2);
+    g[9] = -(sig_r * w9) + z9 * i_sd2;
+    g[10] = -(u10 * i_sd2);
+    g[11] = u11 * i_sd2;
+    g[12] = -(u12 * i_sig2);
+    g[13] = -(sig_th * i_sig2);
+    g[14] = -(u14 * w14) + z14 * i_sd2;
+    // Reverse: the uses of each intermediate, last definitions first.
+    double big_a_b = -gb[0] * i_sd;
+    double i_sd_b = -gb[0] * big_a - gb[1] * ma2r;
+    double ma2r_b = -gb[1] * i_sd;
+    double dlt_b = gb[2] * i_sig;
+    double i_sig_b = gb[2] * dlt + gb[3];
+    const double u4_b = gb[4] * w4, w4_b = gb[4] * u4;
+    dlt_b += u4_b;
+    double a2_b = -u4_b * s2;
+    double s2_b = -u4_b * a2;
+    i_sd_b += w4_b * i_s2;
+    double i_s2_b = w4_b * i_sd;
+    const double u5_b = -gb[5] * i_sd2;
+    double i_sd2_b = -gb[5] * u5 - gb[6] * v6 + gb[9] * z9 - gb[10] * u10 + gb[11] * u11 + gb[14] * z14;
+    double a_r_b = u5_b * sd;
+    double sd_b = u5_b * a_r;
+    big_a_b -= u5_b * sd_r;
+    double sd_r_b = -u5_b * big_a;
+    const double v6_b = -gb[6] * i_sd2;
+    double ma2_b = v6_b * u6;
+    const double u6_b = v6_b * ma2;
+    sd_b += u6_b;
+    double r_bb = -u6_b * sd_r;
+    sd_r_b -= u6_b * r;
+    const double u7_b = gb[7] * i_sig2;
+    double i_sig2_b = gb[7] * u7 - gb[8] * sig_r - gb[12] * u12 - gb[13] * sig_th;
+    double dlt_r_b = u7_b * sig;
+    double sig_b = u7_b * dlt_r;
+    dlt_b -= u7_b * sig_r;
+    double sig_r_b = -u7_b * dlt - gb[8] * i_sig2 - gb[9] * w9;
+    const double w9_b = -gb[9] * sig_r;
+    i_sig2_b += w9_b * i_s2;
+    i_s2_b += w9_b * i_sig2;
+    const double z9_b = gb[9] * i_sd2;
+    a2_b += z9_b * sd_r;
+    sd_r_b += z9_b * a2;
+    const double u10_b = -gb[10] * i_sd2;
+    double a_th_b = u10_b * sd;
+    sd_b += u10_b * a_th;
+    big_a_b -= u10_b * sd_th;
+    double sd_th_b = -u10_b * big_a;
+    const double u11_b = gb[11] * i_sd2;
+    ma2r_b += u11_b * sd_th;
+    sd_th_b += u11_b * ma2r;
+    const double u12_b = -gb[12] * i_sig2;
+    dlt_b += u12_b * sig_th;
+    double sig_th_b = u12_b * dlt - gb[13] * i_sig2;
+    const double u14_b = -gb[14] * w14, w14_b = -gb[14] * u14;
+    sig_th_b += u14_b * s2;
+    s2_b += u14_b * sig_th;
+    sig_b += u14_b * sin2th;
+    double sin2th_b = u14_b * sig;
+    i_sig2_b += w14_b * q14;
+    i_s2_b += 2.0 * (w14_b * i_sig2) * i_s2;
+    const double z14_b = gb[14] * i_sd2;
+    a2_b += z14_b * sd_th;
+    sd_th_b += z14_b * a2;
+    // sd_th = sig_th dlt; sd_r = sig_r dlt + sig dlt_r
+    sig_th_b += sd_th_b * dlt;
+    dlt_b += sd_th_b * sig_th + sd_r_b * sig_r;
+    sig_r_b += sd_r_b * dlt;
+    sig_b += sd_r_b * dlt_r;
+    dlt_r_b += sd_r_b * sig;
+    // a_th = -(a2 dlt) sin2th; a_r = 4 r r2a2 - 2 (a2 (r - m)) s2
+    const double a2dlt_b = -a_th_b * sin2th - big_a_b * s2;
+    sin2th_b -= a_th_b * a2dlt;
+    r_bb += 4.0 * a_r_b * r2a2;
+    double r2a2_b = 4.0 * a_r_b * r + 2.0 * big_a_b * r2a2;
+    const double p_ar_b = -2.0 * a_r_b * s2;
+    s2_b -= 2.0 * a_r_b * p_ar + big_a_b * a2dlt;
+    a2_b += p_ar_b * rm;
+    r_bb += p_ar_b * a2 + 2.0 * dlt_r_b + 2.0 * sig_r_b;
+    // sig_th = -(a2 sin2th)
+    a2_b -= sig_th_b * sin2th;
+    sin2th_b -= sig_th_b * a2;
+    // i_sig2, i_sd2, i_sd, ma2 r
+    i_sig_b += 2.0 * i_sig2_b * i_sig;
+    i_sd_b += 2.0 * i_sd2_b * i_sd;
+    i_sig_b += i_sd_b * i_dlt;
+    const double i_dlt_b = i_sd_b * i_sig;
+    ma2_b += ma2r_b * r;
+    r_bb += ma2r_b * ma2;
+    // the reciprocals
+    s2_b -= i_s2_b * i_s2 * i_s2;
+    dlt_b -= i_dlt_b * i_dlt * i_dlt;
+    sig_b -= i_sig_b * i_sig * i_sig;
+    // sd = sig dlt; a2 dlt; r2a2 = r2 + a2; dlt = r2 - 2 m r + a2; sig = r2 + a2 c^2
+    sig_b += sd_b * dlt;
+    dlt_b += sd_b * sig + a2dlt_b * a2;
+    a2_b += a2dlt_b * dlt + r2a2_b + dlt_b + sig_b * cc;
+    const double r2_b = r2a2_b + dlt_b + sig_b;
+    r_bb -= (2.0 * m) * dlt_b;
+    double c_b = 2.0 * (sig_b * a2) * c;
+    // a2 = a a; r2 = r r; sin2th = 2 s c; s2 = s s; (s, c) = (sin th, cos th)
+    a_b = (2.0 * m) * ma2_b + 2.0 * a * a2_b;
+    r_b = r_bb + 2.0 * r * r2_b;
+    double s_b = 2.0 * sin2th_b * c + 2.0 * s2_b * s;
+    c_b += 2.0 * sin2th_b * s;
+    th_b = s_b * c - c_b * s;
+}
+
 // v^T df, for the right-hand side f(y; p_t, p_phi, a) of the six integrated
 // components (t, r, theta, phi, p_r, p_theta) at the state Y.  Accumulates
-// into ybar (six components), ptbar, pphbar and abar.
+// into ybar (six components), ptbar, pphbar and abar.  f is linear in the
+// metric terms g for given momenta, so v^T df/d(r, theta, a) is
+// gbar^T dg/d(r, theta, a) with gbar = (df/dg)^T v, which metric_vjp takes
+// in one reverse sweep.
 KR_HD void rhs_vjp(double m, double a, const double (&Y)[NA], double pt, double pph, const double (&v)[NA],
                    double (&ybar)[NA], double& ptbar, double& pphbar, double& abar) {
-    Dual<3> g[15];
-    metric_terms(seed<3>(Y[1], 0), seed<3>(Y[2], 1), seed<3>(a, 2), m, g);
-    if (pph == 0.0) {  // polar-axis guard of the primal right-hand side
-        g[4] = constant<3>(0.0);
-        g[9] = constant<3>(0.0);
-        g[14] = constant<3>(0.0);
-    }
     const double pr = Y[4], pth = Y[5];
     const double pt2 = pt * pt, ptpph = pt * pph, pr2 = pr * pr, pth2 = pth * pth, pph2 = pph * pph;
-    double dir[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const double f0 = g[0].d[k] * pt + g[1].d[k] * pph;
-        const double f1 = g[2].d[k] * pr;
-        const double f2 = g[3].d[k] * pth;
-        const double f3 = g[1].d[k] * pt + g[4].d[k] * pph;
-        const double f4 = -0.5 * (g[5].d[k] * pt2 + 2.0 * g[6].d[k] * ptpph + g[7].d[k] * pr2 + g[8].d[k] * pth2 +
-                                  g[9].d[k] * pph2);
-        const double f5 = -0.5 * (g[10].d[k] * pt2 + 2.0 * g[11].d[k] * ptpph + g[12].d[k] * pr2 +
-                                  g[13].d[k] * pth2 + g[14].d[k] * pph2);
-        dir[k] = v[0] * f0 + v[1] * f1 + v[2] * f2 + v[3] * f3 + v[4] * f4 + v[5] * f5;
+    // The polar-axis guard of the primal right-hand side zeroes g[4], g[9]
+    // and g[14] (values and derivatives) when p_phi = 0.
+    const double keep = pph == 0.0 ? 0.0 : 1.0;
+    const double gb[15] = {v[0] * pt,
+                           v[0] * pph + v[3] * pt,
+                           v[1] * pr,
+                           v[2] * pth,
+                           keep * (v[3] * pph),
+                           -0.5 * v[4] * pt2,
+                           -v[4] * ptpph,
+                           -0.5 * v[4] * pr2,
+                           -0.5 * v[4] * pth2,
+                           keep * (-0.5 * v[4] * pph2),
+                           -0.5 * v[5] * pt2,
+                           -v[5] * ptpph,
+                           -0.5 * v[5] * pr2,
+                           -0.5 * v[5] * pth2,
+                           keep * (-0.5 * v[5] * pph2)};
+    double g[15], r_b, th_b, a_b;
+    metric_vjp(Y[1], Y[2], a, m, gb, g, r_b, th_b, a_b);
+    if (pph == 0.0) {
+        g[4] = 0.0;
+        g[9] = 0.0;
+        g[14] = 0.0;
     }
-    ybar[1] += dir[0];
-    ybar[2] += dir[1];
-    abar += dir[2];
-    ybar[4] += v[1] * g[2].v - v[4] * g[7].v * pr - v[5] * g[12].v * pr;
-    ybar[5] += v[2] * g[3].v - v[4] * g[8].v * pth - v[5] * g[13].v * pth;
-    ptbar += v[0] * g[0].v + v[3] * g[1].v - v[4] * (g[5].v * pt + g[6].v * pph) -
-             v[5] * (g[10].v * pt + g[11].v * pph);
-    pphbar += v[0] * g[1].v + v[3] * g[4].v - v[4] * (g[6].v * pt + g[9].v * pph) -
-              v[5] * (g[11].v * pt + g[14].v * pph);
+    ybar[1] += r_b;
+    ybar[2] += th_b;
+    abar += a_b;
+    ybar[4] += v[1] * g[2] - v[4] * g[7] * pr - v[5] * g[12] * pr;
+    ybar[5] += v[2] * g[3] - v[4] * g[8] * pth - v[5] * g[13] * pth;
+    ptbar += v[0] * g[0] + v[3] * g[1] - v[4] * (g[5] * pt + g[6] * pph) - v[5] * (g[10] * pt + g[11] * pph);
+    pphbar += v[0] * g[1] + v[3] * g[4] - v[4] * (g[6] * pt + g[9] * pph) - v[5] * (g[11] * pt + g[14] * pph);
 }
 
 // One accepted step of the primal, recomputed: y <- Phi(y; h).  Uses the
