@@ -1,8 +1,8 @@
-"""The reference-side binding printed in INTEGRATION.md, run as it stands.
+"""The reference-side binding printed in the repository's integration notes, run as it stands.
 
 The module text (``src/kerrray/raytracing/backends/hip_backend.py`` of the
-reference, once a maintainer adds it) is taken from INTEGRATION.md and
-executed with stand-ins for the four reference modules it imports:
+reference, once a maintainer adds it) is taken from the notes' fenced code
+block and executed with stand-ins for the four reference modules it imports:
 
 * CPU: ``_kernel_constants`` and ``_tableaus`` hand back the reference's own
   arrays (tests/golden/packing.npz, written by the reference); the binding's
