@@ -183,3 +183,42 @@ def test_precision_batch_of_more_than_128_images_keeps_every_fp64_mask(hip) -> N
           f"{st.repair_rounds} rounds", flush=True)
     for job, a, b in zip(jobs, got, ref):
         assert np.array_equal(a.state, b.state), job
+
+
+@pytest.mark.parametrize("jobs_kind", ["all_exact", "one_image"])
+def test_precision_batch_edge_cases(hip, jobs_kind) -> None:
+    """A precision batch whose every image is below fp32_min_rtol (one exact launch, no FP32 launch beside it),
+    and a precision batch of one image (constants in the kernel arguments, both launches): rows bitwise the FP64
+    batch's for the first, the FP64 mask for the second, eager and replayed alike."""
+    from kerrray_amd import _lib
+    from kerrray_amd.batch import DeviceBatch, ImageJob
+    from kerrray_amd.planned import PlannedOptions
+
+    if jobs_kind == "all_exact":
+        jobs = [ImageJob(0.5, 60.0, 24, 1e-8, 1e-10), ImageJob(0.9, 17.0, 16, 1e-7, 1e-9)]
+    else:
+        jobs = [ImageJob(0.9, 60.0, 64, 1e-6, 1e-8)]
+    tune = _lib.KrTune(128, 2, 8, 1)
+    exact = DeviceBatch(hip, jobs, tune)
+    exact.run(graph=False)
+    ref = exact.results()
+    exact.close()
+    mixed = DeviceBatch(hip, jobs, tune, precision=PlannedOptions())
+    mixed.run(graph=False)
+    eager = mixed.results()
+    st = mixed.stats()
+    mixed.run(graph=True)
+    again = mixed.results()
+    mixed.close()
+    n = sum(j.n_rays for j in jobs)
+    if jobs_kind == "all_exact":
+        assert st.planned_band == n
+    else:
+        assert 0 < st.planned_band < n
+    for a, b, c in zip(eager, again, ref):
+        for name in FIELDS:
+            assert np.array_equal(getattr(a, name), getattr(b, name)), name
+        assert np.array_equal(a.state, c.state)
+        if jobs_kind == "all_exact":
+            for name in FIELDS:
+                assert np.array_equal(getattr(a, name), getattr(c, name)), name
