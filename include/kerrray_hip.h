@@ -460,9 +460,12 @@ kr_status kr_planned_read_stats(const void* d_work, size_t work_bytes, kr_planne
  * kr_trace_image_planned with fp32_bulk = 1): each image's photons within
  * band_px + 1 pixels of its analytic edge, or with |alpha sin theta_o| <
  * axis_lz, take the exact FP64 attempt, the others FP32 right-hand sides
- * with the FP64 update, error norm and step control; after the persistent
- * launch every image is verified and repaired in FP64 to a fixed point
- * (KR_REPAIR_ROUNDS), all in the same kr_batch_run / graph launch.  Image k
+ * with the FP64 update, error norm and step control.  The two kinds run as
+ * two persistent launches side by side (the FP32 one on a stream the batch
+ * owns, forked from and joined back to the caller's stream); after them
+ * every image is verified and repaired in FP64 to a fixed point
+ * (KR_REPAIR_ROUNDS), all in the same kr_batch_run / graph launch, so the
+ * caller's stream sees one ordered run.  Image k
  * (in job order) has its edge as n_angles radii radii[k * n_angles + j]
  * about (centre_alpha[k], centre_beta[k]) along the angles 2 pi j /
  * n_angles.  Host arrays, copied at creation. */
