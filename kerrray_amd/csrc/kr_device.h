@@ -991,6 +991,16 @@ KR_HD void store_row(const RowsOut<T>& o, int64_t idx, const RayState<T>& R) {
     o.m_q[idx] = r.m_q;
 }
 
+// The disk event of a finished photon into row idx of the event outputs:
+// the crossing state where the photon hit the disk (numba_kernel.py:450-452),
+// zeros elsewhere (the zeroed arrays of numba_backend.py:180-181).
+template <typename T>
+KR_HD void store_event(T* ev_Y, uint8_t* ev_hit, int64_t idx, const RayState<T>& R) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) ev_Y[idx * 8 + c] = R.ev_hit ? R.ev[c] : T(0);
+    ev_hit[idx] = R.ev_hit ? 1 : 0;
+}
+
 // numba_kernel.py:340-463 for one photon whose float64 initial state is y64.
 template <typename T, bool ADAPTIVE, bool DISK>
 KR_HD void integrate_ray(const Params& P, const double (&y64)[8], RayResult<T>& out) {

@@ -211,13 +211,16 @@ __global__ __launch_bounds__(PLAN_BLOCK) void plan_scatter_kernel(int64_t total,
 // photons, the FP64 ones, down.
 //
 // ROWS: the photons are given initial states Y0 (n, 8), not image pixels
-// (FP64 only; B is then unused).
-template <int BLOCK, bool MIXED, int WAVES, bool PARK, bool ROWS = false>
+// (FP64 only; B is then unused).  DISK (ROWS only): the equatorial disk event
+// of numba_kernel.py:439-454 ends photons that cross the disk, and every
+// finished photon's event row goes to ev_Y / ev_hit.
+template <int BLOCK, bool MIXED, int WAVES, bool PARK, bool ROWS = false, bool DISK = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void planned_trace_kernel(
     Params P_arg, BardeenImage B_arg, MixedCfg M, const int64_t* work, PlanCounters* c, int refill_interval,
     int band_priority, int split, uint8_t* used_fp64, RowsOut<double> out, int32_t* flags, const double* Y0,
-    int64_t total) {
+    int64_t total, double* ev_Y, uint8_t* ev_hit) {
     static_assert(!(ROWS && MIXED), "given rows are traced in FP64");
+    static_assert(!DISK || (ROWS && !MIXED), "the disk event runs on given rows in FP64");
     // At one wavefront per SIMD there are registers to spare but scalar
     // registers are short: the launch constants, uniform values the compiler
     // would hold in scalar registers (and spill to vector lanes and reload,
@@ -339,6 +342,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
                             busy = true;
                         } else if (!(pair && odd)) {
                             store_row(out, idx, R);
+                            if (DISK) store_event(ev_Y, ev_hit, idx, R);
                             used_fp64[idx] = 1;
                         }
                     }
@@ -366,12 +370,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
             } else if (MIXED && pair) {
                 ray_attempt_with<double, true, false>(P, R, StoredStepper<double, true, PARK, TrigPair>{K, {odd}});
             } else {
-                ray_attempt_with<double, true, false>(P, R, StoredStepper<double, true, PARK>{K});
+                ray_attempt_with<double, true, DISK>(P, R, StoredStepper<double, true, PARK>{K});
             }
             if (R.state != RUNNING && pair && odd) {
                 busy = false;  // the even lane of the pair writes the row
             } else if (R.state != RUNNING) {
                 store_row(out, idx, R);
+                if (DISK) store_event(ev_Y, ev_hit, idx, R);
                 used_fp64[idx] = exact ? 1 : 0;
                 const int64_t n_fp64 = exact ? R.n_steps + R.n_rej : fp64_att;
                 if (n_fp64) atomicAdd(&c->fp64_attempts, static_cast<unsigned long long>(n_fp64));
@@ -520,7 +525,8 @@ void launch_planned_t(int cus, double frac, hipStream_t s, const Params& P, cons
     const int slots = cus * (4 * WAVES * 64 / BLOCK);
     const int grid = std::max(1, static_cast<int>(slots * frac));
     hipLaunchKernelGGL((planned_trace_kernel<BLOCK, MIXED, WAVES, PARK>), dim3(grid), dim3(BLOCK), 0, s, P, B, M, w.work,
-                       w.counters, refill, band_priority, split, w.used_fp64, o, w.flags, nullptr, B.n * B.n);
+                       w.counters, refill, band_priority, split, w.used_fp64, o, w.flags, nullptr, B.n * B.n, nullptr,
+                       nullptr);
 }
 
 template <int BLOCK, bool MIXED>
@@ -668,9 +674,23 @@ static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
 
 size_t kr::host::rows_workspace_bytes(int64_t n) { return workspace_bytes(n, 0); }
 
-// The FP64 rk45 trace of given rows (kr_integrate_batch[_dev]): the row plan,
-// then the persistent trace of the longest-first work list, exactly as the
-// planned image trace runs it; every row equals the plain kernel's.
+// One launch of the planned kernel over given rows: WAVES wavefronts per SIMD
+// on every slot, with or without the disk event.
+template <int BLOCK, int WAVES, bool PARK, bool DISK>
+static void launch_rows(int cus, const Params& P, const Work& w, const RowsOut<double>& o, const double* d_Y0,
+                        int64_t n, const kr_outputs& d_out, hipStream_t s) {
+    const BardeenImage none{};
+    const MixedCfg M{};
+    hipLaunchKernelGGL((planned_trace_kernel<BLOCK, false, WAVES, PARK, true, DISK>),
+                       dim3(cus * (4 * WAVES * 64 / BLOCK)), dim3(BLOCK), 0, s, P, none, M, w.work, w.counters, 8, 0,
+                       0, w.used_fp64, o, w.flags, d_Y0, n, DISK ? static_cast<double*>(d_out.event_Y) : nullptr,
+                       DISK ? d_out.event_hit : nullptr);
+}
+
+// The FP64 rk45 trace of given rows (kr_integrate_batch[_dev], and the disk
+// render): the row plan, then the persistent trace of the longest-first work
+// list, exactly as the planned image trace runs it; every row (and disk
+// event) equals the plain kernel's.
 kr_status kr::host::enqueue_rows_planned(kr_context* h, const Params& P, const double* d_Y0, int64_t n,
                                          const kr_outputs& d_out, void* d_work, hipStream_t s) {
     if (n <= 0) return KR_OK;
@@ -694,16 +714,20 @@ kr_status kr::host::enqueue_rows_planned(kr_context* h, const Params& P, const d
     // The planned image trace's automatic occupancy: one wavefront per SIMD up
     // to 2^19 rows, two with the error sums parked in LDS above.
     constexpr int BLOCK = 128;
-    const BardeenImage none{};
-    const MixedCfg M{};
-    if (n > (int64_t(1) << 19))
-        hipLaunchKernelGGL((planned_trace_kernel<BLOCK, false, 2, true, true>), dim3(cus * (4 * 2 * 64 / BLOCK)),
-                           dim3(BLOCK), 0, s, P, none, M, w.work, w.counters, 8, 0, 0, w.used_fp64, o, w.flags, d_Y0,
-                           n);
-    else
-        hipLaunchKernelGGL((planned_trace_kernel<BLOCK, false, 1, false, true>), dim3(cus * (4 * 1 * 64 / BLOCK)),
-                           dim3(BLOCK), 0, s, P, none, M, w.work, w.counters, 8, 0, 0, w.used_fp64, o, w.flags, d_Y0,
-                           n);
+    const bool two = n > (int64_t(1) << 19);
+    if (P.ipar[I_DISK]) {
+        if (!d_out.event_Y || !d_out.event_hit)
+            return host::fail(KR_ERR_INVALID, "the disk event needs event_Y and event_hit");
+        if (two)
+            launch_rows<BLOCK, 2, true, true>(cus, P, w, o, d_Y0, n, d_out, s);
+        else
+            launch_rows<BLOCK, 1, false, true>(cus, P, w, o, d_Y0, n, d_out, s);
+    } else {
+        if (two)
+            launch_rows<BLOCK, 2, true, false>(cus, P, w, o, d_Y0, n, d_out, s);
+        else
+            launch_rows<BLOCK, 1, false, false>(cus, P, w, o, d_Y0, n, d_out, s);
+    }
     KR_HIP(hipGetLastError());
     return KR_OK;
 }

@@ -559,14 +559,42 @@ kr_status copy_back(kr_handle h, const BatchBuffers& b, const kr_outputs& out, i
 
 namespace {
 
-// FP64 Dormand-Prince batches of at least this many rows run on the
-// longest-first persistent kernel (kr_mixed.hip); smaller ones, and every
-// other option, on the plain one-photon-per-lane kernel.  Both give the same
-// rows bit for bit.
+// FP64 Dormand-Prince batches of at least this many rows (four times as many
+// with the disk event, whose photons end at the disk and leave less imbalance
+// to remove) run on the longest-first persistent kernel (kr_mixed.hip); smaller
+// ones, and every other option, on the plain one-photon-per-lane kernel.
+// Both give the same rows and events bit for bit.
 constexpr int64_t ROWS_PLANNED_MIN = int64_t(1) << 15;
+constexpr int64_t DISK_ROWS_PLANNED_MIN = int64_t(1) << 17;
 
 bool rows_planned(const Params& P, int dtype, int64_t n) {
-    return dtype == KR_FLOAT64 && P.ipar[I_ADAPTIVE] == 1 && P.ipar[I_DISK] == 0 && n >= ROWS_PLANNED_MIN;
+    const int64_t min = P.ipar[I_DISK] ? DISK_ROWS_PLANNED_MIN : ROWS_PLANNED_MIN;
+    return dtype == KR_FLOAT64 && P.ipar[I_ADAPTIVE] == 1 && n >= min;
+}
+
+// The workspace of a planned disk render: the pixels' initial states (n, 8),
+// then the row planner's workspace.
+size_t round256(size_t bytes) { return (bytes + 255) & ~static_cast<size_t>(255); }
+size_t disk_rows_bytes(int64_t n) { return round256(static_cast<size_t>(n) * 64) + host::rows_workspace_bytes(n); }
+
+// The image trace of a disk render into `o`: planned rows on the image's
+// initial states (in `ws`, disk_rows_bytes(n)) where rows_planned holds, the
+// plain image kernel elsewhere; the same rows and events either way.
+kr_status disk_trace(kr_handle h, const Params& P, int dtype, const BardeenImage& B, const kr_outputs& o,
+                     int32_t* flags, char* ws, hipStream_t s) {
+    const int64_t n = B.n * B.n;
+    if (rows_planned(P, dtype, n)) {
+        double* Y0 = reinterpret_cast<double*>(ws);
+        hipLaunchKernelGGL(image_states_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, B, Y0,
+                           flags);
+        KR_HIP(hipGetLastError());
+        return host::enqueue_rows_planned(h, P, Y0, n, o, ws + round256(static_cast<size_t>(n) * 64), s);
+    }
+    ICArgs ic{};
+    ic.B = B;
+    ic.flags = flags;
+    KR_HIP(launch_trace<IC_IMAGE>(P, dtype, ic, o, n, s));
+    return KR_OK;
 }
 
 }  // namespace
@@ -685,16 +713,23 @@ kr_status image_host(kr_handle h, const kr_params* p, const kr_image* img, const
     }
     KR_HIP(hipSetDevice(h->device));
     BatchBuffers b;
-    // intensity, g, r_hit follow the batch buffers
-    if ((st = carve(h, n, p->dtype, false, events, b, disk ? static_cast<size_t>(n) * 24 : 0)) != KR_OK) return st;
+    // intensity, g, r_hit follow the batch buffers; a render's trace
+    // workspace follows them
+    const size_t shade_bytes = disk ? round256(static_cast<size_t>(n) * 24) : 0;
+    const size_t ws_bytes = disk && rows_planned(P, p->dtype, n) ? disk_rows_bytes(n) : 0;
+    if ((st = carve(h, n, p->dtype, false, events, b, shade_bytes + ws_bytes)) != KR_OK) return st;
     double* shade = reinterpret_cast<double*>(b.extra);
     hipStream_t s = h->stream;
     KR_HIP(fill_i32(b.flags, 1, s));
-    ICArgs ic{};
-    ic.B = B;
-    ic.flags = b.flags;
     KR_HIP(hipEventRecord(h->ev0, s));
-    KR_HIP(launch_trace<IC_IMAGE>(P, p->dtype, ic, b.d, n, s));
+    if (disk) {
+        if ((st = disk_trace(h, P, p->dtype, B, b.d, b.flags, b.extra + shade_bytes, s)) != KR_OK) return st;
+    } else {
+        ICArgs ic{};
+        ic.B = B;
+        ic.flags = b.flags;
+        KR_HIP(launch_trace<IC_IMAGE>(P, p->dtype, ic, b.d, n, s));
+    }
     if (disk)
         KR_HIP(launch_shade(D, p->dtype, b.d.event_Y, b.d.event_hit, n, shade, shade + n, shade + 2 * n, b.flags, s));
     KR_HIP(hipEventRecord(h->ev1, s));
@@ -748,10 +783,8 @@ extern "C" kr_status kr_render_disk_dev(kr_handle h, const kr_params* p, const k
     if (!d_intensity || !d_g || !d_r_hit) return host::fail(KR_ERR_INVALID, "intensity, g or r_hit is NULL");
     KR_HIP(hipSetDevice(h->device));
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    ICArgs ic{};
-    ic.B = B;
-    ic.flags = d_flags;
-    KR_HIP(launch_trace<IC_IMAGE>(P, p->dtype, ic, *d_out, n, s));
+    if (rows_planned(P, p->dtype, n) && (st = host::reserve_ws(h, disk_rows_bytes(n))) != KR_OK) return st;
+    if ((st = disk_trace(h, P, p->dtype, B, *d_out, d_flags, static_cast<char*>(h->ws), s)) != KR_OK) return st;
     KR_HIP(launch_shade(D, p->dtype, d_out->event_Y, d_out->event_hit, n, d_intensity, d_g, d_r_hit, d_flags, s));
     return KR_OK;
 }

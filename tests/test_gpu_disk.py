@@ -10,10 +10,15 @@
 * ``redshift_factor`` on the device against the oracle to a few ulp, and the
   errors the reference raises.
 * The FP32 working dtype renders the same hit set as the oracle's FP32 mode.
+* FP64 renders of at least 2^17 pixels trace on the longest-first persistent
+  kernel over the image's initial states (kr_mixed.hip); every row and every
+  disk event equals the plain image kernel's bit for bit, on the host and
+  device entry points and through integrate_batch with the disk event.
 """
 
 from __future__ import annotations
 
+import ctypes
 import os
 
 import numpy as np
@@ -111,3 +116,91 @@ def test_float32_render_matches_the_oracle_float32_mode(hip, oracle) -> None:
     both = hit & ref.event_hit.astype(bool)
     g_o, _ = oracle.disk_shade(1.0, 0.5, 6.0, 20.0, 3.0, 4, True, ref.event_Y[both].astype(np.float64))
     assert np.median(_rel(img.g.ravel()[both], g_o)) < 1e-5
+
+
+FIELDS = ("Y", "state", "n_steps", "n_rejected", "lam", "max_null_error", "max_energy_drift", "max_lz_drift",
+          "max_carter_drift", "event_Y", "event_hit")
+
+
+def _plain_trace(hip, st, cam, integ, term, disk):
+    """kr_trace_image with the disk event: the plain one-photon-per-lane image kernel at every size."""
+    from kerrray_amd import _lib
+    from kerrray_amd.backends import HostBatch, kernel_params
+    from kerrray_amd.camera import image_struct
+
+    params, img = kernel_params(st, integ, term, event_options(disk)), image_struct(st, cam)
+    out = HostBatch(cam.n_rays, integ.np_dtype, True)
+    ms = ctypes.c_double(0.0)
+    with hip.handle._lock:
+        _lib.check(_lib.lib().kr_trace_image(hip.handle.ptr, ctypes.byref(params), ctypes.byref(img),
+                                             ctypes.byref(out.outputs), ctypes.byref(ms)), "kr_trace_image")
+    return out.result(0.0), ms.value
+
+
+@pytest.mark.parametrize("camera,res", [("zamo", 364), ("bardeen", 366)])
+def test_planned_disk_render_equals_the_plain_kernel(hip, camera, res) -> None:
+    from kerrray_amd.camera import BardeenCamera, bardeen_initial_states, initial_states
+    from kerrray_amd.renderer import DiskParams
+
+    st = Spacetime(1.0, 0.9)
+    disk = DiskModel.from_params(st, DiskParams())
+    integ, term = options(rtol=1e-7, atol=1e-9)
+    if camera == "zamo":
+        cam = Camera(radius=1000.0, inclination_deg=75.0, fov=25.0, resolution=res)
+        Y0 = initial_states(cam, st, backend=hip)
+    else:
+        cam = BardeenCamera(radius=1000.0, inclination_deg=80.0, fov=25.0, resolution=res)
+        Y0 = bardeen_initial_states(cam, st)
+    assert cam.n_rays >= 1 << 17  # the planned kernel's size with the disk event
+    img = render_disk(st, cam, integ, term, disk, backend=hip)
+    ref, plain_ms = _plain_trace(hip, st, cam, integ, term, disk)
+    rows = hip.integrate_batch(st, Y0, integ, term, event_options(disk))
+    rows_ms = hip.last_kernel_ms
+    hits = int(np.sum(ref.event_hit))
+    print(f"[disk {camera} {res}^2] planned render {img.kernel_ms:.3f} ms (trace + shading), plain image "
+          f"kernel {plain_ms:.3f} ms, planned rows {rows_ms:.3f} ms; {hits} disk hits", flush=True)
+    assert hits > 1000
+    for name in FIELDS:
+        assert np.array_equal(getattr(img.result, name), getattr(ref, name)), name
+        assert np.array_equal(getattr(rows, name), getattr(ref, name)), name
+    hit = img.hit.ravel()
+    assert np.array_equal(hit, ref.event_hit.astype(bool))
+    assert np.all(np.isfinite(img.g.ravel()[hit])) and np.all(np.isnan(img.g.ravel()[~hit]))
+
+
+def test_device_render_entry_point_equals_the_host_one(hip) -> None:
+    import torch
+
+    from bench import device_outputs
+    from kerrray_amd import _lib
+    from kerrray_amd.backends import kernel_params
+    from kerrray_amd.camera import image_struct
+    from kerrray_amd.renderer import DiskParams
+
+    st = Spacetime(1.0, 0.5)
+    disk = DiskModel.from_params(st, DiskParams())
+    integ, term = options(rtol=1e-7, atol=1e-9)
+    cam = Camera(radius=1000.0, inclination_deg=70.0, fov=25.0, resolution=364)  # planned: >= 2^17 pixels
+    host = render_disk(st, cam, integ, term, disk, backend=hip)
+    n = cam.n_rays
+    params, img, kd = kernel_params(st, integ, term, event_options(disk)), image_struct(st, cam), disk.kr()
+    bufs, outs = device_outputs(torch, n, torch.float64)
+    ev_Y = torch.empty((n, 8), dtype=torch.float64, device="cuda")
+    ev_hit = torch.empty(n, dtype=torch.uint8, device="cuda")
+    outs.event_Y, outs.event_hit = ev_Y.data_ptr(), ev_hit.data_ptr()
+    shade = torch.empty((3, n), dtype=torch.float64, device="cuda")
+    flags = torch.zeros(1, dtype=torch.int32, device="cuda")
+    for _ in range(2):  # the second call reuses the handle's device workspace
+        _lib.check(_lib.lib().kr_render_disk_dev(hip.handle.ptr, ctypes.byref(params), ctypes.byref(img),
+                                                 ctypes.byref(kd), ctypes.byref(outs), shade[0].data_ptr(),
+                                                 shade[1].data_ptr(), shade[2].data_ptr(), flags.data_ptr(), None),
+                   "kr_render_disk_dev")
+        torch.cuda.synchronize()
+        assert int(flags.item()) == 0
+        assert np.array_equal(bufs["state"].cpu().numpy(), host.result.state)
+        assert np.array_equal(bufs["n_steps"].cpu().numpy(), host.result.n_steps)
+        assert np.array_equal(ev_Y.cpu().numpy(), host.result.event_Y)
+        assert np.array_equal(ev_hit.cpu().numpy(), host.result.event_hit)
+        sh = shade.cpu().numpy()
+        for k, arr in enumerate((host.intensity, host.g, host.r_hit)):
+            assert np.array_equal(sh[k], arr.ravel(), equal_nan=True)
