@@ -504,10 +504,12 @@ def precision_report(hip, log) -> dict:
                      **{f"vs_bardeen_{k}": float(v) for k, v in err.items()}}}
 
 
-# Probes per direction and round of the edge bisection: 720 x 127 lanes keep the
-# GPU busy through 5 rounds (72.9 ms of kernels, against 87.5 ms for 63 probes
-# in 6 rounds and 110.6 ms for 255 in 5).
-EDGE_PROBES = 127
+# Probes per direction and round of the edge bisection: with 63, each of the
+# 720 directions is one wavefront and all 6 rounds run in one launch at one
+# wavefront per SIMD (58-64 ms of kernels, against 63-65 ms for 127 probes in
+# 5 rounds of two launches each, whose 720 workgroups of two wavefronts do not
+# fit at one per SIMD).
+EDGE_PROBES = 63
 
 
 def adjoint_report(hip, log) -> dict:

@@ -17,7 +17,9 @@
  *    copy through library-owned device memory and return after the work is
  *    done.
  *  - A handle owns one device, one stream and a grow-only device buffer; it
- *    is not re-entrant (one call at a time per handle).
+ *    is not re-entrant (one call at a time per handle).  The _dev calls of
+ *    one handle share its device workspace: enqueue them on one stream, or
+ *    synchronise between calls made on different streams.
  */
 #ifndef KERRRAY_HIP_H
 #define KERRRAY_HIP_H

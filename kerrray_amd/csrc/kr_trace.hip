@@ -165,25 +165,90 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if (r.state != ESCAPED && r.state != CAPTURED) atomicAdd(failures, 1ull);
 }
 
-// Bracket update (study_edge_bisection.py:66-67 for n_probe = 1).
+// The bracket update of one direction from its round's probe results
+// escaped[0, n_probe) (study_edge_bisection.py:66-67 for n_probe = 1): the
+// sub-bracket around the first escaping probe.  Returns true when the probes
+// were not ordered captured -> escaped.
+__device__ __forceinline__ bool bisect_update(double& lo, double& hi, const uint8_t* escaped, int n_probe) {
+    const double l = lo, h = hi;
+    int first = -1;
+    bool bad = false;
+    for (int j = 0; j < n_probe; ++j) {
+        const bool e = escaped[j] != 0;
+        if (e && first < 0) first = j;
+        if (!e && first >= 0) bad = true;
+    }
+    if (first < 0) {
+        lo = probe_radius(l, h, n_probe - 1, n_probe);
+    } else {
+        hi = probe_radius(l, h, first, n_probe);
+        if (first > 0) lo = probe_radius(l, h, first - 1, n_probe);
+    }
+    return bad;
+}
+
+// Bracket update kernel, one thread per direction.
 __global__ void bisect_update_kernel(double* lo, double* hi, const uint8_t* escaped, int64_t n_angles, int n_probe,
                                      unsigned long long* nonmonotone) {
     const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= n_angles) return;
-    const double l = lo[k], h = hi[k];
-    int first = -1;
-    bool bad = false;
-    for (int j = 0; j < n_probe; ++j) {
-        const bool e = escaped[k * n_probe + j] != 0;
-        if (e && first < 0) first = j;
-        if (!e && first >= 0) bad = true;
+    double l = lo[k], h = hi[k];
+    if (bisect_update(l, h, escaped + k * n_probe, n_probe)) atomicAdd(nonmonotone, 1ull);
+    lo[k] = l;
+    hi[k] = h;
+}
+
+// Every round of the bisection in one launch (FP64 DP5).  A workgroup of
+// BLOCK lanes carries `dirs` directions (n_probe lanes each, dirs * n_probe
+// <= BLOCK) through all `iterations` rounds, its brackets in LDS between
+// rounds, so a direction waits for its own slowest probe only, never for
+// another workgroup's.  One wavefront per SIMD: the launch is used only when
+// all its workgroups fit on the chip at once.  The probe radii and the update
+// are those of the per-round kernels above: the brackets are the same bit for
+// bit.
+template <typename T, bool ADAPTIVE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1, 1))) void bisect_rounds_kernel(
+    Params P, BardeenImage B, double ca, double cb, const double* cosk, const double* sink, double* lo, double* hi,
+    int64_t n_angles, int n_probe, int dirs, int iterations, unsigned long long* counts, int32_t* flags) {
+    // The directions' brackets and unit vectors stay in LDS, so that the
+    // lanes carry nothing but their photon across the rounds.
+    __shared__ uint8_t esc[BLOCK];
+    __shared__ double lo_s[64], hi_s[64], c_s[64], s_s[64];  // dirs <= 64
+    const int d = static_cast<int>(threadIdx.x) / n_probe;  // direction within the workgroup
+    const int j = static_cast<int>(threadIdx.x) - d * n_probe;
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * dirs + d;
+    const bool active = d < dirs && k < n_angles;
+    if (active && j == 0) {
+        lo_s[d] = lo[k];
+        hi_s[d] = hi[k];
+        c_s[d] = cosk[k];
+        s_s[d] = sink[k];
     }
-    if (bad) atomicAdd(nonmonotone, 1ull);
-    if (first < 0) {
-        lo[k] = probe_radius(l, h, n_probe - 1, n_probe);
-    } else {
-        hi[k] = probe_radius(l, h, first, n_probe);
-        if (first > 0) lo[k] = probe_radius(l, h, first - 1, n_probe);
+    __syncthreads();
+    for (int it = 0; it < iterations; ++it) {
+        if (active) {
+            const double mid = probe_radius(lo_s[d], hi_s[d], j, n_probe);
+            const double alpha = ca + mid * c_s[d];
+            const double beta = cb + mid * s_s[d];
+            double y0[8];
+            if (const int32_t bad = image_state(B, alpha, beta, y0)) atomicOr(flags, bad);
+            RayResult<T> r;
+            integrate_ray<T, ADAPTIVE, false>(P, y0, r);
+            esc[threadIdx.x] = r.state == ESCAPED ? 1 : 0;
+            if (r.state != ESCAPED && r.state != CAPTURED) atomicAdd(&counts[0], 1ull);
+        }
+        __syncthreads();
+        if (active && j == 0) {
+            double l = lo_s[d], h = hi_s[d];
+            if (bisect_update(l, h, esc + d * n_probe, n_probe)) atomicAdd(&counts[1], 1ull);
+            lo_s[d] = l;
+            hi_s[d] = h;
+        }
+        __syncthreads();
+    }
+    if (active && j == 0) {
+        lo[k] = lo_s[d];
+        hi[k] = hi_s[d];
     }
 }
 
@@ -925,6 +990,40 @@ extern "C" kr_status kr_trace_points_dev(kr_handle h, const kr_params* p, const 
     return KR_OK;
 }
 
+namespace {
+
+// Largest workgroup of the one-launch bisection: 256 lanes, one per SIMD of
+// a CU at one wavefront per SIMD.
+constexpr int BISECT_ROUNDS_MAX_PROBE = 256;
+
+// Launches bisect_rounds_kernel on the smallest workgroup (64, 128 or 256
+// lanes) that holds a direction's probes (at most 64 probes: as many
+// directions as fit in one wavefront) and returns true, if every workgroup
+// fits on the chip at one wavefront per SIMD (4 per CU); else launches
+// nothing and returns false.  FP64 Dormand-Prince only.
+bool launch_bisect_rounds(int cus, const Params& P, int dtype, const BardeenImage& B, double ca, double cb,
+                          const double* cosk, const double* sink, double* lo, double* hi, int64_t n_angles,
+                          int n_probe, int iterations, unsigned long long* counts, int32_t* flags, hipStream_t s) {
+    if (dtype != KR_FLOAT64 || P.ipar[I_ADAPTIVE] == 0 || n_probe > BISECT_ROUNDS_MAX_PROBE) return false;
+    const int block = n_probe <= 64 ? 64 : n_probe <= 128 ? 128 : 256;
+    const int dirs = block / n_probe;
+    const int64_t grid = (n_angles + dirs - 1) / dirs;
+    if (grid > static_cast<int64_t>(cus) * (256 / block)) return false;
+    const dim3 g(static_cast<unsigned>(grid)), b(static_cast<unsigned>(block));
+    if (block == 64)
+        hipLaunchKernelGGL((bisect_rounds_kernel<double, true, 64>), g, b, 0, s, P, B, ca, cb, cosk, sink, lo, hi,
+                           n_angles, n_probe, dirs, iterations, counts, flags);
+    else if (block == 128)
+        hipLaunchKernelGGL((bisect_rounds_kernel<double, true, 128>), g, b, 0, s, P, B, ca, cb, cosk, sink, lo, hi,
+                           n_angles, n_probe, dirs, iterations, counts, flags);
+    else
+        hipLaunchKernelGGL((bisect_rounds_kernel<double, true, 256>), g, b, 0, s, P, B, ca, cb, cosk, sink, lo, hi,
+                           n_angles, n_probe, dirs, iterations, counts, flags);
+    return true;
+}
+
+}  // namespace
+
 extern "C" kr_status kr_bisect_edge(kr_handle h, const kr_params* p, const kr_image* img, double centre_alpha,
                                     double centre_beta, const double* cos_angles, const double* sin_angles,
                                     int64_t n_angles, double* lo, double* hi, int32_t iterations, int32_t n_probe,
@@ -965,7 +1064,15 @@ extern "C" kr_status kr_bisect_edge(kr_handle h, const kr_params* p, const kr_im
     KR_HIP(fill_i32(d_flags, 1, s));
     KR_HIP(hipEventRecord(h->ev0, s));
     const bool adaptive = P.ipar[I_ADAPTIVE] != 0;
-    for (int it = 0; it < iterations && nt > 0; ++it) {
+    int cus = 0;
+    KR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+    bool fused = false;
+    if (nt > 0 && iterations > 0) {
+        fused = launch_bisect_rounds(cus, P, p->dtype, B, centre_alpha, centre_beta, d_cos, d_sin, d_lo, d_hi,
+                                     n_angles, n_probe, iterations, d_cnt, d_flags, s);
+        KR_HIP(hipGetLastError());
+    }
+    for (int it = 0; !fused && it < iterations && nt > 0; ++it) {
         const unsigned blocks = static_cast<unsigned>((nt + 255) / 256);
         if (p->dtype == KR_FLOAT64) {
             if (adaptive)

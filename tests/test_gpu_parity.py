@@ -317,6 +317,39 @@ def test_shadow_edge_matches_bardeen_curve(hip, n_probe: int) -> None:
     assert err["rms"] < 5e-8 and err["max"] < 2e-7
 
 
+@pytest.mark.parametrize("n_probe,n_angles,rounds", [(127, 400, 2), (1, 96, 12), (63, 200, 3), (15, 300, 4),
+                                                     (200, 40, 2)])
+def test_fused_bisection_rounds_equal_one_call_per_round(hip, n_probe, n_angles, rounds) -> None:
+    """Every round in one launch (brackets in LDS between rounds) against one call per round: same brackets bit for bit.
+
+    The cases cover workgroups of 64, 128 and 256 lanes and one to 64
+    directions per wavefront.
+    """
+    from kerrray_amd.bisection import bisect_edge, bisection_options
+    from kerrray_amd.boundary import polygon_centroid, polygon_radii
+    from kerrray_amd.geometry import Spacetime
+    from kerrray_amd.orbits import shadow_curve
+
+    st, incl, rtol = Spacetime(1.0, 0.9), 60.0, 1e-8
+    integ, term = bisection_options(rtol, 1000.0)
+    curve = shadow_curve(st, incl, 4000)
+    centre = polygon_centroid(*curve)
+    angles = 2.0 * np.pi * np.arange(n_angles) / n_angles
+    r_ana = polygon_radii(curve[0], curve[1], centre, angles)
+    kw = dict(n_probe=n_probe, integ=integ, term=term)
+    fused = bisect_edge(st, incl, 1000.0, rtol, centre, angles, 0.98 * r_ana, 1.02 * r_ana, rounds, hip, **kw)
+    lo, hi, failures, nonmono, ms = 0.98 * r_ana, 1.02 * r_ana, 0, 0, 0.0
+    for _ in range(rounds):
+        one = bisect_edge(st, incl, 1000.0, rtol, centre, angles, lo, hi, 1, hip, **kw)
+        lo, hi = one.lo, one.hi
+        failures, nonmono, ms = failures + one.failures, nonmono + one.nonmonotone, ms + one.kernel_ms
+    print(f"[bisect rounds n_probe={n_probe} x {n_angles}] {rounds} rounds in one launch "
+          f"{fused.kernel_ms:.2f} ms, one call per round {ms:.2f} ms", flush=True)
+    assert np.array_equal(fused.lo, lo) and np.array_equal(fused.hi, hi)
+    assert (fused.failures, fused.nonmonotone) == (failures, nonmono)
+    assert np.all(fused.hi - fused.lo < 0.04 * r_ana / (n_probe + 1) ** (rounds - 1))
+
+
 def test_kerr_512_every_photon_against_the_oracle(hip, oracle, golden2) -> None:
     """All 262,144 photons of the 512^2, a=0.9, i=60 deg, rtol 1e-6 image, device against the oracle.
 

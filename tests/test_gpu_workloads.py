@@ -68,13 +68,16 @@ def test_ragged64_graph_launch_equals_plain_kernel_per_image(hip) -> None:
     assert not mismatched, mismatched[:10]
 
 
-def test_edge720_a099_i85_fp64_rtol1e8(hip) -> None:
+@pytest.mark.parametrize("n_probe", [63, 127])
+def test_edge720_a099_i85_fp64_rtol1e8(hip, n_probe) -> None:
+    """63 probes per direction: every round in one launch at one wavefront per SIMD; 127: one launch per round."""
     from kerrray_amd.bisection import shadow_edge
     from kerrray_amd.geometry import Spacetime
 
     st = Spacetime(1.0, 0.99)
-    edge, angles, centre, err = shadow_edge(st, 85.0, 1000.0, 1e-8, 720, backend=hip, n_probe=127)
-    print(f"[edge720 a=0.99 i=85] {edge.kernel_ms:.1f} ms of kernels, failures {edge.failures}, nonmonotone "
+    edge, angles, centre, err = shadow_edge(st, 85.0, 1000.0, 1e-8, 720, backend=hip, n_probe=n_probe)
+    print(f"[edge720 a=0.99 i=85 n_probe={n_probe}] {edge.kernel_ms:.1f} ms of kernels, failures {edge.failures}, "
+          f"nonmonotone "
           f"{edge.nonmonotone}, widest bracket {edge.width.max():.2e}, max |r - Bardeen| {err['max']:.2e}, "
           f"rms {err['rms']:.2e}", flush=True)
     assert angles.size == 720
