@@ -267,7 +267,9 @@ kr_status kr_trace_points_dev(kr_handle h, const kr_params* p, const kr_image* i
  * src/kerrray/validation/kerr_bisection.py:12-18,230-246).  failures
  * counts traced photons that neither escaped nor were captured;
  * nonmonotone counts rounds whose probes were not inside...outside ordered.
- * Host arrays. */
+ * Host arrays.  FP64 rk45 searches whose directions all fit on the chip at
+ * one wavefront per SIMD (n_probe <= 256) run every round in one launch;
+ * others launch once per round.  The brackets are the same either way. */
 kr_status kr_bisect_edge(kr_handle h, const kr_params* p, const kr_image* img, double centre_alpha,
                          double centre_beta, const double* cos_angles, const double* sin_angles, int64_t n_angles,
                          double* lo, double* hi, int32_t iterations, int32_t n_probe, int64_t* failures,
