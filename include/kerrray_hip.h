@@ -130,10 +130,10 @@ const char* kr_last_error(void);
  * integrate_batch_numba (numba_backend.py:131-207) ->
  * integrate_rays_kernel (numba_kernel.py:466-474).  Y0 is (n, 8) float64
  * host memory; outputs are host arrays.  kernel_ms (may be NULL) receives
- * the device time of the integration kernel.  FP64 rk45 batches of at least
- * 2^15 rows (2^17 with the disk event) run on the persistent longest-first
- * kernel, smaller ones one photon per lane; the rows are the same bit for
- * bit either way. */
+ * the device time of the integration kernel.  rk45 batches of at least
+ * 2^15 rows (FP64) or 2^17 rows (FP64 with the disk event, float32) run on
+ * the persistent longest-first kernels, the others one photon per lane; the
+ * rows are the same bit for bit either way. */
 kr_status kr_integrate_batch(kr_handle h, const kr_params* p, const double* Y0, int64_t n, const kr_outputs* out,
                              double* kernel_ms);
 
@@ -212,9 +212,10 @@ kr_status kr_disk_shade_dev(kr_handle h, const kr_disk* disk, double mass, doubl
  * of kr_trace_image with the disk event, then the shading above.  Host
  * arrays: the BatchResult fields in `out` (event_Y, event_hit included) and
  * intensity, g, r_hit (n*n each).  kernel_ms (may be NULL) receives the
- * device time of trace and shading.  FP64 rk45 renders of at least 2^17
- * pixels trace the image's initial states on the persistent longest-first
- * kernel (the rows and crossing states of kr_trace_image bit for bit). */
+ * device time of trace and shading.  rk45 renders of at least 2^17 pixels
+ * (FP64 or float32) trace the image's initial states on the persistent
+ * longest-first kernels (the rows and crossing states of kr_trace_image bit
+ * for bit). */
 kr_status kr_render_disk(kr_handle h, const kr_params* p, const kr_image* img, const kr_disk* disk,
                          const kr_outputs* out, double* intensity, double* g, double* r_hit, double* kernel_ms);
 /* Same on device arrays, enqueued on `stream`. */

@@ -189,10 +189,11 @@ inline kr_status bardeen_image(const kr_image* img, bool need_grid, kr::BardeenI
     return err ? fail(KR_ERR_INVALID, err) : KR_OK;
 }
 
-// The FP64 rk45 trace of given rows on the longest-first persistent kernel
-// (kr_mixed.hip), enqueued on s with a workspace of rows_workspace_bytes(n).
+// The trace of given rows (working dtype `dtype`) on the longest-first
+// persistent kernels (kr_mixed.hip), enqueued on s with a workspace of
+// rows_workspace_bytes(n).
 size_t rows_workspace_bytes(int64_t n);
-kr_status enqueue_rows_planned(kr_context* h, const kr::Params& P, const double* d_Y0, int64_t n,
+kr_status enqueue_rows_planned(kr_context* h, const kr::Params& P, int dtype, const double* d_Y0, int64_t n,
                                const kr_outputs& d_out, void* d_work, hipStream_t s);
 
 // Finds the device address of the planned trace's static scratch array on

@@ -387,6 +387,70 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES, WA
     }
 }
 
+// The longest-first trace of given rows in the float32 working dtype (the
+// planned kernels hold FP64 states): one photon per lane with its state in
+// registers; each wavefront refills its idle lanes from the work list every
+// `refill_interval` attempts, or at once when all are idle.  A photon runs
+// ray_init and ray_attempt as integrate_ray does, so every row (and disk
+// event) equals the plain kernel's bit for bit.  (Also instantiable for
+// FP64 and RK4; RK4 photons of one image take about as many steps each, and
+// the plain kernel traces them as fast.)
+template <typename T, bool ADAPTIVE, bool DISK>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void rows_refill_kernel(
+    Params P, const int64_t* work, PlanCounters* c, int refill_interval, RowsOut<T> out, T* ev_Y, uint8_t* ev_hit,
+    const double* Y0, int64_t total) {
+    const int lane = threadIdx.x & 63;
+    RayState<T> R;
+    R.state = RUNNING;
+    bool busy = false, exhausted = false;
+    int64_t idx = -1;
+    int since = refill_interval;
+    while (true) {
+        const bool any_busy = __any(busy);
+        if (!exhausted && (since >= refill_interval || !any_busy)) {
+            since = 0;
+            const uint64_t idle = __ballot(!busy);
+            const int n_idle = __popcll(idle);
+            if (n_idle > 0) {
+                const int leader = __ffsll(static_cast<unsigned long long>(idle)) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(&c->work, static_cast<unsigned long long>(n_idle));
+                base = __shfl(base, leader);
+                if (base + static_cast<unsigned long long>(n_idle) >= static_cast<unsigned long long>(total))
+                    exhausted = true;
+                if (!busy) {
+                    const uint64_t below = lane == 0 ? 0ull : (idle & (~0ull >> (64 - lane)));
+                    const int64_t pos = static_cast<int64_t>(base) + __popcll(below);
+                    if (pos < total) {
+                        idx = work[pos] & INDEX_MASK;
+                        double y0[8];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) y0[k] = Y0[idx * 8 + k];
+                        ray_init<T, ADAPTIVE>(P, y0, R);
+                        if (R.state == RUNNING) {
+                            busy = true;
+                        } else {
+                            store_row(out, idx, R);
+                            if (DISK) store_event(ev_Y, ev_hit, idx, R);
+                        }
+                    }
+                }
+            }
+            continue;
+        }
+        if (!any_busy) break;
+        if (busy) {
+            ray_attempt<T, ADAPTIVE, DISK>(P, R);
+            if (R.state != RUNNING) {
+                store_row(out, idx, R);
+                if (DISK) store_event(ev_Y, ev_hit, idx, R);
+                busy = false;
+            }
+        }
+        ++since;
+    }
+}
+
 // The verify pass: round 0's repair list (kr_repair.h).  Each lane marks
 // its own pixel queued; no lane reads another's mark.
 __global__ void verify_kernel(const int64_t* state, uint8_t* marks, int64_t n, int band_px, int32_t* list,
@@ -687,13 +751,28 @@ static void launch_rows(int cus, const Params& P, const Work& w, const RowsOut<d
                        DISK ? d_out.event_hit : nullptr);
 }
 
-// The FP64 rk45 trace of given rows (kr_integrate_batch[_dev], and the disk
-// render): the row plan, then the persistent trace of the longest-first work
-// list, exactly as the planned image trace runs it; every row (and disk
+// One launch of rows_refill_kernel, two wavefronts per SIMD on every slot.
+template <typename T, bool ADAPTIVE, bool DISK>
+static void launch_refill(int cus, const Params& P, const Work& w, const kr_outputs& d_out, const double* d_Y0,
+                          int64_t n, hipStream_t s) {
+    RowsOut<T> o{static_cast<T*>(d_out.Y), d_out.lam, d_out.n_steps, d_out.n_rejected, d_out.state,
+                 d_out.max_null_error, d_out.max_energy_drift, d_out.max_lz_drift, d_out.max_carter_drift};
+    hipLaunchKernelGGL((rows_refill_kernel<T, ADAPTIVE, DISK>), dim3(cus * (4 * 2 * 64 / 256)), dim3(256), 0, s, P,
+                       w.work, w.counters, 8, o, DISK ? static_cast<T*>(d_out.event_Y) : nullptr,
+                       DISK ? d_out.event_hit : nullptr, d_Y0, n);
+}
+
+// The Dormand-Prince trace of given rows (kr_integrate_batch[_dev], and the
+// disk render): the row plan, then the persistent trace of the longest-first
+// work list, in FP64 on planned_trace_kernel (exactly as the planned image
+// trace runs it), in float32 on rows_refill_kernel.  Every row (and disk
 // event) equals the plain kernel's.
-kr_status kr::host::enqueue_rows_planned(kr_context* h, const Params& P, const double* d_Y0, int64_t n,
+kr_status kr::host::enqueue_rows_planned(kr_context* h, const Params& P, int dtype, const double* d_Y0, int64_t n,
                                          const kr_outputs& d_out, void* d_work, hipStream_t s) {
     if (n <= 0) return KR_OK;
+    if (P.ipar[I_ADAPTIVE] == 0) return host::fail(KR_ERR_UNSUPPORTED, "the row planner takes Dormand-Prince only");
+    if (P.ipar[I_DISK] && (!d_out.event_Y || !d_out.event_hit))
+        return host::fail(KR_ERR_INVALID, "the disk event needs event_Y and event_hit");
     int cus = 0;
     KR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
     Work w = carve_work(d_work, n, 0);
@@ -709,6 +788,14 @@ kr_status kr::host::enqueue_rows_planned(kr_context* h, const Params& P, const d
     hipLaunchKernelGGL(plan_scatter_kernel, dim3(plan_blocks), dim3(PLAN_BLOCK), 0, s, n, w.counters, w.used_fp64,
                        w.work, 0, 1);
     KR_HIP(hipGetLastError());
+    if (dtype != KR_FLOAT64) {
+        if (P.ipar[I_DISK])
+            launch_refill<float, true, true>(cus, P, w, d_out, d_Y0, n, s);
+        else
+            launch_refill<float, true, false>(cus, P, w, d_out, d_Y0, n, s);
+        KR_HIP(hipGetLastError());
+        return KR_OK;
+    }
     RowsOut<double> o{static_cast<double*>(d_out.Y), d_out.lam, d_out.n_steps, d_out.n_rejected, d_out.state,
                       d_out.max_null_error, d_out.max_energy_drift, d_out.max_lz_drift, d_out.max_carter_drift};
     // The planned image trace's automatic occupancy: one wavefront per SIMD up
@@ -716,8 +803,6 @@ kr_status kr::host::enqueue_rows_planned(kr_context* h, const Params& P, const d
     constexpr int BLOCK = 128;
     const bool two = n > (int64_t(1) << 19);
     if (P.ipar[I_DISK]) {
-        if (!d_out.event_Y || !d_out.event_hit)
-            return host::fail(KR_ERR_INVALID, "the disk event needs event_Y and event_hit");
         if (two)
             launch_rows<BLOCK, 2, true, true>(cus, P, w, o, d_Y0, n, d_out, s);
         else

@@ -624,17 +624,19 @@ kr_status copy_back(kr_handle h, const BatchBuffers& b, const kr_outputs& out, i
 
 namespace {
 
-// FP64 Dormand-Prince batches of at least this many rows (four times as many
-// with the disk event, whose photons end at the disk and leave less imbalance
-// to remove) run on the longest-first persistent kernel (kr_mixed.hip); smaller
-// ones, and every other option, on the plain one-photon-per-lane kernel.
-// Both give the same rows and events bit for bit.
+// Dormand-Prince batches of at least this many rows run on the longest-first
+// persistent kernels (kr_mixed.hip): FP64 from 2^15 rows, 2^17 with the disk
+// event (whose photons end at the disk plane, which leaves less imbalance to
+// remove), float32 from 2^17 (at 256^2 neither gains).  Smaller ones, and
+// RK4 (whose photons take about as many steps each), run on the plain
+// one-photon-per-lane kernel.  Both give the same rows and events bit for bit.
 constexpr int64_t ROWS_PLANNED_MIN = int64_t(1) << 15;
-constexpr int64_t DISK_ROWS_PLANNED_MIN = int64_t(1) << 17;
+constexpr int64_t ROWS_PLANNED_MIN_WIDE = int64_t(1) << 17;
 
 bool rows_planned(const Params& P, int dtype, int64_t n) {
-    const int64_t min = P.ipar[I_DISK] ? DISK_ROWS_PLANNED_MIN : ROWS_PLANNED_MIN;
-    return dtype == KR_FLOAT64 && P.ipar[I_ADAPTIVE] == 1 && n >= min;
+    if (P.ipar[I_ADAPTIVE] != 1) return false;
+    const bool fp64 = dtype == KR_FLOAT64;
+    return n >= (fp64 && !P.ipar[I_DISK] ? ROWS_PLANNED_MIN : ROWS_PLANNED_MIN_WIDE);
 }
 
 // The workspace of a planned disk render: the pixels' initial states (n, 8),
@@ -653,7 +655,7 @@ kr_status disk_trace(kr_handle h, const Params& P, int dtype, const BardeenImage
         hipLaunchKernelGGL(image_states_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, B, Y0,
                            flags);
         KR_HIP(hipGetLastError());
-        return host::enqueue_rows_planned(h, P, Y0, n, o, ws + round256(static_cast<size_t>(n) * 64), s);
+        return host::enqueue_rows_planned(h, P, dtype, Y0, n, o, ws + round256(static_cast<size_t>(n) * 64), s);
     }
     ICArgs ic{};
     ic.B = B;
@@ -677,7 +679,7 @@ extern "C" kr_status kr_integrate_batch_dev(kr_handle h, const kr_params* p, con
     const hipStream_t s = static_cast<hipStream_t>(stream);
     if (rows_planned(P, p->dtype, n)) {
         if ((st = host::reserve_ws(h, host::rows_workspace_bytes(n))) != KR_OK) return st;
-        return host::enqueue_rows_planned(h, P, d_Y0, n, *d_out, h->ws, s);
+        return host::enqueue_rows_planned(h, P, p->dtype, d_Y0, n, *d_out, h->ws, s);
     }
     ICArgs ic{};
     ic.Y0 = d_Y0;
@@ -704,7 +706,7 @@ extern "C" kr_status kr_integrate_batch(kr_handle h, const kr_params* p, const d
     ic.Y0 = b.Y0;
     KR_HIP(hipEventRecord(h->ev0, s));
     if (planned) {
-        if ((st = host::enqueue_rows_planned(h, P, b.Y0, n, b.d, b.extra, s)) != KR_OK) return st;
+        if ((st = host::enqueue_rows_planned(h, P, p->dtype, b.Y0, n, b.d, b.extra, s)) != KR_OK) return st;
     } else {
         KR_HIP(launch_trace<IC_Y0>(P, p->dtype, ic, b.d, n, s));
     }

@@ -10,10 +10,11 @@
 * ``redshift_factor`` on the device against the oracle to a few ulp, and the
   errors the reference raises.
 * The FP32 working dtype renders the same hit set as the oracle's FP32 mode.
-* FP64 renders of at least 2^17 pixels trace on the longest-first persistent
-  kernel over the image's initial states (kr_mixed.hip); every row and every
-  disk event equals the plain image kernel's bit for bit, on the host and
-  device entry points and through integrate_batch with the disk event.
+* Renders of at least 2^17 pixels (FP64 and float32 Dormand-Prince) trace on
+  the longest-first persistent kernels over the image's initial states
+  (kr_mixed.hip); every row and every disk event equals the plain image
+  kernel's bit for bit, on the host and device entry points and through
+  integrate_batch with the disk event.
 """
 
 from __future__ import annotations
@@ -137,14 +138,18 @@ def _plain_trace(hip, st, cam, integ, term, disk):
     return out.result(0.0), ms.value
 
 
-@pytest.mark.parametrize("camera,res", [("zamo", 364), ("bardeen", 366)])
-def test_planned_disk_render_equals_the_plain_kernel(hip, camera, res) -> None:
+@pytest.mark.parametrize("camera,res,dtype", [("zamo", 364, "float64"), ("bardeen", 366, "float64"),
+                                              ("zamo", 364, "float32")])
+def test_planned_disk_render_equals_the_plain_kernel(hip, camera, res, dtype) -> None:
     from kerrray_amd.camera import BardeenCamera, bardeen_initial_states, initial_states
     from kerrray_amd.renderer import DiskParams
 
     st = Spacetime(1.0, 0.9)
     disk = DiskModel.from_params(st, DiskParams())
-    integ, term = options(rtol=1e-7, atol=1e-9)
+    if dtype == "float64":
+        integ, term = options(rtol=1e-7, atol=1e-9)
+    else:
+        integ, term = options(rtol=1e-6, atol=1e-8, dtype="float32", horizon_epsilon=1e-3)
     if camera == "zamo":
         cam = Camera(radius=1000.0, inclination_deg=75.0, fov=25.0, resolution=res)
         Y0 = initial_states(cam, st, backend=hip)
@@ -157,7 +162,7 @@ def test_planned_disk_render_equals_the_plain_kernel(hip, camera, res) -> None:
     rows = hip.integrate_batch(st, Y0, integ, term, event_options(disk))
     rows_ms = hip.last_kernel_ms
     hits = int(np.sum(ref.event_hit))
-    print(f"[disk {camera} {res}^2] planned render {img.kernel_ms:.3f} ms (trace + shading), plain image "
+    print(f"[disk {camera} {dtype} {res}^2] planned render {img.kernel_ms:.3f} ms (trace + shading), plain image "
           f"kernel {plain_ms:.3f} ms, planned rows {rows_ms:.3f} ms; {hits} disk hits", flush=True)
     assert hits > 1000
     for name in FIELDS:

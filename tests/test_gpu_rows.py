@@ -1,7 +1,8 @@
 """Given initial states (the reference's trace_rays -> integrate_batch path) on the longest-first kernel.
 
-FP64 Dormand-Prince batches of at least 2^15 rows run on the persistent
-kernel with a work list ordered from each row's constants of motion
+Dormand-Prince batches of at least 2^15 rows (FP64) or 2^17 rows (float32)
+run on a persistent kernel with a work list ordered from each row's
+constants of motion
 (kr_mixed.hip: plan_row). The order changes no photon's arithmetic: every
 row must equal the plain one-photon-per-lane kernel's bit for bit, whatever
 the order of the rows and whichever camera made them.
@@ -28,13 +29,18 @@ def _plain_image(hip, st, cam, integ, term):
     return img.result, hip.last_kernel_ms
 
 
-@pytest.mark.parametrize("camera", ["bardeen", "zamo"])
-def test_rows_equal_the_plain_kernel_at_512(hip, golden2, camera) -> None:
+@pytest.mark.parametrize("camera,dtype", [("bardeen", "float64"), ("zamo", "float64"), ("bardeen", "float32"),
+                                          ("zamo", "float32")])
+def test_rows_equal_the_plain_kernel_at_512(hip, golden2, camera, dtype) -> None:
+    """FP64 rows run on planned_trace_kernel, float32 rows on rows_refill_kernel (2^18 rows: both planned)."""
     from kerrray_amd.camera import BardeenCamera, Camera, bardeen_initial_states, initial_states
     from kerrray_amd.geometry import Spacetime
 
     st = Spacetime(1.0, 0.9)
-    integ, term = options(rtol=1e-6, atol=1e-8)
+    if dtype == "float64":
+        integ, term = options(rtol=1e-6, atol=1e-8)
+    else:
+        integ, term = options(rtol=1e-6, atol=1e-8, dtype="float32", horizon_epsilon=1e-3)
     if camera == "bardeen":
         cam = BardeenCamera(radius=1000.0, inclination_deg=60.0, fov=10.0, resolution=512)
         Y0 = bardeen_initial_states(cam, st)
@@ -46,12 +52,13 @@ def test_rows_equal_the_plain_kernel_at_512(hip, golden2, camera) -> None:
     rows_ms = hip.last_kernel_ms
     perm = np.random.default_rng(7).permutation(len(Y0))
     shuffled = hip.integrate_batch(st, Y0[perm], integ, term)
-    print(f"[rows {camera} 512^2] longest-first rows {rows_ms:.3f} ms, shuffled {hip.last_kernel_ms:.3f} ms, "
+    print(f"[rows {camera} {dtype} 512^2] longest-first rows {rows_ms:.3f} ms, shuffled {hip.last_kernel_ms:.3f} ms, "
           f"plain image kernel {plain_ms:.3f} ms", flush=True)
+    assert rows.Y.dtype == integ.np_dtype
     for name in FIELDS:
         assert np.array_equal(getattr(rows, name), getattr(ref, name)), name
         assert np.array_equal(getattr(shuffled, name), getattr(ref, name)[perm]), name
-    if camera == "bardeen":
+    if camera == "bardeen" and dtype == "float64":
         assert np.array_equal(rows.state, golden2["numpy_state"].astype(np.int64))
 
 
