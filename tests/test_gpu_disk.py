@@ -173,6 +173,36 @@ def test_planned_disk_render_equals_the_plain_kernel(hip, camera, res, dtype) ->
     assert np.all(np.isfinite(img.g.ravel()[hit])) and np.all(np.isnan(img.g.ravel()[~hit]))
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_planned_disk_render_equals_the_plain_kernel_on_random_images(hip, seed) -> None:
+    """Seeded spins, inclinations, fields of view, tolerances, disks, cameras and working dtypes."""
+    from kerrray_amd.camera import BardeenCamera
+    from kerrray_amd.renderer import DiskParams
+
+    rng = np.random.default_rng(1000 + seed)
+    st = Spacetime(1.0, float(rng.uniform(0.0, 0.99)))
+    incl, fov = float(rng.uniform(20.0, 88.0)), float(rng.uniform(15.0, 40.0))
+    disk = DiskModel.from_params(st, DiskParams(r_out=float(rng.uniform(10.0, 40.0))))
+    dtype = "float32" if seed % 3 == 2 else "float64"
+    if dtype == "float64":
+        rtol = float(10.0 ** rng.uniform(-8.0, -5.0))
+        integ, term = options(rtol=rtol, atol=1e-2 * rtol)
+    else:
+        rtol = float(10.0 ** rng.uniform(-6.0, -4.0))
+        integ, term = options(rtol=rtol, atol=1e-2 * rtol, dtype="float32", horizon_epsilon=1e-3)
+    if seed % 2 == 0:
+        cam = Camera(radius=1000.0, inclination_deg=incl, fov=fov, resolution=364)
+    else:
+        cam = BardeenCamera(radius=1000.0, inclination_deg=incl, fov=fov, resolution=366)
+    img = render_disk(st, cam, integ, term, disk, backend=hip)
+    ref, plain_ms = _plain_trace(hip, st, cam, integ, term, disk)
+    print(f"[disk random {seed}] a={st.spin:.3f} i={incl:.1f} fov={fov:.1f} rtol={rtol:.1e} {dtype} "
+          f"{type(cam).__name__}: planned render {img.kernel_ms:.3f} ms, plain trace {plain_ms:.3f} ms, "
+          f"{int(np.sum(ref.event_hit))} hits", flush=True)
+    for name in FIELDS:
+        assert np.array_equal(getattr(img.result, name), getattr(ref, name)), name
+
+
 def test_device_render_entry_point_equals_the_host_one(hip) -> None:
     import torch
 

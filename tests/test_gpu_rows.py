@@ -62,6 +62,33 @@ def test_rows_equal_the_plain_kernel_at_512(hip, golden2, camera, dtype) -> None
         assert np.array_equal(rows.state, golden2["numpy_state"].astype(np.int64))
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_float32_rows_equal_the_plain_kernel_on_random_images(hip, seed) -> None:
+    """Seeded spins, inclinations, fields of view and tolerances; float32 rows of both cameras at 2^17+ rows."""
+    from kerrray_amd.camera import BardeenCamera, Camera, bardeen_initial_states, initial_states
+    from kerrray_amd.geometry import Spacetime
+
+    rng = np.random.default_rng(2000 + seed)
+    st = Spacetime(1.0, float(rng.uniform(0.0, 0.99)))
+    incl, fov = float(rng.uniform(5.0, 89.0)), float(rng.uniform(8.0, 30.0))
+    rtol = float(10.0 ** rng.uniform(-6.0, -4.0))
+    integ, term = options(rtol=rtol, atol=1e-2 * rtol, dtype="float32", horizon_epsilon=1e-3)
+    if seed % 2 == 0:
+        cam = BardeenCamera(radius=1000.0, inclination_deg=incl, fov=fov, resolution=366)
+        Y0 = bardeen_initial_states(cam, st)
+    else:
+        cam = Camera(radius=1000.0, inclination_deg=incl, fov=fov, resolution=364)
+        Y0 = initial_states(cam, st, backend=hip)
+    assert len(Y0) >= 1 << 17
+    ref, plain_ms = _plain_image(hip, st, cam, integ, term)
+    rows = hip.integrate_batch(st, Y0, integ, term)
+    print(f"[rows float32 random {seed}] a={st.spin:.3f} i={incl:.1f} fov={fov:.1f} rtol={rtol:.1e} "
+          f"{type(cam).__name__}: longest-first rows {hip.last_kernel_ms:.3f} ms, plain image {plain_ms:.3f} ms",
+          flush=True)
+    for name in FIELDS:
+        assert np.array_equal(getattr(rows, name), getattr(ref, name)), name
+
+
 def test_device_rows_entry_point_equals_the_host_one(hip) -> None:
     import torch
 
