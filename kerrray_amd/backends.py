@@ -11,7 +11,10 @@ cannot run here. ``"hip"`` is added to the reference's names.
 constants as the reference's Numba wrapper does
 (numba_backend.py:87-128) and hands them, with the float64 initial states,
 to ``kr_integrate_batch`` in ``libkrhip.so``, where one GPU lane integrates
-one photon. This package implements only ``hip``; the reference's ``numpy``
+one photon at a time (Dormand-Prince batches of 2^15 rows and more, 2^17 in
+float32 or with the disk event, on persistent kernels that hand out the
+longest photons first; the rows are the same either way). This package
+implements only ``hip``; the reference's ``numpy``
 and ``numba`` backends belong to the reference package and are reported
 unavailable here rather than emulated.
 """
