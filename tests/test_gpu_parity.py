@@ -350,6 +350,31 @@ def test_fused_bisection_rounds_equal_one_call_per_round(hip, n_probe, n_angles,
     assert np.all(fused.hi - fused.lo < 0.04 * r_ana / (n_probe + 1) ** (rounds - 1))
 
 
+def test_bisection_edge_cases(hip) -> None:
+    """No rounds leaves the brackets as given; the largest one-launch workgroup (256 probes) and the first
+    per-round width (257 probes) both narrow every bracket around Bardeen's curve."""
+    from kerrray_amd.bisection import bisect_edge
+    from kerrray_amd.boundary import polygon_centroid, polygon_radii
+    from kerrray_amd.geometry import Spacetime
+    from kerrray_amd.orbits import shadow_curve
+
+    st, incl = Spacetime(1.0, 0.9), 60.0
+    curve = shadow_curve(st, incl, 4000)
+    centre = polygon_centroid(*curve)
+    angles = 2.0 * np.pi * np.arange(24) / 24
+    r_ana = polygon_radii(curve[0], curve[1], centre, angles)
+    lo0, hi0 = 0.98 * r_ana, 1.02 * r_ana
+    none = bisect_edge(st, incl, 1000.0, 1e-8, centre, angles, lo0, hi0, 0, hip, n_probe=63)
+    assert np.array_equal(none.lo, lo0) and np.array_equal(none.hi, hi0) and none.failures == 0
+    for n_probe in (256, 257):
+        res = bisect_edge(st, incl, 1000.0, 1e-8, centre, angles, lo0, hi0, 2, hip, n_probe=n_probe)
+        print(f"[bisect n_probe={n_probe}] width {res.width.max():.2e}, |r - Bardeen| max "
+              f"{np.max(np.abs(res.radius - r_ana)):.2e}, {res.kernel_ms:.2f} ms", flush=True)
+        assert res.failures == 0 and res.nonmonotone == 0
+        assert np.all(res.width <= 0.04 * r_ana / (n_probe + 1) ** 2 * 1.0001)
+        assert np.all((res.lo <= r_ana * (1 + 1e-6)) & (res.hi >= r_ana * (1 - 1e-6)))
+
+
 def test_kerr_512_every_photon_against_the_oracle(hip, oracle, golden2) -> None:
     """All 262,144 photons of the 512^2, a=0.9, i=60 deg, rtol 1e-6 image, device against the oracle.
 
