@@ -121,9 +121,10 @@ def extract_boundary_from_mask(alpha: ArrayLike, beta: ArrayLike, captured: Arra
                                centre: tuple[float, float] | None = None, backend: Any = "hip") -> tuple[Float, Float]:
     """Sub-pixel edge radii ``r(phi_k)``, ``phi_k = 2 pi k / n_angles`` (boundary.py:116-170).
 
-    The marching runs on the device (``kr_mask_edge_dev``, one lane per
-    direction); the checks, the captured centroid and the directions'
-    cosines and sines are the host's, as the reference computes them.
+    The marching runs on the device (``kr_mask_edge_dev``, one wavefront per
+    direction, 64 samples at a time); the checks, the captured centroid and
+    the directions' cosines and sines are the host's, as the reference
+    computes them.
     """
     import torch
 

@@ -412,18 +412,55 @@ __global__ __launch_bounds__(256) void disk_shade_kernel(DiskShade D, const T* e
     r_hit[idx] = rr;
 }
 
-// Mask edge radii, one lane per direction; the mask (at most a few MB) is
-// read through the caches by every lane.
+// Mask edge radii, one wavefront per direction: the lanes evaluate 64
+// consecutive samples at a time and a ballot finds the first below 0.5, so a
+// direction's march takes n_samples / 64 steps instead of n_samples.  Each
+// sample's value and the crossing are mask_edge_radius's expressions, so the
+// radii are the same bit for bit.  The mask (at most a few MB) is read
+// through the caches.
 __global__ __launch_bounds__(256) void mask_edge_kernel(const uint8_t* mask, int64_t rows, int64_t cols, double a0,
                                                         double b0, double d, double ca, double cb, const double* cosk,
                                                         const double* sink, int64_t n_angles, int64_t n_samples,
                                                         double* radii, int32_t* flags) {
-    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= n_angles) return;
+    const int64_t k = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (k >= n_angles) return;  // whole wavefronts leave together
+    const double ds = 0.125 * d;
+    const double cos_k = cosk[k], sin_k = sink[k];
+    double f_carry = 0.0;  // the value of the last sample of the previous step
     double r = __builtin_nan("");
-    if (!mask_edge_radius(mask, rows, cols, a0, b0, d, ca, cb, cosk[k], sink[k], n_samples, r) && flags)
-        atomicOr(flags, FLAG_EDGE);
-    radii[k] = r;
+    bool found = false;
+    for (int64_t m0 = 0; m0 < n_samples; m0 += 64) {
+        const int64_t m = m0 + lane;
+        double f = 1.0;
+        if (m < n_samples) {
+            const double sm = ds * static_cast<double>(m);
+            const double x = ca + cos_k * sm;
+            const double y = cb + sin_k * sm;
+            f = mask_bilinear(mask, rows, cols, (x - a0) / d, (b0 - y) / d);
+        }
+        const uint64_t below = __ballot(m < n_samples && f < 0.5);
+        const double f_before = __shfl_up(f, 1);  // lane L gets lane L - 1's value
+        if (below != 0ull) {
+            const int first = __ffsll(static_cast<unsigned long long>(below)) - 1;
+            if (lane == first) {
+                if (m > 0) {
+                    const double f_prev = lane == 0 ? f_carry : f_before;
+                    const double s_prev = ds * static_cast<double>(m - 1);
+                    r = s_prev + ds * (f_prev - 0.5) / (f_prev - f);
+                    found = true;
+                }
+                radii[k] = r;
+                if (!found && flags) atomicOr(flags, FLAG_EDGE);
+            }
+            return;
+        }
+        f_carry = __shfl(f, 63);
+    }
+    if (lane == 0) {
+        radii[k] = r;
+        if (flags) atomicOr(flags, FLAG_EDGE);
+    }
 }
 
 __global__ void fill_u64_kernel(unsigned long long* p, int64_t n) {
@@ -874,7 +911,7 @@ extern "C" kr_status kr_mask_edge_dev(kr_handle h, const uint8_t* d_mask, int64_
     if (!(samples <= 1e8)) return host::fail(KR_ERR_INVALID, "reach / (d / 8) must stay below 1e8 samples");
     if (n_angles == 0) return KR_OK;
     KR_HIP(hipSetDevice(h->device));
-    hipLaunchKernelGGL(mask_edge_kernel, dim3(static_cast<unsigned>((n_angles + 255) / 256)), dim3(256), 0,
+    hipLaunchKernelGGL(mask_edge_kernel, dim3(static_cast<unsigned>((n_angles + 3) / 4)), dim3(256), 0,
                        static_cast<hipStream_t>(stream), d_mask, rows, cols, alpha0, beta0, d, centre_alpha,
                        centre_beta, d_cos, d_sin, n_angles, static_cast<int64_t>(samples), d_radii, d_flags);
     KR_HIP(hipGetLastError());
